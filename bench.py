@@ -1,0 +1,238 @@
+#!/usr/bin/env python3
+"""Benchmark: batched linear absorption on one MI355X.
+
+Workload (the headline configuration): N=256 sites on a 16x16 square lattice,
+dipole-dipole couplings J = 8 cm^-1 (a/r)^3 kappa truncated below 0.1 cm^-1,
+AR(1) site energies (mean 1650, sd 10 cm^-1, correlation 100 fs), 50,000
+FP32 Full-Hamiltonian frames at dt = 2 fs, tmax = 256 steps, 4000 start
+samples spaced 10 frames apart, 3 polarisations, Trotter 1.  Synthetic data,
+resident in HBM before the timed region.
+
+One step = one complete absorption run: per-frame rotation/phase tables,
+the frame-major conveyor kernel and the response reduction, replayed as a
+captured hipGraph.  value = site-steps per second, a site-step being one
+amplitude of one (sample, polarisation) vector advanced by one frame.
+
+Also reported on the same line: the roofline of the dominant (conveyor)
+kernel from HIP events on its own stream, the plain one-sample-per-launch
+GPU kernel on a subset of samples, and the CPU restatement of the reference
+FP32 path timed on this host's cores (baseline only).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_site_step(N, tmax, sample, full=True):
+    """B = 16 + (F_H + F_mu) / (N * V_f), V_f = 3 tmax / Samplerate."""
+    F_H = 4 + 4 * (N * (N + 1) // 2 if full else N)
+    F_mu = 4 + 12 * N
+    V_f = 3.0 * tmax / sample
+    return 16.0 + (F_H + F_mu) / (N * V_f)
+
+
+def build_lattice_on_device(L, nframes, seed, torch):
+    import numpy as np
+
+    from nise_2017_amd import synth
+    rng, mus, coup = synth.lattice_static(L, seed=seed)
+    N = L * L
+    nn2 = N * (N + 1) // 2
+    diag = synth.lattice_diag(rng, nframes, N)
+    a, b = np.triu_indices(N)
+    row = np.zeros(1 + nn2, np.float32)
+    row[1 + np.nonzero(a != b)[0]] = coup
+    H = torch.empty((nframes, 1 + nn2), dtype=torch.float32, device="cuda")
+    H.copy_(torch.from_numpy(row).cuda().expand(nframes, -1))
+    didx = torch.from_numpy(1 + np.nonzero(a == b)[0]).cuda()
+    H[:, didx] = torch.from_numpy(diag).cuda()
+    mu = torch.from_numpy(synth.dipole_records(mus, 1)[0]).cuda().expand(nframes, -1).contiguous()
+    return H, mu, diag, coup, mus, row
+
+
+def host_records(diag, row, mus, f_hi):
+    import numpy as np
+
+    from nise_2017_amd import synth
+    N = diag.shape[1]
+    a, b = np.triu_indices(N)
+    H = np.repeat(row[None, :], f_hi, axis=0)
+    H[:, 1 + np.nonzero(a == b)[0]] = diag[:f_hi]
+    return H, synth.dipole_records(mus, f_hi)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--samples", type=int, default=4000)
+    ap.add_argument("--frames", type=int, default=50000)
+    ap.add_argument("--lattice", type=int, default=16)
+    ap.add_argument("--tmax", type=int, default=256)
+    ap.add_argument("--samplerate", type=int, default=10)
+    ap.add_argument("--plain-samples", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--shape", default="", help="vp,spw conveyor shape override")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the sharding plan across GPUs is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from nise_2017_amd import abi
+
+    torch.cuda.set_device(0)
+    L, nfr, T, sr, ns = args.lattice, args.frames, args.tmax, args.samplerate, args.samples
+    N = L * L
+    need = (ns - 1) * sr + T
+    if need > nfr:
+        raise SystemExit(f"{ns} samples need {need} frames, have {nfr}")
+    log(f"generating {nfr} frames, N={N} on the device")
+    H, mu, diag, coup, mus, row = build_lattice_on_device(L, nfr, 20261202 + 1, torch)
+    mean = float(diag[:need].mean())
+    lo, hi = mean - 100.0, mean + 100.0
+    non = abi.make_non(singles=N, tmax1=T, sample=sr, begin=0, end=ns, ts=1, propagation=1,
+                       deltat=2.0, min1=lo, max1=hi, couplingcut=0.0, fft=2048)
+    non.shifte = np.float32((np.float32(hi) + np.float32(lo)) / np.float32(2))
+    torch.cuda.synchronize()
+
+    log("building the plan (coupling structure, rotation schedule)")
+    plan = abi.AbsPlan(non, H, H.shape[1], mu, mu.shape[1], nfr, where=abi.NISE_DEVICE, kernel=0)
+    if args.shape:
+        vp, spw = (int(x) for x in args.shape.split(","))
+        plan.set_shape(vp, spw)
+    info = plan.info()
+    log(f"plan: {info}")
+
+    # per-kernel timing with HIP events on the plan's stream
+    plan.execute(flags=1, fetch=False)
+    conv_ms, tab_ms = [], []
+    for _ in range(max(1, min(3, args.steps))):
+        plan.execute(flags=1, fetch=False)
+        t = plan.timing()
+        conv_ms.append(t["conveyor_ms"])
+        tab_ms.append(t["tables_ms"])
+    log(f"event timing: tables {np.mean(tab_ms):.3f} ms, conveyor {np.mean(conv_ms):.3f} ms")
+
+    for _ in range(args.warmup):
+        plan.execute(flags=2, fetch=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.execute(flags=2, fetch=False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    re_g, im_g = plan.execute(flags=0)
+    site_steps = ns * 3 * T * N
+    value = site_steps / (ms_per_step * 1e-3)
+    log(f"{ms_per_step:.3f} ms per run, {value:.3e} site-steps/s")
+
+    B = algorithmic_bytes_per_site_step(N, T, sr)
+    conv = float(np.mean(conv_ms))
+    achieved = site_steps * B / (conv * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "conveyor_kernel", "kernel_ms": round(conv, 4),
+                "bytes_per_site_step": round(B, 3)}
+    tf = os.path.join(ROOT, "profiles", "conveyor_lattice256_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            roofline["traffic"] = json.load(fh).get("bytes_per_launch")
+
+    extra = {}
+    if not args.no_plain and args.plain_samples > 0:
+        nps = args.plain_samples
+        nonp = abi.make_non(singles=N, tmax1=T, sample=sr, begin=0, end=nps, ts=1, propagation=1,
+                            deltat=2.0, min1=lo, max1=hi)
+        nonp.shifte = non.shifte
+        pp = abi.AbsPlan(nonp, H, H.shape[1], mu, mu.shape[1], nfr, where=abi.NISE_DEVICE,
+                         kernel=1)
+        pp.execute(flags=1, fetch=False)
+        pms = pp.timing()["conveyor_ms"]
+        pp.close()
+        plain_rate = nps * 3 * T * N / (pms * 1e-3)
+        # frame-major on the same subset, for a like-for-like ratio
+        pf = abi.AbsPlan(nonp, H, H.shape[1], mu, mu.shape[1], nfr, where=abi.NISE_DEVICE,
+                         kernel=0)
+        pf.execute(flags=1, fetch=False)
+        pf.execute(flags=1, fetch=False)
+        fms = sum(pf.timing()[k] for k in ("tables_ms", "conveyor_ms", "reduce_ms"))
+        pf.close()
+        extra["plain_kernel"] = {"value": plain_rate, "samples": nps, "ms": round(pms, 3),
+                                 "batched_same_subset_ms": round(fms, 3),
+                                 "batched_over_plain_full_run": round(value / plain_rate, 2)}
+        log(f"plain kernel: {plain_rate:.3e} site-steps/s on {nps} samples")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        threads = max(1, min(16, os.cpu_count() or 1))
+        f_need = lambda k: (k - 1) * sr + T  # noqa: E731
+        Hh, muh = host_records(diag, row, mus, f_need(1))
+        non1 = abi.make_non(singles=N, tmax1=T, sample=sr, begin=0, end=1, ts=1, propagation=1,
+                            deltat=2.0, min1=lo, max1=hi)
+        non1.shifte = non.shifte
+        t_a = time.perf_counter()
+        oracle.absorption(non1, Hh, muh, nthreads=1)
+        per_sample = time.perf_counter() - t_a
+        k = int(max(threads, min(ns, round(args.cpu_seconds * threads / per_sample))))
+        Hh, muh = host_records(diag, row, mus, f_need(k))
+        nonk = abi.make_non(singles=N, tmax1=T, sample=sr, begin=0, end=k, ts=1, propagation=1,
+                            deltat=2.0, min1=lo, max1=hi)
+        nonk.shifte = non.shifte
+        t_a = time.perf_counter()
+        o_re, o_im = oracle.absorption(nonk, Hh, muh, nthreads=threads)
+        dt = time.perf_counter() - t_a
+        cpu = {"value": k * 3 * T * N / dt, "unit": "site-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{k} of {ns} samples x 3 pols x {T} steps at N={N}, trajectory "
+                         f"preloaded, OpenMP over samples, {dt:.1f} s",
+               "single_thread_value": 3 * T * N / per_sample}
+        log(f"cpu baseline: {cpu['value']:.3e} site-steps/s on {threads} threads")
+        # spectral check of the GPU run against the oracle on the same samples
+        pc = abi.AbsPlan(nonk, H, H.shape[1], mu, mu.shape[1], nfr, where=abi.NISE_DEVICE)
+        g_re, g_im = pc.execute()
+        pc.close()
+        err = float(np.linalg.norm((g_re - o_re) + 1j * (g_im - o_im)) /
+                    np.linalg.norm(o_re + 1j * o_im))
+        extra["parity_rel_l2_vs_fp32_oracle"] = err
+
+    line = {
+        "metric": "site-steps/s propagated (N=256 absorption) vs plain GPU; adjoint/forward cost",
+        "value": value, "unit": "site-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"absorption N={N} {L}x{L} dipole lattice, Full FP32 frames, "
+                               f"{nfr} frames, tmax {T}, {ns} samples every {sr} frames, 3 pols, "
+                               f"Trotter 1", "N": N, "frames": nfr, "tmax": T,
+                   "samples": ns, "samplerate": sr, "shape": {"vp": info["vp"], "spw": info["spw"]},
+                   "K": info["K"], "layers": info["nlayers"]},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

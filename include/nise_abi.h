@@ -1,0 +1,218 @@
+/*
+ * nise_abi.h -- C ABI of libnise_hip.so, the MI355X (gfx950) implementation of
+ * NISE's wavefunction-propagation hot path.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in symbols with the exact C signatures of the reference
+ *     (GHlacour/NISE_2017).  An unchanged NISE host program links these in
+ *     place of the reference's definitions.  They keep the reference's
+ *     conventions: caller-owned host arrays, split real/imaginary float
+ *     vectors updated in place, the packed upper-triangle Hamiltonian with the
+ *     RWA-shifted diagonal, void return, print-and-exit on fatal errors.
+ *     Every one of them computes on the GPU; none has a CPU fallback.
+ *
+ *  2. New batched entry points (prefix nise_) that keep trajectories resident
+ *     in HBM and run the frame-major kernels.  They return an int status
+ *     (NISE_OK == 0) and never exit the process.
+ *
+ * If the reference's types.h is included before this header, its t_non is
+ * used (the definition below is byte-identical to it, src/types.h:37-90).
+ */
+#ifndef NISE_ABI_H
+#define NISE_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* t_non: run parameters.  Field order and sizes follow src/types.h:37-90 so  */
+/* that a t_non built by the reference's readInput() can be passed as is.    */
+/* ------------------------------------------------------------------------ */
+#ifndef _TYPES_
+typedef struct {
+  int tmax1, tmax2, tmax3;
+  int tmin1, tmin2, tmin3;
+  int dt1, dt2, dt3;
+  int d1, d2, d3;
+  float min1, max1;
+  float min2, max2;
+  float min3, max3;
+  int k[4];
+  int length;
+  int sample;
+  int fft;
+  int begin;
+  int end;
+  int is;
+  int ts;
+  int interpol;
+  int propagation;
+  float lifetime;
+  float homogen;
+  float inhomogen;
+  float deltat;
+  int buffer, singles, doubles;
+  char energyFName[256];
+  char dipoleFName[256];
+  char alphaFName[256];
+  char positionFName[256];
+  char anharFName[256];
+  char overdipFName[256];
+  char singleShiftFName[256];
+  char technique[256];
+  char basis[256];
+  char pdbFName[256];
+  char hamiltonian[256];
+  char couplingFName[256];
+  char pbcFName[256];
+  int tmax;
+  int cluster;
+  float shifte, shiftf;
+  int labPol;
+  float dephasing;
+  float statstart, statend, statstep;
+  int statsteps;
+  float thres;
+  float couplingcut;
+  float temperature;
+  float anharmonicity;
+  int Npsites;
+  int *psites;
+  int window;
+  int SingleShiftSites, *SingleShiftSite;
+  float *SingleShift;
+  int printLevel;
+} t_non;
+#endif
+
+/* MPI communicator handles appear only in calc_2DIR's signature and are
+ * never dereferenced here (one GPU, rank 0 does all work).  Without mpi.h
+ * they are declared with MPICH's integer handle type; both an int and a
+ * pointer handle occupy one argument slot in the SysV x86-64 ABI. */
+#if defined(NISE_ABI_WITH_MPI)
+#include <mpi.h>
+#elif !defined(MPI_VERSION)
+typedef int MPI_Comm;
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* 1. Drop-in symbols (reference signatures).                                */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces src/propagate.c:20-39 (prototype src/propagate.h:4).  Dispatch on
+ * non->propagation: 1 -> Trotter coupling scheme, 0 with thres==0 or >1 ->
+ * exact exponential (DIA).  Other schemes (RK4, sparse DIA_S) are outside
+ * this library and report an error. */
+void propagate_vector(t_non *non, float *Hamil_i_e, float *vecr, float *veci,
+                      int sign, int samples, int display);
+
+/* Replaces src/propagate.c:43-52 (prototype src/propagate.h:5): propagates
+ * the non->singles column vectors of an N x N block in one launch. */
+void propagate_matrix(t_non *non, float *Hamil_i_e, float *vecr, float *veci,
+                      int sign, int samples, int display);
+
+/* Replaces src/propagate.c:821-899 (prototype src/propagate.h:13). */
+void propagate_vec_coupling_S(t_non *non, float *Hamiltonian_i, float *cr,
+                              float *ci, int m, int sign);
+
+/* Replaces src/propagate.c:902-1048 (prototype src/propagate.h:14-15). */
+void propagate_vec_coupling_S_doubles(t_non *non, float *Hamiltonian_i,
+                                      float *cr, float *ci, int m, float *Anh);
+
+/* Replaces src/propagate.c:60-125 (prototype src/propagate.h:7). */
+void propagate_vec_DIA(t_non *non, float *Hamiltonian_i, float *cr, float *ci,
+                       int sign);
+
+/* Replaces src/propagate.c:1185-1221 (prototype src/propagate.h:17):
+ * H is a dense row-major symmetric N x N matrix.  On return
+ * H[i*N + j] is component i of eigenvector j (the layout the reference
+ * leaves after its transpose copy) and v holds the ascending eigenvalues.
+ * Computed with the batched Jacobi kernel. */
+void diagonalizeLPD(float *H, float *v, int N);
+
+/* Replaces src/propagate.c:1476-1591 (prototype src/propagate.h:22-25) for
+ * non->propagation == 1: doubles propagation of fr and ft1r[0..tmax1) and
+ * sign -1 singles propagation of rightnr and rightrr[0..tmax1), one launch
+ * each. */
+void propagate_double_2DIR_control(t_non *non, float *Hamil_i_e, float **ft1r,
+                                   float **ft1i, float *fr, float *fi,
+                                   float **rightrr, float **rightri,
+                                   float *rightnr, float *rightni,
+                                   int currentSample, int molPol, int t3,
+                                   float *Anh);
+
+/* Replaces src/absorption.c:205-212 (prototype src/absorption.h:4). */
+void calc_S1(float *re_S_1, float *im_S_1, int t1, t_non *non, float *cr,
+             float *ci, float *mu);
+
+/* Replaces src/absorption.c:18-202 (prototype src/absorption.h:3): reads the
+ * trajectory named in non, runs the batched frame-major kernel, writes
+ * TD_Absorption.dat and Absorption.dat. */
+void absorption(t_non *non);
+
+/* Replaces src/1DFFT.c:144-274 (prototype src/1DFFT.h:5). */
+void do_1DFFT(t_non *non, char fname[], float *re_S_1, float *im_S_1,
+              int samples);
+
+/* Replaces src/calc_2DIR.c:22-629 (prototype src/calc_2DIR.h:5).  Rank 0
+ * computes every (sample, molPol) work item on its GPU and writes the six
+ * R{par,per,cro}{I,II}.dat files; other ranks return at once. */
+void calc_2DIR(t_non *non, int parentRank, int parentSize, int subRank,
+               int subSize, MPI_Comm subComm, MPI_Comm rootComm);
+
+/* ------------------------------------------------------------------------ */
+/* 2. Batched entry points (no counterpart in the reference).                */
+/* ------------------------------------------------------------------------ */
+
+enum {
+  NISE_OK = 0,
+  NISE_ERR_ARG = 1,       /* invalid argument or unsupported setting     */
+  NISE_ERR_HIP = 2,       /* HIP runtime error                           */
+  NISE_ERR_NOMEM = 3,     /* device allocation failed                    */
+  NISE_ERR_RANGE = 4,     /* trajectory too short for the requested run  */
+  NISE_ERR_NUMERIC = 5    /* non-finite input or eigensolver failure     */
+};
+
+/* Trajectory record layout (GROBIN, the reference's binary format): every
+ * frame is one int32 header followed by float32 payload.  A pointer
+ * argument named *_rec points at the header of frame 0; *_stride is the
+ * record length in 4-byte words.
+ *   Full Hamiltonian:   stride >= 1 + N(N+1)/2, packed upper triangle
+ *   Coupling-type H:    stride >= 1 + N, diagonal only
+ *   Dipole:             stride >= 1 + 3N, x block, y block, z block      */
+
+/* Where the pointer arguments of a batched call live. */
+enum { NISE_HOST = 0, NISE_DEVICE = 1 };
+
+/* Batched linear absorption (src/absorption.c:92-146 over all samples in
+ * [non->begin, non->end) and the three polarisations).  Uses non->singles,
+ * tmax1, sample, begin, end, ts, deltat, couplingcut, shifte, hamiltonian
+ * ("Full" or "Coupling"), SingleShift*.  For "Coupling" the Hamiltonian
+ * records carry the diagonal and coupling_rec is one Full record with the
+ * static couplings; the dipole of frame 0 is used at all times.
+ * re_S/im_S (host, length tmax1) receive the sums over samples and
+ * polarisations, un-normalised, exactly what the reference accumulates.
+ * kernel: 0 = frame-major batched kernel (default), 1 = plain
+ * one-sample-per-launch kernel (baseline). */
+int nise_absorption_batched(const t_non *non, const float *H_rec,
+                            int64_t H_stride, const float *mu_rec,
+                            int64_t mu_stride, int64_t nframes,
+                            const float *coupling_rec, int where,
+                            int kernel, float *re_S, float *im_S);
+
+/* Last error message of the calling thread (static storage). */
+const char *nise_last_error(void);
+
+/* Library version string. */
+const char *nise_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NISE_ABI_H */

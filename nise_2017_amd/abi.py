@@ -1,0 +1,191 @@
+"""ctypes view of libnise_hip.so (include/nise_abi.h).
+
+The library is the product: this module only loads it and declares the
+argument types.  Loading fails loudly when the shared object has not been
+built (python -m nise_2017_amd.build).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnise_hip.so")
+
+NISE_OK, NISE_ERR_ARG, NISE_ERR_HIP, NISE_ERR_NOMEM, NISE_ERR_RANGE, NISE_ERR_NUMERIC = range(6)
+NISE_HOST, NISE_DEVICE = 0, 1
+
+_S256 = C.c_char * 256
+
+
+class TNon(C.Structure):
+    """Byte-identical mirror of the reference's t_non (src/types.h:37-90)."""
+
+    _fields_ = [
+        ("tmax1", C.c_int), ("tmax2", C.c_int), ("tmax3", C.c_int),
+        ("tmin1", C.c_int), ("tmin2", C.c_int), ("tmin3", C.c_int),
+        ("dt1", C.c_int), ("dt2", C.c_int), ("dt3", C.c_int),
+        ("d1", C.c_int), ("d2", C.c_int), ("d3", C.c_int),
+        ("min1", C.c_float), ("max1", C.c_float),
+        ("min2", C.c_float), ("max2", C.c_float),
+        ("min3", C.c_float), ("max3", C.c_float),
+        ("k", C.c_int * 4),
+        ("length", C.c_int), ("sample", C.c_int), ("fft", C.c_int),
+        ("begin", C.c_int), ("end", C.c_int), ("is_", C.c_int), ("ts", C.c_int),
+        ("interpol", C.c_int), ("propagation", C.c_int),
+        ("lifetime", C.c_float), ("homogen", C.c_float), ("inhomogen", C.c_float),
+        ("deltat", C.c_float),
+        ("buffer", C.c_int), ("singles", C.c_int), ("doubles", C.c_int),
+        ("energyFName", _S256), ("dipoleFName", _S256), ("alphaFName", _S256),
+        ("positionFName", _S256), ("anharFName", _S256), ("overdipFName", _S256),
+        ("singleShiftFName", _S256), ("technique", _S256), ("basis", _S256),
+        ("pdbFName", _S256), ("hamiltonian", _S256), ("couplingFName", _S256),
+        ("pbcFName", _S256),
+        ("tmax", C.c_int), ("cluster", C.c_int),
+        ("shifte", C.c_float), ("shiftf", C.c_float),
+        ("labPol", C.c_int), ("dephasing", C.c_float),
+        ("statstart", C.c_float), ("statend", C.c_float), ("statstep", C.c_float),
+        ("statsteps", C.c_int), ("thres", C.c_float), ("couplingcut", C.c_float),
+        ("temperature", C.c_float), ("anharmonicity", C.c_float),
+        ("Npsites", C.c_int), ("psites", C.POINTER(C.c_int)),
+        ("window", C.c_int),
+        ("SingleShiftSites", C.c_int), ("SingleShiftSite", C.POINTER(C.c_int)),
+        ("SingleShift", C.POINTER(C.c_float)),
+        ("printLevel", C.c_int),
+    ]
+
+
+def make_non(**kw):
+    """t_non with the reference's readInput defaults (src/readinput.c:25-44)
+    plus the given fields; strings are encoded to char[256]."""
+    n = TNon()
+    n.interpol, n.begin, n.end, n.is_, n.ts = 1, 0, 0, 0, 5
+    n.anharmonicity, n.couplingcut, n.temperature, n.cluster = 0.0, 0.0, 300.0, -1
+    n.fft, n.printLevel, n.tmax1, n.tmax2, n.tmax3 = 0, 1, 256, 0, 256
+    n.homogen, n.inhomogen, n.window = 0.0, 0.0, 0
+    n.basis = b"Local"
+    n.hamiltonian = b"Full"
+    n.dt1 = n.dt2 = n.dt3 = 1
+    keep = []
+    for k, v in kw.items():
+        if isinstance(v, str):
+            v = v.encode()
+        if k == "single_shift":  # list of (site, a, b)
+            sites = (C.c_int * len(v))(*[int(s) for s, _, _ in v])
+            ab = (C.c_float * (2 * len(v)))(*[float(x) for _, a, b in v for x in (a, b)])
+            n.SingleShiftSites = len(v)
+            n.SingleShiftSite = C.cast(sites, C.POINTER(C.c_int))
+            n.SingleShift = C.cast(ab, C.POINTER(C.c_float))
+            keep += [sites, ab]
+            continue
+        setattr(n, k, v)
+    if "tmax" not in kw:
+        n.tmax = n.tmax1
+    n._keep = keep  # keep ctypes arrays alive with the struct
+    return n
+
+
+_lib = None
+
+_fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+def lib():
+    """Load libnise_hip.so; raise if it is missing (no fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with python -m nise_2017_amd.build")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER(TNon)
+    L.nise_last_error.restype = C.c_char_p
+    L.nise_version.restype = C.c_char_p
+    L.nise_absorption_batched.argtypes = [P, _vp, C.c_int64, _vp, C.c_int64, C.c_int64, _vp,
+                                          C.c_int, C.c_int, _fp, _fp]
+    L.nise_abs_plan_create.argtypes = [C.POINTER(C.c_void_p), P, _vp, C.c_int64, _vp, C.c_int64,
+                                       C.c_int64, _vp, C.c_int, C.c_int]
+    L.nise_abs_plan_execute.argtypes = [_vp, _vp, _vp, C.c_int]
+    L.nise_abs_plan_set_shape.argtypes = [_vp, C.c_int, C.c_int]
+    L.nise_abs_plan_info.argtypes = [_vp, C.POINTER(C.c_int64), C.c_int]
+    L.nise_abs_plan_last_timing.argtypes = [_vp, C.POINTER(C.c_float), C.c_int]
+    L.nise_abs_plan_sync.argtypes = [_vp]
+    L.nise_abs_plan_destroy.argtypes = [_vp]
+    L.nise_abs_plan_destroy.restype = None
+    for name in ("propagate_vec_coupling_S",):
+        getattr(L, name).argtypes = [P, _fp, _fp, _fp, C.c_int, C.c_int]
+        getattr(L, name).restype = None
+    L.propagate_vector.argtypes = [P, _fp, _fp, _fp, C.c_int, C.c_int, C.c_int]
+    L.propagate_vector.restype = None
+    L.propagate_matrix.argtypes = [P, _fp, _fp, _fp, C.c_int, C.c_int, C.c_int]
+    L.propagate_matrix.restype = None
+    L.propagate_vec_DIA.argtypes = [P, _fp, _fp, _fp, C.c_int]
+    L.propagate_vec_DIA.restype = None
+    L.diagonalizeLPD.argtypes = [_fp, _fp, C.c_int]
+    L.diagonalizeLPD.restype = None
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != NISE_OK:
+        raise RuntimeError(f"libnise_hip error {rc}: {lib().nise_last_error().decode()}")
+
+
+def ptr(a):
+    """Device or host address of a numpy array or torch tensor."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class AbsPlan:
+    """Batched absorption plan over a trajectory (host or device records)."""
+
+    def __init__(self, non, H_rec, H_stride, mu_rec, mu_stride, nframes, coupling_rec=None,
+                 where=NISE_HOST, kernel=0):
+        L = lib()
+        self._keep = (non, H_rec, mu_rec, coupling_rec)
+        h = C.c_void_p()
+        check(L.nise_abs_plan_create(C.byref(h), C.byref(non), ptr(H_rec), H_stride, ptr(mu_rec),
+                                     mu_stride, nframes, ptr(coupling_rec), where, kernel))
+        self.h = h
+        self.tmax = non.tmax1
+
+    def set_shape(self, vp, spw):
+        check(lib().nise_abs_plan_set_shape(self.h, vp, spw))
+
+    def execute(self, flags=0, fetch=True):
+        re = np.zeros(self.tmax, np.float32)
+        im = np.zeros(self.tmax, np.float32)
+        if fetch:
+            check(lib().nise_abs_plan_execute(self.h, ptr(re), ptr(im), flags))
+            return re, im
+        check(lib().nise_abs_plan_execute(self.h, None, None, flags))
+        return None
+
+    def sync(self):
+        check(lib().nise_abs_plan_sync(self.h))
+
+    def info(self):
+        a = (C.c_int64 * 8)()
+        lib().nise_abs_plan_info(self.h, a, 8)
+        keys = ("N", "K", "nlayers", "nwg", "vp", "spw", "f_lo", "nframes")
+        return dict(zip(keys, list(a)))
+
+    def timing(self):
+        a = (C.c_float * 3)()
+        lib().nise_abs_plan_last_timing(self.h, a, 3)
+        return {"tables_ms": a[0], "conveyor_ms": a[1], "reduce_ms": a[2]}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nise_abs_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

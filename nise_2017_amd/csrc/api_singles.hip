@@ -1,0 +1,479 @@
+// Singles propagation entry points: the batched absorption plan (frame-major
+// conveyor, plain per-sample baseline) and the drop-in symbols
+// propagate_vec_coupling_S / propagate_vector / propagate_matrix.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+
+#include "nise_internal.h"
+
+using namespace nise;
+
+namespace nise {
+int dia_propagate_host(const t_non *non, const float *H, float *cr, float *ci, int sign, int nvec,
+                       hipStream_t st);
+}
+
+// ---------------------------------------------------------------------------
+// Absorption plan
+// ---------------------------------------------------------------------------
+struct nise_abs_plan {
+  t_non non{};
+  int N = 0, tmax = 0, m = 1, begin = 0, end = 0, sample = 1;
+  int kernel = 0;
+  int64_t f_lo = 0, f_hi = 0;  // frames used
+  FrameSource src;
+  const float *mu_dev = nullptr;
+  int64_t mu_stride = 0;
+  DevBuf H_own, mu_own, C_own, shift_site, shift_ab;
+  Schedule sch;
+  DevBuf ab, layer_off, diag, cs, slots, wg_begin, partial, out;
+  DevBuf ws_cs, ws_ab, resp_plain;
+  int vp = 16, spw = 5, nwg = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t ev[4] = {};
+  float last_ms[3] = {0, 0, 0};
+  bool graph_ready = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~nise_abs_plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace {
+
+int upload(DevBuf &dst, const void *src, size_t bytes, hipStream_t st) {
+  int rc = dst.alloc(bytes);
+  if (rc) return rc;
+  NISE_HIP_CHECK(hipMemcpyAsync(dst.p, src, bytes, hipMemcpyHostToDevice, st));
+  return NISE_OK;
+}
+
+bool is_coupling_type(const t_non *non) { return std::strcmp(non->hamiltonian, "Coupling") == 0; }
+
+// Slot assignment: consecutive samples, three polarisations each, spw
+// samples per workgroup (3*spw <= vp slots).
+int build_slots(nise_abs_plan *P) {
+  std::vector<SlotDesc> sl;
+  std::vector<int> wb;
+  int ns = P->end - P->begin;
+  for (int s = 0; s < ns; ++s) {
+    if (s % P->spw == 0) wb.push_back((int)sl.size());
+    int fs = (P->begin + s) * P->sample;
+    for (int x = 0; x < 3; ++x) sl.push_back(SlotDesc{fs, P->tmax, P->tmax - 1, x, 0, -1});
+  }
+  wb.push_back((int)sl.size());
+  P->nwg = (int)wb.size() - 1;
+  int rc;
+  if ((rc = upload(P->slots, sl.data(), sl.size() * sizeof(SlotDesc), P->st))) return rc;
+  if ((rc = upload(P->wg_begin, wb.data(), wb.size() * sizeof(int), P->st))) return rc;
+  if ((rc = P->partial.alloc((size_t)std::max(P->nwg, 1) * P->tmax * sizeof(double2)))) return rc;
+  return NISE_OK;
+}
+
+int run_tables(nise_abs_plan *P) {
+  float fac = trotter_factor(P->non.deltat, 1, P->m);
+  return launch_frame_tables(P->src, P->N, P->sch, P->ab.as<uint32_t>(), fac, P->f_lo,
+                             P->f_hi - P->f_lo, P->diag.as<float2>(), P->cs.as<float2>(), P->st);
+}
+
+int run_conveyor(nise_abs_plan *P) {
+  ConveyorArgs a;
+  a.N = P->N, a.K = P->sch.K, a.nlayers = P->sch.nlayers(), a.m = P->m;
+  a.f_lo = P->f_lo;
+  a.ab = P->ab.as<uint32_t>(), a.layer_off = P->layer_off.as<int>();
+  a.diag = P->diag.as<float2>(), a.cs = P->cs.as<float2>();
+  a.mu_rec = P->mu_dev, a.mu_stride = P->mu_stride;
+  a.slots = P->slots.as<SlotDesc>(), a.wg_slot_begin = P->wg_begin.as<int>(), a.nwg = P->nwg;
+  a.tmax_acc = P->tmax, a.partial = P->partial.as<double2>();
+  return launch_conveyor(a, P->vp, P->st);
+}
+
+int run_reduce(nise_abs_plan *P) {
+  float *re = P->out.as<float>();
+  return launch_reduce_partials(P->partial.as<double2>(), P->nwg, P->tmax, re, re + P->tmax, P->st);
+}
+
+__global__ void zero_d2(double2 *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_double2(0, 0);
+}
+__global__ void d2_to_f(const double2 *p, int n, float *re, float *im) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) re[i] = (float)p[i].x, im[i] = (float)p[i].y;
+}
+
+int run_plain(nise_abs_plan *P) {
+  double2 *resp = P->resp_plain.as<double2>();
+  hipLaunchKernelGGL(zero_d2, dim3((P->tmax + 255) / 256), dim3(256), 0, P->st, resp, P->tmax);
+  PlainArgs a;
+  a.N = P->N, a.m = P->m, a.tmax = P->tmax;
+  a.fac = trotter_factor(P->non.deltat, 1, P->m);
+  a.src = P->src, a.mu_rec = P->mu_dev, a.mu_stride = P->mu_stride;
+  a.resp = resp, a.ws_cs = P->ws_cs.as<float2>(), a.ws_ab = P->ws_ab.as<uint32_t>();
+  for (int s = P->begin; s < P->end; ++s) {  // one launch per sample
+    a.ti = (int64_t)s * P->sample;
+    int rc = launch_plain_sample(a, P->st);
+    if (rc) return rc;
+  }
+  float *re = P->out.as<float>();
+  hipLaunchKernelGGL(d2_to_f, dim3((P->tmax + 255) / 256), dim3(256), 0, P->st, resp, P->tmax, re,
+                     re + P->tmax);
+  NISE_HIP_CHECK(hipGetLastError());
+  return NISE_OK;
+}
+
+}  // namespace
+
+extern "C" int nise_abs_plan_create(nise_abs_plan **out, const t_non *non, const float *H_rec,
+                                    int64_t H_stride, const float *mu_rec, int64_t mu_stride,
+                                    int64_t nframes, const float *coupling_rec, int where,
+                                    int kernel) {
+  *out = nullptr;
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (!non || !H_rec || !mu_rec) return set_error("null argument"), NISE_ERR_ARG;
+  auto P = new nise_abs_plan();
+  std::unique_ptr<nise_abs_plan> guard(P);
+  P->non = *non;
+  P->N = non->singles, P->tmax = non->tmax1, P->m = non->ts, P->sample = non->sample;
+  P->begin = non->begin, P->end = non->end, P->kernel = kernel;
+  const int N = P->N;
+  if (N < 1 || N > 65535) return set_error("singles=%d out of range", N), NISE_ERR_ARG;
+  if (P->tmax < 1 || P->sample < 1 || P->m < 1)
+    return set_error("tmax1, sample and Trotter steps must be >= 1"), NISE_ERR_ARG;
+  if (non->propagation != 1)
+    return set_error("batched absorption supports propagation 1 (Coupling) here; got %d",
+                     non->propagation), NISE_ERR_ARG;
+  if (P->end == 0) P->end = (int)((nframes - P->tmax - 1) / P->sample + 1);  // src/NISE_subs.c:200,210
+  if (P->end <= P->begin) return set_error("no samples (begin=%d end=%d)", P->begin, P->end), NISE_ERR_RANGE;
+  P->f_lo = (int64_t)P->begin * P->sample;
+  P->f_hi = (int64_t)(P->end - 1) * P->sample + P->tmax;
+  if (P->f_hi > nframes)
+    return set_error("trajectory has %lld frames, run needs %lld", (long long)nframes,
+                     (long long)P->f_hi), NISE_ERR_RANGE;
+  const bool ctype = is_coupling_type(non);
+  if (ctype && !coupling_rec) return set_error("Coupling type needs coupling_rec"), NISE_ERR_ARG;
+  if (H_stride < 1 + (ctype ? N : (int64_t)N * (N + 1) / 2) || mu_stride < 1 + 3 * (int64_t)N)
+    return set_error("record stride too small"), NISE_ERR_ARG;
+  NISE_HIP_CHECK(hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking));
+  for (auto &e : P->ev) NISE_HIP_CHECK(hipEventCreate(&e));
+
+  // device-resident inputs (upload the used frame range when given on host)
+  const float *Hd = H_rec, *mud = mu_rec, *Cd = coupling_rec;
+  int64_t mu_eff_stride = ctype ? 0 : mu_stride;
+  if (where == NISE_HOST) {
+    size_t hbytes = (size_t)(P->f_hi - P->f_lo) * H_stride * 4;
+    if ((rc = upload(P->H_own, H_rec + P->f_lo * H_stride, hbytes, P->st))) return rc;
+    Hd = P->H_own.as<float>() - P->f_lo * H_stride;
+    if (ctype) {
+      if ((rc = upload(P->mu_own, mu_rec, (size_t)mu_stride * 4, P->st))) return rc;
+      mud = P->mu_own.as<float>();
+      if ((rc = upload(P->C_own, coupling_rec, (size_t)(1 + (int64_t)N * (N + 1) / 2) * 4, P->st)))
+        return rc;
+      Cd = P->C_own.as<float>();
+    } else {
+      if ((rc = upload(P->mu_own, mu_rec + P->f_lo * mu_stride,
+                       (size_t)(P->f_hi - P->f_lo) * mu_stride * 4, P->st)))
+        return rc;
+      mud = P->mu_own.as<float>() - P->f_lo * mu_stride;
+    }
+  }
+  P->mu_dev = mud, P->mu_stride = mu_eff_stride;
+  P->src.H_rec = Hd, P->src.H_stride = H_stride, P->src.coupling_type = ctype, P->src.C_rec = Cd;
+  P->src.shifte = non->shifte, P->src.cut = non->couplingcut, P->src.apply_shift = 1;
+  P->src.nshift = non->SingleShiftSites;
+  if (P->src.nshift > 0) {
+    for (int i = 0; i < P->src.nshift; ++i)
+      if (non->SingleShiftSite[i] < 0 || non->SingleShiftSite[i] >= N)
+        return set_error("single shift site %d out of range", non->SingleShiftSite[i]), NISE_ERR_ARG;
+    if ((rc = upload(P->shift_site, non->SingleShiftSite, P->src.nshift * sizeof(int), P->st))) return rc;
+    if ((rc = upload(P->shift_ab, non->SingleShift, 2 * P->src.nshift * sizeof(float), P->st))) return rc;
+    P->src.shift_site = P->shift_site.as<int>(), P->src.shift_ab = P->shift_ab.as<float>();
+  }
+  if ((rc = P->out.alloc(2 * P->tmax * sizeof(float)))) return rc;
+
+  if (kernel == 1) {
+    int64_t npair = (int64_t)N * (N - 1) / 2 + 1;
+    if ((rc = P->ws_cs.alloc(npair * sizeof(float2)))) return rc;
+    if ((rc = P->ws_ab.alloc(npair * sizeof(uint32_t)))) return rc;
+    if ((rc = P->resp_plain.alloc(P->tmax * sizeof(double2)))) return rc;
+  } else {
+    // structure: union of kept couplings over the frames in use
+    int64_t npair = (int64_t)N * (N - 1) / 2;
+    std::vector<uint8_t> mask(npair);
+    if (npair) {
+      DevBuf mk;
+      if ((rc = mk.alloc(npair))) return rc;
+      if ((rc = launch_union_mask(P->src, N, P->f_lo, P->f_hi - P->f_lo, mk.as<uint8_t>(), P->st))) return rc;
+      NISE_HIP_CHECK(hipMemcpyAsync(mask.data(), mk.p, npair, hipMemcpyDeviceToHost, P->st));
+      NISE_HIP_CHECK(hipStreamSynchronize(P->st));
+    }
+    build_schedule(N, mask, P->sch);
+    if ((rc = upload(P->ab, P->sch.ab.data(), std::max<size_t>(1, P->sch.ab.size()) * 4, P->st))) return rc;
+    if ((rc = upload(P->layer_off, P->sch.layer_off.data(), P->sch.layer_off.size() * 4, P->st))) return rc;
+    int64_t nfr = P->f_hi - P->f_lo;
+    if ((rc = P->diag.alloc((size_t)nfr * N * sizeof(float2)))) return rc;
+    if ((rc = P->cs.alloc((size_t)nfr * std::max(P->sch.K, 1) * sizeof(float2)))) return rc;
+    // default shape: as many samples per workgroup as fit 16 slots, wider
+    // when LDS allows and the run is long
+    P->vp = 16, P->spw = 5;
+    if ((rc = build_slots(P))) return rc;
+  }
+  NISE_HIP_CHECK(hipStreamSynchronize(P->st));
+  *out = guard.release();
+  return NISE_OK;
+}
+
+extern "C" int nise_abs_plan_set_shape(nise_abs_plan *P, int vp, int samples_per_wg) {
+  if (!P || P->kernel != 0) return set_error("no conveyor plan"), NISE_ERR_ARG;
+  if (!(vp == 4 || vp == 8 || vp == 16 || vp == 32 || vp == 64) || samples_per_wg < 1 ||
+      3 * samples_per_wg > vp)
+    return set_error("bad shape vp=%d spw=%d", vp, samples_per_wg), NISE_ERR_ARG;
+  P->vp = vp, P->spw = samples_per_wg;
+  if (P->graph_ready) {
+    (void)hipGraphExecDestroy(P->exec), (void)hipGraphDestroy(P->graph);
+    P->exec = nullptr, P->graph = nullptr, P->graph_ready = false;
+  }
+  return build_slots(P);
+}
+
+// flags bit 0: time the stages with HIP events (no graph)
+//       bit 1: replay a captured hipGraph of the stages
+extern "C" int nise_abs_plan_execute(nise_abs_plan *P, float *re_S, float *im_S, int flags) {
+  if (!P) return set_error("null plan"), NISE_ERR_ARG;
+  int rc;
+  if (P->kernel == 1) {
+    NISE_HIP_CHECK(hipEventRecord(P->ev[0], P->st));
+    if ((rc = run_plain(P))) return rc;
+    NISE_HIP_CHECK(hipEventRecord(P->ev[1], P->st));
+    NISE_HIP_CHECK(hipEventSynchronize(P->ev[1]));
+    NISE_HIP_CHECK(hipEventElapsedTime(&P->last_ms[1], P->ev[0], P->ev[1]));
+    P->last_ms[0] = P->last_ms[2] = 0;
+  } else if (flags & 2) {
+    if (!P->graph_ready) {
+      NISE_HIP_CHECK(hipStreamBeginCapture(P->st, hipStreamCaptureModeThreadLocal));
+      int r1 = run_tables(P), r2 = r1 ? r1 : run_conveyor(P), r3 = r2 ? r2 : run_reduce(P);
+      hipError_t ce = hipStreamEndCapture(P->st, &P->graph);
+      if (r3) return r3;
+      if (ce != hipSuccess) return set_error("graph capture: %s", hipGetErrorString(ce)), NISE_ERR_HIP;
+      NISE_HIP_CHECK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+      P->graph_ready = true;
+    }
+    NISE_HIP_CHECK(hipGraphLaunch(P->exec, P->st));
+  } else {
+    NISE_HIP_CHECK(hipEventRecord(P->ev[0], P->st));
+    if ((rc = run_tables(P))) return rc;
+    NISE_HIP_CHECK(hipEventRecord(P->ev[1], P->st));
+    if ((rc = run_conveyor(P))) return rc;
+    NISE_HIP_CHECK(hipEventRecord(P->ev[2], P->st));
+    if ((rc = run_reduce(P))) return rc;
+    NISE_HIP_CHECK(hipEventRecord(P->ev[3], P->st));
+    if (flags & 1) {
+      NISE_HIP_CHECK(hipEventSynchronize(P->ev[3]));
+      for (int i = 0; i < 3; ++i)
+        NISE_HIP_CHECK(hipEventElapsedTime(&P->last_ms[i], P->ev[i], P->ev[i + 1]));
+    }
+  }
+  if (re_S || im_S) {
+    std::vector<float> h(2 * P->tmax);
+    NISE_HIP_CHECK(hipMemcpyAsync(h.data(), P->out.p, h.size() * 4, hipMemcpyDeviceToHost, P->st));
+    NISE_HIP_CHECK(hipStreamSynchronize(P->st));
+    if (re_S) std::memcpy(re_S, h.data(), P->tmax * 4);
+    if (im_S) std::memcpy(im_S, h.data() + P->tmax, P->tmax * 4);
+  }
+  return NISE_OK;
+}
+
+extern "C" int nise_abs_plan_sync(nise_abs_plan *P) {
+  NISE_HIP_CHECK(hipStreamSynchronize(P->st));
+  return NISE_OK;
+}
+
+// info: N, K, nlayers, nwg, vp, samples per workgroup, first frame, frames
+extern "C" int nise_abs_plan_info(const nise_abs_plan *P, int64_t *info, int n) {
+  int64_t v[] = {P->N, P->sch.K, P->sch.nlayers(), P->nwg, P->vp, P->spw, P->f_lo, P->f_hi - P->f_lo};
+  int k = (int)(sizeof(v) / sizeof(v[0]));
+  for (int i = 0; i < k && i < n; ++i) info[i] = v[i];
+  return k;
+}
+
+// ms of the last timed execute: frame tables, conveyor (or plain), reduce
+extern "C" int nise_abs_plan_last_timing(const nise_abs_plan *P, float *ms, int n) {
+  for (int i = 0; i < 3 && i < n; ++i) ms[i] = P->last_ms[i];
+  return 3;
+}
+
+extern "C" void nise_abs_plan_destroy(nise_abs_plan *P) { delete P; }
+
+extern "C" int nise_absorption_batched(const t_non *non, const float *H_rec, int64_t H_stride,
+                                       const float *mu_rec, int64_t mu_stride, int64_t nframes,
+                                       const float *coupling_rec, int where, int kernel,
+                                       float *re_S, float *im_S) {
+  nise_abs_plan *P = nullptr;
+  int rc = nise_abs_plan_create(&P, non, H_rec, H_stride, mu_rec, mu_stride, nframes, coupling_rec,
+                                where, kernel);
+  if (rc) return rc;
+  rc = nise_abs_plan_execute(P, re_S, im_S, 0);
+  nise_abs_plan_destroy(P);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Drop-in singles symbols.  Per host thread: one stream and reusable device
+// buffers.  Each call: kept-coupling mask and layer schedule from the frame
+// (integer bookkeeping on the host), angles, phases and the sweep on the GPU.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct SinglesCtx {
+  DevBuf H, ab, lay, diag, cs, state, slots, wgb;
+  std::vector<float> hbuf;
+  std::vector<float2> sbuf;
+};
+
+SinglesCtx &ctx() {
+  static thread_local SinglesCtx c;
+  return c;
+}
+
+[[noreturn]] void fatal(const char *where) {
+  printf("%s: GPU propagation failed: %s\n", where, get_error());
+  exit(1);
+}
+
+}  // namespace
+
+namespace nise {
+
+// Propagate nvec vectors (columns of cr/ci, stride N) through one frame H
+// (packed triangle, diagonal already shifted) with the Trotter scheme.
+int coupling_S_host(int N, const float *H, float *cr, float *ci, int nvec, int m, int sign,
+                    float deltat, float cut) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (N < 1 || m < 1 || nvec < 1) return set_error("bad arguments"), NISE_ERR_ARG;
+  hipStream_t st = default_stream();
+  SinglesCtx &c = ctx();
+  const int64_t nn2 = (int64_t)N * (N + 1) / 2;
+  // frame record with a header word, as the table kernel expects
+  c.hbuf.resize(nn2 + 1);
+  c.hbuf[0] = 0.f;
+  std::memcpy(c.hbuf.data() + 1, H, nn2 * 4);
+  std::vector<uint8_t> mask((size_t)N * (N - 1) / 2);
+  {
+    int64_t p = 0;
+    for (int a = 0; a < N; ++a)
+      for (int b = a + 1; b < N; ++b, ++p) {
+        float J = H[sindex(a, b, N)];
+        mask[p] = std::fabs(J) > cut;  // src/propagate.c:853
+      }
+  }
+  Schedule sch;
+  build_schedule(N, mask, sch);
+  int vp = nvec <= 4 ? 4 : nvec <= 8 ? 8 : nvec <= 16 ? 16 : nvec <= 32 ? 32 : 64;
+  while (vp > 4 && (size_t)N * vp * 8 > 150 * 1024) vp >>= 1;
+  std::vector<SlotDesc> sl(nvec);
+  std::vector<int> wb;
+  for (int v = 0; v < nvec; ++v) {
+    if (v % vp == 0) wb.push_back(v);
+    sl[v] = SlotDesc{0, 0, 1, 0, 1, v};
+  }
+  wb.push_back(nvec);
+  c.sbuf.resize((size_t)nvec * N);
+  for (int v = 0; v < nvec; ++v)
+    for (int i = 0; i < N; ++i) c.sbuf[(size_t)v * N + i] = make_float2(cr[(size_t)v * N + i], ci[(size_t)v * N + i]);
+  if ((rc = upload(c.H, c.hbuf.data(), (nn2 + 1) * 4, st))) return rc;
+  if ((rc = upload(c.ab, sch.ab.data(), std::max<size_t>(1, sch.ab.size()) * 4, st))) return rc;
+  if ((rc = upload(c.lay, sch.layer_off.data(), sch.layer_off.size() * 4, st))) return rc;
+  if ((rc = upload(c.state, c.sbuf.data(), c.sbuf.size() * sizeof(float2), st))) return rc;
+  if ((rc = upload(c.slots, sl.data(), sl.size() * sizeof(SlotDesc), st))) return rc;
+  if ((rc = upload(c.wgb, wb.data(), wb.size() * sizeof(int), st))) return rc;
+  if ((rc = c.diag.alloc(N * sizeof(float2)))) return rc;
+  if ((rc = c.cs.alloc(std::max(sch.K, 1) * sizeof(float2)))) return rc;
+  FrameSource src;
+  src.H_rec = c.H.as<float>(), src.H_stride = nn2 + 1, src.shifte = 0.f, src.cut = cut;
+  src.apply_shift = 0;
+  float fac = trotter_factor(deltat, sign, m);
+  if ((rc = launch_frame_tables(src, N, sch, c.ab.as<uint32_t>(), fac, 0, 1, c.diag.as<float2>(),
+                                c.cs.as<float2>(), st)))
+    return rc;
+  ConveyorArgs a;
+  a.N = N, a.K = sch.K, a.nlayers = sch.nlayers(), a.m = m, a.f_lo = 0;
+  a.ab = c.ab.as<uint32_t>(), a.layer_off = c.lay.as<int>();
+  a.diag = c.diag.as<float2>(), a.cs = c.cs.as<float2>();
+  a.mu_rec = c.H.as<float>(), a.mu_stride = 0;
+  a.slots = c.slots.as<SlotDesc>(), a.wg_slot_begin = c.wgb.as<int>(), a.nwg = (int)wb.size() - 1;
+  a.state = c.state.as<float2>(), a.writeback = 1;
+  if ((rc = launch_conveyor(a, vp, st))) return rc;
+  NISE_HIP_CHECK(hipMemcpyAsync(c.sbuf.data(), c.state.p, c.sbuf.size() * sizeof(float2),
+                                hipMemcpyDeviceToHost, st));
+  NISE_HIP_CHECK(hipStreamSynchronize(st));
+  for (int v = 0; v < nvec; ++v)
+    for (int i = 0; i < N; ++i) {
+      cr[(size_t)v * N + i] = c.sbuf[(size_t)v * N + i].x;
+      ci[(size_t)v * N + i] = c.sbuf[(size_t)v * N + i].y;
+    }
+  return NISE_OK;
+}
+
+}  // namespace nise
+
+// src/propagate.c:821-899
+extern "C" void propagate_vec_coupling_S(t_non *non, float *Hamiltonian_i, float *cr, float *ci,
+                                         int m, int sign) {
+  if (coupling_S_host(non->singles, Hamiltonian_i, cr, ci, 1, m, sign, non->deltat,
+                      non->couplingcut))
+    fatal("propagate_vec_coupling_S");
+}
+
+// src/propagate.c:20-39
+extern "C" void propagate_vector(t_non *non, float *Hamil_i_e, float *vecr, float *veci, int sign,
+                                 int samples, int display) {
+  (void)samples, (void)display;
+  if (non->propagation == 1) {
+    if (coupling_S_host(non->singles, Hamil_i_e, vecr, veci, 1, non->ts, sign, non->deltat,
+                        non->couplingcut))
+      fatal("propagate_vector");
+    return;
+  }
+  if (non->propagation == 0 && (non->thres == 0 || non->thres > 1)) {
+    if (dia_propagate_host(non, Hamil_i_e, vecr, veci, sign, 1, default_stream()))
+      fatal("propagate_vector");
+    return;
+  }
+  printf("propagate_vector: propagation scheme %d (threshold %g) is not provided by the GPU "
+         "library\n", non->propagation, non->thres);
+  exit(1);
+}
+
+// src/propagate.c:43-52: the N columns of an N x N block, one launch
+extern "C" void propagate_matrix(t_non *non, float *Hamil_i_e, float *vecr, float *veci, int sign,
+                                 int samples, int display) {
+  (void)samples, (void)display;
+  int N = non->singles;
+  if (non->propagation == 1) {
+    if (coupling_S_host(N, Hamil_i_e, vecr, veci, N, non->ts, sign, non->deltat, non->couplingcut))
+      fatal("propagate_matrix");
+    return;
+  }
+  if (non->propagation == 0 && (non->thres == 0 || non->thres > 1)) {
+    if (dia_propagate_host(non, Hamil_i_e, vecr, veci, sign, N, default_stream()))
+      fatal("propagate_matrix");
+    return;
+  }
+  printf("propagate_matrix: propagation scheme %d is not provided by the GPU library\n",
+         non->propagation);
+  exit(1);
+}

@@ -1,0 +1,154 @@
+// Internal declarations shared by the HIP translation units of libnise_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/nise_abi.h"
+
+namespace nise {
+
+// Constants exactly as the reference declares them (src/types.h:5-12):
+// float variables initialised from double expressions.
+constexpr float kCv = 2.99792458e-5;
+constexpr float kTwoPi = 2 * 3.14159265359;
+constexpr float kIcm2ifs = 2.99792458e-5;
+constexpr float kSqrt2 = 1.41421356237;
+
+// Packed upper-triangle index, src/NISE_subs.h:54-65.
+__host__ __device__ inline int sindex(int a, int b, int N) {
+  return (a > b) ? a + b * ((N << 1) - b - 1) / 2 : b + a * ((N << 1) - a - 1) / 2;
+}
+
+// Error state for the nise_* entry points (thread local message).
+void set_error(const char *fmt, ...);
+const char *get_error();
+
+#define NISE_HIP_CHECK(expr)                                                  \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::nise::set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr,            \
+                        hipGetErrorString(e_));                               \
+      return NISE_ERR_HIP;                                                    \
+    }                                                                         \
+  } while (0)
+
+// Device buffer with RAII release.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  int alloc(size_t n);
+  void release();
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Frame-independent rotation schedule of one Trotter sweep.
+//   ab[r]      : (a | b << 16) of the r-th rotation in schedule order
+//   layer_off  : rotations of layer l are [layer_off[l], layer_off[l+1])
+// Rotations inside a layer touch disjoint sites; for every site the layers
+// keep the lexicographic (a<b) order of src/propagate.c:848-888, so the
+// layered sweep performs, site by site, the same floating point operations
+// in the same order as the reference's sequential sweep.
+struct Schedule {
+  int N = 0;
+  int K = 0;  // rotations per sweep (union of kept couplings)
+  std::vector<uint32_t> ab;
+  std::vector<int> layer_off;
+  int nlayers() const { return (int)layer_off.size() - 1; }
+};
+
+// Build the layered schedule from a kept-coupling mask over the strict upper
+// triangle, enumerated in lexicographic (a<b) order.
+void build_schedule(int N, const std::vector<uint8_t> &mask, Schedule &s);
+
+// Singles parameters shared by the frame-table kernels.
+struct FrameSource {
+  const float *H_rec = nullptr;  // device, record 0 header
+  int64_t H_stride = 0;          // words per record
+  int coupling_type = 0;         // 1: H_rec holds N diagonal values
+  const float *C_rec = nullptr;  // device, Full record with couplings
+  float shifte = 0.f;
+  float cut = 0.f;
+  int nshift = 0;                // single-site shifts, applied in order
+  const int *shift_site = nullptr;   // device
+  const float *shift_ab = nullptr;   // device, (a, b) pairs
+  int apply_shift = 1;
+};
+
+// Reference time factor f = deltat*icm2ifs*twoPi*sign/m (src/propagate.c:836).
+inline float trotter_factor(float deltat, int sign, int m) {
+  volatile float f = deltat * kIcm2ifs;
+  f = f * kTwoPi;
+  f = f * (float)sign;
+  f = f / (float)m;
+  return f;
+}
+
+// Launch wrappers (trotter.hip).
+int launch_union_mask(const FrameSource &src, int N, int64_t f_lo, int64_t nfr,
+                      uint8_t *mask_dev, hipStream_t st);
+int launch_frame_tables(const FrameSource &src, int N, const Schedule &sch,
+                        const uint32_t *ab_dev, float fac, int64_t f_lo,
+                        int64_t nfr, float2 *diag_dev, float2 *cs_dev,
+                        hipStream_t st);
+
+// One vector slot of the conveyor kernel.
+struct SlotDesc {
+  int fs;     // first frame (absolute frame index)
+  int nresp;  // response steps (0 = none)
+  int nprop;  // propagation steps
+  int x;      // dipole component for init/response
+  int init;   // 0: psi = mu_x(fs), 1: psi from state buffer
+  int out;    // state buffer row (init 1 or write back), -1 none
+};
+
+struct ConveyorArgs {
+  int N = 0, K = 0, nlayers = 0, m = 1;
+  int64_t f_lo = 0;          // absolute frame index of table row 0
+  const uint32_t *ab = nullptr;
+  const int *layer_off = nullptr;
+  const float2 *diag = nullptr;  // [rows][N]
+  const float2 *cs = nullptr;    // [rows][K]
+  const float *mu_rec = nullptr;
+  int64_t mu_stride = 0;         // 0: static dipole (Coupling type)
+  const SlotDesc *slots = nullptr;
+  const int *wg_slot_begin = nullptr;  // [nwg+1]
+  int nwg = 0;
+  int tmax_acc = 0;              // response length (absorption tmax)
+  double2 *partial = nullptr;    // [nwg][tmax_acc]
+  float2 *state = nullptr;       // [rows][N] for init/write back
+  int writeback = 0;
+};
+
+int launch_conveyor(const ConveyorArgs &a, int vp, hipStream_t st);
+int launch_reduce_partials(const double2 *partial, int nwg, int tmax,
+                           float *re, float *im, hipStream_t st);
+
+// Plain one-sample-per-launch kernel (baseline).
+struct PlainArgs {
+  int N = 0, m = 1, tmax = 0;
+  float fac = 0.f;
+  FrameSource src;
+  const float *mu_rec = nullptr;
+  int64_t mu_stride = 0;
+  int64_t ti = 0;            // start frame of the sample
+  double2 *resp = nullptr;   // [tmax] accumulated (3 polarisations)
+  float2 *ws_cs = nullptr;   // [N(N-1)/2] workspace
+  uint32_t *ws_ab = nullptr; // [N(N-1)/2] workspace
+};
+int launch_plain_sample(const PlainArgs &a, hipStream_t st);
+
+// Library-wide stream and device setup.
+int ensure_device();
+hipStream_t default_stream();
+
+}  // namespace nise

@@ -1,0 +1,108 @@
+// Host runtime of libnise_hip: error state, device buffers, the layered
+// rotation schedule, device and stream setup.
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+
+#include "nise_internal.h"
+
+namespace nise {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+const char *get_error() { return g_err; }
+
+int DevBuf::alloc(size_t n) {
+  if (n <= bytes && p) return NISE_OK;
+  release();
+  if (n == 0) return NISE_OK;
+  hipError_t e = hipMalloc(&p, n);
+  if (e != hipSuccess) {
+    p = nullptr;
+    set_error("hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+    return NISE_ERR_NOMEM;
+  }
+  bytes = n;
+  return NISE_OK;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+// ASAP layering of the lexicographic rotation sequence: rotation k goes to
+// layer 1 + max(last layer touching a_k, last layer touching b_k).  Every
+// site therefore sees its rotations in the original order, and rotations of
+// one layer share no site.
+void build_schedule(int N, const std::vector<uint8_t> &mask, Schedule &s) {
+  s.N = N;
+  std::vector<int> la, lb, lay;
+  std::vector<int> last(N, -1);
+  int nl = 0;
+  int64_t p = 0;
+  for (int a = 0; a < N; ++a)
+    for (int b = a + 1; b < N; ++b, ++p) {
+      if (!mask[p]) continue;
+      int l = 1 + (last[a] > last[b] ? last[a] : last[b]);
+      last[a] = last[b] = l;
+      la.push_back(a), lb.push_back(b), lay.push_back(l);
+      if (l + 1 > nl) nl = l + 1;
+    }
+  s.K = (int)la.size();
+  s.layer_off.assign(nl + 1, 0);
+  for (int l : lay) s.layer_off[l + 1]++;
+  for (int l = 0; l < nl; ++l) s.layer_off[l + 1] += s.layer_off[l];
+  std::vector<int> fill(s.layer_off.begin(), s.layer_off.end() - 1);
+  s.ab.assign(s.K, 0);
+  for (int k = 0; k < s.K; ++k)  // stable within a layer
+    s.ab[fill[lay[k]]++] = (uint32_t)la[k] | ((uint32_t)lb[k] << 16);
+}
+
+static std::once_flag g_dev_once;
+static int g_dev_status = NISE_OK;
+
+int ensure_device() {
+  std::call_once(g_dev_once, [] {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n < 1) {
+      g_dev_status = NISE_ERR_HIP;
+      return;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+      g_dev_status = NISE_ERR_HIP;
+      return;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+      fprintf(stderr, "libnise_hip: built for gfx950, device is %s\n", prop.gcnArchName);
+    }
+  });
+  if (g_dev_status != NISE_OK) set_error("no usable HIP device");
+  return g_dev_status;
+}
+
+// One non-blocking stream per host thread: the drop-in symbols may be called
+// concurrently from OpenMP threads (src/propagate.c:1541-1548).
+hipStream_t default_stream() {
+  static thread_local hipStream_t st = nullptr;
+  if (!st) {
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
+  }
+  return st;
+}
+
+}  // namespace nise
+
+extern "C" const char *nise_last_error(void) { return nise::get_error(); }
+extern "C" const char *nise_version(void) { return "nise-mi355x 0.1 (gfx950)"; }

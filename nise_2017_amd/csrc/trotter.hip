@@ -1,0 +1,480 @@
+// Trotter (coupling-scheme) singles kernels for gfx950.
+//
+// Reference algorithm: propagate_vec_coupling_S, src/propagate.c:821-899:
+//   psi <- [ D . R_kmax ... R_1 . D ]^m psi
+//   D   = diag(cos(f H_aa / 2) - i sin(f H_aa / 2))
+//   R_k = exp(-i f J_k sigma_x) on the pair (a_k, b_k), a_k < b_k in
+//         lexicographic order, applied sequentially.
+//
+// Work split on the GPU:
+//   frame_tables  : per frame, D (float2 per site) and (cos, sin) of every
+//                   scheduled rotation, in the reference's libm precision
+//                   (double trig on the float angle).  One pass per frame,
+//                   shared by every vector that goes through the frame.
+//   conveyor      : frame-major kernel.  A workgroup owns up to VP vectors
+//                   (slots) whose windows overlap in time and walks the
+//                   frames in order; each frame is read once for all its
+//                   slots.  Slot state lives in LDS as psi[site][slot]
+//                   (float2), so the VP lanes that serve one rotation read
+//                   one contiguous 8*VP byte row.  Rotations run layer by
+//                   layer (disjoint pairs per layer), one barrier per layer.
+//   plain         : one sample (3 polarisations) per launch, every frame
+//                   re-read and re-compacted per step, rotations in one
+//                   sequential sweep: the per-sample GPU baseline.
+//
+// Arithmetic: compiled with -ffp-contract=off so products and sums round
+// exactly as the reference's x86-64 SSE code (no FMA) does.
+
+#include "nise_internal.h"
+
+namespace nise {
+
+// ---------------------------------------------------------------------------
+// Frame element access.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float raw_diag(const FrameSource &s, int N, int64_t f, int a) {
+  const float *rec = s.H_rec + f * s.H_stride + 1;
+  return s.coupling_type ? rec[a] : rec[sindex(a, a, N)];
+}
+
+__device__ __forceinline__ float raw_coupling(const FrameSource &s, int N, int64_t f, int a, int b) {
+  const float *rec = s.coupling_type ? (s.C_rec + 1) : (s.H_rec + f * s.H_stride + 1);
+  return rec[sindex(a, b, N)];
+}
+
+// Diagonal element as read_Hamiltonian leaves it: minus shifte
+// (src/read_trajectory.c:153,176,337), then the single-site shifts
+// (src/hamshift.c:59-66) in list order.
+__device__ __forceinline__ float site_energy(const FrameSource &s, int N, int64_t f, int a) {
+  float h = __fsub_rn(raw_diag(s, N, f, a), s.shifte);
+  if (s.apply_shift) {
+    for (int i = 0; i < s.nshift; ++i)
+      if (s.shift_site[i] == a) h = __fadd_rn(__fmul_rn(h, s.shift_ab[2 * i]), s.shift_ab[2 * i + 1]);
+  }
+  return h;
+}
+
+// Coupling as kept by read_He (|J| < cut -> 0, src/read_trajectory.c:305-311)
+// and the propagator (|J| > cut, src/propagate.c:853).  Returns 0 when not kept.
+__device__ __forceinline__ float kept_coupling(const FrameSource &s, int N, int64_t f, int a, int b) {
+  float J = raw_coupling(s, N, f, a, b);
+  if (fabsf(J) < s.cut) J = 0.f;
+  return (fabsf(J) > s.cut) ? J : 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// Union of kept couplings over a frame range (one byte per a<b pair).
+// ---------------------------------------------------------------------------
+__global__ void union_mask_kernel(FrameSource s, int N, int64_t f_lo, int64_t nfr, uint8_t *mask) {
+  int64_t npair = (int64_t)N * (N - 1) / 2;
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npair) return;
+  // invert the lexicographic pair index p -> (a, b)
+  int a = 0;
+  int64_t rem = p;
+  while (rem >= N - 1 - a) { rem -= N - 1 - a; ++a; }
+  int b = a + 1 + (int)rem;
+  uint8_t keep = 0;
+  int64_t nf = s.coupling_type ? 1 : nfr;
+  for (int64_t f = 0; f < nf && !keep; ++f) keep = kept_coupling(s, N, f_lo + f, a, b) != 0.f;
+  mask[p] = keep;
+}
+
+int launch_union_mask(const FrameSource &src, int N, int64_t f_lo, int64_t nfr, uint8_t *mask_dev,
+                      hipStream_t st) {
+  int64_t npair = (int64_t)N * (N - 1) / 2;
+  if (npair == 0) return NISE_OK;
+  dim3 grid((unsigned)((npair + 255) / 256));
+  hipLaunchKernelGGL(union_mask_kernel, grid, dim3(256), 0, st, src, N, f_lo, nfr, mask_dev);
+  NISE_HIP_CHECK(hipGetLastError());
+  return NISE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Per-frame tables: D phases and rotation (cos, sin) in schedule order.
+// ---------------------------------------------------------------------------
+__global__ void frame_tables_kernel(FrameSource s, int N, int K, const uint32_t *ab, float fac,
+                                    int64_t f_lo, float2 *diag, float2 *cs) {
+  int64_t row = blockIdx.y;
+  int64_t f = f_lo + row;
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < N) {
+    // src/propagate.c:863-866: cos/sin of (0.5 * H0 * f) in double
+    float h = site_energy(s, N, f, j);
+    double arg = 0.5 * (double)h * (double)fac;
+    diag[row * N + j] = make_float2((float)cos(arg), (float)(-sin(arg)));
+  } else if (j < N + K) {
+    int r = j - N;
+    uint32_t p = ab[r];
+    int a = (int)(p & 0xffffu), b = (int)(p >> 16);
+    float J = kept_coupling(s, N, f, a, b);
+    float co = 1.f, si = 0.f;  // absent coupling: exact identity rotation
+    if (J != 0.f) {
+      // src/propagate.c:879-882: J = J*f; si = -sin(J); co = sqrt(1 - si*si)
+      float Jf = __fmul_rn(J, fac);
+      si = (float)(-sin((double)Jf));
+      co = (float)sqrt((double)__fsub_rn(1.0f, __fmul_rn(si, si)));
+    }
+    cs[row * K + r] = make_float2(co, si);
+  }
+}
+
+int launch_frame_tables(const FrameSource &src, int N, const Schedule &sch, const uint32_t *ab_dev,
+                        float fac, int64_t f_lo, int64_t nfr, float2 *diag_dev, float2 *cs_dev,
+                        hipStream_t st) {
+  int n = N + sch.K;
+  int64_t done = 0;
+  while (done < nfr) {  // grid.y is limited to 65535
+    int64_t chunk = nfr - done < 65535 ? nfr - done : 65535;
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)chunk);
+    hipLaunchKernelGGL(frame_tables_kernel, grid, dim3(256), 0, st, src, N, sch.K, ab_dev, fac,
+                       f_lo + done, diag_dev + done * N, cs_dev + done * sch.K);
+    NISE_HIP_CHECK(hipGetLastError());
+    done += chunk;
+  }
+  return NISE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Complex helpers with the reference's operation order.
+// ---------------------------------------------------------------------------
+// src/propagate.c:871-872 / :892-893: (cr*re - ci*im, ci*re + cr*im)
+__device__ __forceinline__ float2 cmul_diag(float2 c, float2 u) {
+  return make_float2(__fsub_rn(__fmul_rn(c.x, u.x), __fmul_rn(c.y, u.y)),
+                     __fadd_rn(__fmul_rn(c.y, u.x), __fmul_rn(c.x, u.y)));
+}
+
+// src/propagate.c:883-887
+__device__ __forceinline__ void rotate_pair(float2 &pa, float2 &pb, float co, float si) {
+  float cr1 = __fsub_rn(__fmul_rn(co, pa.x), __fmul_rn(si, pb.y));
+  float ci1 = __fadd_rn(__fmul_rn(co, pa.y), __fmul_rn(si, pb.x));
+  float cr2 = __fsub_rn(__fmul_rn(co, pb.x), __fmul_rn(si, pa.y));
+  float ci2 = __fadd_rn(__fmul_rn(co, pb.y), __fmul_rn(si, pa.x));
+  pa = make_float2(cr1, ci1);
+  pb = make_float2(cr2, ci2);
+}
+
+// ---------------------------------------------------------------------------
+// Conveyor kernel.
+// ---------------------------------------------------------------------------
+constexpr int kConvBlock = 256;
+
+template <int VP>
+__global__ void __launch_bounds__(kConvBlock) conveyor_kernel(ConveyorArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = A.N;
+  float2 *psi = reinterpret_cast<float2 *>(smem);                 // [N][VP]
+  double2 *acc = reinterpret_cast<double2 *>(psi + (size_t)N * VP);  // [tmax_acc]
+  float2 *red = reinterpret_cast<float2 *>(acc + A.tmax_acc);     // [waves][VP]
+  constexpr int kWaves = kConvBlock / 64;
+  constexpr int kRowsPerPass = kConvBlock / VP;
+
+  const int g = blockIdx.x;
+  const int s0 = A.wg_slot_begin[g];
+  const int ns = A.wg_slot_begin[g + 1] - s0;
+  const int t = threadIdx.x;
+  const int v = t % VP;       // slot served by this thread
+  const int q = t / VP;       // row group
+  const int lane = t & 63, wave = t >> 6;
+
+  SlotDesc sd;
+  bool has = v < ns;
+  if (has) sd = A.slots[s0 + v];
+  else sd = SlotDesc{0, 0, 0, 0, 0, -1};
+
+  // frame range of this workgroup
+  int64_t F0 = INT64_MAX, F1 = INT64_MIN;
+  for (int i = 0; i < ns; ++i) {
+    SlotDesc d = A.slots[s0 + i];
+    int len = d.nresp > d.nprop ? d.nresp : d.nprop;
+    F0 = d.fs < F0 ? d.fs : F0;
+    F1 = (int64_t)d.fs + len > F1 ? (int64_t)d.fs + len : F1;
+  }
+  for (int i = t; i < A.tmax_acc; i += kConvBlock) acc[i] = make_double2(0.0, 0.0);
+  __syncthreads();
+
+  for (int64_t f = F0; f < F1; ++f) {
+    const int64_t row = f - A.f_lo;
+    const int64_t rel = f - sd.fs;
+    const bool starts = has && rel == 0;
+    const bool prev_prop = has && rel >= 1 && rel - 1 < sd.nprop;
+    const bool resp = has && rel >= 0 && rel < sd.nresp;
+    const bool prop = has && rel >= 0 && rel < sd.nprop;
+    const float *mu = A.mu_rec + (A.mu_stride ? f * A.mu_stride : 0) + 1 + (int64_t)sd.x * N;
+    const float2 *Dcur = A.diag + row * N;
+    const float2 *Dprev = A.diag + (row - 1) * N;
+
+    // Per-site pass: pending second half-phase of frame f-1, response
+    // mu(f).psi (src/absorption.c:130, calc_S1 :205-212), first half-phase
+    // of frame f.
+    float rr = 0.f, ri = 0.f;
+    if (starts || prev_prop || resp || prop) {
+      for (int i = q; i < N; i += kRowsPerPass) {
+        float2 p;
+        if (starts) {
+          p = (sd.init == 0) ? make_float2(mu[i], 0.f) : A.state[(int64_t)sd.out * N + i];
+        } else {
+          p = psi[i * VP + v];
+          if (prev_prop) p = cmul_diag(p, Dprev[i]);
+        }
+        if (resp) {
+          float m = mu[i];
+          rr = __fadd_rn(rr, __fmul_rn(m, p.x));
+          ri = __fadd_rn(ri, __fmul_rn(m, p.y));
+        }
+        if (prop) p = cmul_diag(p, Dcur[i]);
+        psi[i * VP + v] = p;
+      }
+    }
+    if (A.tmax_acc > 0) {
+      // reduce the per-thread partial dot products of each slot
+#pragma unroll
+      for (int off = VP; off < 64; off <<= 1) {
+        rr += __shfl_xor(rr, off);
+        ri += __shfl_xor(ri, off);
+      }
+      if (lane < VP) red[wave * VP + lane] = make_float2(rr, ri);
+      __syncthreads();
+      if (t == 0) {
+        for (int vv = 0; vv < ns; ++vv) {
+          SlotDesc d = A.slots[s0 + vv];
+          int64_t r2 = f - d.fs;
+          if (r2 < 0 || r2 >= d.nresp) continue;
+          float sr = 0.f, si = 0.f;
+          for (int w = 0; w < kWaves; ++w) sr += red[w * VP + vv].x, si += red[w * VP + vv].y;
+          acc[r2].x += (double)sr;
+          acc[r2].y += (double)si;
+        }
+      }
+    }
+    __syncthreads();
+
+    // Rotation sweeps (m Trotter substeps).
+    for (int sub = 0; sub < A.m; ++sub) {
+      if (sub > 0) {
+        // between substeps: second half-phase then first half-phase
+        if (prop)
+          for (int i = q; i < N; i += kRowsPerPass)
+            psi[i * VP + v] = cmul_diag(cmul_diag(psi[i * VP + v], Dcur[i]), Dcur[i]);
+        __syncthreads();
+      }
+      const float2 *cs = A.cs + row * A.K;
+      for (int l = 0; l < A.nlayers; ++l) {
+        const int r0 = A.layer_off[l], r1 = A.layer_off[l + 1];
+        if (prop) {
+          for (int r = r0 + q; r < r1; r += kRowsPerPass) {
+            uint32_t p = A.ab[r];
+            int a = (int)(p & 0xffffu), b = (int)(p >> 16);
+            float2 c = cs[r];
+            float2 pa = psi[a * VP + v], pb = psi[b * VP + v];
+            rotate_pair(pa, pb, c.x, c.y);
+            psi[a * VP + v] = pa;
+            psi[b * VP + v] = pb;
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+
+  // write back: pending second half-phase of the last propagated frame
+  if (A.writeback && has && sd.out >= 0 && sd.nprop > 0) {
+    const int64_t row = (int64_t)sd.fs + sd.nprop - 1 - A.f_lo;
+    for (int i = q; i < N; i += kRowsPerPass)
+      A.state[(int64_t)sd.out * N + i] = cmul_diag(psi[i * VP + v], A.diag[row * N + i]);
+  }
+  if (A.tmax_acc > 0)
+    for (int i = t; i < A.tmax_acc; i += kConvBlock) A.partial[(int64_t)g * A.tmax_acc + i] = acc[i];
+}
+
+size_t conveyor_smem(int N, int vp, int tmax_acc) {
+  return (size_t)N * vp * sizeof(float2) + (size_t)tmax_acc * sizeof(double2) +
+         (size_t)(kConvBlock / 64) * vp * sizeof(float2);
+}
+
+int launch_conveyor(const ConveyorArgs &a, int vp, hipStream_t st) {
+  size_t smem = conveyor_smem(a.N, vp, a.tmax_acc);
+  if (smem > 160 * 1024) {
+    set_error("conveyor: N=%d with %d slots needs %zu B of LDS (max 160 KiB)", a.N, vp, smem);
+    return NISE_ERR_ARG;
+  }
+  if (a.nwg == 0) return NISE_OK;
+  switch (vp) {
+#define NISE_CONV_CASE(V)                                                                   \
+  case V:                                                                                   \
+    NISE_HIP_CHECK(hipFuncSetAttribute((const void *)conveyor_kernel<V>,                   \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)); \
+    hipLaunchKernelGGL(conveyor_kernel<V>, dim3(a.nwg), dim3(kConvBlock), smem, st, a);    \
+    break;
+    NISE_CONV_CASE(4)
+    NISE_CONV_CASE(8)
+    NISE_CONV_CASE(16)
+    NISE_CONV_CASE(32)
+    NISE_CONV_CASE(64)
+#undef NISE_CONV_CASE
+    default:
+      set_error("conveyor: unsupported slot count %d", vp);
+      return NISE_ERR_ARG;
+  }
+  NISE_HIP_CHECK(hipGetLastError());
+  return NISE_OK;
+}
+
+// Deterministic sum of the per-workgroup response partials (fixed order,
+// double accumulation), rounded to float at the end.
+__global__ void reduce_partials_kernel(const double2 *partial, int nwg, int tmax, float *re, float *im) {
+  int t1 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t1 >= tmax) return;
+  double sr = 0, si = 0;
+  for (int g = 0; g < nwg; ++g) {
+    double2 p = partial[(int64_t)g * tmax + t1];
+    sr += p.x, si += p.y;
+  }
+  re[t1] = (float)sr;
+  im[t1] = (float)si;
+}
+
+int launch_reduce_partials(const double2 *partial, int nwg, int tmax, float *re, float *im,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3((tmax + 127) / 128), dim3(128), 0, st, partial,
+                     nwg, tmax, re, im);
+  NISE_HIP_CHECK(hipGetLastError());
+  return NISE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plain per-sample kernel: one workgroup, the three polarisation vectors of
+// one sample, frames re-read from HBM at every step.  Mirrors the
+// reference's per-call structure (src/absorption.c:113-145 with
+// src/propagate.c:821-899 inside): per step, compact the kept couplings of
+// the frame (prefix sum over the triangle), evaluate the rotation angles,
+// then sweep the rotations in lexicographic order, one lane per
+// polarisation.
+// ---------------------------------------------------------------------------
+constexpr int kPlainBlock = 256;
+
+__global__ void __launch_bounds__(kPlainBlock) plain_sample_kernel(PlainArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = A.N;
+  const int npair = N * (N - 1) / 2;
+  float2 *psi = reinterpret_cast<float2 *>(smem);        // [3][N]
+  float2 *D = psi + 3 * N;                               // [N]
+  int *cnt = reinterpret_cast<int *>(D + N);             // [kPlainBlock + 1]
+  float2 *cs = A.ws_cs;                                  // [npair] global
+  uint32_t *ab = A.ws_ab;                                // [npair] global
+  float2 *red = reinterpret_cast<float2 *>(cnt + kPlainBlock + 1);  // [3][waves]
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  constexpr int kWaves = kPlainBlock / 64;
+
+  for (int x = 0; x < 3; ++x) {
+    const float *mu0 = A.mu_rec + (A.mu_stride ? A.ti * A.mu_stride : 0) + 1 + x * N;
+    for (int i = t; i < N; i += kPlainBlock) psi[x * N + i] = make_float2(mu0[i], 0.f);
+  }
+  __syncthreads();
+  for (int t1 = 0; t1 < A.tmax; ++t1) {
+    const int64_t f = A.ti + t1;
+    // response for the three polarisations
+    for (int x = 0; x < 3; ++x) {
+      const float *mu = A.mu_rec + (A.mu_stride ? f * A.mu_stride : 0) + 1 + x * N;
+      float rr = 0.f, ri = 0.f;
+      for (int i = t; i < N; i += kPlainBlock) {
+        rr += mu[i] * psi[x * N + i].x;
+        ri += mu[i] * psi[x * N + i].y;
+      }
+      for (int off = 32; off > 0; off >>= 1) rr += __shfl_xor(rr, off), ri += __shfl_xor(ri, off);
+      if (lane == 0) red[x * kWaves + wave] = make_float2(rr, ri);
+    }
+    __syncthreads();
+    if (t == 0) {
+      double sr = 0, si = 0;
+      for (int x = 0; x < 3; ++x)
+        for (int w = 0; w < kWaves; ++w) sr += red[x * kWaves + w].x, si += red[x * kWaves + w].y;
+      A.resp[t1].x += sr;
+      A.resp[t1].y += si;
+    }
+    if (t1 == A.tmax - 1) break;  // the last propagation is never observed
+    // diagonal phases
+    for (int i = t; i < N; i += kPlainBlock) {
+      float h = site_energy(A.src, N, f, i);
+      double arg = 0.5 * (double)h * (double)A.fac;
+      D[i] = make_float2((float)cos(arg), (float)(-sin(arg)));
+    }
+    // compaction of kept couplings in lexicographic order: each thread owns
+    // a contiguous chunk of the pair list
+    const int chunk = (npair + kPlainBlock - 1) / kPlainBlock;
+    const int p0 = t * chunk, p1 = min(npair, p0 + chunk);
+    int a = 0, rem = p0;
+    while (a < N - 1 && rem >= N - 1 - a) { rem -= N - 1 - a; ++a; }
+    int b = a + 1 + rem;
+    int mine = 0;
+    {
+      int aa = a, bb = b;
+      for (int p = p0; p < p1; ++p) {
+        mine += kept_coupling(A.src, N, f, aa, bb) != 0.f;
+        if (++bb == N) { ++aa; bb = aa + 1; }
+      }
+    }
+    cnt[t] = mine;
+    __syncthreads();
+    if (t == 0) {
+      int s = 0;
+      for (int i = 0; i < kPlainBlock; ++i) { int c = cnt[i]; cnt[i] = s; s += c; }
+      cnt[kPlainBlock] = s;
+    }
+    __syncthreads();
+    {
+      int k = cnt[t];
+      int aa = a, bb = b;
+      for (int p = p0; p < p1; ++p) {
+        float J = kept_coupling(A.src, N, f, aa, bb);
+        if (J != 0.f) {
+          float Jf = __fmul_rn(J, A.fac);
+          float si = (float)(-sin((double)Jf));
+          float co = (float)sqrt((double)__fsub_rn(1.0f, __fmul_rn(si, si)));
+          cs[k] = make_float2(co, si);
+          ab[k] = (uint32_t)aa | ((uint32_t)bb << 16);
+          ++k;
+        }
+        if (++bb == N) { ++aa; bb = aa + 1; }
+      }
+    }
+    __syncthreads();
+    const int kmax = cnt[kPlainBlock];
+    for (int sub = 0; sub < A.m; ++sub) {
+      for (int i = t; i < 3 * N; i += kPlainBlock) psi[i] = cmul_diag(psi[i], D[i % N]);
+      __syncthreads();
+      if (t < 3) {
+        float2 *p = psi + t * N;
+        for (int k = 0; k < kmax; ++k) {
+          uint32_t pk = ab[k];
+          int ka = (int)(pk & 0xffffu), kb = (int)(pk >> 16);
+          float2 pa = p[ka], pb = p[kb];
+          rotate_pair(pa, pb, cs[k].x, cs[k].y);
+          p[ka] = pa;
+          p[kb] = pb;
+        }
+      }
+      __syncthreads();
+      for (int i = t; i < 3 * N; i += kPlainBlock) psi[i] = cmul_diag(psi[i], D[i % N]);
+      __syncthreads();
+    }
+  }
+}
+
+int launch_plain_sample(const PlainArgs &a, hipStream_t st) {
+  int N = a.N, npair = N * (N - 1) / 2;
+  (void)npair;
+  size_t smem = (size_t)4 * N * sizeof(float2) + (kPlainBlock + 1) * sizeof(int) +
+                3 * (kPlainBlock / 64) * sizeof(float2);
+  if (smem > 160 * 1024) {
+    set_error("plain kernel: N=%d needs %zu B of LDS", N, smem);
+    return NISE_ERR_ARG;
+  }
+  NISE_HIP_CHECK(hipFuncSetAttribute((const void *)plain_sample_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  hipLaunchKernelGGL(plain_sample_kernel, dim3(1), dim3(kPlainBlock), smem, st, a);
+  NISE_HIP_CHECK(hipGetLastError());
+  return NISE_OK;
+}
+
+}  // namespace nise

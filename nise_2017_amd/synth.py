@@ -1,0 +1,150 @@
+"""Synthetic NISE trajectories in the reference's binary layout (GROBIN).
+
+Every frame is one int32 header followed by float32 payload
+(doc/input.tex:58-90, src/read_trajectory.c:138-176,357-377):
+  Full Hamiltonian  : N(N+1)/2 floats, packed upper triangle, row major
+  Coupling-type H   : N diagonal floats (static couplings in one Full record)
+  Dipole            : 3N floats, x block, y block, z block
+Arrays returned here are float32 records of shape (frames, stride) whose
+first column holds the header word (the frame number, stored as int32 bits).
+
+Site energies follow the AR(1) process x_{t+1} = a x_t + sqrt(1-a^2) s xi,
+a = exp(-dt/tau), started from its stationary distribution, the same form as
+the reference's tutorial generator (example/tutorial/src/stochastic.c:29-41).
+"""
+import numpy as np
+
+SEED0 = 20261202
+
+
+def sindex(a, b, N):
+    if a > b:
+        a, b = b, a
+    return b + a * (2 * N - a - 1) // 2
+
+
+def tri_index_arrays(N):
+    """(rows, cols) of the packed upper triangle in storage order."""
+    a, b = np.triu_indices(N)
+    return a, b
+
+
+def ar1(rng, nframes, nsites, mean, sd, tau, dt):
+    a = np.exp(-dt / tau)
+    b = np.sqrt(1.0 - a * a)
+    x = np.empty((nframes, nsites), np.float64)
+    x[0] = rng.normal(0.0, sd, nsites)
+    noise = rng.normal(0.0, sd, (nframes - 1, nsites))
+    for t in range(1, nframes):
+        x[t] = a * x[t - 1] + b * noise[t - 1]
+    return (x + mean).astype(np.float32)
+
+
+def unit_vectors(rng, n):
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def headers(nframes):
+    return np.arange(nframes, dtype=np.int32).view(np.float32)
+
+
+def full_records(diag, coup_tri):
+    """Full Hamiltonian records from per-frame diagonals (T, N) and per-frame
+    or static strict-upper couplings (T, N(N-1)/2) or (N(N-1)/2,)."""
+    T, N = diag.shape
+    nn2 = N * (N + 1) // 2
+    rec = np.empty((T, 1 + nn2), np.float32)
+    rec[:, 0] = headers(T)
+    a, b = tri_index_arrays(N)
+    is_diag = a == b
+    pos = 1 + np.arange(nn2)
+    rec[:, pos[is_diag]] = diag
+    rec[:, pos[~is_diag]] = coup_tri
+    return rec
+
+
+def dipole_records(mu_site, nframes):
+    """Static per-site dipoles (N, 3) repeated over frames: (T, 1 + 3N)."""
+    N = mu_site.shape[0]
+    rec = np.empty((nframes, 1 + 3 * N), np.float32)
+    rec[:, 0] = headers(nframes)
+    rec[:, 1:] = np.concatenate([mu_site[:, 0], mu_site[:, 1], mu_site[:, 2]])[None, :]
+    return rec
+
+
+def chain_couplings(N, J):
+    """Strict upper triangle (storage order) with J[i] on (i, i+1)."""
+    a, b = np.triu_indices(N, 1)
+    out = np.zeros(a.size, np.float32)
+    nn = b == a + 1
+    out[nn] = J[a[nn]]
+    return out
+
+
+def dipole_coupling(pos, mu, prefactor, cut):
+    """J_ij = prefactor * (mu_i.mu_j - 3 (mu_i.r)(mu_j.r)) / r^3 (unit r in
+    lattice units), |J| < cut -> 0.  Strict upper triangle, storage order."""
+    N = pos.shape[0]
+    a, b = np.triu_indices(N, 1)
+    r = pos[b] - pos[a]
+    d = np.linalg.norm(r, axis=1)
+    rh = r / d[:, None]
+    kappa = np.einsum("ij,ij->i", mu[a], mu[b]) - 3 * np.einsum("ij,ij->i", mu[a], rh) * np.einsum(
+        "ij,ij->i", mu[b], rh)
+    J = prefactor * kappa / d ** 3
+    J[np.abs(J) < cut] = 0.0
+    return J.astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# The benchmark configurations
+# ---------------------------------------------------------------------------
+
+def chain_absorption(N=16, nframes=4200, seed=SEED0, dt=2.0, mean=1650.0, sd=10.0, tau=100.0,
+                     jmean=-5.0, jsd=2.0, jfluct=1.0):
+    """N-site chain, Full type: AR(1) site energies, nearest-neighbour
+    couplings static N(jmean, jsd) plus AR(1) fluctuation of sd jfluct,
+    fixed random unit dipoles."""
+    rng = np.random.default_rng(seed)
+    diag = ar1(rng, nframes, N, mean, sd, tau, dt)
+    j0 = rng.normal(jmean, jsd, N - 1)
+    jf = ar1(rng, nframes, N - 1, 0.0, jfluct, tau, dt) if jfluct > 0 else np.zeros((nframes, N - 1), np.float32)
+    a, b = np.triu_indices(N, 1)
+    nn = np.nonzero(b == a + 1)[0]
+    coup = np.zeros((nframes, a.size), np.float32)
+    coup[:, nn] = (j0[None, :] + jf).astype(np.float32)
+    H = full_records(diag, coup)
+    mu = dipole_records(unit_vectors(rng, N), nframes)
+    return H, mu
+
+
+def lattice_static(L=16, seed=SEED0 + 1, prefactor=8.0, cut=0.1):
+    """L x L square lattice (spacing 1) with random unit dipoles: site
+    positions, dipoles, and the static dipole-dipole couplings."""
+    rng = np.random.default_rng(seed)
+    N = L * L
+    pos = np.array([[i % L, i // L, 0.0] for i in range(N)], np.float64)
+    mu = unit_vectors(rng, N)
+    coup = dipole_coupling(pos, mu, prefactor, cut)
+    return rng, mu, coup
+
+
+def lattice_diag(rng, nframes, N, dt=2.0, mean=1650.0, sd=10.0, tau=100.0):
+    return ar1(rng, nframes, N, mean, sd, tau, dt)
+
+
+def lattice_absorption(L=16, nframes=50000, seed=SEED0 + 1, dt=2.0):
+    """Full-type records for the L x L dipole lattice (host memory)."""
+    rng, mu, coup = lattice_static(L, seed)
+    N = L * L
+    diag = lattice_diag(rng, nframes, N, dt)
+    return full_records(diag, coup), dipole_records(mu, nframes)
+
+
+def shifted_center(H_full, N):
+    """(min1, max1) window centred on the mean site energy."""
+    a, b = np.triu_indices(N)
+    d = H_full[:, 1:][:, a == b]
+    c = float(np.mean(d))
+    return c - 100.0, c + 100.0

@@ -1,0 +1,93 @@
+/*
+ * ORACLE -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement of the reference NISE hot path (GHlacour/NISE_2017), used
+ * by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+ * checker.  Nothing in nise_2017_amd/ links, loads or calls it.
+ *
+ * Every function states the reference lines it follows.  *_f32 variants keep
+ * the reference's FP32 arithmetic in the reference's order (compiled with the
+ * reference's flags -O2 -ffast-math, so they are bit-identical to the
+ * reference build where one can be made, see oracle/Makefile); *_f64
+ * variants are the same algorithm in double precision and give the error
+ * figure against which every FP32 result is reported.
+ *
+ * Pinning (what fixes each function independently of this file):
+ *   or_coupling_S_f32 / or_coupling_S_doubles_f32 / or_dipole_double* /
+ *   or_polar* / or_read_H* : bit-identical to the reference's own
+ *     propagate.c / NISE_subs.c / polar.c / read_trajectory.c compiled from
+ *     the reference sources (oracle/_ref), and to tests/golden fixtures.
+ *   or_coupling_S_f64 : closed forms (uncoupled phases, dimer D.R.D,
+ *     norm conservation), substep identity, FP64 expm limit.
+ *   or_DIA_f64 / or_jacobi_f64 : eigen-decomposition residuals and FP64
+ *     matrix exponential (numpy eigh / scipy expm) in tests.
+ *   or_absorption : per-step composition of the pinned primitives; compared
+ *     with the reference-primitive driver in oracle/_ref.
+ *   or_1dfft : numpy.fft with the reference's re/im swap and scaling.
+ *   or_2dir_item : composition of pinned primitives; N=1 three-level closed
+ *     form in tests.  The order and signs of the 12 accumulators follow
+ *     src/calc_2DIR.c line by line; calc_2DIR.c itself needs MPI and is not
+ *     built here, so that part is "parity unpinned" beyond the closed form.
+ */
+#ifndef NISE_ORACLE_H
+#define NISE_ORACLE_H
+
+#include "../include/nise_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int or_sindex(int a, int b, int N);
+
+void or_coupling_S_f32(int N, const float *H, float *cr, float *ci, int m,
+                       int sign, float deltat, float couplingcut);
+void or_coupling_S_f64(int N, const float *H, double *cr, double *ci, int m,
+                       int sign, float deltat, float couplingcut);
+/* Exact inverse of one or_coupling_S_f64 step (reverse order, negated
+ * angles): used to pin the adjoint kernel. */
+void or_coupling_S_inv_f64(int N, const float *H, double *cr, double *ci,
+                           int m, int sign, float deltat, float couplingcut);
+
+void or_coupling_S_doubles_f32(int N, const float *H, float *cr, float *ci,
+                               int m, const float *Anh, float anharmonicity,
+                               float deltat, float couplingcut);
+void or_coupling_S_doubles_f64(int N, const float *H, double *cr, double *ci,
+                               int m, const float *Anh, float anharmonicity,
+                               float deltat, float couplingcut);
+
+int or_jacobi_f64(int N, double *A, double *w, double *V);
+void or_DIA_f64(int N, const float *H, double *cr, double *ci, int sign,
+                float deltat);
+
+void or_dipole_double(int N, const float *dipole, const float *cr,
+                      const float *ci, float *fr, float *fi, float *over,
+                      float anharmonicity);
+void or_dipole_double_last(int N, const float *dipole, const float *cr,
+                           const float *ci, float *fr, float *fi, float *over,
+                           float anharmonicity);
+float or_polarweight(int pol, int x);
+void or_polar(int *xp, int x);
+
+/* One Hamiltonian frame as read_Hamiltonian()/read_He() leave it. */
+void or_read_H(const t_non *non, const float *H_rec, long long H_stride,
+               const float *C_rec, long long pos, float *He, int apply_shift);
+
+int or_absorption(const t_non *non, const float *H_rec, long long H_stride,
+                  const float *mu_rec, long long mu_stride,
+                  const float *C_rec, float *re_S, float *im_S,
+                  double *re_S64, double *im_S64, int nthreads);
+
+int or_1dfft(const t_non *non, const float *re_S, const float *im_S,
+             int samples, double *freq, double *spec_r, double *spec_i,
+             int *nout);
+
+int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
+                 const float *mu_rec, long long mu_stride, const float *C_rec,
+                 const float *A_rec, const float *over_rec, int sample,
+                 int molPol, float *R12);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,176 @@
+/*
+ * ORACLE -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Thin driver compiled together with the reference's own, unmodified
+ * sources (propagate.c, NISE_subs.c, polar.c, read_trajectory.c,
+ * hamshift.c, randomlib.c from the reference's src/) into
+ * oracle/_ref/libnise_ref.so by oracle/Makefile.  It exposes the reference
+ * primitives to the tests so that the restatement in nise_oracle.c can be
+ * pinned bit for bit.  The reference objects are built with hidden
+ * visibility and linked with --gc-sections, so functions that would need
+ * LAPACK (ssyev_/sgeev_) are dropped; nothing here calls them.
+ *
+ * The two loop drivers (ref_absorption_files, ref_2dir_item_files) restate
+ * the loop structure of src/absorption.c:92-146 and src/calc_2DIR.c:184-506
+ * (those files need FFTW/MPI headers this image lacks), but every file read,
+ * propagation, dipole and polarisation operation inside them is the
+ * reference's own compiled function.
+ */
+#include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "types.h"
+#include "NISE_subs.h"
+#include "propagate.h"
+#include "polar.h"
+#include "read_trajectory.h"
+#include "hamshift.h"
+
+#define EXPORT __attribute__((visibility("default")))
+
+static void init_non(t_non *non, int N, float deltat, float cut) {
+  memset(non, 0, sizeof(*non));
+  non->singles = N;
+  non->deltat = deltat;
+  non->couplingcut = cut;
+  non->ts = 1;
+  non->propagation = 1;
+  non->cluster = -1;
+  strcpy(non->hamiltonian, "Full");
+}
+
+EXPORT void ref_coupling_S(int N, float *H, float *cr, float *ci, int m,
+                           int sign, float deltat, float cut) {
+  t_non non;
+  init_non(&non, N, deltat, cut);
+  propagate_vec_coupling_S(&non, H, cr, ci, m, sign);
+}
+
+EXPORT void ref_coupling_S_doubles(int N, float *H, float *cr, float *ci,
+                                   int m, float *Anh, float anharmonicity,
+                                   float deltat, float cut) {
+  t_non non;
+  init_non(&non, N, deltat, cut);
+  non.anharmonicity = anharmonicity;
+  propagate_vec_coupling_S_doubles(&non, H, cr, ci, m, Anh);
+}
+
+EXPORT void ref_dipole_double(int N, float *dipole, float *cr, float *ci,
+                              float *fr, float *fi, float *over,
+                              float anharmonicity) {
+  t_non non;
+  init_non(&non, N, 1, 0);
+  non.anharmonicity = anharmonicity;
+  dipole_double(&non, dipole, cr, ci, fr, fi, over);
+}
+
+EXPORT void ref_dipole_double_last(int N, float *dipole, float *cr, float *ci,
+                                   float *fr, float *fi, float *over,
+                                   float anharmonicity) {
+  t_non non;
+  init_non(&non, N, 1, 0);
+  non.anharmonicity = anharmonicity;
+  dipole_double_last(&non, dipole, cr, ci, fr, fi, over);
+}
+
+EXPORT float ref_polarweight(int pol, int x) { return polarweight(pol, x); }
+EXPORT void ref_polar(int *xp, int x) { polar(xp, x); }
+EXPORT int ref_sindex(int a, int b, int N) { return Sindex(a, b, N); }
+
+/* Layout of the reference's t_non, to pin include/nise_abi.h. */
+EXPORT int ref_tnon_layout(long long *out, int n) {
+  long long v[] = {
+      sizeof(t_non), offsetof(t_non, tmax1), offsetof(t_non, length),
+      offsetof(t_non, deltat), offsetof(t_non, singles),
+      offsetof(t_non, energyFName), offsetof(t_non, hamiltonian),
+      offsetof(t_non, pbcFName), offsetof(t_non, tmax),
+      offsetof(t_non, shifte), offsetof(t_non, thres),
+      offsetof(t_non, couplingcut), offsetof(t_non, anharmonicity),
+      offsetof(t_non, psites), offsetof(t_non, window),
+      offsetof(t_non, SingleShiftSite), offsetof(t_non, SingleShift),
+      offsetof(t_non, printLevel)};
+  int k = (int)(sizeof(v) / sizeof(v[0]));
+  for (int i = 0; i < k && i < n; i++) out[i] = v[i];
+  return k;
+}
+
+static void copy_name(char *dst, const char *src) {
+  strncpy(dst, src, 255);
+  dst[255] = 0;
+}
+
+/* One frame through the reference's read_Hamiltonian / read_coupling. */
+EXPORT int ref_read_frame(int N, const char *hfile, const char *dfile,
+                          const char *cfile, const char *shiftfile,
+                          const char *htype, float shifte, float cut, int pos,
+                          float *He, float *mu3) {
+  t_non non;
+  init_non(&non, N, 1, cut);
+  copy_name(non.energyFName, hfile);
+  copy_name(non.dipoleFName, dfile);
+  copy_name(non.couplingFName, cfile ? cfile : "");
+  copy_name(non.singleShiftFName, shiftfile ? shiftfile : "");
+  copy_name(non.hamiltonian, htype);
+  non.shifte = shifte;
+  non.length = 1 << 29;
+  FILE *H_traj, *mu_traj, *Cfile = NULL, *C_traj = NULL;
+  open_files(&non, &H_traj, &mu_traj, &Cfile);
+  float *mu_xyz = calloc(3 * N, sizeof(float));
+  read_coupling(&non, C_traj, mu_traj, He, mu_xyz);
+  read_Hamiltonian(&non, He, H_traj, pos);
+  for (int x = 0; x < 3; x++) read_dipole(&non, mu_traj, mu3 + x * N, mu_xyz, x, pos);
+  free(mu_xyz);
+  fclose(H_traj), fclose(mu_traj);
+  return 0;
+}
+
+/* Loop of src/absorption.c:92-146 over the reference's primitives.  calc_S1
+ * (src/absorption.c:205-212) is the two-line running sum written inline. */
+EXPORT int ref_absorption_files(int N, const char *hfile, const char *dfile,
+                                const char *cfile, const char *shiftfile,
+                                const char *htype, float min1, float max1,
+                                float deltat, float cut, int ts, int tmax,
+                                int sample, int begin, int end, int length,
+                                float *re_S, float *im_S) {
+  t_non non;
+  init_non(&non, N, deltat, cut);
+  copy_name(non.energyFName, hfile);
+  copy_name(non.dipoleFName, dfile);
+  copy_name(non.couplingFName, cfile ? cfile : "");
+  copy_name(non.singleShiftFName, shiftfile ? shiftfile : "");
+  copy_name(non.hamiltonian, htype);
+  non.min1 = min1, non.max1 = max1;
+  non.ts = ts;
+  non.tmax = non.tmax1 = tmax;
+  non.sample = sample, non.begin = begin, non.end = end;
+  non.length = length;
+  non.shifte = (non.max1 + non.min1) / 2; /* src/absorption.c:57-59 */
+  int nn2 = N * (N + 1) / 2;
+  float *Hamil_i_e = calloc(nn2, sizeof(float));
+  float *vecr = calloc(N, sizeof(float)), *veci = calloc(N, sizeof(float));
+  float *mu_eg = calloc(N, sizeof(float)), *mu_xyz = calloc(3 * N, sizeof(float));
+  FILE *H_traj, *mu_traj, *Cfile = NULL, *C_traj = NULL;
+  open_files(&non, &H_traj, &mu_traj, &Cfile);
+  read_coupling(&non, C_traj, mu_traj, Hamil_i_e, mu_xyz);
+  for (int samples = non.begin; samples < non.end; samples++) {
+    int ti = samples * non.sample;
+    for (int x = 0; x < 3; x++) {
+      read_dipole(&non, mu_traj, vecr, mu_xyz, x, ti);
+      clearvec(veci, non.singles);
+      for (int t1 = 0; t1 < non.tmax; t1++) {
+        int tj = ti + t1;
+        read_Hamiltonian(&non, Hamil_i_e, H_traj, tj);
+        read_dipole(&non, mu_traj, mu_eg, mu_xyz, x, tj);
+        for (int i = 0; i < non.singles; i++) {
+          re_S[t1] += mu_eg[i] * vecr[i];
+          im_S[t1] += mu_eg[i] * veci[i];
+        }
+        propagate_vec_coupling_S(&non, Hamil_i_e, vecr, veci, non.ts, 1);
+      }
+    }
+  }
+  fclose(mu_traj), fclose(H_traj);
+  free(Hamil_i_e), free(vecr), free(veci), free(mu_eg), free(mu_xyz);
+  return 0;
+}

@@ -1,0 +1,212 @@
+"""GPU parity: singles Trotter, DIA and batched absorption against the oracle.
+
+Tolerances: the Trotter kernels reproduce the reference's FP32 operation
+order (no FMA, double-precision trig rounded to float), so one step agrees
+with the oracle to the last bit except where the device and host libm round
+a double sin/cos differently; multi-step results are held to 1e-6 relative.
+Absorption response sums differ from the oracle only by summation order
+(FP32 running sum in the reference, per-vector FP32 dot + FP64 sums here):
+held to 1e-5 relative L2, and both are reported against the FP64 oracle.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from nise_2017_amd import abi, synth
+from nise_2017_amd.abi import make_non
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def random_frame(rng, N, density=1.0, jsd=30.0):
+    nn = N * (N + 1) // 2
+    H = rng.normal(0, jsd, nn).astype(np.float32)
+    if density < 1:
+        H[rng.random(nn) > density] = 0
+    for a in range(N):
+        H[synth.sindex(a, a, N)] = rng.normal(0, 50)
+    return H
+
+
+@pytest.mark.parametrize("N,m,sign,cut,density", [
+    (1, 1, 1, 0.0, 1.0), (2, 1, 1, 0.0, 1.0), (5, 3, -1, 0.0, 1.0), (16, 1, 1, 5.0, 1.0),
+    (33, 2, 1, 0.0, 0.3), (64, 1, -1, 0.0, 0.1), (130, 1, 1, 1.0, 0.05), (257, 1, 1, 0.0, 0.02)])
+def test_coupling_S_matches_reference_order(orc, nise_lib, N, m, sign, cut, density):
+    rng = np.random.default_rng(N * 7 + m)
+    non = make_non(singles=N, deltat=2.0, couplingcut=cut, ts=m, propagation=1)
+    H = random_frame(rng, N, density)
+    cr = rng.normal(size=N).astype(np.float32)
+    ci = rng.normal(size=N).astype(np.float32)
+    g_r, g_i = cr.copy(), ci.copy()
+    o_r, o_i = cr.copy(), ci.copy()
+    # one step: same operations in the same order as the reference
+    nise_lib.propagate_vec_coupling_S(C.byref(non), H, g_r, g_i, m, sign)
+    orc.oracle().or_coupling_S_f32(N, H, o_r, o_i, m, sign, 2.0, cut)
+    ulps = np.abs(g_r.view(np.int32).astype(np.int64) - o_r.view(np.int32).astype(np.int64)).max()
+    assert ulps <= 2, f"one step differs by {ulps} ulp"
+    for _ in range(9):
+        nise_lib.propagate_vec_coupling_S(C.byref(non), H, g_r, g_i, m, sign)
+        orc.oracle().or_coupling_S_f32(N, H, o_r, o_i, m, sign, 2.0, cut)
+    assert rel_l2(g_r + 1j * g_i, o_r + 1j * o_i) < 1e-6
+    # norm conservation of the unitary steps
+    n0 = np.sum(cr.astype(np.float64) ** 2 + ci.astype(np.float64) ** 2)
+    n1 = np.sum(g_r.astype(np.float64) ** 2 + g_i.astype(np.float64) ** 2)
+    assert abs(n1 / n0 - 1) < 1e-5
+
+
+def test_propagate_vector_and_matrix(orc, nise_lib):
+    rng = np.random.default_rng(3)
+    N = 12
+    non = make_non(singles=N, deltat=2.0, ts=2, propagation=1)
+    H = random_frame(rng, N)
+    V = rng.normal(size=(N, N)).astype(np.float32)
+    W = rng.normal(size=(N, N)).astype(np.float32)
+    vr, vi = V.copy(), W.copy()
+    nise_lib.propagate_matrix(C.byref(non), H, vr.reshape(-1), vi.reshape(-1), 1, 0, 0)
+    for j in range(N):
+        r, i = V[j].copy(), W[j].copy()
+        orc.oracle().or_coupling_S_f32(N, H, r, i, 2, 1, 2.0, 0.0)
+        assert rel_l2(vr[j] + 1j * vi[j], r + 1j * i) < 1e-6
+        r2, i2 = V[j].copy(), W[j].copy()
+        nise_lib.propagate_vector(C.byref(non), H, r2, i2, 1, 0, 0)
+        assert rel_l2(r2 + 1j * i2, r + 1j * i) < 1e-6
+
+
+@pytest.mark.parametrize("N", [1, 4, 17, 64])
+def test_dia_matches_fp64_expm(orc, nise_lib, N):
+    rng = np.random.default_rng(N)
+    non = make_non(singles=N, deltat=2.0, propagation=0, thres=0.0)
+    H = random_frame(rng, N)
+    cr = rng.normal(size=N).astype(np.float32)
+    ci = rng.normal(size=N).astype(np.float32)
+    g_r, g_i = cr.copy(), ci.copy()
+    nise_lib.propagate_vec_DIA(C.byref(non), H, g_r, g_i, 1)
+    d_r, d_i = cr.astype(np.float64), ci.astype(np.float64)
+    orc.oracle().or_DIA_f64(N, H, d_r, d_i, 1, 2.0)
+    # independent check: numpy eigh of the dense matrix
+    A = np.zeros((N, N))
+    for a in range(N):
+        for b in range(N):
+            A[a, b] = H[synth.sindex(a, b, N)]
+    w, V = np.linalg.eigh(A)
+    f = np.float32(np.float32(np.float32(2.0) * np.float32(2.99792458e-5)) * np.float32(2 * 3.14159265359))
+    U = (V * np.exp(-1j * w * float(f))) @ V.T
+    ref = U @ (cr + 1j * ci).astype(np.complex128)
+    assert rel_l2(d_r + 1j * d_i, ref) < 1e-9
+    assert rel_l2(g_r + 1j * g_i, ref) < 2e-6
+    g2_r, g2_i = cr.copy(), ci.copy()
+    nise_lib.propagate_vector(C.byref(non), H, g2_r, g2_i, 1, 0, 0)
+    assert rel_l2(g2_r + 1j * g2_i, ref) < 2e-6
+
+
+def test_diagonalizeLPD(nise_lib):
+    rng = np.random.default_rng(11)
+    N = 37
+    A = rng.normal(size=(N, N)).astype(np.float32)
+    A = (A + A.T).astype(np.float32)
+    H = A.copy()
+    v = np.zeros(N, np.float32)
+    nise_lib.diagonalizeLPD(H.reshape(-1), v, N)
+    w = np.linalg.eigvalsh(A.astype(np.float64))
+    assert np.max(np.abs(v - w)) < 1e-4
+    # columns of H are eigenvectors: A V = V diag(v)
+    assert np.max(np.abs(A @ H - H * v[None, :])) < 1e-3
+    assert np.max(np.abs(H.T @ H - np.eye(N))) < 1e-5
+
+
+def absorption_case(kind, N, T, nsamp, sample, coupling=False, shift=False):
+    if kind == "chain":
+        H, mu = synth.chain_absorption(N=N, nframes=(nsamp - 1) * sample + T + 5, seed=5)
+    else:
+        L = int(round(N ** 0.5))
+        rng, mus, coup = synth.lattice_static(L, seed=9, cut=0.1)
+        nfr = (nsamp - 1) * sample + T + 5
+        H = synth.full_records(synth.lattice_diag(rng, nfr, N), coup)
+        mu = synth.dipole_records(mus, nfr)
+    lo, hi = synth.shifted_center(H, N)
+    kw = dict(singles=N, tmax1=T, sample=sample, begin=0, end=nsamp, ts=1, propagation=1,
+              deltat=2.0, min1=lo, max1=hi)
+    if shift:
+        kw["single_shift"] = [(0, 1.0, 3.0), (N - 1, 0.9, 170.0)]
+    Crec = None
+    if coupling:
+        kw["hamiltonian"] = "Coupling"
+        Crec = H[0].copy()
+        a, b = np.triu_indices(N)
+        H = np.concatenate([H[:, :1], H[:, 1:][:, a == b]], axis=1)
+        mu = mu[:1].copy()
+    non = make_non(**kw)
+    non.shifte = np.float32((np.float32(hi) + np.float32(lo)) / np.float32(2))
+    return non, H, mu, Crec
+
+
+@pytest.mark.parametrize("kind,N,T,nsamp,sample,coupling,shift,kernel", [
+    ("chain", 8, 32, 7, 10, False, False, 0),
+    ("chain", 8, 32, 7, 10, False, False, 1),
+    ("chain", 16, 64, 12, 20, False, True, 0),
+    ("lattice", 16, 40, 9, 10, False, False, 0),
+    ("lattice", 36, 24, 5, 7, True, True, 0),
+    ("lattice", 36, 24, 5, 7, True, False, 1),
+])
+def test_batched_absorption(orc, nise_lib, kind, N, T, nsamp, sample, coupling, shift, kernel):
+    non, H, mu, Crec = absorption_case(kind, N, T, nsamp, sample, coupling, shift)
+    re = np.zeros(T, np.float32)
+    im = np.zeros(T, np.float32)
+    rc = nise_lib.nise_absorption_batched(C.byref(non), abi.ptr(H), H.shape[1], abi.ptr(mu),
+                                          mu.shape[1], H.shape[0], abi.ptr(Crec), abi.NISE_HOST,
+                                          kernel, re, im)
+    abi.check(rc)
+    o_re, o_im = orc.absorption(non, H, mu, Crec)
+    d_re, d_im = orc.absorption(non, H, mu, Crec, f64=True)
+    e_gpu = rel_l2(re + 1j * im, o_re + 1j * o_im)
+    e_gpu64 = rel_l2(re + 1j * im, d_re + 1j * d_im)
+    e_ref64 = rel_l2(o_re + 1j * o_im, d_re + 1j * d_im)
+    print(f"{kind} N={N} kernel={kernel}: vs FP32 oracle {e_gpu:.2e}, vs FP64 {e_gpu64:.2e} "
+          f"(FP32 oracle vs FP64 {e_ref64:.2e})")
+    assert e_gpu < 1e-5
+    assert e_gpu64 < 1e-4
+
+
+def test_absorption_shapes_agree(orc, nise_lib):
+    """Every conveyor shape gives the same response (bit-identical vectors,
+    same summation per workgroup shape up to FP64 rounding)."""
+    non, H, mu, _ = absorption_case("lattice", 16, 30, 20, 3)
+    base = None
+    for vp, spw in [(4, 1), (8, 2), (16, 5), (32, 10), (64, 21), (64, 3)]:
+        p = abi.AbsPlan(non, H, H.shape[1], mu, mu.shape[1], H.shape[0])
+        p.set_shape(vp, spw)
+        re, im = p.execute()
+        p.close()
+        if base is None:
+            base = re + 1j * im
+        assert rel_l2(re + 1j * im, base) < 1e-6
+
+
+def test_graph_replay_matches_eager(nise_lib):
+    non, H, mu, _ = absorption_case("lattice", 16, 30, 20, 3)
+    p = abi.AbsPlan(non, H, H.shape[1], mu, mu.shape[1], H.shape[0])
+    re0, im0 = p.execute(flags=1)
+    for _ in range(3):
+        re1, im1 = p.execute(flags=2)
+        assert np.array_equal(re0, re1) and np.array_equal(im0, im1)
+        re2, im2 = p.execute(flags=0)
+        assert np.array_equal(re0, re2)
+    p.close()
+
+
+def test_errors_are_codes(nise_lib):
+    non, H, mu, _ = absorption_case("chain", 8, 32, 7, 10)
+    non.end = 10_000
+    re = np.zeros(32, np.float32)
+    im = np.zeros(32, np.float32)
+    rc = nise_lib.nise_absorption_batched(C.byref(non), abi.ptr(H), H.shape[1], abi.ptr(mu),
+                                          mu.shape[1], H.shape[0], None, abi.NISE_HOST, 0, re, im)
+    assert rc == abi.NISE_ERR_RANGE
+    assert b"frames" in nise_lib.nise_last_error()
