@@ -86,7 +86,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--shape", default="", help="vp,spw conveyor shape override")
+    ap.add_argument("--shape", default="", help="vp,spw,block conveyor shape override")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -119,8 +119,8 @@ def main():
     log("building the plan (coupling structure, rotation schedule)")
     plan = abi.AbsPlan(non, H, H.shape[1], mu, mu.shape[1], nfr, where=abi.NISE_DEVICE, kernel=0)
     if args.shape:
-        vp, spw = (int(x) for x in args.shape.split(","))
-        plan.set_shape(vp, spw)
+        vp, spw, blk = (int(x) for x in args.shape.split(","))
+        plan.set_shape(vp, spw, blk)
     info = plan.info()
     log(f"plan: {info}")
 
@@ -226,7 +226,7 @@ def main():
         "config": {"workload": f"absorption N={N} {L}x{L} dipole lattice, Full FP32 frames, "
                                f"{nfr} frames, tmax {T}, {ns} samples every {sr} frames, 3 pols, "
                                f"Trotter 1", "N": N, "frames": nfr, "tmax": T,
-                   "samples": ns, "samplerate": sr, "shape": {"vp": info["vp"], "spw": info["spw"]},
+                   "samples": ns, "samplerate": sr, "shape": {"vp": info["vp"], "spw": info["spw"], "block": info["block"]},
                    "K": info["K"], "layers": info["nlayers"]},
         "roofline": roofline, "cpu_baseline": cpu,
     }

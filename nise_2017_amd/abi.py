@@ -108,7 +108,7 @@ def lib():
     L.nise_abs_plan_create.argtypes = [C.POINTER(C.c_void_p), P, _vp, C.c_int64, _vp, C.c_int64,
                                        C.c_int64, _vp, C.c_int, C.c_int]
     L.nise_abs_plan_execute.argtypes = [_vp, _vp, _vp, C.c_int]
-    L.nise_abs_plan_set_shape.argtypes = [_vp, C.c_int, C.c_int]
+    L.nise_abs_plan_set_shape.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.nise_abs_plan_info.argtypes = [_vp, C.POINTER(C.c_int64), C.c_int]
     L.nise_abs_plan_last_timing.argtypes = [_vp, C.POINTER(C.c_float), C.c_int]
     L.nise_abs_plan_sync.argtypes = [_vp]
@@ -139,7 +139,11 @@ def ptr(a):
     if a is None:
         return None
     if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor passed to libnise_hip must be contiguous")
         return C.c_void_p(a.data_ptr())
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array passed to libnise_hip must be C-contiguous (row-major records)")
     return a.ctypes.data_as(C.c_void_p)
 
 
@@ -156,8 +160,8 @@ class AbsPlan:
         self.h = h
         self.tmax = non.tmax1
 
-    def set_shape(self, vp, spw):
-        check(lib().nise_abs_plan_set_shape(self.h, vp, spw))
+    def set_shape(self, vp, spw, block=256):
+        check(lib().nise_abs_plan_set_shape(self.h, vp, spw, block))
 
     def execute(self, flags=0, fetch=True):
         re = np.zeros(self.tmax, np.float32)
@@ -172,9 +176,9 @@ class AbsPlan:
         check(lib().nise_abs_plan_sync(self.h))
 
     def info(self):
-        a = (C.c_int64 * 8)()
-        lib().nise_abs_plan_info(self.h, a, 8)
-        keys = ("N", "K", "nlayers", "nwg", "vp", "spw", "f_lo", "nframes")
+        a = (C.c_int64 * 10)()
+        lib().nise_abs_plan_info(self.h, a, 10)
+        keys = ("N", "K", "nlayers", "nwg", "vp", "spw", "f_lo", "nframes", "block", "nsteps")
         return dict(zip(keys, list(a)))
 
     def timing(self):
@@ -183,9 +187,9 @@ class AbsPlan:
         return {"tables_ms": a[0], "conveyor_ms": a[1], "reduce_ms": a[2]}
 
     def close(self):
-        if getattr(self, "h", None):
-            lib().nise_abs_plan_destroy(self.h)
-            self.h = None
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.nise_abs_plan_destroy(self.h)
+        self.h = None
 
     def __del__(self):
         self.close()

@@ -33,9 +33,10 @@ struct nise_abs_plan {
   int64_t mu_stride = 0;
   DevBuf H_own, mu_own, C_own, shift_site, shift_ab;
   Schedule sch;
-  DevBuf ab, layer_off, diag, cs, slots, wg_begin, partial, out;
+  DevBuf ab, layer_off, diag, cs, slots, wg_begin, partial, out, steps;
+  int nsteps = 0;
   DevBuf ws_cs, ws_ab, resp_plain;
-  int vp = 16, spw = 5, nwg = 0;
+  int vp = 16, spw = 5, nwg = 0, block = 256;
   hipStream_t st = nullptr;
   hipEvent_t ev[4] = {};
   float last_ms[3] = {0, 0, 0};
@@ -53,10 +54,14 @@ struct nise_abs_plan {
 
 namespace {
 
+// Allocates at least 16 bytes so that empty inputs (no couplings) still get
+// a valid device pointer; copies only when there is data.
 int upload(DevBuf &dst, const void *src, size_t bytes, hipStream_t st) {
-  int rc = dst.alloc(bytes);
+  int rc = dst.alloc(bytes < 16 ? 16 : bytes);
   if (rc) return rc;
-  NISE_HIP_CHECK(hipMemcpyAsync(dst.p, src, bytes, hipMemcpyHostToDevice, st));
+  (void)st;
+  // synchronous: the source may be pageable memory owned by the caller
+  if (bytes && src) NISE_HIP_CHECK(hipMemcpy(dst.p, src, bytes, hipMemcpyHostToDevice));
   return NISE_OK;
 }
 
@@ -71,11 +76,15 @@ int build_slots(nise_abs_plan *P) {
   for (int s = 0; s < ns; ++s) {
     if (s % P->spw == 0) wb.push_back((int)sl.size());
     int fs = (P->begin + s) * P->sample;
-    for (int x = 0; x < 3; ++x) sl.push_back(SlotDesc{fs, P->tmax, P->tmax - 1, x, 0, -1});
+    for (int x = 0; x < 3; ++x) sl.push_back(SlotDesc{fs, P->tmax, P->tmax - 1, x, 0, -1, x == 0 ? 3 : 0});
   }
   wb.push_back((int)sl.size());
   P->nwg = (int)wb.size() - 1;
   int rc;
+  std::vector<int2> stp;
+  build_steps(P->sch, P->vp, P->block, stp);
+  P->nsteps = (int)stp.size();
+  if ((rc = upload(P->steps, stp.data(), stp.size() * sizeof(int2), P->st))) return rc;
   if ((rc = upload(P->slots, sl.data(), sl.size() * sizeof(SlotDesc), P->st))) return rc;
   if ((rc = upload(P->wg_begin, wb.data(), wb.size() * sizeof(int), P->st))) return rc;
   if ((rc = P->partial.alloc((size_t)std::max(P->nwg, 1) * P->tmax * sizeof(double2)))) return rc;
@@ -97,7 +106,8 @@ int run_conveyor(nise_abs_plan *P) {
   a.mu_rec = P->mu_dev, a.mu_stride = P->mu_stride;
   a.slots = P->slots.as<SlotDesc>(), a.wg_slot_begin = P->wg_begin.as<int>(), a.nwg = P->nwg;
   a.tmax_acc = P->tmax, a.partial = P->partial.as<double2>();
-  return launch_conveyor(a, P->vp, P->st);
+  a.steps = P->steps.as<int2>(), a.nsteps = P->nsteps;
+  return launch_conveyor(a, P->vp, P->block, P->st);
 }
 
 int run_reduce(nise_abs_plan *P) {
@@ -221,7 +231,7 @@ extern "C" int nise_abs_plan_create(nise_abs_plan **out, const t_non *non, const
       NISE_HIP_CHECK(hipStreamSynchronize(P->st));
     }
     build_schedule(N, mask, P->sch);
-    if ((rc = upload(P->ab, P->sch.ab.data(), std::max<size_t>(1, P->sch.ab.size()) * 4, P->st))) return rc;
+    if ((rc = upload(P->ab, P->sch.ab.data(), P->sch.ab.size() * 4, P->st))) return rc;
     if ((rc = upload(P->layer_off, P->sch.layer_off.data(), P->sch.layer_off.size() * 4, P->st))) return rc;
     int64_t nfr = P->f_hi - P->f_lo;
     if ((rc = P->diag.alloc((size_t)nfr * N * sizeof(float2)))) return rc;
@@ -236,12 +246,13 @@ extern "C" int nise_abs_plan_create(nise_abs_plan **out, const t_non *non, const
   return NISE_OK;
 }
 
-extern "C" int nise_abs_plan_set_shape(nise_abs_plan *P, int vp, int samples_per_wg) {
+extern "C" int nise_abs_plan_set_shape(nise_abs_plan *P, int vp, int samples_per_wg, int block) {
   if (!P || P->kernel != 0) return set_error("no conveyor plan"), NISE_ERR_ARG;
   if (!(vp == 4 || vp == 8 || vp == 16 || vp == 32 || vp == 64) || samples_per_wg < 1 ||
-      3 * samples_per_wg > vp)
-    return set_error("bad shape vp=%d spw=%d", vp, samples_per_wg), NISE_ERR_ARG;
-  P->vp = vp, P->spw = samples_per_wg;
+      3 * samples_per_wg > vp || !(block == 64 || block == 256 || block == 512 || block == 1024) ||
+      vp > block)
+    return set_error("bad shape vp=%d spw=%d block=%d", vp, samples_per_wg, block), NISE_ERR_ARG;
+  P->vp = vp, P->spw = samples_per_wg, P->block = block;
   if (P->graph_ready) {
     (void)hipGraphExecDestroy(P->exec), (void)hipGraphDestroy(P->graph);
     P->exec = nullptr, P->graph = nullptr, P->graph_ready = false;
@@ -303,7 +314,8 @@ extern "C" int nise_abs_plan_sync(nise_abs_plan *P) {
 
 // info: N, K, nlayers, nwg, vp, samples per workgroup, first frame, frames
 extern "C" int nise_abs_plan_info(const nise_abs_plan *P, int64_t *info, int n) {
-  int64_t v[] = {P->N, P->sch.K, P->sch.nlayers(), P->nwg, P->vp, P->spw, P->f_lo, P->f_hi - P->f_lo};
+  int64_t v[] = {P->N, P->sch.K, P->sch.nlayers(), P->nwg, P->vp, P->spw, P->f_lo, P->f_hi - P->f_lo,
+                  P->block, P->nsteps};
   int k = (int)(sizeof(v) / sizeof(v[0]));
   for (int i = 0; i < k && i < n; ++i) info[i] = v[i];
   return k;
@@ -338,7 +350,7 @@ extern "C" int nise_absorption_batched(const t_non *non, const float *H_rec, int
 namespace {
 
 struct SinglesCtx {
-  DevBuf H, ab, lay, diag, cs, state, slots, wgb;
+  DevBuf H, ab, lay, diag, cs, state, slots, wgb, steps;
   std::vector<float> hbuf;
   std::vector<float2> sbuf;
 };
@@ -350,6 +362,8 @@ SinglesCtx &ctx() {
 
 [[noreturn]] void fatal(const char *where) {
   printf("%s: GPU propagation failed: %s\n", where, get_error());
+  fflush(stdout);
+  fprintf(stderr, "%s: GPU propagation failed: %s\n", where, get_error());
   exit(1);
 }
 
@@ -388,18 +402,21 @@ int coupling_S_host(int N, const float *H, float *cr, float *ci, int nvec, int m
   std::vector<int> wb;
   for (int v = 0; v < nvec; ++v) {
     if (v % vp == 0) wb.push_back(v);
-    sl[v] = SlotDesc{0, 0, 1, 0, 1, v};
+    sl[v] = SlotDesc{0, 0, 1, 0, 1, v, 0};
   }
   wb.push_back(nvec);
   c.sbuf.resize((size_t)nvec * N);
   for (int v = 0; v < nvec; ++v)
     for (int i = 0; i < N; ++i) c.sbuf[(size_t)v * N + i] = make_float2(cr[(size_t)v * N + i], ci[(size_t)v * N + i]);
   if ((rc = upload(c.H, c.hbuf.data(), (nn2 + 1) * 4, st))) return rc;
-  if ((rc = upload(c.ab, sch.ab.data(), std::max<size_t>(1, sch.ab.size()) * 4, st))) return rc;
+  if ((rc = upload(c.ab, sch.ab.data(), sch.ab.size() * 4, st))) return rc;
   if ((rc = upload(c.lay, sch.layer_off.data(), sch.layer_off.size() * 4, st))) return rc;
   if ((rc = upload(c.state, c.sbuf.data(), c.sbuf.size() * sizeof(float2), st))) return rc;
   if ((rc = upload(c.slots, sl.data(), sl.size() * sizeof(SlotDesc), st))) return rc;
   if ((rc = upload(c.wgb, wb.data(), wb.size() * sizeof(int), st))) return rc;
+  std::vector<int2> stp;
+  build_steps(sch, vp, 256, stp);
+  if ((rc = upload(c.steps, stp.data(), stp.size() * sizeof(int2), st))) return rc;
   if ((rc = c.diag.alloc(N * sizeof(float2)))) return rc;
   if ((rc = c.cs.alloc(std::max(sch.K, 1) * sizeof(float2)))) return rc;
   FrameSource src;
@@ -416,7 +433,8 @@ int coupling_S_host(int N, const float *H, float *cr, float *ci, int nvec, int m
   a.mu_rec = c.H.as<float>(), a.mu_stride = 0;
   a.slots = c.slots.as<SlotDesc>(), a.wg_slot_begin = c.wgb.as<int>(), a.nwg = (int)wb.size() - 1;
   a.state = c.state.as<float2>(), a.writeback = 1;
-  if ((rc = launch_conveyor(a, vp, st))) return rc;
+  a.steps = c.steps.as<int2>(), a.nsteps = (int)stp.size();
+  if ((rc = launch_conveyor(a, vp, 256, st))) return rc;
   NISE_HIP_CHECK(hipMemcpyAsync(c.sbuf.data(), c.state.p, c.sbuf.size() * sizeof(float2),
                                 hipMemcpyDeviceToHost, st));
   NISE_HIP_CHECK(hipStreamSynchronize(st));
@@ -476,4 +494,63 @@ extern "C" void propagate_matrix(t_non *non, float *Hamil_i_e, float *vecr, floa
   printf("propagate_matrix: propagation scheme %d is not provided by the GPU library\n",
          non->propagation);
   exit(1);
+}
+
+// Frame tables of one frame for inspection: D phases (N float2) and the
+// (cos, sin) of the kept couplings in lexicographic order (count returned
+// in *K).  Same code path as the batched plan (FrameSource + table kernel).
+extern "C" int nise_frame_tables(const t_non *non, const float *H_rec, int64_t H_stride,
+                                 const float *coupling_rec, int64_t pos, int sign, float *diag_out,
+                                 float *cs_out, int *K_out) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  const int N = non->singles;
+  const bool ctype = is_coupling_type(non);
+  hipStream_t st = default_stream();
+  DevBuf H, Cb, ab, dg, cs, ss, sab;
+  const int64_t nn2 = (int64_t)N * (N + 1) / 2;
+  if ((rc = upload(H, H_rec + pos * H_stride, H_stride * 4, st))) return rc;
+  FrameSource src;
+  src.H_rec = H.as<float>() - pos * H_stride, src.H_stride = H_stride;
+  src.coupling_type = ctype;
+  if (ctype) {
+    if ((rc = upload(Cb, coupling_rec, (1 + nn2) * 4, st))) return rc;
+    src.C_rec = Cb.as<float>();
+  }
+  src.shifte = non->shifte, src.cut = non->couplingcut, src.apply_shift = 1;
+  src.nshift = non->SingleShiftSites;
+  if (src.nshift > 0) {
+    if ((rc = upload(ss, non->SingleShiftSite, src.nshift * sizeof(int), st))) return rc;
+    if ((rc = upload(sab, non->SingleShift, 2 * src.nshift * sizeof(float), st))) return rc;
+    src.shift_site = ss.as<int>(), src.shift_ab = sab.as<float>();
+  }
+  std::vector<uint8_t> mask((size_t)N * (N - 1) / 2, 1);
+  Schedule sch;
+  build_schedule(N, mask, sch);
+  // lexicographic order instead of layer order for inspection
+  std::vector<uint32_t> lex;
+  for (int a = 0; a < N; ++a)
+    for (int b = a + 1; b < N; ++b) lex.push_back((uint32_t)a | ((uint32_t)b << 16));
+  sch.ab = lex;
+  if ((rc = upload(ab, lex.data(), lex.size() * 4, st))) return rc;
+  if ((rc = dg.alloc(N * sizeof(float2))) || (rc = cs.alloc((lex.size() + 1) * sizeof(float2))))
+    return rc;
+  float fac = trotter_factor(non->deltat, sign, non->ts);
+  if (getenv("NISE_DEBUG")) {
+    std::vector<float> back(H_stride);
+    NISE_HIP_CHECK(hipMemcpy(back.data(), H.p, H_stride * 4, hipMemcpyDeviceToHost));
+    fprintf(stderr, "nise_frame_tables: ctype=%d stride=%lld pos=%lld H_rec=%p Hdev=%p src.H_rec=%p\n",
+            (int)ctype, (long long)H_stride, (long long)pos, (const void *)H_rec, H.p,
+            (const void *)src.H_rec);
+    for (int64_t i = 0; i < H_stride && i < 12; ++i)
+      fprintf(stderr, "  host %g dev %g\n", H_rec[pos * H_stride + i], back[i]);
+  }
+  if ((rc = launch_frame_tables(src, N, sch, ab.as<uint32_t>(), fac, pos, 1, dg.as<float2>(),
+                                cs.as<float2>(), st)))
+    return rc;
+  NISE_HIP_CHECK(hipMemcpyAsync(diag_out, dg.p, N * sizeof(float2), hipMemcpyDeviceToHost, st));
+  NISE_HIP_CHECK(hipMemcpyAsync(cs_out, cs.p, lex.size() * sizeof(float2), hipMemcpyDeviceToHost, st));
+  NISE_HIP_CHECK(hipStreamSynchronize(st));
+  *K_out = (int)lex.size();
+  return NISE_OK;
 }

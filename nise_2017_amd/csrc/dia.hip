@@ -193,9 +193,9 @@ DiaCtx &dctx() {
   return c;
 }
 int up(DevBuf &d, const void *s, size_t n, hipStream_t st) {
-  int rc = d.alloc(n);
+  int rc = d.alloc(n < 16 ? 16 : n);
   if (rc) return rc;
-  NISE_HIP_CHECK(hipMemcpyAsync(d.p, s, n, hipMemcpyHostToDevice, st));
+  if (n && s) NISE_HIP_CHECK(hipMemcpyAsync(d.p, s, n, hipMemcpyHostToDevice, st));
   return NISE_OK;
 }
 }  // namespace

@@ -109,6 +109,8 @@ struct SlotDesc {
   int x;      // dipole component for init/response
   int init;   // 0: psi = mu_x(fs), 1: psi from state buffer
   int out;    // state buffer row (init 1 or write back), -1 none
+  int grp;    // response group leader: number of consecutive slots (same fs)
+              // whose responses it sums; 0 for members
 };
 
 struct ConveyorArgs {
@@ -127,9 +129,13 @@ struct ConveyorArgs {
   double2 *partial = nullptr;    // [nwg][tmax_acc]
   float2 *state = nullptr;       // [rows][N] for init/write back
   int writeback = 0;
+  const int2 *steps = nullptr;   // rotation steps (build_steps)
+  int nsteps = 0;
 };
 
-int launch_conveyor(const ConveyorArgs &a, int vp, hipStream_t st);
+int launch_conveyor(const ConveyorArgs &a, int vp, int block, hipStream_t st);
+void build_steps(const Schedule &s, int vp, int block, std::vector<int2> &steps);
+size_t conveyor_smem(int N, int vp, int block, int tmax_acc, int nsteps);
 int launch_reduce_partials(const double2 *partial, int nwg, int tmax,
                            float *re, float *im, hipStream_t st);
 

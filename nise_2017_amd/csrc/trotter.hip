@@ -156,41 +156,111 @@ __device__ __forceinline__ void rotate_pair(float2 &pa, float2 &pb, float co, fl
 
 // ---------------------------------------------------------------------------
 // Conveyor kernel.
+//
+// Rotation work of one sweep is a list of steps (build_steps).  Step s
+// covers schedule rotations [x, y & 0x7fffffff) (at most BLOCK/VP of them,
+// all in one layer); a set sign bit of y closes the layer with a workgroup
+// barrier.  Thread (q, v) applies rotation x + q of each step to slot v.
+// The list is padded with empty steps to a multiple of kRing so that a
+// register ring of the next kRing steps' operands ((cos, sin) and packed
+// (a, b)) can run continuously across sweeps and frames; the barrier waits
+// only for LDS traffic, so the ring's global loads stay in flight.
+//
+// Per frame, the sites' half-step phases D(f) and dipoles mu(f) are read
+// from LDS; the next frame's are fetched into registers while the current
+// frame's rotations run.
 // ---------------------------------------------------------------------------
-constexpr int kConvBlock = 256;
+constexpr int kRing = 8;
 
-template <int VP>
-__global__ void __launch_bounds__(kConvBlock) conveyor_kernel(ConveyorArgs A) {
+__device__ __forceinline__ void lds_barrier() {
+  // LDS traffic of this wave complete, then the workgroup barrier; global
+  // loads already issued remain outstanding.
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+struct ConvSmem {
+  static __host__ __device__ size_t bytes(int N, int vp, int block, int tmax_acc, int nsteps) {
+    return (size_t)N * vp * 8        // psi
+           + (size_t)2 * N * 8       // D, two frames
+           + (size_t)3 * N * 4       // mu
+           + (size_t)tmax_acc * 16   // acc
+           + (size_t)(block / 64) * vp * 8  // red
+           + (size_t)nsteps * 8      // steps
+           + (size_t)vp * sizeof(SlotDesc) + 64;
+  }
+};
+
+template <int VP, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) conveyor_kernel(ConveyorArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = A.N;
+  constexpr int kWaves = BLOCK / 64;
+  constexpr int kRows = BLOCK / VP;      // rotations per step, sites per pass
+  constexpr int kPF = 4;                 // next-frame D prefetch per thread
+  // byte offsets from smem keep every pointer in the LDS address space
+  const size_t o_D = (size_t)N * VP * 8;
+  const size_t o_mu = o_D + (size_t)2 * N * 8;
+  const size_t o_acc = (o_mu + (size_t)3 * N * 4 + 15) & ~(size_t)15;
+  const size_t o_red = o_acc + (size_t)A.tmax_acc * 16;
+  const size_t o_steps = o_red + (size_t)kWaves * VP * 8;
+  const size_t o_sls = o_steps + (size_t)A.nsteps * 8;
   float2 *psi = reinterpret_cast<float2 *>(smem);                 // [N][VP]
-  double2 *acc = reinterpret_cast<double2 *>(psi + (size_t)N * VP);  // [tmax_acc]
-  float2 *red = reinterpret_cast<float2 *>(acc + A.tmax_acc);     // [waves][VP]
-  constexpr int kWaves = kConvBlock / 64;
-  constexpr int kRowsPerPass = kConvBlock / VP;
+  float2 *Dsh = reinterpret_cast<float2 *>(smem + o_D);           // [2][N]
+  float *mush = reinterpret_cast<float *>(smem + o_mu);           // [3][N]
+  double2 *acc = reinterpret_cast<double2 *>(smem + o_acc);       // [tmax_acc]
+  float2 *red = reinterpret_cast<float2 *>(smem + o_red);         // [waves][VP]
+  int2 *steps = reinterpret_cast<int2 *>(smem + o_steps);         // [nsteps]
+  SlotDesc *sls = reinterpret_cast<SlotDesc *>(smem + o_sls);     // [VP]
 
   const int g = blockIdx.x;
   const int s0 = A.wg_slot_begin[g];
   const int ns = A.wg_slot_begin[g + 1] - s0;
   const int t = threadIdx.x;
-  const int v = t % VP;       // slot served by this thread
-  const int q = t / VP;       // row group
+  const int v = t % VP;
+  const int q = t / VP;
   const int lane = t & 63, wave = t >> 6;
+  const int nsteps = A.nsteps;
+  const int K = A.K;
 
-  SlotDesc sd;
-  bool has = v < ns;
-  if (has) sd = A.slots[s0 + v];
-  else sd = SlotDesc{0, 0, 0, 0, 0, -1};
+  for (int i = t; i < nsteps; i += BLOCK) steps[i] = A.steps[i];
+  for (int i = t; i < ns; i += BLOCK) sls[i] = A.slots[s0 + i];
+  for (int i = t; i < A.tmax_acc; i += BLOCK) acc[i] = make_double2(0.0, 0.0);
+  __syncthreads();
 
-  // frame range of this workgroup
+  const bool has = v < ns;
+  SlotDesc sd = has ? sls[v] : SlotDesc{0, 0, 0, 0, 0, -1, 0};
   int64_t F0 = INT64_MAX, F1 = INT64_MIN;
   for (int i = 0; i < ns; ++i) {
-    SlotDesc d = A.slots[s0 + i];
-    int len = d.nresp > d.nprop ? d.nresp : d.nprop;
+    const SlotDesc d = sls[i];
+    const int len = d.nresp > d.nprop ? d.nresp : d.nprop;
     F0 = d.fs < F0 ? d.fs : F0;
     F1 = (int64_t)d.fs + len > F1 ? (int64_t)d.fs + len : F1;
   }
-  for (int i = t; i < A.tmax_acc; i += kConvBlock) acc[i] = make_double2(0.0, 0.0);
+  if (ns == 0) return;
+
+  // frame F0's D and mu into LDS
+  {
+    const int64_t row = F0 - A.f_lo;
+    const float *mu = A.mu_rec + (A.mu_stride ? F0 * A.mu_stride : 0) + 1;
+    for (int i = t; i < N; i += BLOCK) Dsh[(F0 & 1) * N + i] = A.diag[row * N + i];
+    for (int i = t; i < 3 * N; i += BLOCK) mush[i] = mu[i];
+  }
+  // operand ring for the first sweep
+  int2 rst[kRing];
+  float2 rc[kRing];
+  uint32_t rab[kRing];
+  {
+    const float2 *cs0 = A.cs + (F0 - A.f_lo) * K;
+#pragma unroll
+    for (int j = 0; j < kRing; ++j) {
+      const int jj = j < nsteps ? j : 0;
+      const int2 st = nsteps ? A.steps[jj] : make_int2(0, 0);
+      const int r = max(0, min(st.x + q, (st.y & 0x7fffffff) - 1));
+      rst[j] = st;
+      rc[j] = nsteps ? cs0[r] : make_float2(1.f, 0.f);
+      rab[j] = nsteps ? A.ab[r] : 0u;
+    }
+  }
   __syncthreads();
 
   for (int64_t f = F0; f < F1; ++f) {
@@ -200,16 +270,16 @@ __global__ void __launch_bounds__(kConvBlock) conveyor_kernel(ConveyorArgs A) {
     const bool prev_prop = has && rel >= 1 && rel - 1 < sd.nprop;
     const bool resp = has && rel >= 0 && rel < sd.nresp;
     const bool prop = has && rel >= 0 && rel < sd.nprop;
-    const float *mu = A.mu_rec + (A.mu_stride ? f * A.mu_stride : 0) + 1 + (int64_t)sd.x * N;
-    const float2 *Dcur = A.diag + row * N;
-    const float2 *Dprev = A.diag + (row - 1) * N;
+    const float2 *Dcur = Dsh + (f & 1) * N;
+    const float2 *Dprev = Dsh + ((f + 1) & 1) * N;
+    const float *mu = mush + sd.x * N;
 
     // Per-site pass: pending second half-phase of frame f-1, response
     // mu(f).psi (src/absorption.c:130, calc_S1 :205-212), first half-phase
     // of frame f.
     float rr = 0.f, ri = 0.f;
     if (starts || prev_prop || resp || prop) {
-      for (int i = q; i < N; i += kRowsPerPass) {
+      for (int i = q; i < N; i += kRows) {
         float2 p;
         if (starts) {
           p = (sd.init == 0) ? make_float2(mu[i], 0.f) : A.state[(int64_t)sd.out * N + i];
@@ -218,7 +288,7 @@ __global__ void __launch_bounds__(kConvBlock) conveyor_kernel(ConveyorArgs A) {
           if (prev_prop) p = cmul_diag(p, Dprev[i]);
         }
         if (resp) {
-          float m = mu[i];
+          const float m = mu[i];
           rr = __fadd_rn(rr, __fmul_rn(m, p.x));
           ri = __fadd_rn(ri, __fmul_rn(m, p.y));
         }
@@ -227,7 +297,6 @@ __global__ void __launch_bounds__(kConvBlock) conveyor_kernel(ConveyorArgs A) {
       }
     }
     if (A.tmax_acc > 0) {
-      // reduce the per-thread partial dot products of each slot
 #pragma unroll
       for (int off = VP; off < 64; off <<= 1) {
         rr += __shfl_xor(rr, off);
@@ -235,88 +304,147 @@ __global__ void __launch_bounds__(kConvBlock) conveyor_kernel(ConveyorArgs A) {
       }
       if (lane < VP) red[wave * VP + lane] = make_float2(rr, ri);
       __syncthreads();
-      if (t == 0) {
-        for (int vv = 0; vv < ns; ++vv) {
-          SlotDesc d = A.slots[s0 + vv];
-          int64_t r2 = f - d.fs;
-          if (r2 < 0 || r2 >= d.nresp) continue;
-          float sr = 0.f, si = 0.f;
-          for (int w = 0; w < kWaves; ++w) sr += red[w * VP + vv].x, si += red[w * VP + vv].y;
-          acc[r2].x += (double)sr;
-          acc[r2].y += (double)si;
-        }
+      // A group leader (the x=0 slot of a sample) adds its group's sums in a
+      // fixed order; groups of one workgroup start on distinct frames, so
+      // their response indices differ and the adds do not collide.
+      if (q == 0 && resp && sd.grp > 0) {
+        float sr = 0.f, si = 0.f;
+        for (int gg = 0; gg < sd.grp; ++gg)
+          for (int w = 0; w < kWaves; ++w) sr += red[w * VP + v + gg].x, si += red[w * VP + v + gg].y;
+        acc[rel].x += (double)sr;
+        acc[rel].y += (double)si;
       }
     }
     __syncthreads();
 
-    // Rotation sweeps (m Trotter substeps).
+    // next frame's D and mu into registers (consumed after the sweeps)
+    const int64_t fn = f + 1 < F1 ? f + 1 : f;
+    float2 dn[kPF];
+    float mn[3 * kPF];
+#pragma unroll
+    for (int k = 0; k < kPF; ++k) {
+      const int i = min(t + k * BLOCK, N - 1);
+      dn[k] = A.diag[(fn - A.f_lo) * N + i];
+      const float *mup = A.mu_rec + (A.mu_stride ? fn * A.mu_stride : 0) + 1;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) mn[3 * k + x] = mup[x * N + i];
+    }
+
     for (int sub = 0; sub < A.m; ++sub) {
       if (sub > 0) {
-        // between substeps: second half-phase then first half-phase
         if (prop)
-          for (int i = q; i < N; i += kRowsPerPass)
+          for (int i = q; i < N; i += kRows)
             psi[i * VP + v] = cmul_diag(cmul_diag(psi[i * VP + v], Dcur[i]), Dcur[i]);
         __syncthreads();
       }
-      const float2 *cs = A.cs + row * A.K;
-      for (int l = 0; l < A.nlayers; ++l) {
-        const int r0 = A.layer_off[l], r1 = A.layer_off[l + 1];
-        if (prop) {
-          for (int r = r0 + q; r < r1; r += kRowsPerPass) {
-            uint32_t p = A.ab[r];
-            int a = (int)(p & 0xffffu), b = (int)(p >> 16);
-            float2 c = cs[r];
+      // operands of the following sweep (next substep, else next frame)
+      const float2 *csn = A.cs + ((sub + 1 < A.m) ? row : (fn - A.f_lo)) * K;
+      for (int b0 = 0; b0 < nsteps; b0 += kRing) {
+#pragma unroll
+        for (int j = 0; j < kRing; ++j) {
+          const int sidx = b0 + j;
+          // the step entry is the same for every lane
+          const int2 st = make_int2(__builtin_amdgcn_readfirstlane(rst[j].x),
+                                    __builtin_amdgcn_readfirstlane(rst[j].y));
+          if (prop && st.x + q < (st.y & 0x7fffffff)) {
+            const float2 c = rc[j];
+            const uint32_t pk = rab[j];
+            const int a = (int)(pk & 0xffffu), b = (int)(pk >> 16);
             float2 pa = psi[a * VP + v], pb = psi[b * VP + v];
             rotate_pair(pa, pb, c.x, c.y);
             psi[a * VP + v] = pa;
             psi[b * VP + v] = pb;
           }
+          {
+            // refill this slot with step sidx + kRing of this sweep, or of
+            // the next sweep once this one runs out (nsteps % kRing == 0)
+            const int sp = sidx + kRing;
+            const bool nxt = sp >= nsteps;
+            const int2 stp = steps[nxt ? sp - nsteps : sp];
+            const int rp = max(0, min(stp.x + q, (stp.y & 0x7fffffff) - 1));
+            const float2 *src = nxt ? csn : A.cs + row * K;
+            rst[j] = stp;
+            rc[j] = src[rp];
+            rab[j] = A.ab[rp];
+          }
+          if (st.y < 0) lds_barrier();
         }
-        __syncthreads();
+      }
+      __syncthreads();
+    }
+    // publish frame f+1's D and mu
+#pragma unroll
+    for (int k = 0; k < kPF; ++k) {
+      const int i = t + k * BLOCK;
+      if (i < N) {
+        Dsh[(fn & 1) * N + i] = dn[k];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) mush[x * N + i] = mn[3 * k + x];
       }
     }
+    for (int i = t + kPF * BLOCK; i < N; i += BLOCK) {  // N > kPF*BLOCK
+      Dsh[(fn & 1) * N + i] = A.diag[(fn - A.f_lo) * N + i];
+      const float *mup = A.mu_rec + (A.mu_stride ? fn * A.mu_stride : 0) + 1;
+      for (int x = 0; x < 3; ++x) mush[x * N + i] = mup[x * N + i];
+    }
+    __syncthreads();
   }
 
   // write back: pending second half-phase of the last propagated frame
   if (A.writeback && has && sd.out >= 0 && sd.nprop > 0) {
     const int64_t row = (int64_t)sd.fs + sd.nprop - 1 - A.f_lo;
-    for (int i = q; i < N; i += kRowsPerPass)
+    for (int i = q; i < N; i += kRows)
       A.state[(int64_t)sd.out * N + i] = cmul_diag(psi[i * VP + v], A.diag[row * N + i]);
   }
   if (A.tmax_acc > 0)
-    for (int i = t; i < A.tmax_acc; i += kConvBlock) A.partial[(int64_t)g * A.tmax_acc + i] = acc[i];
+    for (int i = t; i < A.tmax_acc; i += BLOCK) A.partial[(int64_t)g * A.tmax_acc + i] = acc[i];
 }
 
-size_t conveyor_smem(int N, int vp, int tmax_acc) {
-  return (size_t)N * vp * sizeof(float2) + (size_t)tmax_acc * sizeof(double2) +
-         (size_t)(kConvBlock / 64) * vp * sizeof(float2);
+// Steps for a given shape: each layer split into chunks of at most
+// block/vp rotations, the layer's last chunk flagged (sign bit of y), the
+// list padded with empty steps to a multiple of the operand ring length.
+void build_steps(const Schedule &s, int vp, int block, std::vector<int2> &steps) {
+  const int rpp = block / vp;
+  steps.clear();
+  for (int l = 0; l < s.nlayers(); ++l) {
+    const int r0 = s.layer_off[l], r1 = s.layer_off[l + 1];
+    for (int r = r0; r < r1; r += rpp) {
+      const int e = r + rpp < r1 ? r + rpp : r1;
+      int2 st;
+      st.x = r;
+      st.y = (e == r1) ? (int)(0x80000000u | (uint32_t)e) : e;
+      steps.push_back(st);
+    }
+  }
+  while (!steps.empty() && steps.size() % kRing) steps.push_back(make_int2(0, 0));
 }
 
-int launch_conveyor(const ConveyorArgs &a, int vp, hipStream_t st) {
-  size_t smem = conveyor_smem(a.N, vp, a.tmax_acc);
+size_t conveyor_smem(int N, int vp, int block, int tmax_acc, int nsteps) {
+  return ConvSmem::bytes(N, vp, block, tmax_acc, nsteps);
+}
+
+int launch_conveyor(const ConveyorArgs &a, int vp, int block, hipStream_t st) {
+  const size_t smem = conveyor_smem(a.N, vp, block, a.tmax_acc, a.nsteps);
   if (smem > 160 * 1024) {
-    set_error("conveyor: N=%d with %d slots needs %zu B of LDS (max 160 KiB)", a.N, vp, smem);
+    set_error("conveyor: N=%d with %d slots and %d steps needs %zu B of LDS (max 160 KiB)", a.N,
+              vp, a.nsteps, smem);
     return NISE_ERR_ARG;
   }
+  if (a.nsteps % kRing) return set_error("conveyor: step list not padded"), NISE_ERR_ARG;
   if (a.nwg == 0) return NISE_OK;
-  switch (vp) {
-#define NISE_CONV_CASE(V)                                                                   \
-  case V:                                                                                   \
-    NISE_HIP_CHECK(hipFuncSetAttribute((const void *)conveyor_kernel<V>,                   \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)); \
-    hipLaunchKernelGGL(conveyor_kernel<V>, dim3(a.nwg), dim3(kConvBlock), smem, st, a);    \
-    break;
-    NISE_CONV_CASE(4)
-    NISE_CONV_CASE(8)
-    NISE_CONV_CASE(16)
-    NISE_CONV_CASE(32)
-    NISE_CONV_CASE(64)
-#undef NISE_CONV_CASE
-    default:
-      set_error("conveyor: unsupported slot count %d", vp);
-      return NISE_ERR_ARG;
-  }
-  NISE_HIP_CHECK(hipGetLastError());
+  const void *fn = nullptr;
+#define NISE_CONV(V, B) \
+  if (vp == V && block == B) fn = (const void *)conveyor_kernel<V, B>;
+  NISE_CONV(4, 64) NISE_CONV(8, 64) NISE_CONV(16, 64)
+  NISE_CONV(4, 256) NISE_CONV(8, 256) NISE_CONV(16, 256) NISE_CONV(32, 256) NISE_CONV(64, 256)
+  NISE_CONV(16, 512) NISE_CONV(32, 512) NISE_CONV(64, 512)
+  NISE_CONV(32, 1024) NISE_CONV(64, 1024)
+#undef NISE_CONV
+  if (!fn) return set_error("conveyor: unsupported shape vp=%d block=%d", vp, block), NISE_ERR_ARG;
+  NISE_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  ConveyorArgs args = a;
+  void *kargs[] = {&args};
+  NISE_HIP_CHECK(hipLaunchKernel(fn, dim3(a.nwg), dim3(block), kargs, smem, st));
   return NISE_OK;
 }
 

@@ -1,0 +1,154 @@
+"""Write tests/golden/reference_vectors.npz from the reference's own code.
+
+Runs only where the reference tree is present (this build container): it
+loads oracle/_ref/libnise_ref.so, which oracle/Makefile compiles from the
+reference's unmodified sources (propagate.c, NISE_subs.c, polar.c,
+read_trajectory.c, hamshift.c, randomlib.c).  The fixtures are data: inputs
+and the outputs the reference functions produced for them.
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from nise_2017_amd import synth  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.npz")
+
+
+def frame(rng, N, density=1.0, jsd=30.0):
+    nn = N * (N + 1) // 2
+    H = rng.normal(0, jsd, nn).astype(np.float32)
+    if density < 1:
+        H[rng.random(nn) > density] = 0
+    for a in range(N):
+        H[synth.sindex(a, a, N)] = rng.normal(0, 50)
+    return H
+
+
+def main():
+    R = O.ref()
+    rng = np.random.default_rng(20261202)
+    out = {}
+    # propagate_vec_coupling_S: (N, m, sign, cut, density), 3 steps
+    cases = [(2, 1, 1, 0.0, 1.0), (7, 2, -1, 0.0, 1.0), (16, 1, 1, 8.0, 1.0), (24, 3, 1, 0.0, 0.4),
+             (40, 1, -1, 2.0, 0.2), (65, 1, 1, 0.0, 0.05)]
+    for i, (N, m, sign, cut, dens) in enumerate(cases):
+        H = frame(rng, N, dens)
+        cr = rng.normal(size=N).astype(np.float32)
+        ci = rng.normal(size=N).astype(np.float32)
+        r, q = cr.copy(), ci.copy()
+        for _ in range(3):
+            R.ref_coupling_S(N, H, r, q, m, sign, 2.0, cut)
+        out[f"cS{i}_args"] = np.array([N, m, sign, cut], np.float64)
+        out[f"cS{i}_H"], out[f"cS{i}_in"] = H, np.stack([cr, ci])
+        out[f"cS{i}_out"] = np.stack([r, q])
+    # propagate_vec_coupling_S_doubles: (N, m, anharmonicity, cut)
+    dcases = [(1, 1, 16.0, 0.0), (3, 1, 0.0, 0.0), (6, 2, 16.0, 0.0), (11, 1, 20.0, 5.0)]
+    for i, (N, m, anh, cut) in enumerate(dcases):
+        H = frame(rng, N)
+        N2 = N * (N + 1) // 2
+        Anh = rng.normal(16, 3, N).astype(np.float32)
+        fr = rng.normal(size=N2).astype(np.float32)
+        fi = rng.normal(size=N2).astype(np.float32)
+        r, q = fr.copy(), fi.copy()
+        for _ in range(2):
+            R.ref_coupling_S_doubles(N, H, r, q, m, Anh, anh, 2.0, cut)
+        out[f"cD{i}_args"] = np.array([N, m, anh, cut], np.float64)
+        out[f"cD{i}_H"], out[f"cD{i}_Anh"] = H, Anh
+        out[f"cD{i}_in"], out[f"cD{i}_out"] = np.stack([fr, fi]), np.stack([r, q])
+    # dipole_double / dipole_double_last
+    for i, (N, anh) in enumerate([(4, 16.0), (9, 0.0)]):
+        N2 = N * (N + 1) // 2
+        dip = rng.normal(size=N).astype(np.float32)
+        over = rng.normal(size=N).astype(np.float32)
+        cr = rng.normal(size=N).astype(np.float32)
+        ci = rng.normal(size=N).astype(np.float32)
+        fr = np.zeros(N2, np.float32)
+        fi = np.zeros(N2, np.float32)
+        ov = over.copy()
+        R.ref_dipole_double(N, dip, cr, ci, fr, fi, ov, anh)
+        gr = rng.normal(size=N2).astype(np.float32)
+        gi = rng.normal(size=N2).astype(np.float32)
+        lr = np.zeros(N, np.float32)
+        li = np.zeros(N, np.float32)
+        ov2 = over.copy()
+        R.ref_dipole_double_last(N, dip, gr, gi, lr, li, ov2, anh)
+        out[f"dd{i}_args"] = np.array([N, anh])
+        out[f"dd{i}_in"] = np.concatenate([dip, over, cr, ci])
+        out[f"dd{i}_out"] = np.concatenate([fr, fi])
+        out[f"dl{i}_in"] = np.concatenate([gr, gi])
+        out[f"dl{i}_out"] = np.concatenate([lr, li])
+    # polarisation tables
+    xp = np.zeros((21, 4), np.int32)
+    w = np.zeros((3, 21), np.float32)
+    for x in range(21):
+        t = np.zeros(4, np.int32)
+        R.ref_polar(t, x)
+        xp[x] = t
+        for p in range(3):
+            w[p, x] = R.ref_polarweight(p, x)
+    out["polar"], out["polarweight"] = xp, w
+    # t_non layout
+    lay = np.zeros(32, np.int64)
+    k = R.ref_tnon_layout(lay, 32)
+    out["tnon_layout"] = lay[:k]
+    # frame reads and absorption loops over real GROBIN files
+    with tempfile.TemporaryDirectory() as d:
+        N = 6
+        H, mu = synth.chain_absorption(N=N, nframes=120, seed=77)
+        hf, df = os.path.join(d, "H.bin"), os.path.join(d, "mu.bin")
+        H.tofile(hf)
+        mu.tofile(df)
+        sf = os.path.join(d, "shift.txt")
+        with open(sf, "w") as fh:
+            fh.write("2\n1 1.0 5.5\n4 0.5 820.0\n")
+        He = np.zeros(N * (N + 1) // 2, np.float32)
+        m3 = np.zeros(3 * N, np.float32)
+        R.ref_read_frame(N, hf.encode(), df.encode(), b"", sf.encode(), b"Full", 1650.0, 4.9, 37,
+                         He, m3)
+        out["read_full_H"], out["read_full_mu_rec"] = H, mu
+        out["read_full_out"], out["read_full_mu"] = He, m3
+        lo, hi = synth.shifted_center(H, N)
+        T, sr, ns = 24, 9, 6
+        re = np.zeros(T, np.float32)
+        im = np.zeros(T, np.float32)
+        R.ref_absorption_files(N, hf.encode(), df.encode(), b"", b"", b"Full", lo, hi, 2.0, 0.0, 1,
+                               T, sr, 0, ns, H.shape[0], re, im)
+        out["abs_full_args"] = np.array([N, T, sr, ns, lo, hi], np.float64)
+        out["abs_full_re"], out["abs_full_im"] = re, im
+        # Coupling type: diagonal-only records, one coupling record, static dipoles
+        N = 9
+        rng2, mus, coup = synth.lattice_static(3, seed=5, cut=0.1)
+        nfr = 80
+        Hf = synth.full_records(synth.lattice_diag(rng2, nfr, N), coup)
+        a, b = np.triu_indices(N)
+        Hc = np.ascontiguousarray(np.concatenate([Hf[:, :1], Hf[:, 1:][:, a == b]], axis=1))
+        Cr = Hf[0].copy()
+        mur = synth.dipole_records(mus, 1)
+        hf2, df2, cf2 = (os.path.join(d, x) for x in ("Hc.bin", "muc.bin", "C.bin"))
+        Hc.tofile(hf2)
+        mur.tofile(df2)
+        Cr.tofile(cf2)
+        lo, hi = synth.shifted_center(Hf, N)
+        T, sr, ns = 20, 7, 5
+        re = np.zeros(T, np.float32)
+        im = np.zeros(T, np.float32)
+        R.ref_absorption_files(N, hf2.encode(), df2.encode(), cf2.encode(), sf.encode(),
+                               b"Coupling", lo, hi, 2.0, 0.0, 2, T, sr, 0, ns, nfr, re, im)
+        out["abs_coup_args"] = np.array([N, T, sr, ns, lo, hi], np.float64)
+        out["abs_coup_H"], out["abs_coup_mu"], out["abs_coup_C"] = Hc, mur, Cr
+        out["abs_coup_re"], out["abs_coup_im"] = re, im
+    np.savez_compressed(OUT, **out)
+    print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {len(out)} arrays)")
+
+
+if __name__ == "__main__":
+    main()

@@ -20,8 +20,8 @@ pytestmark = pytest.mark.gpu
 
 
 def rel_l2(a, b):
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
@@ -140,7 +140,7 @@ def absorption_case(kind, N, T, nsamp, sample, coupling=False, shift=False):
         kw["hamiltonian"] = "Coupling"
         Crec = H[0].copy()
         a, b = np.triu_indices(N)
-        H = np.concatenate([H[:, :1], H[:, 1:][:, a == b]], axis=1)
+        H = np.ascontiguousarray(np.concatenate([H[:, :1], H[:, 1:][:, a == b]], axis=1))
         mu = mu[:1].copy()
     non = make_non(**kw)
     non.shifte = np.float32((np.float32(hi) + np.float32(lo)) / np.float32(2))
@@ -179,9 +179,10 @@ def test_absorption_shapes_agree(orc, nise_lib):
     same summation per workgroup shape up to FP64 rounding)."""
     non, H, mu, _ = absorption_case("lattice", 16, 30, 20, 3)
     base = None
-    for vp, spw in [(4, 1), (8, 2), (16, 5), (32, 10), (64, 21), (64, 3)]:
+    for vp, spw, block in [(4, 1, 64), (8, 2, 256), (16, 5, 256), (16, 5, 64), (32, 10, 512),
+                           (64, 21, 256), (64, 3, 1024), (32, 7, 256)]:
         p = abi.AbsPlan(non, H, H.shape[1], mu, mu.shape[1], H.shape[0])
-        p.set_shape(vp, spw)
+        p.set_shape(vp, spw, block)
         re, im = p.execute()
         p.close()
         if base is None:

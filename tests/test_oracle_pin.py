@@ -1,0 +1,373 @@
+"""Pin the oracle (CPU restatement) before trusting it.
+
+1. Bit-exact against golden vectors written by the reference's own compiled
+   functions (tests/golden/reference_vectors.npz, make_golden.py).
+2. Bit-exact against the live reference build where oracle/_ref exists.
+3. Against what the mathematics fixes: uncoupled phases, the dimer's closed
+   form, norm conservation, convergence order of the splitting, FP64 matrix
+   exponentials, the three-level single site of the doubles manifold, and
+   numpy's FFT for the 1D spectrum.  Each check is chosen so that a dropped
+   term, wrong sign, wrong index or transposed operand fails it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+from nise_2017_amd import synth
+from nise_2017_amd.abi import make_non
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz")
+F32 = np.float32
+TWO_PI_F = F32(2 * 3.14159265359)
+ICM = F32(2.99792458e-5)
+
+
+def fac(dt, sign=1, m=1):
+    f = F32(F32(F32(dt) * ICM) * TWO_PI_F)
+    return F32(F32(f * F32(sign)) / F32(m))
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(GOLD)
+
+
+def dense(H, N):
+    A = np.zeros((N, N))
+    for a in range(N):
+        for b in range(N):
+            A[a, b] = H[synth.sindex(a, b, N)]
+    return A
+
+
+# --------------------------------------------------------------- golden --
+def test_coupling_S_golden(orc, gold):
+    for i in range(6):
+        N, m, sign, cut = gold[f"cS{i}_args"]
+        N, m, sign = int(N), int(m), int(sign)
+        H = gold[f"cS{i}_H"]
+        r, q = gold[f"cS{i}_in"][0].copy(), gold[f"cS{i}_in"][1].copy()
+        for _ in range(3):
+            orc.oracle().or_coupling_S_f32(N, H, r, q, m, sign, 2.0, float(cut))
+        assert np.array_equal(r, gold[f"cS{i}_out"][0]) and np.array_equal(q, gold[f"cS{i}_out"][1])
+
+
+def test_doubles_golden(orc, gold):
+    for i in range(4):
+        N, m, anh, cut = gold[f"cD{i}_args"]
+        N, m = int(N), int(m)
+        r, q = gold[f"cD{i}_in"][0].copy(), gold[f"cD{i}_in"][1].copy()
+        for _ in range(2):
+            orc.oracle().or_coupling_S_doubles_f32(N, gold[f"cD{i}_H"], r, q, m, gold[f"cD{i}_Anh"],
+                                                   float(anh), 2.0, float(cut))
+        assert np.array_equal(r, gold[f"cD{i}_out"][0]) and np.array_equal(q, gold[f"cD{i}_out"][1])
+
+
+def test_dipole_double_golden(orc, gold):
+    for i in range(2):
+        N, anh = gold[f"dd{i}_args"]
+        N = int(N)
+        N2 = N * (N + 1) // 2
+        x = gold[f"dd{i}_in"]
+        dip, over, cr, ci = x[:N], x[N:2 * N].copy(), x[2 * N:3 * N], x[3 * N:]
+        fr = np.zeros(N2, np.float32)
+        fi = np.zeros(N2, np.float32)
+        orc.oracle().or_dipole_double(N, dip, cr, ci, fr, fi, over, float(anh))
+        assert np.array_equal(np.concatenate([fr, fi]), gold[f"dd{i}_out"])
+        g = gold[f"dl{i}_in"]
+        lr = np.zeros(N, np.float32)
+        li = np.zeros(N, np.float32)
+        over = x[N:2 * N].copy()
+        orc.oracle().or_dipole_double_last(N, dip, g[:N2].copy(), g[N2:].copy(), lr, li, over,
+                                           float(anh))
+        assert np.array_equal(np.concatenate([lr, li]), gold[f"dl{i}_out"])
+
+
+def test_polar_golden(orc, gold):
+    for x in range(21):
+        t = np.zeros(4, np.int32)
+        orc.oracle().or_polar(t, x)
+        assert np.array_equal(t, gold["polar"][x])
+        for p in range(3):
+            assert orc.oracle().or_polarweight(p, x) == gold["polarweight"][p, x]
+
+
+def test_tnon_layout_golden(gold):
+    from nise_2017_amd.abi import TNon
+    T = TNon
+    mine = [C.sizeof(T)] + [getattr(T, f).offset for f in (
+        "tmax1", "length", "deltat", "singles", "energyFName", "hamiltonian", "pbcFName", "tmax",
+        "shifte", "thres", "couplingcut", "anharmonicity", "psites", "window", "SingleShiftSite",
+        "SingleShift", "printLevel")]
+    assert mine == list(gold["tnon_layout"])
+
+
+def test_read_frame_golden(orc, gold):
+    N = 6
+    H, mu = gold["read_full_H"], gold["read_full_mu_rec"]
+    non = make_non(singles=N, couplingcut=4.9, single_shift=[(1, 1.0, 5.5), (4, 0.5, 820.0)])
+    non.shifte = 1650.0
+    He = np.zeros(N * (N + 1) // 2, np.float32)
+    orc.oracle().or_read_H(C.byref(non), orc.ptr(H), H.shape[1], None, 37, He, 1)
+    assert np.array_equal(He, gold["read_full_out"])
+    m3 = mu[37, 1:]
+    assert np.array_equal(m3, gold["read_full_mu"])
+
+
+def test_absorption_golden_full(orc, gold):
+    N, T, sr, ns, lo, hi = gold["abs_full_args"]
+    H, mu = gold["read_full_H"], gold["read_full_mu_rec"]
+    non = make_non(singles=int(N), tmax1=int(T), sample=int(sr), begin=0, end=int(ns), ts=1,
+                   propagation=1, deltat=2.0, min1=lo, max1=hi)
+    non.shifte = F32((F32(hi) + F32(lo)) / F32(2))
+    re, im = orc.absorption(non, H, mu)
+    assert np.array_equal(re, gold["abs_full_re"]) and np.array_equal(im, gold["abs_full_im"])
+
+
+def test_absorption_golden_coupling(orc, gold):
+    N, T, sr, ns, lo, hi = gold["abs_coup_args"]
+    non = make_non(singles=int(N), tmax1=int(T), sample=int(sr), begin=0, end=int(ns), ts=2,
+                   propagation=1, deltat=2.0, min1=lo, max1=hi, hamiltonian="Coupling",
+                   single_shift=[(1, 1.0, 5.5), (4, 0.5, 820.0)])
+    non.shifte = F32((F32(hi) + F32(lo)) / F32(2))
+    re, im = orc.absorption(non, gold["abs_coup_H"], gold["abs_coup_mu"], gold["abs_coup_C"])
+    assert np.array_equal(re, gold["abs_coup_re"]) and np.array_equal(im, gold["abs_coup_im"])
+
+
+# ---------------------------------------------------------- live reference --
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(
+    os.path.abspath(__file__))), "oracle", "_ref", "libnise_ref.so")), reason="reference not built")
+def test_live_reference_random(orc):
+    R = orc.ref()
+    rng = np.random.default_rng(5)
+    for trial in range(40):
+        N = int(rng.integers(1, 48))
+        H = synth_frame(rng, N, density=float(rng.choice([1.0, 0.3])))
+        m = int(rng.integers(1, 4))
+        sign = int(rng.choice([-1, 1]))
+        cut = float(rng.choice([0.0, 3.0]))
+        cr = rng.normal(size=N).astype(np.float32)
+        ci = rng.normal(size=N).astype(np.float32)
+        a, b, c, d = cr.copy(), ci.copy(), cr.copy(), ci.copy()
+        for _ in range(5):
+            orc.oracle().or_coupling_S_f32(N, H, a, b, m, sign, 2.0, cut)
+            R.ref_coupling_S(N, H, c, d, m, sign, 2.0, cut)
+        assert np.array_equal(a, c) and np.array_equal(b, d)
+        N2 = N * (N + 1) // 2
+        fr = rng.normal(size=N2).astype(np.float32)
+        fi = rng.normal(size=N2).astype(np.float32)
+        Anh = rng.normal(16, 2, N).astype(np.float32)
+        anh = float(rng.choice([0.0, 12.0]))
+        a, b, c, d = fr.copy(), fi.copy(), fr.copy(), fi.copy()
+        orc.oracle().or_coupling_S_doubles_f32(N, H, a, b, m, Anh, anh, 2.0, cut)
+        R.ref_coupling_S_doubles(N, H, c, d, m, Anh, anh, 2.0, cut)
+        assert np.array_equal(a, c) and np.array_equal(b, d)
+
+
+def synth_frame(rng, N, density=1.0):
+    nn = N * (N + 1) // 2
+    H = rng.normal(0, 30, nn).astype(np.float32)
+    if density < 1:
+        H[rng.random(nn) > density] = 0
+    for a in range(N):
+        H[synth.sindex(a, a, N)] = rng.normal(0, 50)
+    return H
+
+
+# ------------------------------------------------------------ closed forms --
+def test_uncoupled_phases(orc):
+    N = 5
+    w = np.array([-30.0, 0.0, 12.5, 40.0, -7.0], np.float32)
+    H = np.zeros(N * (N + 1) // 2, np.float32)
+    for a in range(N):
+        H[synth.sindex(a, a, N)] = w[a]
+    cr = np.ones(N)
+    ci = np.zeros(N)
+    for _ in range(10):
+        orc.oracle().or_coupling_S_f64(N, H, cr, ci, 1, 1, 2.0, 0.0)
+    f = float(fac(2.0))
+    assert np.allclose(cr + 1j * ci, np.exp(-1j * w * f * 10), atol=1e-12)
+
+
+def test_dimer_closed_form(orc):
+    """One coupling: D R D is the symmetric splitting of exp(-i f H)."""
+    e1, e2, J = 10.0, -15.0, 25.0
+    H = np.array([e1, J, e2], np.float32)
+    f = float(fac(2.0))
+    D = np.diag(np.exp(-0.5j * f * np.array([e1, e2])))
+    R = np.array([[np.cos(J * f), -1j * np.sin(J * f)], [-1j * np.sin(J * f), np.cos(J * f)]])
+    U = D @ R @ D
+    psi = np.array([0.3 + 0.1j, -0.8 + 0.5j])
+    cr, ci = psi.real.copy(), psi.imag.copy()
+    orc.oracle().or_coupling_S_f64(2, H, cr, ci, 1, 1, 2.0, 0.0)
+    assert np.allclose(cr + 1j * ci, U @ psi, atol=1e-14)
+    # the f32 path agrees to float precision and conserves the norm
+    r32, i32 = psi.real.astype(np.float32), psi.imag.astype(np.float32)
+    orc.oracle().or_coupling_S_f32(2, H, r32, i32, 1, 1, 2.0, 0.0)
+    assert np.allclose(r32 + 1j * i32, U @ psi, atol=3e-7)
+    # sign -1 applies exp(+i f H) in the same splitting (not the inverse order)
+    cr, ci = psi.real.copy(), psi.imag.copy()
+    orc.oracle().or_coupling_S_f64(2, H, cr, ci, 1, -1, 2.0, 0.0)
+    assert np.allclose(cr + 1j * ci, np.conj(U) @ psi, atol=1e-14)
+
+
+def test_splitting_converges_to_expm(orc):
+    """Error of m substeps vs the exact exponential falls with m (first
+    order in the coupling commutators for a chain)."""
+    rng = np.random.default_rng(1)
+    N = 6
+    H = np.zeros(N * (N + 1) // 2, np.float32)
+    for a in range(N):
+        H[synth.sindex(a, a, N)] = rng.normal(0, 40)
+        if a + 1 < N:
+            H[synth.sindex(a, a + 1, N)] = rng.normal(0, 60)
+    A = dense(H, N)
+    psi = rng.normal(size=N) + 1j * rng.normal(size=N)
+    f1 = float(F32(F32(F32(2.0) * ICM) * TWO_PI_F))
+    exact = expm(-1j * A * f1) @ psi
+    errs = []
+    for m in (1, 2, 4, 8):
+        cr, ci = psi.real.copy(), psi.imag.copy()
+        orc.oracle().or_coupling_S_f64(N, H, cr, ci, m, 1, 2.0, 0.0)
+        errs.append(np.linalg.norm(cr + 1j * ci - exact))
+    assert errs[0] > 0 and all(errs[i + 1] < 0.6 * errs[i] for i in range(3))
+
+
+def test_rotation_order_matters(orc):
+    """Two rotations sharing a site do not commute: the oracle's order is
+    visible (swapping the couplings' order changes the result)."""
+    N = 3
+    H = np.array([0, 40, 0, 0, 25, 0], np.float32)  # J01=40, J12=25 (J02=0)
+    psi = np.array([1, 0, 0], complex)
+    cr, ci = psi.real.copy(), psi.imag.copy()
+    orc.oracle().or_coupling_S_f64(N, H, cr, ci, 1, 1, 2.0, 0.0)
+    f = float(fac(2.0))
+
+    def R(a, b, J):
+        M = np.eye(N, dtype=complex)
+        M[a, a] = M[b, b] = np.cos(J * f)
+        M[a, b] = M[b, a] = -1j * np.sin(J * f)
+        return M
+    assert np.allclose(cr + 1j * ci, R(1, 2, 25) @ R(0, 1, 40) @ psi, atol=1e-14)
+    assert not np.allclose(cr + 1j * ci, R(0, 1, 40) @ R(1, 2, 25) @ psi, atol=1e-6)
+
+
+def test_inverse_step(orc):
+    rng = np.random.default_rng(2)
+    N = 9
+    H = synth_frame(rng, N)
+    psi = rng.normal(size=N) + 1j * rng.normal(size=N)
+    cr, ci = psi.real.copy(), psi.imag.copy()
+    orc.oracle().or_coupling_S_f64(N, H, cr, ci, 2, 1, 2.0, 0.0)
+    orc.oracle().or_coupling_S_inv_f64(N, H, cr, ci, 2, 1, 2.0, 0.0)
+    assert np.allclose(cr + 1j * ci, psi, atol=1e-13)
+
+
+def test_doubles_single_site_and_expm(orc):
+    # N=1: the |aa> state only picks up the phase of 2 e - anharmonicity
+    H = np.array([30.0], np.float32)
+    fr, fi = np.array([1.0]), np.array([0.0])
+    orc.oracle().or_coupling_S_doubles_f64(1, H, fr, fi, 1, np.zeros(1, np.float32), 16.0, 2.0, 0.0)
+    f = float(fac(2.0))
+    assert np.allclose(fr + 1j * fi, np.exp(-1j * (60.0 - 16.0) * f), atol=1e-14)
+    # small N: doubles Trotter step vs exact exponential of the two-exciton
+    # Hamiltonian built with the harmonic sqrt(2) rule
+    rng = np.random.default_rng(4)
+    N = 4
+    Hs = synth_frame(rng, N)
+    for a in range(N):
+        for b in range(a + 1, N):
+            Hs[synth.sindex(a, b, N)] *= 0.1
+    A = dense(Hs, N)
+    anh = 16.0
+    idx = [(a, b) for a in range(N) for b in range(a, N)]
+    pos = {p: synth.sindex(p[0], p[1], N) for p in idx}
+    N2 = len(idx)
+    H2 = np.zeros((N2, N2))
+    for (a, b) in idx:
+        i = pos[(a, b)]
+        H2[i, i] = A[a, a] + A[b, b] - (anh if a == b else 0)
+    for (a, b) in idx:  # hop one quantum: a -> c
+        for (c, d) in idx:
+            i, j = pos[(a, b)], pos[(c, d)]
+            if i >= j:
+                continue
+            ab, cd = [a, b], [c, d]
+            common = None
+            for x in ab:
+                if x in cd:
+                    common = x
+                    break
+            if common is None:
+                continue
+            r1 = list(ab)
+            r1.remove(common)
+            r2 = list(cd)
+            r2.remove(common)
+            if r1[0] == r2[0]:
+                continue
+            J = A[r1[0], r2[0]]
+            fac2 = np.sqrt(2.0) if (a == b or c == d) else 1.0
+            H2[i, j] = H2[j, i] = J * fac2
+    psi = rng.normal(size=N2) + 1j * rng.normal(size=N2)
+    exact = expm(-1j * H2 * f) @ psi
+    fr, fi = psi.real.copy(), psi.imag.copy()
+    orc.oracle().or_coupling_S_doubles_f64(N, Hs, fr, fi, 1, np.zeros(N, np.float32), anh, 2.0, 0.0)
+    err1 = np.linalg.norm(fr + 1j * fi - exact) / np.linalg.norm(exact)
+    fr, fi = psi.real.copy(), psi.imag.copy()
+    orc.oracle().or_coupling_S_doubles_f64(N, Hs, fr, fi, 4, np.zeros(N, np.float32), anh, 2.0, 0.0)
+    err4 = np.linalg.norm(fr + 1j * fi - exact) / np.linalg.norm(exact)
+    assert err1 < 1e-3 and err4 < err1 / 2
+
+
+def test_dia_oracle_is_expm(orc):
+    rng = np.random.default_rng(6)
+    N = 7
+    H = synth_frame(rng, N)
+    A = dense(H, N)
+    Ad = A.copy()
+    w = np.zeros(N)
+    V = np.zeros(N * N)
+    sweeps = orc.oracle().or_jacobi_f64(N, Ad.reshape(-1), w, V)
+    assert sweeps >= 0
+    V = V.reshape(N, N)
+    assert np.allclose(A @ V, V * w[None, :], atol=1e-9)
+    assert np.all(np.diff(w) >= 0)
+    psi = rng.normal(size=N) + 1j * rng.normal(size=N)
+    cr, ci = psi.real.copy(), psi.imag.copy()
+    orc.oracle().or_DIA_f64(N, H, cr, ci, -1, 2.0)
+    f = float(fac(2.0, -1))
+    assert np.allclose(cr + 1j * ci, expm(-1j * A * f) @ psi, atol=1e-11)
+
+
+def test_1dfft_matches_numpy(orc):
+    rng = np.random.default_rng(8)
+    T, fft = 64, 256
+    re = rng.normal(size=T).astype(np.float32)
+    im = rng.normal(size=T).astype(np.float32)
+    non = make_non(tmax1=T, fft=fft, deltat=2.0, lifetime=300.0, min1=1500.0, max1=1800.0)
+    non.shifte = 1650.0
+    samples = 7
+    fr, sr, si = orc.fft1d(non, re, im, samples)
+    # numpy restatement of src/1DFFT.c:174-206
+    x = np.zeros(fft, complex)
+    i = np.arange(T)
+    x[:T] = (im / F32(samples)).astype(np.float64) + 1j * (re / F32(samples)).astype(np.float64)
+    x[:T] *= np.exp(-(i * F32(2.0)).astype(np.float32) / F32(2 * 300.0))
+    x[0] *= 0.5
+    X = np.fft.fft(x)
+    spec_r = X.imag * 2 * 2.0 * float(F32(2.99792458e-5))
+    spec_i = X.real * 2 * 2.0 * float(F32(2.99792458e-5))
+    k = np.r_[np.arange(fft // 2, fft), np.arange(0, fft // 2)]
+    w = np.where(k >= fft // 2, -((fft - k) / 2.0 / 2.99792458e-5 / fft - 1650.0),
+                 -((-k) / 2.0 / 2.99792458e-5 / fft - 1650.0))
+    sel = (w > 1500.0) & (w < 1800.0)
+    assert len(fr) == sel.sum()
+    assert np.allclose(fr, w[sel], atol=5e-3)
+    assert np.allclose(sr, spec_r[k[sel]], rtol=1e-6, atol=1e-7 * np.abs(spec_r).max())
+    assert np.allclose(si, spec_i[k[sel]], rtol=1e-6, atol=1e-7 * np.abs(spec_i).max())
+    # a planted mistake (no re/im swap) would fail: the two parts differ
+    assert not np.allclose(sr, spec_i[k[sel]], rtol=1e-3)
