@@ -76,7 +76,7 @@ int build_slots(nise_abs_plan *P) {
   for (int s = 0; s < ns; ++s) {
     if (s % P->spw == 0) wb.push_back((int)sl.size());
     int fs = (P->begin + s) * P->sample;
-    for (int x = 0; x < 3; ++x) sl.push_back(SlotDesc{fs, P->tmax, P->tmax - 1, x, 0, -1, x == 0 ? 3 : 0});
+    for (int x = 0; x < 3; ++x) sl.push_back(SlotDesc{fs, P->tmax, P->tmax - 1, x, 0, -1, x == 0 ? 3 : 0, -1});
   }
   wb.push_back((int)sl.size());
   P->nwg = (int)wb.size() - 1;
@@ -249,8 +249,9 @@ extern "C" int nise_abs_plan_create(nise_abs_plan **out, const t_non *non, const
 extern "C" int nise_abs_plan_set_shape(nise_abs_plan *P, int vp, int samples_per_wg, int block) {
   if (!P || P->kernel != 0) return set_error("no conveyor plan"), NISE_ERR_ARG;
   if (!(vp == 4 || vp == 8 || vp == 16 || vp == 32 || vp == 64) || samples_per_wg < 1 ||
-      3 * samples_per_wg > vp || !(block == 64 || block == 256 || block == 512 || block == 1024) ||
-      vp > block)
+      3 * samples_per_wg > vp ||
+      !(block == 64 || block == 256 || block == 512 || block == -2 || block == -4) ||
+      (block > 0 && vp > block))
     return set_error("bad shape vp=%d spw=%d block=%d", vp, samples_per_wg, block), NISE_ERR_ARG;
   P->vp = vp, P->spw = samples_per_wg, P->block = block;
   if (P->graph_ready) {
@@ -402,7 +403,7 @@ int coupling_S_host(int N, const float *H, float *cr, float *ci, int nvec, int m
   std::vector<int> wb;
   for (int v = 0; v < nvec; ++v) {
     if (v % vp == 0) wb.push_back(v);
-    sl[v] = SlotDesc{0, 0, 1, 0, 1, v, 0};
+    sl[v] = SlotDesc{0, 0, 1, 0, 1, v, 0, -1};
   }
   wb.push_back(nvec);
   c.sbuf.resize((size_t)nvec * N);

@@ -1,0 +1,333 @@
+// Two-exciton (doubles) Trotter propagation for gfx950.
+//
+// Reference: propagate_vec_coupling_S_doubles, src/propagate.c:902-1048.
+// Basis: the N(N+1)/2 states |ab>, a <= b, packed with Sindex.  One Trotter
+// substep is D2 . R_kmax ... R_1 . D2 with
+//   D2     = exp(-i f H0/2),  H0_ab = e_a + e_b - delta_ab * anharmonicity
+//   R_k    = for c = 0..N-1 the 2x2 rotation of (|ca>, |cb>) with angle
+//            f J_k, or sqrt(2) f J_k when c is a or b, in ascending c.
+// The N rotations of one coupling touch disjoint pairs except c = a and
+// c = b, which share |ab>: they run as two ordered rounds.  Couplings follow
+// each other strictly (any two share a state).  The reference evaluates the
+// trigonometry in float libm (cosf, sinf, sqrtf); here each is computed in
+// double and rounded once, the correctly rounded float result.
+//
+// One wave propagates 64/NP vectors (NP = lanes per vector, N <= NP <= 64,
+// or NP = 64 with a loop over site chunks when N > 64); the wave's vectors
+// share one frame, live in LDS as psi[state][vector], and LDS operations of
+// a wave execute in order, so rounds need no barrier.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "nise_internal.h"
+
+namespace nise {
+
+__device__ __forceinline__ float2 cmul_diag2(float2 c, float2 u) {
+  // src/propagate.c:960-961: (cr*re - ci*im, cr*im + ci*re)
+  return make_float2(__fsub_rn(__fmul_rn(c.x, u.x), __fmul_rn(c.y, u.y)),
+                     __fadd_rn(__fmul_rn(c.x, u.y), __fmul_rn(c.y, u.x)));
+}
+
+__device__ __forceinline__ void rot2(float2 &p1, float2 &p2, float co, float si) {
+  // src/propagate.c:986-990 (same form for all five c ranges)
+  float cr1 = __fsub_rn(__fmul_rn(co, p1.x), __fmul_rn(si, p2.y));
+  float ci1 = __fadd_rn(__fmul_rn(co, p1.y), __fmul_rn(si, p2.x));
+  float cr2 = __fsub_rn(__fmul_rn(co, p2.x), __fmul_rn(si, p1.y));
+  float ci2 = __fadd_rn(__fmul_rn(co, p2.y), __fmul_rn(si, p1.x));
+  p1 = make_float2(cr1, ci1);
+  p2 = make_float2(cr2, ci2);
+}
+
+// Per-frame doubles tables: D2[row][N2] and (co, si, co2, si2)[row][K2]
+// for the kept couplings in lexicographic order (ab2).  anh_rec != null:
+// per-site anharmonicities from records (1 + N words per frame).
+__global__ void doubles_tables_kernel(FrameSource s, int N, int K2, const uint32_t *ab2, float fac,
+                                      float anh_fixed, const float *anh_rec, int64_t f_lo,
+                                      float2 *D2, float4 *cs2) {
+  const int64_t row = blockIdx.y;
+  const int64_t f = f_lo + row;
+  const int N2 = N * (N + 1) / 2;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < N2) {
+    // invert the packed index j -> (a, b), a <= b
+    int a = 0, rem = j;
+    while (rem >= N - a) { rem -= N - a; ++a; }
+    const int b = a + rem;
+    const float ea = __fsub_rn(raw_diag(s, N, f, a), s.shifte);
+    const float eb = __fsub_rn(raw_diag(s, N, f, b), s.shifte);
+    float h0 = __fadd_rn(ea, eb);
+    if (a == b) {
+      const float d = (anh_fixed == 0.f) ? anh_rec[f * (1 + (int64_t)N) + 1 + a] : anh_fixed;
+      h0 = __fsub_rn(h0, d);
+    }
+    const float arg = __fmul_rn(__fmul_rn(0.5f, h0), fac);
+    D2[row * N2 + j] = make_float2((float)cos((double)arg), (float)(-sin((double)arg)));
+  } else if (j < N2 + K2) {
+    const int k = j - N2;
+    const uint32_t p = ab2[k];
+    const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
+    float J = kept_coupling(s, N, f, a, b);
+    float4 out = make_float4(1.f, 0.f, 1.f, 0.f);
+    if (J != 0.f) {
+      const float Jf = __fmul_rn(J, fac);
+      const float si = (float)(-sin((double)Jf));
+      const float co = (float)sqrt((double)__fsub_rn(1.0f, __fmul_rn(si, si)));
+      const float si2 = (float)(-sin((double)__fmul_rn(Jf, kSqrt2)));
+      const float co2 = (float)sqrt((double)__fsub_rn(1.0f, __fmul_rn(si2, si2)));
+      out = make_float4(co, si, co2, si2);
+    }
+    cs2[row * K2 + k] = out;
+  }
+}
+
+int launch_doubles_tables(const FrameSource &src, int N, int K2, const uint32_t *ab2, float fac,
+                          float anh_fixed, const float *anh_rec, int64_t f_lo, int64_t nfr,
+                          float2 *D2, float4 *cs2, hipStream_t st) {
+  const int N2 = N * (N + 1) / 2;
+  const int n = N2 + K2;
+  int64_t done = 0;
+  while (done < nfr) {
+    const int64_t chunk = std::min<int64_t>(nfr - done, 65535);
+    hipLaunchKernelGGL(doubles_tables_kernel, dim3((n + 255) / 256, (unsigned)chunk), dim3(256), 0, st,
+                       src, N, K2, ab2, fac, anh_fixed, anh_rec, f_lo + done, D2 + done * N2,
+                       cs2 + done * K2);
+    NISE_HIP_CHECK(hipGetLastError());
+    done += chunk;
+  }
+  return NISE_OK;
+}
+
+// One frame of m Trotter substeps for groups of vectors.  Group g (one
+// wave) holds vectors vec_begin[g] .. vec_begin[g+1]-1 of the state buffer
+// (at most 64/NP of them) and uses table row rows[g].
+template <int NP>
+__global__ void __launch_bounds__(64) doubles_step_kernel(DoublesArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int VW = 64 / NP;  // vectors per wave
+  const int N = A.N, N2 = N * (N + 1) / 2, K2 = A.K2;
+  float2 *psi = reinterpret_cast<float2 *>(smem);  // [N2][VW]
+  const int g = blockIdx.x, t = threadIdx.x;
+  const int v0 = A.vec_begin[g], nv = A.vec_begin[g + 1] - v0;
+  const int64_t row = A.rows[g];
+  const float2 *D2 = A.D2 + row * N2;
+  const float4 *cs2 = A.cs2 + row * K2;
+  // load: state[v0 + w][j] -> psi[j][w]
+  for (int i = t; i < N2 * VW; i += 64) {
+    const int w = i / N2, j = i % N2;
+    psi[j * VW + w] = (w < nv) ? A.state[(int64_t)(v0 + w) * N2 + j] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const int w = t / NP, c0 = t % NP;
+  const bool on = w < nv;
+  for (int sub = 0; sub < A.m; ++sub) {
+    for (int j = t; j < N2 * VW; j += 64) {
+      const int jj = j / VW;
+      psi[j] = cmul_diag2(psi[j], D2[jj]);
+    }
+    asm volatile("" ::: "memory");
+    for (int k = 0; k < K2; ++k) {
+      const uint32_t p = A.ab2[k];
+      const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
+      const float4 cc = cs2[k];
+      for (int cb = 0; cb < N; cb += NP) {
+        const int c = cb + c0;
+        // round 1: every c except b (c == a uses the sqrt(2) angle)
+        if (on && c < N && c != b) {
+          const int i1 = sindex(c, a, N), i2 = sindex(c, b, N);
+          float2 p1 = psi[i1 * VW + w], p2 = psi[i2 * VW + w];
+          if (c == a) rot2(p1, p2, cc.z, cc.w);
+          else rot2(p1, p2, cc.x, cc.y);
+          psi[i1 * VW + w] = p1;
+          psi[i2 * VW + w] = p2;
+        }
+        asm volatile("" ::: "memory");
+      }
+      // round 2: c == b, after c == a (they share |ab>)
+      if (on && c0 == (b % NP)) {
+        const int i1 = sindex(a, b, N), i2 = sindex(b, b, N);
+        float2 p1 = psi[i1 * VW + w], p2 = psi[i2 * VW + w];
+        rot2(p1, p2, cc.z, cc.w);
+        psi[i1 * VW + w] = p1;
+        psi[i2 * VW + w] = p2;
+      }
+      asm volatile("" ::: "memory");
+    }
+    for (int j = t; j < N2 * VW; j += 64) {
+      const int jj = j / VW;
+      psi[j] = cmul_diag2(psi[j], D2[jj]);
+    }
+    asm volatile("" ::: "memory");
+  }
+  __syncthreads();
+  for (int i = t; i < N2 * VW; i += 64) {
+    const int ww = i / N2, j = i % N2;
+    if (ww < nv) A.state[(int64_t)(v0 + ww) * N2 + j] = psi[j * VW + ww];
+  }
+}
+
+int doubles_np(int N) { return N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64; }
+
+int launch_doubles_step(const DoublesArgs &a, int ngroups, hipStream_t st) {
+  if (ngroups == 0) return NISE_OK;
+  const int np = doubles_np(a.N);
+  const int N2 = a.N * (a.N + 1) / 2;
+  const size_t smem = (size_t)N2 * (64 / np) * sizeof(float2);
+  if (smem > 160 * 1024) return set_error("doubles: N=%d too large for LDS", a.N), NISE_ERR_ARG;
+  const void *fn = np == 8 ? (const void *)doubles_step_kernel<8>
+                 : np == 16 ? (const void *)doubles_step_kernel<16>
+                 : np == 32 ? (const void *)doubles_step_kernel<32>
+                            : (const void *)doubles_step_kernel<64>;
+  NISE_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  DoublesArgs args = a;
+  void *kargs[] = {&args};
+  NISE_HIP_CHECK(hipLaunchKernel(fn, dim3(ngroups), dim3(64), kargs, smem, st));
+  return NISE_OK;
+}
+
+// Lexicographic list of kept couplings of one packed frame (host side).
+void kept_pairs(int N, const float *H, float cut, std::vector<uint32_t> &ab) {
+  ab.clear();
+  for (int a = 0; a < N; ++a)
+    for (int b = a + 1; b < N; ++b)
+      if (std::fabs(H[sindex(a, b, N)]) > cut) ab.push_back((uint32_t)a | ((uint32_t)b << 16));
+}
+
+namespace {
+struct DoublesCtx {
+  DevBuf H, ab, D2, cs2, state, vb, rows, anh;
+  std::vector<float2> sbuf;
+};
+DoublesCtx &dctx2() {
+  static thread_local DoublesCtx c;
+  return c;
+}
+int up2(DevBuf &d, const void *s, size_t n) {
+  int rc = d.alloc(n < 16 ? 16 : n);
+  if (rc) return rc;
+  if (n && s) NISE_HIP_CHECK(hipMemcpy(d.p, s, n, hipMemcpyHostToDevice));
+  return NISE_OK;
+}
+}  // namespace
+
+// nvec doubles vectors (stride N(N+1)/2) through one frame H (packed,
+// diagonal already shifted), m substeps.
+int doubles_host(int N, const float *H, float *const *cr, float *const *ci, int nvec, int m,
+                 const float *Anh, float anharmonicity, float deltat, float cut) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (N < 1 || m < 1 || nvec < 1) return set_error("doubles: bad arguments"), NISE_ERR_ARG;
+  hipStream_t st = default_stream();
+  DoublesCtx &c = dctx2();
+  const int N2 = N * (N + 1) / 2;
+  std::vector<float> rec(1 + N2);
+  rec[0] = 0.f;
+  std::memcpy(rec.data() + 1, H, N2 * 4);
+  std::vector<uint32_t> ab;
+  kept_pairs(N, H, cut, ab);
+  const int K2 = (int)ab.size();
+  const int np = doubles_np(N), vw = 64 / np;
+  const int ng = (nvec + vw - 1) / vw;
+  std::vector<int> vb(ng + 1);
+  for (int g = 0; g <= ng; ++g) vb[g] = std::min(nvec, g * vw);
+  c.sbuf.resize((size_t)nvec * N2);
+  for (int v = 0; v < nvec; ++v)
+    for (int j = 0; j < N2; ++j) c.sbuf[(size_t)v * N2 + j] = make_float2(cr[v][j], ci[v][j]);
+  if ((rc = up2(c.H, rec.data(), rec.size() * 4)) || (rc = up2(c.ab, ab.data(), ab.size() * 4)) ||
+      (rc = up2(c.vb, vb.data(), vb.size() * 4)) ||
+      (rc = up2(c.state, c.sbuf.data(), c.sbuf.size() * sizeof(float2))))
+    return rc;
+  std::vector<int64_t> rows64(ng, 0);
+  if ((rc = up2(c.rows, rows64.data(), rows64.size() * 8))) return rc;
+  if (anharmonicity == 0.f) {
+    std::vector<float> arec(1 + N);
+    arec[0] = 0.f;
+    std::memcpy(arec.data() + 1, Anh, N * 4);
+    if ((rc = up2(c.anh, arec.data(), arec.size() * 4))) return rc;
+  }
+  if ((rc = c.D2.alloc((size_t)N2 * sizeof(float2))) || (rc = c.cs2.alloc((size_t)std::max(K2, 1) * sizeof(float4))))
+    return rc;
+  FrameSource src;
+  src.H_rec = c.H.as<float>(), src.H_stride = 1 + N2, src.shifte = 0.f, src.cut = cut, src.apply_shift = 0;
+  const float fac = trotter_factor(deltat, 1, m);
+  if ((rc = launch_doubles_tables(src, N, K2, c.ab.as<uint32_t>(), fac, anharmonicity,
+                                  anharmonicity == 0.f ? c.anh.as<float>() : nullptr, 0, 1,
+                                  c.D2.as<float2>(), c.cs2.as<float4>(), st)))
+    return rc;
+  DoublesArgs a;
+  a.N = N, a.K2 = K2, a.m = m, a.ab2 = c.ab.as<uint32_t>();
+  a.D2 = c.D2.as<float2>(), a.cs2 = c.cs2.as<float4>();
+  a.state = c.state.as<float2>(), a.vec_begin = c.vb.as<int>(), a.rows = c.rows.as<int64_t>();
+  if ((rc = launch_doubles_step(a, ng, st))) return rc;
+  NISE_HIP_CHECK(hipMemcpyAsync(c.sbuf.data(), c.state.p, c.sbuf.size() * sizeof(float2),
+                                hipMemcpyDeviceToHost, st));
+  NISE_HIP_CHECK(hipStreamSynchronize(st));
+  for (int v = 0; v < nvec; ++v)
+    for (int j = 0; j < N2; ++j) {
+      cr[v][j] = c.sbuf[(size_t)v * N2 + j].x;
+      ci[v][j] = c.sbuf[(size_t)v * N2 + j].y;
+    }
+  return NISE_OK;
+}
+
+int coupling_S_host(int N, const float *H, float *cr, float *ci, int nvec, int m, int sign,
+                    float deltat, float cut);
+
+}  // namespace nise
+
+using namespace nise;
+
+// src/propagate.c:902-1048
+extern "C" void propagate_vec_coupling_S_doubles(t_non *non, float *Hamiltonian_i, float *cr,
+                                                 float *ci, int m, float *Anh) {
+  float *r[1] = {cr}, *i[1] = {ci};
+  if (doubles_host(non->singles, Hamiltonian_i, r, i, 1, m, Anh, non->anharmonicity, non->deltat,
+                   non->couplingcut)) {
+    printf("propagate_vec_coupling_S_doubles: GPU propagation failed: %s\n", get_error());
+    exit(1);
+  }
+}
+
+// src/propagate.c:1476-1591, Coupling scheme (non->propagation == 1): the
+// doubles vectors fr and ft1r[0..tmax1) in one launch, the singles vectors
+// rightnr and rightrr[0..tmax1) with sign -1 in another.
+extern "C" void propagate_double_2DIR_control(t_non *non, float *Hamil_i_e, float **ft1r,
+                                              float **ft1i, float *fr, float *fi, float **rightrr,
+                                              float **rightri, float *rightnr, float *rightni,
+                                              int currentSample, int molPol, int t3, float *Anh) {
+  (void)currentSample, (void)molPol, (void)t3;
+  if (non->propagation != 1) {
+    printf("propagate_double_2DIR_control: propagation scheme %d is not provided by the GPU "
+           "library\n", non->propagation);
+    exit(1);
+  }
+  const int T1 = non->tmax1, N = non->singles;
+  std::vector<float *> r(T1 + 1), i(T1 + 1);
+  r[0] = fr, i[0] = fi;
+  for (int t = 0; t < T1; ++t) r[t + 1] = ft1r[t], i[t + 1] = ft1i[t];
+  if (doubles_host(N, Hamil_i_e, r.data(), i.data(), T1 + 1, non->ts, Anh, non->anharmonicity,
+                   non->deltat, non->couplingcut)) {
+    printf("propagate_double_2DIR_control: GPU propagation failed: %s\n", get_error());
+    exit(1);
+  }
+  std::vector<float> sr((size_t)(T1 + 1) * N), si((size_t)(T1 + 1) * N);
+  std::memcpy(sr.data(), rightnr, N * 4);
+  std::memcpy(si.data(), rightni, N * 4);
+  for (int t = 0; t < T1; ++t) {
+    std::memcpy(sr.data() + (size_t)(t + 1) * N, rightrr[t], N * 4);
+    std::memcpy(si.data() + (size_t)(t + 1) * N, rightri[t], N * 4);
+  }
+  if (coupling_S_host(N, Hamil_i_e, sr.data(), si.data(), T1 + 1, non->ts, -1, non->deltat,
+                      non->couplingcut)) {
+    printf("propagate_double_2DIR_control: GPU propagation failed: %s\n", get_error());
+    exit(1);
+  }
+  std::memcpy(rightnr, sr.data(), N * 4);
+  std::memcpy(rightni, si.data(), N * 4);
+  for (int t = 0; t < T1; ++t) {
+    std::memcpy(rightrr[t], sr.data() + (size_t)(t + 1) * N, N * 4);
+    std::memcpy(rightri[t], si.data() + (size_t)(t + 1) * N, N * 4);
+  }
+}

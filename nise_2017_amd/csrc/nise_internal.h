@@ -84,6 +84,39 @@ struct FrameSource {
   int apply_shift = 1;
 };
 
+// ---------------------------------------------------------------------------
+// Frame element access.
+// ---------------------------------------------------------------------------
+__device__ inline float raw_diag(const FrameSource &s, int N, int64_t f, int a) {
+  const float *rec = s.H_rec + f * s.H_stride + 1;
+  return s.coupling_type ? rec[a] : rec[sindex(a, a, N)];
+}
+
+__device__ inline float raw_coupling(const FrameSource &s, int N, int64_t f, int a, int b) {
+  const float *rec = s.coupling_type ? (s.C_rec + 1) : (s.H_rec + f * s.H_stride + 1);
+  return rec[sindex(a, b, N)];
+}
+
+// Diagonal element as read_Hamiltonian leaves it: minus shifte
+// (src/read_trajectory.c:153,176,337), then the single-site shifts
+// (src/hamshift.c:59-66) in list order.
+__device__ inline float site_energy(const FrameSource &s, int N, int64_t f, int a) {
+  float h = __fsub_rn(raw_diag(s, N, f, a), s.shifte);
+  if (s.apply_shift) {
+    for (int i = 0; i < s.nshift; ++i)
+      if (s.shift_site[i] == a) h = __fadd_rn(__fmul_rn(h, s.shift_ab[2 * i]), s.shift_ab[2 * i + 1]);
+  }
+  return h;
+}
+
+// Coupling as kept by read_He (|J| < cut -> 0, src/read_trajectory.c:305-311)
+// and the propagator (|J| > cut, src/propagate.c:853).  Returns 0 when not kept.
+__device__ inline float kept_coupling(const FrameSource &s, int N, int64_t f, int a, int b) {
+  float J = raw_coupling(s, N, f, a, b);
+  if (fabsf(J) < s.cut) J = 0.f;
+  return (fabsf(J) > s.cut) ? J : 0.f;
+}
+
 // Reference time factor f = deltat*icm2ifs*twoPi*sign/m (src/propagate.c:836).
 inline float trotter_factor(float deltat, int sign, int m) {
   volatile float f = deltat * kIcm2ifs;
@@ -111,6 +144,9 @@ struct SlotDesc {
   int out;    // state buffer row (init 1 or write back), -1 none
   int grp;    // response group leader: number of consecutive slots (same fs)
               // whose responses it sums; 0 for members
+  int hist;   // history row: psi at frames fs .. fs+nresp-1 (after the
+              // pending half-phase, before the next) goes to rows hist+rel
+              // of ConveyorArgs::hist; -1 none
 };
 
 struct ConveyorArgs {
@@ -131,6 +167,7 @@ struct ConveyorArgs {
   int writeback = 0;
   const int2 *steps = nullptr;   // rotation steps (build_steps)
   int nsteps = 0;
+  float2 *hist = nullptr;        // [rows][N] recorded states
 };
 
 int launch_conveyor(const ConveyorArgs &a, int vp, int block, hipStream_t st);
@@ -152,6 +189,22 @@ struct PlainArgs {
   uint32_t *ws_ab = nullptr; // [N(N-1)/2] workspace
 };
 int launch_plain_sample(const PlainArgs &a, hipStream_t st);
+
+// Doubles propagation (doubles.hip).
+struct DoublesArgs {
+  int N = 0, K2 = 0, m = 1;
+  const uint32_t *ab2 = nullptr;   // kept couplings, lexicographic
+  const float2 *D2 = nullptr;      // [rows][N2]
+  const float4 *cs2 = nullptr;     // [rows][K2] (co, si, co2, si2)
+  float2 *state = nullptr;         // [vectors][N2]
+  const int *vec_begin = nullptr;  // [groups + 1]
+  const int64_t *rows = nullptr;   // [groups] table row of each group
+};
+int launch_doubles_tables(const FrameSource &src, int N, int K2, const uint32_t *ab2, float fac,
+                          float anh_fixed, const float *anh_rec, int64_t f_lo, int64_t nfr,
+                          float2 *D2, float4 *cs2, hipStream_t st);
+int launch_doubles_step(const DoublesArgs &a, int ngroups, hipStream_t st);
+int doubles_np(int N);
 
 // Library-wide stream and device setup.
 int ensure_device();

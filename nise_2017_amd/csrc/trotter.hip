@@ -30,39 +30,6 @@
 namespace nise {
 
 // ---------------------------------------------------------------------------
-// Frame element access.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ float raw_diag(const FrameSource &s, int N, int64_t f, int a) {
-  const float *rec = s.H_rec + f * s.H_stride + 1;
-  return s.coupling_type ? rec[a] : rec[sindex(a, a, N)];
-}
-
-__device__ __forceinline__ float raw_coupling(const FrameSource &s, int N, int64_t f, int a, int b) {
-  const float *rec = s.coupling_type ? (s.C_rec + 1) : (s.H_rec + f * s.H_stride + 1);
-  return rec[sindex(a, b, N)];
-}
-
-// Diagonal element as read_Hamiltonian leaves it: minus shifte
-// (src/read_trajectory.c:153,176,337), then the single-site shifts
-// (src/hamshift.c:59-66) in list order.
-__device__ __forceinline__ float site_energy(const FrameSource &s, int N, int64_t f, int a) {
-  float h = __fsub_rn(raw_diag(s, N, f, a), s.shifte);
-  if (s.apply_shift) {
-    for (int i = 0; i < s.nshift; ++i)
-      if (s.shift_site[i] == a) h = __fadd_rn(__fmul_rn(h, s.shift_ab[2 * i]), s.shift_ab[2 * i + 1]);
-  }
-  return h;
-}
-
-// Coupling as kept by read_He (|J| < cut -> 0, src/read_trajectory.c:305-311)
-// and the propagator (|J| > cut, src/propagate.c:853).  Returns 0 when not kept.
-__device__ __forceinline__ float kept_coupling(const FrameSource &s, int N, int64_t f, int a, int b) {
-  float J = raw_coupling(s, N, f, a, b);
-  if (fabsf(J) < s.cut) J = 0.f;
-  return (fabsf(J) > s.cut) ? J : 0.f;
-}
-
-// ---------------------------------------------------------------------------
 // Union of kept couplings over a frame range (one byte per a<b pair).
 // ---------------------------------------------------------------------------
 __global__ void union_mask_kernel(FrameSource s, int N, int64_t f_lo, int64_t nfr, uint8_t *mask) {
@@ -228,7 +195,7 @@ __global__ void __launch_bounds__(BLOCK) conveyor_kernel(ConveyorArgs A) {
   __syncthreads();
 
   const bool has = v < ns;
-  SlotDesc sd = has ? sls[v] : SlotDesc{0, 0, 0, 0, 0, -1, 0};
+  SlotDesc sd = has ? sls[v] : SlotDesc{0, 0, 0, 0, 0, -1, 0, -1};
   int64_t F0 = INT64_MAX, F1 = INT64_MIN;
   for (int i = 0; i < ns; ++i) {
     const SlotDesc d = sls[i];
@@ -291,6 +258,7 @@ __global__ void __launch_bounds__(BLOCK) conveyor_kernel(ConveyorArgs A) {
           const float m = mu[i];
           rr = __fadd_rn(rr, __fmul_rn(m, p.x));
           ri = __fadd_rn(ri, __fmul_rn(m, p.y));
+          if (sd.hist >= 0) A.hist[((int64_t)sd.hist + rel) * N + i] = p;
         }
         if (prop) p = cmul_diag(p, Dcur[i]);
         psi[i * VP + v] = p;
@@ -400,11 +368,491 @@ __global__ void __launch_bounds__(BLOCK) conveyor_kernel(ConveyorArgs A) {
     for (int i = t; i < A.tmax_acc; i += BLOCK) A.partial[(int64_t)g * A.tmax_acc + i] = acc[i];
 }
 
+// ---------------------------------------------------------------------------
+// Single-wave conveyor.  A workgroup is one wave: LDS operations of one wave
+// execute in order, so no barrier separates dependent layers.  A step is a
+// chunk of one layer of at most (64/VP)*kBatch rotations; each lane applies
+// up to kBatch of them to its slot, issuing all of their LDS reads before
+// any arithmetic so that one LDS round trip serves kBatch rotations.
+// ---------------------------------------------------------------------------
+constexpr int kBatch = 4;
+constexpr int kRing1 = 4;
+
+__device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
+
+template <int VP>
+__global__ void __launch_bounds__(64) conveyor_w1_kernel(ConveyorArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = A.N;
+  constexpr int kRows = 64 / VP;
+  const size_t o_D = (size_t)N * VP * 8;
+  const size_t o_mu = o_D + (size_t)2 * N * 8;
+  const size_t o_acc = (o_mu + (size_t)3 * N * 4 + 15) & ~(size_t)15;
+  const size_t o_steps = o_acc + (size_t)A.tmax_acc * 16;
+  const size_t o_sls = o_steps + (size_t)A.nsteps * 8;
+  float2 *psi = reinterpret_cast<float2 *>(smem);
+  float2 *Dsh = reinterpret_cast<float2 *>(smem + o_D);
+  float *mush = reinterpret_cast<float *>(smem + o_mu);
+  double2 *acc = reinterpret_cast<double2 *>(smem + o_acc);
+  int2 *steps = reinterpret_cast<int2 *>(smem + o_steps);
+  SlotDesc *sls = reinterpret_cast<SlotDesc *>(smem + o_sls);
+
+  const int g = blockIdx.x;
+  const int s0 = A.wg_slot_begin[g];
+  const int ns = A.wg_slot_begin[g + 1] - s0;
+  const int t = threadIdx.x;
+  const int v = t % VP, q = t / VP;
+  const int nsteps = A.nsteps;
+  const int K = A.K;
+  for (int i = t; i < nsteps; i += 64) steps[i] = A.steps[i];
+  for (int i = t; i < ns; i += 64) sls[i] = A.slots[s0 + i];
+  for (int i = t; i < A.tmax_acc; i += 64) acc[i] = make_double2(0.0, 0.0);
+  __syncthreads();
+  if (ns == 0) return;
+  const bool has = v < ns;
+  const SlotDesc sd = has ? sls[v] : SlotDesc{0, 0, 0, 0, 0, -1, 0, -1};
+  int64_t F0 = INT64_MAX, F1 = INT64_MIN;
+  for (int i = 0; i < ns; ++i) {
+    const SlotDesc d = sls[i];
+    const int len = d.nresp > d.nprop ? d.nresp : d.nprop;
+    F0 = d.fs < F0 ? d.fs : F0;
+    F1 = (int64_t)d.fs + len > F1 ? (int64_t)d.fs + len : F1;
+  }
+  {
+    const int64_t row = F0 - A.f_lo;
+    const float *mu = A.mu_rec + (A.mu_stride ? F0 * A.mu_stride : 0) + 1;
+    for (int i = t; i < N; i += 64) Dsh[(F0 & 1) * N + i] = A.diag[row * N + i];
+    for (int i = t; i < 3 * N; i += 64) mush[i] = mu[i];
+  }
+  // operand ring: kRing1 steps x kBatch rounds
+  int2 rst[kRing1];
+  float2 rc[kRing1][kBatch];
+  uint32_t rab[kRing1][kBatch];
+  {
+    const float2 *cs0 = A.cs + (F0 - A.f_lo) * K;
+#pragma unroll
+    for (int j = 0; j < kRing1; ++j) {
+      const int2 st = nsteps ? A.steps[j < nsteps ? j : 0] : make_int2(0, 0);
+      rst[j] = st;
+      const int e = st.y & 0x7fffffff;
+#pragma unroll
+      for (int i = 0; i < kBatch; ++i) {
+        const int r = max(0, min(st.x + i * kRows + q, e - 1));
+        rc[j][i] = nsteps ? cs0[r] : make_float2(1.f, 0.f);
+        rab[j][i] = nsteps ? A.ab[r] : 0u;
+      }
+    }
+  }
+  __syncthreads();
+
+  for (int64_t f = F0; f < F1; ++f) {
+    const int64_t row = f - A.f_lo;
+    const int64_t rel = f - sd.fs;
+    const bool starts = has && rel == 0;
+    const bool prev_prop = has && rel >= 1 && rel - 1 < sd.nprop;
+    const bool resp = has && rel >= 0 && rel < sd.nresp;
+    const bool prop = has && rel >= 0 && rel < sd.nprop;
+    const float2 *Dcur = Dsh + (f & 1) * N;
+    const float2 *Dprev = Dsh + ((f + 1) & 1) * N;
+    const float *mu = mush + sd.x * N;
+    float rr = 0.f, ri = 0.f;
+    if (starts || prev_prop || resp || prop) {
+      for (int i = q; i < N; i += kRows) {
+        float2 p;
+        if (starts) {
+          p = (sd.init == 0) ? make_float2(mu[i], 0.f) : A.state[(int64_t)sd.out * N + i];
+        } else {
+          p = psi[i * VP + v];
+          if (prev_prop) p = cmul_diag(p, Dprev[i]);
+        }
+        if (resp) {
+          const float m = mu[i];
+          rr = __fadd_rn(rr, __fmul_rn(m, p.x));
+          ri = __fadd_rn(ri, __fmul_rn(m, p.y));
+          if (sd.hist >= 0) A.hist[((int64_t)sd.hist + rel) * N + i] = p;
+        }
+        if (prop) p = cmul_diag(p, Dcur[i]);
+        psi[i * VP + v] = p;
+      }
+    }
+    if (A.tmax_acc > 0) {
+#pragma unroll
+      for (int off = VP; off < 64; off <<= 1) {
+        rr += __shfl_xor(rr, off);
+        ri += __shfl_xor(ri, off);
+      }
+      // group leader (x = 0 slot of a sample) sums its polarisations in order
+      float gr = rr, gi = ri;
+      for (int gg = 1; gg < 3; ++gg) {
+        const float xr = __shfl(rr, (t + gg) & 63), xi = __shfl(ri, (t + gg) & 63);
+        if (gg < sd.grp) gr += xr, gi += xi;
+      }
+      if (q == 0 && resp && sd.grp > 0) {
+        acc[rel].x += (double)gr;
+        acc[rel].y += (double)gi;
+      }
+    }
+    compiler_fence();
+
+    const int64_t fn = f + 1 < F1 ? f + 1 : f;
+    float2 dn[4];
+    float mn[12];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int i = min(t + k * 64, N - 1);
+      dn[k] = A.diag[(fn - A.f_lo) * N + i];
+      const float *mup = A.mu_rec + (A.mu_stride ? fn * A.mu_stride : 0) + 1;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) mn[3 * k + x] = mup[x * N + i];
+    }
+
+    for (int sub = 0; sub < A.m; ++sub) {
+      if (sub > 0) {
+        if (prop)
+          for (int i = q; i < N; i += kRows)
+            psi[i * VP + v] = cmul_diag(cmul_diag(psi[i * VP + v], Dcur[i]), Dcur[i]);
+        compiler_fence();
+      }
+      const float2 *csn = A.cs + ((sub + 1 < A.m) ? row : (fn - A.f_lo)) * K;
+      const float2 *csc = A.cs + row * K;
+      for (int b0 = 0; b0 < nsteps; b0 += kRing1) {
+#pragma unroll
+        for (int j = 0; j < kRing1; ++j) {
+          const int sidx = b0 + j;
+          const int sx = __builtin_amdgcn_readfirstlane(rst[j].x);
+          const int se = __builtin_amdgcn_readfirstlane(rst[j].y) & 0x7fffffff;
+          float2 pa[kBatch], pb[kBatch];
+          bool on[kBatch];
+#pragma unroll
+          for (int i = 0; i < kBatch; ++i) {
+            on[i] = prop && sx + i * kRows + q < se;
+            const uint32_t pk = rab[j][i];
+            const int a = (int)(pk & 0xffffu), b = (int)(pk >> 16);
+            if (on[i]) pa[i] = psi[a * VP + v], pb[i] = psi[b * VP + v];
+          }
+#pragma unroll
+          for (int i = 0; i < kBatch; ++i) {
+            if (on[i]) {
+              const uint32_t pk = rab[j][i];
+              const int a = (int)(pk & 0xffffu), b = (int)(pk >> 16);
+              rotate_pair(pa[i], pb[i], rc[j][i].x, rc[j][i].y);
+              psi[a * VP + v] = pa[i];
+              psi[b * VP + v] = pb[i];
+            }
+          }
+          compiler_fence();
+          {
+            const int sp = sidx + kRing1;
+            const bool nxt = sp >= nsteps;
+            const int2 stp = steps[nxt ? sp - nsteps : sp];
+            const int e = stp.y & 0x7fffffff;
+            const float2 *src = nxt ? csn : csc;
+            rst[j] = stp;
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+              const int rp = max(0, min(stp.x + i * kRows + q, e - 1));
+              rc[j][i] = src[rp];
+              rab[j][i] = A.ab[rp];
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int i = t + k * 64;
+      if (i < N) {
+        Dsh[(fn & 1) * N + i] = dn[k];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) mush[x * N + i] = mn[3 * k + x];
+      }
+    }
+    for (int i = t + 256; i < N; i += 64) {
+      Dsh[(fn & 1) * N + i] = A.diag[(fn - A.f_lo) * N + i];
+      const float *mup = A.mu_rec + (A.mu_stride ? fn * A.mu_stride : 0) + 1;
+      for (int x = 0; x < 3; ++x) mush[x * N + i] = mup[x * N + i];
+    }
+    compiler_fence();
+  }
+  if (A.writeback && has && sd.out >= 0 && sd.nprop > 0) {
+    const int64_t row = (int64_t)sd.fs + sd.nprop - 1 - A.f_lo;
+    for (int i = q; i < N; i += kRows)
+      A.state[(int64_t)sd.out * N + i] = cmul_diag(psi[i * VP + v], A.diag[row * N + i]);
+  }
+  if (A.tmax_acc > 0)
+    for (int i = t; i < A.tmax_acc; i += 64) A.partial[(int64_t)g * A.tmax_acc + i] = acc[i];
+}
+
+// ---------------------------------------------------------------------------
+// Single-wave conveyor with S slots per lane.  Lane (row q, group l) applies
+// rotation x + q of each step to slots l*S .. l*S+S-1, which sit next to
+// each other in LDS (psi[site][slot]), so one 8*S-byte access moves all of
+// them.  A step covers up to 64*S/VP rotations of one layer.  No barriers:
+// the workgroup is one wave and its LDS operations execute in order.
+// ---------------------------------------------------------------------------
+template <int S> struct SlotVec;
+template <> struct SlotVec<2> { using T = float4; };
+template <> struct SlotVec<4> { using T = float4; };
+
+template <int S>
+__device__ __forceinline__ void load_slots(const float2 *p, float2 (&o)[S]) {
+  const float4 *p4 = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+  for (int k = 0; k < S / 2; ++k) {
+    const float4 w = p4[k];
+    o[2 * k] = make_float2(w.x, w.y);
+    o[2 * k + 1] = make_float2(w.z, w.w);
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void store_slots(float2 *p, const float2 (&o)[S]) {
+  float4 *p4 = reinterpret_cast<float4 *>(p);
+#pragma unroll
+  for (int k = 0; k < S / 2; ++k) p4[k] = make_float4(o[2 * k].x, o[2 * k].y, o[2 * k + 1].x, o[2 * k + 1].y);
+}
+
+constexpr int kRingS = 8;
+
+template <int VP, int S>
+__global__ void __launch_bounds__(64) conveyor_s_kernel(ConveyorArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = A.N;
+  constexpr int LR = VP / S;        // lanes per rotation row
+  constexpr int R = 64 / LR;        // rotation rows per step
+  const size_t o_D = (size_t)N * VP * 8;
+  const size_t o_mu = o_D + (size_t)2 * N * 8;
+  const size_t o_acc = (o_mu + (size_t)3 * N * 4 + 15) & ~(size_t)15;
+  const size_t o_red = o_acc + (size_t)A.tmax_acc * 16;
+  const size_t o_steps = o_red + (size_t)VP * 8;
+  const size_t o_sls = o_steps + (size_t)A.nsteps * 8;
+  float2 *psi = reinterpret_cast<float2 *>(smem);
+  float2 *Dsh = reinterpret_cast<float2 *>(smem + o_D);
+  float *mush = reinterpret_cast<float *>(smem + o_mu);
+  double2 *acc = reinterpret_cast<double2 *>(smem + o_acc);
+  float2 *red = reinterpret_cast<float2 *>(smem + o_red);
+  int2 *steps = reinterpret_cast<int2 *>(smem + o_steps);
+  SlotDesc *sls = reinterpret_cast<SlotDesc *>(smem + o_sls);
+
+  const int g = blockIdx.x;
+  const int s0 = A.wg_slot_begin[g];
+  const int ns = A.wg_slot_begin[g + 1] - s0;
+  const int t = threadIdx.x;
+  const int l = t % LR, q = t / LR;
+  const int v0 = l * S;                 // first slot of this lane
+  const int nsteps = A.nsteps;
+  const int K = A.K;
+  for (int i = t; i < nsteps; i += 64) steps[i] = A.steps[i];
+  for (int i = t; i < ns; i += 64) sls[i] = A.slots[s0 + i];
+  for (int i = t; i < A.tmax_acc; i += 64) acc[i] = make_double2(0.0, 0.0);
+  __syncthreads();
+  if (ns == 0) return;
+  SlotDesc sd[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) sd[k] = (v0 + k < ns) ? sls[v0 + k] : SlotDesc{0, 0, 0, 0, 0, -1, 0, -1};
+  int64_t F0 = INT64_MAX, F1 = INT64_MIN;
+  for (int i = 0; i < ns; ++i) {
+    const SlotDesc d = sls[i];
+    const int len = d.nresp > d.nprop ? d.nresp : d.nprop;
+    F0 = d.fs < F0 ? d.fs : F0;
+    F1 = (int64_t)d.fs + len > F1 ? (int64_t)d.fs + len : F1;
+  }
+  {
+    const int64_t row = F0 - A.f_lo;
+    const float *mu = A.mu_rec + (A.mu_stride ? F0 * A.mu_stride : 0) + 1;
+    for (int i = t; i < N; i += 64) Dsh[(F0 & 1) * N + i] = A.diag[row * N + i];
+    for (int i = t; i < 3 * N; i += 64) mush[i] = mu[i];
+  }
+  int2 rst[kRingS];
+  float2 rc[kRingS];
+  uint32_t rab[kRingS];
+  {
+    const float2 *cs0 = A.cs + (F0 - A.f_lo) * K;
+#pragma unroll
+    for (int j = 0; j < kRingS; ++j) {
+      const int2 st = nsteps ? A.steps[j < nsteps ? j : 0] : make_int2(0, 0);
+      const int r = max(0, min(st.x + q, (st.y & 0x7fffffff) - 1));
+      rst[j] = st;
+      rc[j] = nsteps ? cs0[r] : make_float2(1.f, 0.f);
+      rab[j] = nsteps ? A.ab[r] : 0u;
+    }
+  }
+  __syncthreads();
+
+  for (int64_t f = F0; f < F1; ++f) {
+    const int64_t row = f - A.f_lo;
+    bool starts[S], pprop[S], resp[S], prop[S];
+    bool any_site = false, any_prop = false;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const bool has = v0 + k < ns;
+      const int64_t rel = f - sd[k].fs;
+      starts[k] = has && rel == 0;
+      pprop[k] = has && rel >= 1 && rel - 1 < sd[k].nprop;
+      resp[k] = has && rel >= 0 && rel < sd[k].nresp;
+      prop[k] = has && rel >= 0 && rel < sd[k].nprop;
+      any_site |= starts[k] || pprop[k] || resp[k] || prop[k];
+      any_prop |= prop[k];
+    }
+    const float2 *Dcur = Dsh + (f & 1) * N;
+    const float2 *Dprev = Dsh + ((f + 1) & 1) * N;
+    float rr[S], ri[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) rr[k] = 0.f, ri[k] = 0.f;
+    if (any_site) {
+      for (int i = q; i < N; i += R) {
+        float2 p[S];
+        load_slots<S>(psi + i * VP + v0, p);
+        const float2 dp = Dprev[i], dc = Dcur[i];
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+          const float m = mush[sd[k].x * N + i];
+          if (starts[k]) p[k] = (sd[k].init == 0) ? make_float2(m, 0.f) : A.state[(int64_t)sd[k].out * N + i];
+          else if (pprop[k]) p[k] = cmul_diag(p[k], dp);
+          if (resp[k]) {
+            rr[k] = __fadd_rn(rr[k], __fmul_rn(m, p[k].x));
+            ri[k] = __fadd_rn(ri[k], __fmul_rn(m, p[k].y));
+            if (sd[k].hist >= 0) A.hist[((int64_t)sd[k].hist + (f - sd[k].fs)) * N + i] = p[k];
+          }
+          if (prop[k]) p[k] = cmul_diag(p[k], dc);
+        }
+        store_slots<S>(psi + i * VP + v0, p);
+      }
+    }
+    if (A.tmax_acc > 0) {
+#pragma unroll
+      for (int k = 0; k < S; ++k) {
+#pragma unroll
+        for (int off = LR; off < 64; off <<= 1) {
+          rr[k] += __shfl_xor(rr[k], off);
+          ri[k] += __shfl_xor(ri[k], off);
+        }
+      }
+      if (q == 0) {
+#pragma unroll
+        for (int k = 0; k < S; ++k) red[v0 + k] = make_float2(rr[k], ri[k]);
+      }
+      compiler_fence();
+      // one lane per slot: group leaders (x = 0 slot of a sample) sum their
+      // polarisations in order; groups start on distinct frames
+      if (t < ns) {
+        const SlotDesc d = sls[t];
+        const int64_t rel = f - d.fs;
+        if (d.grp > 0 && rel >= 0 && rel < d.nresp) {
+          float gr = 0.f, gi = 0.f;
+          for (int gg = 0; gg < d.grp; ++gg) gr += red[t + gg].x, gi += red[t + gg].y;
+          acc[rel].x += (double)gr;
+          acc[rel].y += (double)gi;
+        }
+      }
+    }
+    compiler_fence();
+
+    const int64_t fn = f + 1 < F1 ? f + 1 : f;
+    float2 dn[4];
+    float mn[12];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int i = min(t + k * 64, N - 1);
+      dn[k] = A.diag[(fn - A.f_lo) * N + i];
+      const float *mup = A.mu_rec + (A.mu_stride ? fn * A.mu_stride : 0) + 1;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) mn[3 * k + x] = mup[x * N + i];
+    }
+
+    for (int sub = 0; sub < A.m; ++sub) {
+      if (sub > 0) {
+        if (any_prop)
+          for (int i = q; i < N; i += R) {
+            float2 p[S];
+            load_slots<S>(psi + i * VP + v0, p);
+            const float2 dc = Dcur[i];
+#pragma unroll
+            for (int k = 0; k < S; ++k)
+              if (prop[k]) p[k] = cmul_diag(cmul_diag(p[k], dc), dc);
+            store_slots<S>(psi + i * VP + v0, p);
+          }
+        compiler_fence();
+      }
+      const float2 *csn = A.cs + ((sub + 1 < A.m) ? row : (fn - A.f_lo)) * K;
+      const float2 *csc = A.cs + row * K;
+      for (int b0 = 0; b0 < nsteps; b0 += kRingS) {
+#pragma unroll
+        for (int j = 0; j < kRingS; ++j) {
+          const int sidx = b0 + j;
+          const int sx = __builtin_amdgcn_readfirstlane(rst[j].x);
+          const int se = __builtin_amdgcn_readfirstlane(rst[j].y) & 0x7fffffff;
+          if (any_prop && sx + q < se) {
+            const uint32_t pk = rab[j];
+            const int a = (int)(pk & 0xffffu), b = (int)(pk >> 16);
+            float2 pa[S], pb[S];
+            load_slots<S>(psi + a * VP + v0, pa);
+            load_slots<S>(psi + b * VP + v0, pb);
+            const float co = rc[j].x, si = rc[j].y;
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+              float2 na = pa[k], nb = pb[k];
+              rotate_pair(na, nb, co, si);
+              // slots that do not propagate in this frame keep their values
+              pa[k] = prop[k] ? na : pa[k];
+              pb[k] = prop[k] ? nb : pb[k];
+            }
+            store_slots<S>(psi + a * VP + v0, pa);
+            store_slots<S>(psi + b * VP + v0, pb);
+          }
+          compiler_fence();
+          {
+            const int sp = sidx + kRingS;
+            const bool nxt = sp >= nsteps;
+            const int2 stp = steps[nxt ? sp - nsteps : sp];
+            const int rp = max(0, min(stp.x + q, (stp.y & 0x7fffffff) - 1));
+            const float2 *src = nxt ? csn : csc;
+            rst[j] = stp;
+            rc[j] = src[rp];
+            rab[j] = A.ab[rp];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int i = t + k * 64;
+      if (i < N) {
+        Dsh[(fn & 1) * N + i] = dn[k];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) mush[x * N + i] = mn[3 * k + x];
+      }
+    }
+    for (int i = t + 256; i < N; i += 64) {
+      Dsh[(fn & 1) * N + i] = A.diag[(fn - A.f_lo) * N + i];
+      const float *mup = A.mu_rec + (A.mu_stride ? fn * A.mu_stride : 0) + 1;
+      for (int x = 0; x < 3; ++x) mush[x * N + i] = mup[x * N + i];
+    }
+    compiler_fence();
+  }
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const SlotDesc d = sd[k];
+    if (A.writeback && v0 + k < ns && d.out >= 0 && d.nprop > 0) {
+      const int64_t row = (int64_t)d.fs + d.nprop - 1 - A.f_lo;
+      for (int i = q; i < N; i += R)
+        A.state[(int64_t)d.out * N + i] = cmul_diag(psi[i * VP + v0 + k], A.diag[row * N + i]);
+    }
+  }
+  if (A.tmax_acc > 0)
+    for (int i = t; i < A.tmax_acc; i += 64) A.partial[(int64_t)g * A.tmax_acc + i] = acc[i];
+}
+
 // Steps for a given shape: each layer split into chunks of at most
 // block/vp rotations, the layer's last chunk flagged (sign bit of y), the
 // list padded with empty steps to a multiple of the operand ring length.
 void build_steps(const Schedule &s, int vp, int block, std::vector<int2> &steps) {
-  const int rpp = block / vp;
+  // block > 0: multi-wave kernel (block threads) or, for 64, the single-wave
+  // kernel; block < 0: single-wave kernel with -block slots per lane
+  const bool w1 = block == 64;
+  const int sl = block < 0 ? -block : 0;
+  const int rpp = sl ? 64 * sl / vp : w1 ? (64 / vp) * kBatch : block / vp;
+  const int ring = sl ? kRingS : w1 ? kRing1 : kRing;
   steps.clear();
   for (int l = 0; l < s.nlayers(); ++l) {
     const int r0 = s.layer_off[l], r1 = s.layer_off[l + 1];
@@ -416,11 +864,42 @@ void build_steps(const Schedule &s, int vp, int block, std::vector<int2> &steps)
       steps.push_back(st);
     }
   }
-  while (!steps.empty() && steps.size() % kRing) steps.push_back(make_int2(0, 0));
+  while (!steps.empty() && steps.size() % ring) steps.push_back(make_int2(0, 0));
 }
 
 size_t conveyor_smem(int N, int vp, int block, int tmax_acc, int nsteps) {
-  return ConvSmem::bytes(N, vp, block, tmax_acc, nsteps);
+  return ConvSmem::bytes(N, vp, block < 0 ? 64 : block, tmax_acc, nsteps);
+}
+
+// block == 64 selects the single-wave kernel, block = -S the single-wave
+// kernel with S slots per lane
+static const void *conveyor_fn(int vp, int block) {
+  if (block == -2) {
+    if (vp == 8) return (const void *)conveyor_s_kernel<8, 2>;
+    if (vp == 16) return (const void *)conveyor_s_kernel<16, 2>;
+    if (vp == 32) return (const void *)conveyor_s_kernel<32, 2>;
+    return nullptr;
+  }
+  if (block == -4) {
+    if (vp == 16) return (const void *)conveyor_s_kernel<16, 4>;
+    if (vp == 32) return (const void *)conveyor_s_kernel<32, 4>;
+    if (vp == 64) return (const void *)conveyor_s_kernel<64, 4>;
+    return nullptr;
+  }
+  if (block == 64) {
+    if (vp == 4) return (const void *)conveyor_w1_kernel<4>;
+    if (vp == 8) return (const void *)conveyor_w1_kernel<8>;
+    if (vp == 16) return (const void *)conveyor_w1_kernel<16>;
+    if (vp == 32) return (const void *)conveyor_w1_kernel<32>;
+    return nullptr;
+  }
+  const void *fn = nullptr;
+#define NISE_CONV(V, B) \
+  if (vp == V && block == B) fn = (const void *)conveyor_kernel<V, B>;
+  NISE_CONV(4, 256) NISE_CONV(8, 256) NISE_CONV(16, 256) NISE_CONV(32, 256) NISE_CONV(64, 256)
+  NISE_CONV(16, 512) NISE_CONV(32, 512) NISE_CONV(64, 512)
+#undef NISE_CONV
+  return fn;
 }
 
 int launch_conveyor(const ConveyorArgs &a, int vp, int block, hipStream_t st) {
@@ -430,21 +909,15 @@ int launch_conveyor(const ConveyorArgs &a, int vp, int block, hipStream_t st) {
               vp, a.nsteps, smem);
     return NISE_ERR_ARG;
   }
-  if (a.nsteps % kRing) return set_error("conveyor: step list not padded"), NISE_ERR_ARG;
+  if (a.nsteps % (block < 0 ? kRingS : block == 64 ? kRing1 : kRing))
+    return set_error("conveyor: step list not padded"), NISE_ERR_ARG;
   if (a.nwg == 0) return NISE_OK;
-  const void *fn = nullptr;
-#define NISE_CONV(V, B) \
-  if (vp == V && block == B) fn = (const void *)conveyor_kernel<V, B>;
-  NISE_CONV(4, 64) NISE_CONV(8, 64) NISE_CONV(16, 64)
-  NISE_CONV(4, 256) NISE_CONV(8, 256) NISE_CONV(16, 256) NISE_CONV(32, 256) NISE_CONV(64, 256)
-  NISE_CONV(16, 512) NISE_CONV(32, 512) NISE_CONV(64, 512)
-  NISE_CONV(32, 1024) NISE_CONV(64, 1024)
-#undef NISE_CONV
+  const void *fn = conveyor_fn(vp, block);
   if (!fn) return set_error("conveyor: unsupported shape vp=%d block=%d", vp, block), NISE_ERR_ARG;
   NISE_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
   ConveyorArgs args = a;
   void *kargs[] = {&args};
-  NISE_HIP_CHECK(hipLaunchKernel(fn, dim3(a.nwg), dim3(block), kargs, smem, st));
+  NISE_HIP_CHECK(hipLaunchKernel(fn, dim3(a.nwg), dim3(block < 0 ? 64 : block), kargs, smem, st));
   return NISE_OK;
 }
 
