@@ -205,6 +205,23 @@ int nise_absorption_batched(const t_non *non, const float *H_rec,
                             const float *coupling_rec, int where,
                             int kernel, float *re_S, float *im_S);
 
+/* Batched 2DIR (src/calc_2DIR.c:184-506 over all samples in
+ * [non->begin, non->end) and the 21 molecular polarisation items).  Uses
+ * non->singles, tmax1, tmax3 (tmax2 must be 0), sample, begin, end (0: as
+ * many samples as the trajectory holds), ts, deltat, couplingcut, min1 and
+ * max1 (shifte = (min1+max1)/2), lifetime, anharmonicity, technique
+ * ("2DIR", "GBIR", "SEIR", "EAIR", "noEAIR"), hamiltonian ("Full" or
+ * "Coupling"); propagation must be 1.  anh_rec (1 + N words per frame) and
+ * over_rec (1 + 3N words per frame) are read when anharmonicity == 0 and
+ * the technique has the EA pathway.  R12 (host, 12 x tmax3 x tmax1, t1
+ * fastest) receives the sums over samples and items of rrIpar, riIpar,
+ * rrIIpar, riIIpar, then the per and cro planes in the same order,
+ * un-normalised: what the reference reduces before print2D. */
+int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H_stride,
+                      const float *mu_rec, int64_t mu_stride, int64_t nframes,
+                      const float *coupling_rec, const float *anh_rec,
+                      const float *over_rec, int where, float *R12);
+
 /* Last error message of the calling thread (static storage). */
 const char *nise_last_error(void);
 

@@ -114,6 +114,8 @@ def lib():
     L.nise_abs_plan_sync.argtypes = [_vp]
     L.nise_abs_plan_destroy.argtypes = [_vp]
     L.nise_abs_plan_destroy.restype = None
+    L.nise_2dir_batched.argtypes = [P, _vp, C.c_int64, _vp, C.c_int64, C.c_int64, _vp, _vp, _vp,
+                                    C.c_int, _fp]
     for name in ("propagate_vec_coupling_S",):
         getattr(L, name).argtypes = [P, _fp, _fp, _fp, C.c_int, C.c_int]
         getattr(L, name).restype = None
@@ -193,3 +195,21 @@ class AbsPlan:
 
     def __del__(self):
         self.close()
+
+
+def twodir_batched(non, H, mu, nframes=None, coupling_rec=None, anh_rec=None, over_rec=None,
+                   where=None):
+    """nise_2dir_batched: the 12 response planes (12, tmax3, tmax1) float32
+    summed over samples and polarisation items (un-normalised).  Records
+    are numpy arrays (host) or contiguous device tensors (where inferred)."""
+    if where is None:
+        where = NISE_DEVICE if hasattr(H, "data_ptr") and H.is_cuda else NISE_HOST
+    if where == NISE_HOST:
+        for a in (H, mu, coupling_rec, anh_rec, over_rec):
+            if a is not None and a.dtype != np.float32:
+                raise ValueError("records must be float32")
+    R = np.zeros((12, non.tmax3, non.tmax1), np.float32)
+    nfr = H.shape[0] if nframes is None else nframes
+    check(lib().nise_2dir_batched(C.byref(non), ptr(H), H.shape[1], ptr(mu), mu.shape[1], nfr,
+                                  ptr(coupling_rec), ptr(anh_rec), ptr(over_rec), where, R))
+    return R

@@ -67,13 +67,6 @@ bool read_records(const char *name, int64_t rec_words, int64_t first, int64_t co
 
 static bool is_coupling(const t_non *non) { return strcmp(non->hamiltonian, "Coupling") == 0; }
 
-struct Pinned {
-  float *p = nullptr;
-  ~Pinned() {
-    if (p) (void)hipHostFree(p);
-  }
-  bool alloc(size_t words) { return hipHostMalloc((void **)&p, words * 4, hipHostMallocDefault) == hipSuccess; }
-};
 
 // ---------------------------------------------------------------------------
 // do_1DFFT on the device

@@ -210,4 +210,21 @@ int doubles_np(int N);
 int ensure_device();
 hipStream_t default_stream();
 
+// Host file helpers (driver.hip).
+// Read records [first, first+count) of a GROBIN file into dst; false if the
+// file holds fewer records.
+bool read_records(const char *name, int64_t rec_words, int64_t first, int64_t count, float *dst);
+
+// Pinned host staging buffer (hipHostMalloc) with RAII release.
+struct Pinned {
+  float *p = nullptr;
+  Pinned() = default;
+  Pinned(const Pinned &) = delete;
+  Pinned &operator=(const Pinned &) = delete;
+  ~Pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+  bool alloc(size_t words) { return hipHostMalloc((void **)&p, words * 4, hipHostMallocDefault) == hipSuccess; }
+};
+
 }  // namespace nise

@@ -358,11 +358,15 @@ __global__ void __launch_bounds__(BLOCK) conveyor_kernel(ConveyorArgs A) {
     __syncthreads();
   }
 
-  // write back: pending second half-phase of the last propagated frame
+  // write back; the pending second half-phase of the last propagated frame
+  // is still due only when the frame loop ended right after that frame
   if (A.writeback && has && sd.out >= 0 && sd.nprop > 0) {
     const int64_t row = (int64_t)sd.fs + sd.nprop - 1 - A.f_lo;
-    for (int i = q; i < N; i += kRows)
-      A.state[(int64_t)sd.out * N + i] = cmul_diag(psi[i * VP + v], A.diag[row * N + i]);
+    const bool due = (int64_t)sd.fs + sd.nprop >= F1;
+    for (int i = q; i < N; i += kRows) {
+      const float2 p = psi[i * VP + v];
+      A.state[(int64_t)sd.out * N + i] = due ? cmul_diag(p, A.diag[row * N + i]) : p;
+    }
   }
   if (A.tmax_acc > 0)
     for (int i = t; i < A.tmax_acc; i += BLOCK) A.partial[(int64_t)g * A.tmax_acc + i] = acc[i];
@@ -574,10 +578,15 @@ __global__ void __launch_bounds__(64) conveyor_w1_kernel(ConveyorArgs A) {
     }
     compiler_fence();
   }
+  // write back; the pending second half-phase of the last propagated frame
+  // is still due only when the frame loop ended right after that frame
   if (A.writeback && has && sd.out >= 0 && sd.nprop > 0) {
     const int64_t row = (int64_t)sd.fs + sd.nprop - 1 - A.f_lo;
-    for (int i = q; i < N; i += kRows)
-      A.state[(int64_t)sd.out * N + i] = cmul_diag(psi[i * VP + v], A.diag[row * N + i]);
+    const bool due = (int64_t)sd.fs + sd.nprop >= F1;
+    for (int i = q; i < N; i += kRows) {
+      const float2 p = psi[i * VP + v];
+      A.state[(int64_t)sd.out * N + i] = due ? cmul_diag(p, A.diag[row * N + i]) : p;
+    }
   }
   if (A.tmax_acc > 0)
     for (int i = t; i < A.tmax_acc; i += 64) A.partial[(int64_t)g * A.tmax_acc + i] = acc[i];
@@ -835,8 +844,11 @@ __global__ void __launch_bounds__(64) conveyor_s_kernel(ConveyorArgs A) {
     const SlotDesc d = sd[k];
     if (A.writeback && v0 + k < ns && d.out >= 0 && d.nprop > 0) {
       const int64_t row = (int64_t)d.fs + d.nprop - 1 - A.f_lo;
-      for (int i = q; i < N; i += R)
-        A.state[(int64_t)d.out * N + i] = cmul_diag(psi[i * VP + v0 + k], A.diag[row * N + i]);
+      const bool due = (int64_t)d.fs + d.nprop >= F1;  // half-phase not yet applied
+      for (int i = q; i < N; i += R) {
+        const float2 p = psi[i * VP + v0 + k];
+        A.state[(int64_t)d.out * N + i] = due ? cmul_diag(p, A.diag[row * N + i]) : p;
+      }
     }
   }
   if (A.tmax_acc > 0)

@@ -1,0 +1,663 @@
+// Batched 2DIR response (src/calc_2DIR.c:184-506) for gfx950.
+//
+// The reference runs 21 molecular-frame polarisation items per sample, each
+// repeating propagations that depend on one dipole component only.  Here
+// every distinct vector is propagated once per sample:
+//   L0[x][t1]  GB seeds: mu_x(tj - t1) propagated t1 frames    (3 x T1)
+//   L[x][t1]   L0[x][t1] continued through t3                   (3 x T1)
+//   M[x]       mu_x(tk) propagated through t3                    (3)
+//   fr[x1][x2] dipole_double(mu_x2(tk), M[x1](tk))               (9, doubles)
+//   ft[x0][x2][t1] dipole_double(mu_x2(tk), L0[x0][t1])          (9 x T1)
+// with tj = sample*Samplerate + T1 and tk = tj (no waiting time).  The EA
+// "right" vectors are not propagated: the sign -1 step of conj(psi) is, bit
+// for bit, conj of the sign +1 step of psi (negation commutes with every
+// rounded product and sum of the step), so rightrr[t1](t3) =
+// conj(L[x0][t1](t3)) and rightnr(t3) = conj(M[x1](t3)).
+//
+// Stages: seeds (conveyor, write back), history (conveyor recording L and M
+// at every t3), GB/SE responses from the history, doubles setup, then per
+// t3 the EA responses and one doubles step, and finally a fixed-order FP64
+// reduction over samples and polarisation items into the 12 planes
+// [T3][T1] of src/calc_2DIR.c:540-542.  Each dot product keeps the
+// reference's element order.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "nise_internal.h"
+
+namespace nise {
+
+// src/polar.c:36-105 (directions) and :12-33 (weights)
+__constant__ int c_polar[21][4] = {
+    {0, 0, 0, 0}, {1, 1, 1, 1}, {2, 2, 2, 2}, {0, 0, 1, 1}, {0, 0, 2, 2}, {1, 1, 0, 0},
+    {1, 1, 2, 2}, {2, 2, 0, 0}, {2, 2, 1, 1}, {0, 1, 0, 1}, {0, 2, 0, 2}, {1, 0, 1, 0},
+    {1, 2, 1, 2}, {2, 0, 2, 0}, {2, 1, 2, 1}, {0, 1, 1, 0}, {0, 2, 2, 0}, {1, 0, 0, 1},
+    {1, 2, 2, 1}, {2, 0, 0, 2}, {2, 1, 1, 2}};
+
+static float polarweight_h(int pol, int x) {
+  const float ithirty = 1.0 / 30;
+  float w = 0;
+  if (pol == 0) w = (x <= 2) ? 6 * ithirty : 2 * ithirty;
+  if (pol == 1) w = (x <= 2) ? 2 * ithirty : (x <= 8) ? 4 * ithirty : -1 * ithirty;
+  if (pol == 2) w = (x <= 8) ? 0.0f : (x <= 14) ? (float)(2.5 * ithirty) : (float)(-2.5 * ithirty);
+  return w;
+}
+
+struct TwoDirArgs {
+  int N, T1, T3, S;         // sites, t1 and t3 points, samples in this chunk
+  int s0, sample;           // first sample (absolute), Samplerate
+  int do_gb, do_se, do_ea;
+  float anh;                // fixed anharmonicity (0: per-frame records)
+  const float *mu_rec;      // dipole records (absolute frame index)
+  int64_t mu_stride;        // 0: static dipoles
+  const float *over_rec;    // overtone dipole records (1 + 3N) when anh == 0
+  const float2 *L0;         // [S][3][T1][N]
+  const float2 *hist;       // [S][3T1+3][T3][N]: rows of L then M
+  float2 *dbl;              // [S][9 + 9 T1][N2]
+  float *E;                 // [S][21][4][T3][T1]
+};
+
+__device__ inline const float *mu_at(const TwoDirArgs &A, int64_t f, int x) {
+  return A.mu_rec + (A.mu_stride ? f * A.mu_stride : 0) + 1 + (int64_t)x * A.N;
+}
+
+__device__ inline int64_t tj_of(const TwoDirArgs &A, int s) {
+  return (int64_t)(A.s0 + s) * A.sample + A.T1;
+}
+
+// history row of L[x][t1] / M[x]
+__device__ inline int64_t hrow_L(const TwoDirArgs &A, int s, int x, int t1) {
+  return (((int64_t)s * (3 * A.T1 + 3)) + x * A.T1 + t1) * A.T3;
+}
+__device__ inline int64_t hrow_M(const TwoDirArgs &A, int s, int x) {
+  return (((int64_t)s * (3 * A.T1 + 3)) + 3 * A.T1 + x) * A.T3;
+}
+
+// L0[s][x][0] = mu_x(tj) (no propagation for t1 = 0)
+__global__ void seed0_kernel(TwoDirArgs A, float2 *L0) {
+  const int s = blockIdx.x, x = blockIdx.y;
+  const float *mu = mu_at(A, tj_of(A, s), x);
+  for (int i = threadIdx.x; i < A.N; i += blockDim.x)
+    L0[(((int64_t)s * 3 + x) * A.T1) * A.N + i] = make_float2(mu[i], 0.f);
+}
+
+// GB and SE terms from the recorded history, one workgroup per (s, t3),
+// one thread per t1.  Writes E = -GB - SE for all 21 items.
+__global__ void gbse_kernel(TwoDirArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int s = blockIdx.x, t3 = blockIdx.y, N = A.N;
+  float2 *sc = reinterpret_cast<float2 *>(smem);  // [3][3] t3n_GB, [3][3] t3r_SE, [3][3] t3n_SE
+  const int64_t tj = tj_of(A, s), tk = tj, tl = tk + t3;
+  const int t = threadIdx.x;
+  if (t < 27) {
+    const int kind = t / 9, a = (t / 3) % 3, b = t % 3;
+    float r = 0.f, q = 0.f;
+    if (kind == 0) {  // GB: t3n[px2=a][px3=b] = mu_px3(tl) . M[px2](t3)  (:265-268)
+      const float *m4 = mu_at(A, tl, b);
+      const float2 *v = A.hist + (hrow_M(A, s, a) + t3) * N;
+      for (int i = 0; i < N; ++i) r = __fadd_rn(r, __fmul_rn(m4[i], v[i].x)), q = __fadd_rn(q, __fmul_rn(m4[i], v[i].y));
+    } else if (kind == 1) {  // SE: t3r[px1=a][px3=b] = mu_px3(tl) . leftrr(t3)  (:378-381)
+      const float *m4 = mu_at(A, tl, b);
+      const float2 *v = A.hist + (hrow_M(A, s, a) + t3) * N;
+      for (int i = 0; i < N; ++i) r = __fadd_rn(r, __fmul_rn(m4[i], v[i].x)), q = __fadd_rn(q, __fmul_rn(m4[i], v[i].y));
+    } else {  // SE static: t3n[px1=a][px2=b] = leftrr(tk) . mu_px2(tk), im with -=  (:353-357)
+      const float *m3 = mu_at(A, tk, b);
+      const float2 *v = A.hist + (hrow_M(A, s, a) + 0) * N;
+      for (int i = 0; i < N; ++i) r = __fadd_rn(r, __fmul_rn(v[i].x, m3[i])), q = __fsub_rn(q, __fmul_rn(v[i].y, m3[i]));
+    }
+    sc[t] = make_float2(r, q);
+  }
+  __syncthreads();
+  for (int t1 = t; t1 < A.T1; t1 += blockDim.x) {
+    // per-t1 dots: GB t1n[x0][x1] (static, :244-250), SE t1n[x0][x3] (:384-390),
+    // SE static t1r[x0][x2] (:363-369)
+    float2 gb1[3][3], se1[3][3], se1r[3][3];
+    for (int x0 = 0; x0 < 3; ++x0) {
+      const float2 *l0 = A.L0 + (((int64_t)s * 3 + x0) * A.T1 + t1) * N;
+      const float2 *lt = A.hist + (hrow_L(A, s, x0, t1) + t3) * N;
+      for (int y = 0; y < 3; ++y) {
+        const float *mj = mu_at(A, tj, y), *ml = mu_at(A, tl, y), *mk = mu_at(A, tk, y);
+        float ar = 0.f, ai = 0.f, br = 0.f, bi = 0.f, cr = 0.f, ci = 0.f;
+        for (int i = 0; i < N; ++i) {
+          ar = __fadd_rn(ar, __fmul_rn(mj[i], l0[i].x)), ai = __fadd_rn(ai, __fmul_rn(mj[i], l0[i].y));
+          br = __fadd_rn(br, __fmul_rn(lt[i].x, ml[i])), bi = __fadd_rn(bi, __fmul_rn(lt[i].y, ml[i]));
+          cr = __fadd_rn(cr, __fmul_rn(l0[i].x, mk[i])), ci = __fsub_rn(ci, __fmul_rn(l0[i].y, mk[i]));
+        }
+        gb1[x0][y] = make_float2(ar, ai);
+        se1[x0][y] = make_float2(br, bi);
+        se1r[x0][y] = make_float2(cr, ci);
+      }
+    }
+    for (int m = 0; m < 21; ++m) {
+      const int x0 = c_polar[m][0], x1 = c_polar[m][1], x2 = c_polar[m][2], x3 = c_polar[m][3];
+      float e[4] = {0.f, 0.f, 0.f, 0.f};
+      if (A.do_gb) {  // src/calc_2DIR.c:272-275
+        const float t3nr = sc[x2 * 3 + x3].x, t3ni = sc[x2 * 3 + x3].y;
+        const float t1nr = gb1[x0][x1].x, t1ni = gb1[x0][x1].y;
+        e[0] -= __fsub_rn(__fmul_rn(t3ni, t1nr), __fmul_rn(t3nr, t1ni));
+        e[1] -= __fadd_rn(__fmul_rn(t3nr, t1nr), __fmul_rn(t3ni, t1ni));
+        e[2] -= __fadd_rn(__fmul_rn(t3ni, t1nr), __fmul_rn(t3nr, t1ni));
+        e[3] -= __fsub_rn(__fmul_rn(t3nr, t1nr), __fmul_rn(t3ni, t1ni));
+      }
+      if (A.do_se) {  // src/calc_2DIR.c:397-400
+        const float t3rr = sc[9 + x1 * 3 + x3].x, t3ri = sc[9 + x1 * 3 + x3].y;
+        const float t3nr = sc[18 + x1 * 3 + x2].x, t3ni = sc[18 + x1 * 3 + x2].y;
+        const float t1rr = se1r[x0][x2].x, t1ri = se1r[x0][x2].y;
+        const float t1nr = se1[x0][x3].x, t1ni = se1[x0][x3].y;
+        e[0] -= __fadd_rn(__fmul_rn(t3rr, t1ri), __fmul_rn(t3ri, t1rr));
+        e[1] -= __fsub_rn(__fmul_rn(t3rr, t1rr), __fmul_rn(t3ri, t1ri));
+        e[2] -= __fadd_rn(__fmul_rn(t3ni, t1nr), __fmul_rn(t3nr, t1ni));
+        e[3] -= __fsub_rn(__fmul_rn(t3nr, t1nr), __fmul_rn(t3ni, t1ni));
+      }
+      float *E = A.E + (((int64_t)s * 21 + m) * 4) * A.T3 * A.T1;
+      for (int q = 0; q < 4; ++q) E[((int64_t)q * A.T3 + t3) * A.T1 + t1] = e[q];
+    }
+  }
+}
+
+// fr and ft = dipole_double(mu_x2(tk), ...) (src/NISE_subs.c:485-509):
+// |aa> gets over_a c_a, |ab> gets (0 + mu_a c_b) + mu_b c_a.
+__global__ void dd_kernel(TwoDirArgs A) {
+  const int N = A.N, N2 = N * (N + 1) / 2;
+  const int s = blockIdx.y, vec = blockIdx.z;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N2) return;
+  const int64_t tk = tj_of(A, s);
+  int x2;
+  const float2 *c;
+  if (vec < 9) {
+    const int x1 = vec / 3;
+    x2 = vec % 3;
+    c = A.hist + hrow_M(A, s, x1) * N;  // M[x1] at t3 = 0 (= leftrr at tk)
+  } else {
+    const int idx = vec - 9, x0 = idx / (3 * A.T1), t1 = idx % A.T1;
+    x2 = (idx / A.T1) % 3;
+    c = A.L0 + (((int64_t)s * 3 + x0) * A.T1 + t1) * N;
+  }
+  const float *dip = mu_at(A, tk, x2);
+  int a = 0, rem = j;
+  while (rem >= N - a) { rem -= N - a; ++a; }
+  const int b = a + rem;
+  float2 o;
+  if (a == b) {
+    const float ov = (A.anh != 0.f) ? __fmul_rn(kSqrt2, dip[a])
+                                    : A.over_rec[tk * (1 + 3 * (int64_t)N) + 1 + (int64_t)x2 * N + a];
+    o = make_float2(__fmul_rn(ov, c[a].x), __fmul_rn(ov, c[a].y));
+  } else {
+    o = make_float2(__fadd_rn(__fmul_rn(dip[a], c[b].x), __fmul_rn(dip[b], c[a].x)),
+                    __fadd_rn(__fmul_rn(dip[a], c[b].y), __fmul_rn(dip[b], c[a].y)));
+  }
+  A.dbl[((int64_t)s * (9 + 9 * A.T1) + vec) * N2 + j] = o;
+}
+
+// Element k of dipole_double_last(dip, f) in the reference's order
+// (src/NISE_subs.c:552-575): sum over i < k of dip_i f_ik, then over_k f_kk,
+// then dip_j f_kj for j > k.
+__device__ inline float2 ddl_elem(const float *dip, const float *over, const float2 *f, int N, int k) {
+  float r = 0.f, q = 0.f;
+  for (int i = 0; i < k; ++i) {
+    const float2 v = f[sindex(i, k, N)];
+    r = __fadd_rn(r, __fmul_rn(dip[i], v.x)), q = __fadd_rn(q, __fmul_rn(dip[i], v.y));
+  }
+  const float2 d = f[sindex(k, k, N)];
+  r = __fadd_rn(r, __fmul_rn(over[k], d.x)), q = __fadd_rn(q, __fmul_rn(over[k], d.y));
+  for (int j = k + 1; j < N; ++j) {
+    const float2 v = f[sindex(k, j, N)];
+    r = __fadd_rn(r, __fmul_rn(dip[j], v.x)), q = __fadd_rn(q, __fmul_rn(dip[j], v.y));
+  }
+  return make_float2(r, q);
+}
+
+// EA terms at one t3 (src/calc_2DIR.c:448-488), one workgroup per
+// (s, x2, x3), one thread per t1; adds into E.
+__global__ void ea_kernel(TwoDirArgs A, int t3) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = A.N, N2 = N * (N + 1) / 2;
+  const int s = blockIdx.x, x2 = blockIdx.y / 3, x3 = blockIdx.y % 3;
+  float *dip = reinterpret_cast<float *>(smem);   // [N]
+  float *over = dip + N;                          // [N]
+  float2 *lrr = reinterpret_cast<float2 *>(over + N + (N & 1));  // [3][N] leftrr_EA per x1
+  const int64_t tk = tj_of(A, s), tl = tk + t3;
+  const float *m4 = mu_at(A, tl, x3);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    dip[i] = m4[i];
+    over[i] = (A.anh != 0.f) ? __fmul_rn(kSqrt2, m4[i])
+                             : A.over_rec[tl * (1 + 3 * (int64_t)N) + 1 + (int64_t)x3 * N + i];
+  }
+  __syncthreads();
+  const float2 *dbl = A.dbl + (int64_t)s * (9 + 9 * A.T1) * N2;
+  for (int e = threadIdx.x; e < 3 * N; e += blockDim.x) {
+    const int x1 = e / N, k = e % N;
+    lrr[e] = ddl_elem(dip, over, dbl + (int64_t)(x1 * 3 + x2) * N2, N, k);
+  }
+  __syncthreads();
+  for (int t1 = threadIdx.x; t1 < A.T1; t1 += blockDim.x) {
+    for (int m = 0; m < 21; ++m) {
+      if (c_polar[m][2] != x2 || c_polar[m][3] != x3) continue;
+      const int x0 = c_polar[m][0], x1 = c_polar[m][1];
+      const float2 *R = A.hist + (hrow_L(A, s, x0, t1) + t3) * N;  // conj -> rightrr[t1]
+      const float2 *Rn = A.hist + (hrow_M(A, s, x1) + t3) * N;     // conj -> rightnr
+      const float2 *ft = dbl + (int64_t)(9 + (x0 * 3 + x2) * A.T1 + t1) * N2;
+      const float2 *lr = lrr + x1 * N;
+      float rrI = 0.f, riI = 0.f, rrII = 0.f, riII = 0.f;
+      for (int i = 0; i < N; ++i) {
+        const float Rr = R[i].x, Ri = -R[i].y, Nr = Rn[i].x, Ni = -Rn[i].y;
+        const float2 Lv = ddl_elem(dip, over, ft, N, i);
+        rrI = __fadd_rn(rrI, __fadd_rn(__fmul_rn(lr[i].y, Rr), __fmul_rn(lr[i].x, Ri)));
+        riI = __fadd_rn(riI, __fsub_rn(__fmul_rn(lr[i].x, Rr), __fmul_rn(Ri, lr[i].y)));
+        rrII = __fadd_rn(rrII, __fadd_rn(__fmul_rn(Nr, Lv.y), __fmul_rn(Ni, Lv.x)));
+        riII = __fadd_rn(riII, __fsub_rn(__fmul_rn(Nr, Lv.x), __fmul_rn(Ni, Lv.y)));
+      }
+      float *E = A.E + (((int64_t)s * 21 + m) * 4) * A.T3 * A.T1 + (int64_t)t3 * A.T1 + t1;
+      const int64_t plane = (int64_t)A.T3 * A.T1;
+      E[0] += rrI, E[plane] += riI, E[2 * plane] += rrII, E[3 * plane] += riII;
+    }
+  }
+}
+
+// R[k][t3][t1] += sum_s sum_m E[s][m][q][t3][t1] * (w(p, m) lt(t3, t1)),
+// k = 4p + q, in a fixed order (double accumulation).
+__global__ void reduce2d_kernel(const float *E, int S, int T3, int T1, const float *wpl,
+                                const float *lt, double *R) {
+  const int64_t plane = (int64_t)T3 * T1;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 12 * plane) return;
+  const int k = (int)(idx / plane), p = k / 4, q = k % 4;
+  const int64_t e0 = idx % plane;
+  const float l = lt[e0];
+  double acc = 0.0;
+  for (int s = 0; s < S; ++s)
+    for (int m = 0; m < 21; ++m) {
+      const float pw = __fmul_rn(wpl[p * 21 + m], l);
+      if (pw == 0.f) continue;
+      acc += (double)__fmul_rn(E[((((int64_t)s * 21 + m) * 4) + q) * plane + e0], pw);
+    }
+  R[idx] += acc;
+}
+
+}  // namespace nise
+
+using namespace nise;
+
+namespace {
+int upl(DevBuf &d, const void *s, size_t n) {
+  int rc = d.alloc(n < 16 ? 16 : n);
+  if (rc) return rc;
+  if (n && s) NISE_HIP_CHECK(hipMemcpy(d.p, s, n, hipMemcpyHostToDevice));
+  return NISE_OK;
+}
+bool tech_is(const t_non *n, const char *t) { return strcmp(n->technique, t) == 0; }
+}  // namespace
+
+extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H_stride,
+                                 const float *mu_rec, int64_t mu_stride, int64_t nframes,
+                                 const float *coupling_rec, const float *anh_rec,
+                                 const float *over_rec, int where, float *R12) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  const int N = non->singles, T1 = non->tmax1, T3 = non->tmax3;
+  if (N < 1 || T1 < 1 || T3 < 1 || non->sample < 1 || non->ts < 1 || !R12)
+    return set_error("2DIR: bad sizes"), NISE_ERR_ARG;
+  if (N > 65535) return set_error("2DIR: N=%d exceeds 65535 sites", N), NISE_ERR_ARG;
+  if (non->propagation != 1)
+    return set_error("2DIR: only propagation 1 (Coupling) is provided"), NISE_ERR_ARG;
+  if (non->tmax2 != 0) return set_error("2DIR: waiting time tmax2 > 0 is not provided"), NISE_ERR_ARG;
+  if (non->cluster != -1) return set_error("2DIR: cluster selection is not provided"), NISE_ERR_ARG;
+  const int do_gb = tech_is(non, "GBIR") || tech_is(non, "2DIR") || tech_is(non, "noEAIR");
+  const int do_se = tech_is(non, "SEIR") || tech_is(non, "2DIR") || tech_is(non, "noEAIR");
+  const int do_ea = tech_is(non, "EAIR") || tech_is(non, "2DIR");
+  if (!do_gb && !do_se && !do_ea) return set_error("2DIR: unknown technique %s", non->technique), NISE_ERR_ARG;
+  const bool ctype = strcmp(non->hamiltonian, "Coupling") == 0;
+  if (!ctype && strcmp(non->hamiltonian, "Full") != 0)
+    return set_error("2DIR: HamiltonianType %s is not provided", non->hamiltonian), NISE_ERR_ARG;
+  if (ctype && !coupling_rec) return set_error("2DIR: Coupling type needs coupling_rec"), NISE_ERR_ARG;
+  if (non->anharmonicity == 0.f && do_ea && (!anh_rec || !over_rec))
+    return set_error("2DIR: anharmonicity 0 needs anharmonicity and overtone records"), NISE_ERR_ARG;
+  const int begin = non->begin;
+  int end = non->end;
+  if (end == 0) end = (int)((nframes - T1 - non->tmax2 - T3 - 1) / non->sample + 1);  // src/MPI_subs.c:51,62
+  if (end <= begin) return set_error("2DIR: no samples (begin %d, end %d)", begin, end), NISE_ERR_RANGE;
+  const int64_t f_lo = (int64_t)begin * non->sample;
+  const int64_t f_hi = (int64_t)(end - 1) * non->sample + T1 + T3;
+  if (f_hi > nframes)
+    return set_error("2DIR: trajectory has %lld frames, the samples need %lld", (long long)nframes,
+                     (long long)f_hi), NISE_ERR_RANGE;
+  const int64_t nfr = f_hi - f_lo;
+  hipStream_t st = default_stream();
+  const int N2 = N * (N + 1) / 2;
+  const bool anh_file = do_ea && non->anharmonicity == 0.f;
+
+  // inputs on the device (host records are uploaded for [f_lo, f_hi) and
+  // indexed by absolute frame through a shifted base pointer)
+  DevBuf Hb, mub, Cb, Ab, Ob;
+  const float *Hd = H_rec, *mud = mu_rec, *Cd = coupling_rec, *Ad = anh_rec, *Od = over_rec;
+  if (where == NISE_HOST) {
+    if ((rc = upl(Hb, H_rec + f_lo * H_stride, (size_t)(nfr * H_stride * 4)))) return rc;
+    Hd = Hb.as<float>() - f_lo * H_stride;
+    if (ctype) {
+      if ((rc = upl(mub, mu_rec, (size_t)mu_stride * 4))) return rc;
+      mud = mub.as<float>();
+      if ((rc = upl(Cb, coupling_rec, (size_t)(1 + N2) * 4))) return rc;
+      Cd = Cb.as<float>();
+    } else {
+      if ((rc = upl(mub, mu_rec + f_lo * mu_stride, (size_t)(nfr * mu_stride * 4)))) return rc;
+      mud = mub.as<float>() - f_lo * mu_stride;
+    }
+    if (anh_file) {
+      if ((rc = upl(Ab, anh_rec + f_lo * (1 + N), (size_t)(nfr * (1 + N) * 4)))) return rc;
+      Ad = Ab.as<float>() - f_lo * (1 + N);
+      if ((rc = upl(Ob, over_rec + f_lo * (1 + 3 * N), (size_t)(nfr * (1 + 3 * N) * 4)))) return rc;
+      Od = Ob.as<float>() - f_lo * (1 + 3 * N);
+    }
+  } else if (where != NISE_DEVICE) {
+    return set_error("2DIR: where must be NISE_HOST or NISE_DEVICE"), NISE_ERR_ARG;
+  }
+  FrameSource src;
+  src.H_rec = Hd, src.H_stride = H_stride, src.coupling_type = ctype, src.C_rec = Cd;
+  src.shifte = (non->max1 + non->min1) / 2;  // src/calc_2DIR.c:79-81
+  src.cut = non->couplingcut, src.apply_shift = 0;  // read_He applies no single-site shifts
+
+  // singles schedule over the union of kept couplings, per-frame tables
+  const int64_t npair = (int64_t)N * (N - 1) / 2;
+  std::vector<uint8_t> mask(npair);
+  if (npair) {
+    DevBuf mk;
+    if ((rc = mk.alloc(npair))) return rc;
+    if ((rc = launch_union_mask(src, N, f_lo, nfr, mk.as<uint8_t>(), st))) return rc;
+    NISE_HIP_CHECK(hipMemcpyAsync(mask.data(), mk.p, npair, hipMemcpyDeviceToHost, st));
+    NISE_HIP_CHECK(hipStreamSynchronize(st));
+  }
+  Schedule sch;
+  build_schedule(N, mask, sch);
+  std::vector<uint32_t> lex;  // the same couplings in lexicographic order (doubles)
+  {
+    int64_t p = 0;
+    for (int a = 0; a < N; ++a)
+      for (int b = a + 1; b < N; ++b, ++p)
+        if (mask[p]) lex.push_back((uint32_t)a | ((uint32_t)b << 16));
+  }
+  const float fac = trotter_factor(non->deltat, 1, non->ts);
+  DevBuf ab, lay, diag, cs, ab2, D2, cs2, stp;
+  if ((rc = upl(ab, sch.ab.data(), sch.ab.size() * 4)) ||
+      (rc = upl(lay, sch.layer_off.data(), sch.layer_off.size() * 4)))
+    return rc;
+  if ((rc = diag.alloc((size_t)nfr * N * 8)) || (rc = cs.alloc((size_t)nfr * std::max(sch.K, 1) * 8)))
+    return rc;
+  if ((rc = launch_frame_tables(src, N, sch, ab.as<uint32_t>(), fac, f_lo, nfr, diag.as<float2>(),
+                                cs.as<float2>(), st)))
+    return rc;
+  if (do_ea) {
+    if ((rc = upl(ab2, lex.data(), lex.size() * 4))) return rc;
+    if ((rc = D2.alloc((size_t)nfr * N2 * 8)) ||
+        (rc = cs2.alloc((size_t)nfr * std::max((int)lex.size(), 1) * 16)))
+      return rc;
+    if ((rc = launch_doubles_tables(src, N, (int)lex.size(), ab2.as<uint32_t>(), fac,
+                                    non->anharmonicity, Ad, f_lo, nfr, D2.as<float2>(),
+                                    cs2.as<float4>(), st)))
+      return rc;
+  }
+  // conveyor shape: the largest slot count whose LDS image fits
+  const int block = 256;
+  int vp = 0;
+  std::vector<int2> steps;
+  for (int v : {64, 32, 16, 8, 4}) {
+    build_steps(sch, v, block, steps);
+    if (conveyor_smem(N, v, block, 0, (int)steps.size()) <= 160 * 1024) {
+      vp = v;
+      break;
+    }
+  }
+  if (!vp) return set_error("2DIR: N=%d does not fit the conveyor's LDS image", N), NISE_ERR_ARG;
+  if ((rc = upl(stp, steps.data(), steps.size() * 8))) return rc;
+
+  // response weights: polarweight x lifetime factor (src/calc_2DIR.c:105-115)
+  std::vector<float> wh(63), lth((size_t)T3 * T1);
+  for (int p = 0; p < 3; ++p)
+    for (int m = 0; m < 21; ++m) wh[p * 21 + m] = polarweight_h(p, m);
+  for (int t3 = 0; t3 < T3; ++t3)
+    for (int t1 = 0; t1 < T1; ++t1)
+      lth[(size_t)t3 * T1 + t1] =
+          (non->lifetime > 0) ? (float)exp(-(double)(t1 + t3) * non->deltat / (2 * non->lifetime)) : 1.0f;
+  DevBuf wpl, ltb, Rd;
+  const int64_t plane = (int64_t)T3 * T1;
+  if ((rc = upl(wpl, wh.data(), wh.size() * 4)) || (rc = upl(ltb, lth.data(), lth.size() * 4))) return rc;
+  if ((rc = Rd.alloc((size_t)12 * plane * 8))) return rc;
+  NISE_HIP_CHECK(hipMemsetAsync(Rd.p, 0, (size_t)12 * plane * 8, st));
+
+  // samples per chunk, sized to a device memory budget
+  const int total = end - begin;
+  const int nvec = 9 + 9 * T1;  // doubles vectors per sample
+  const double per_sample = 8.0 * 3 * T1 * N + 8.0 * (3 * T1 + 3) * T3 * N +
+                            (do_ea ? 8.0 * nvec * N2 : 0.0) + 4.0 * 84 * plane;
+  const int chunk = (int)std::max(1.0, std::min<double>({(double)total, 1024.0, 8.0e9 / per_sample}));
+  const int vw = 64 / doubles_np(N);
+  const int gps = (nvec + vw - 1) / vw;  // doubles groups per sample
+
+  DevBuf L0, hist, dbl, E, slots, wgb, vb, rows;
+  ConveyorArgs a;
+  a.N = N, a.K = sch.K, a.nlayers = sch.nlayers(), a.m = non->ts, a.f_lo = f_lo;
+  a.ab = ab.as<uint32_t>(), a.layer_off = lay.as<int>(), a.diag = diag.as<float2>(), a.cs = cs.as<float2>();
+  a.mu_rec = mud, a.mu_stride = ctype ? 0 : mu_stride;
+  a.steps = stp.as<int2>(), a.nsteps = (int)steps.size();
+
+  for (int c0 = 0; c0 < total; c0 += chunk) {
+    const int S = std::min(chunk, total - c0);
+    TwoDirArgs A;
+    A.N = N, A.T1 = T1, A.T3 = T3, A.S = S, A.s0 = begin + c0, A.sample = non->sample;
+    A.do_gb = do_gb, A.do_se = do_se, A.do_ea = do_ea, A.anh = non->anharmonicity;
+    A.mu_rec = mud, A.mu_stride = ctype ? 0 : mu_stride, A.over_rec = Od;
+    if ((rc = L0.alloc((size_t)S * 3 * T1 * N * 8)) ||
+        (rc = hist.alloc((size_t)S * (3 * T1 + 3) * T3 * N * 8)) ||
+        (rc = E.alloc((size_t)S * 84 * plane * 4)))
+      return rc;
+    if (do_ea && (rc = dbl.alloc((size_t)S * nvec * N2 * 8))) return rc;
+    A.L0 = L0.as<float2>(), A.hist = hist.as<float2>(), A.dbl = dbl.as<float2>(), A.E = E.as<float>();
+
+    // (1) GB seeds L0[x][t1]: mu_x(tj - t1) propagated through frames
+    //     tj-t1 .. tj-1 (src/calc_2DIR.c:215-239); t1 = 0 is mu_x(tj)
+    hipLaunchKernelGGL(seed0_kernel, dim3(S, 3), dim3(64), 0, st, A, L0.as<float2>());
+    NISE_HIP_CHECK(hipGetLastError());
+    std::vector<SlotDesc> sl;
+    std::vector<int> wb;
+    for (int s = 0; s < S; ++s) {
+      const int tj = (begin + c0 + s) * non->sample + T1;
+      for (int x = 0; x < 3; ++x) {
+        const int first = (int)sl.size();
+        for (int t1 = 1; t1 < T1; ++t1)
+          sl.push_back(SlotDesc{tj - t1, 0, t1, x, 0, (s * 3 + x) * T1 + t1, 0, -1});
+        for (int b = first; b < (int)sl.size(); b += vp) wb.push_back(b);
+      }
+    }
+    wb.push_back((int)sl.size());
+    if (!sl.empty()) {
+      if ((rc = upl(slots, sl.data(), sl.size() * sizeof(SlotDesc))) || (rc = upl(wgb, wb.data(), wb.size() * 4)))
+        return rc;
+      a.slots = slots.as<SlotDesc>(), a.wg_slot_begin = wgb.as<int>(), a.nwg = (int)wb.size() - 1;
+      a.state = L0.as<float2>(), a.writeback = 1, a.hist = nullptr;
+      if ((rc = launch_conveyor(a, vp, block, st))) return rc;
+    }
+    // (2) history: L[x][t1] (from L0) and M[x] (from mu_x(tk)) at t3 = 0..T3-1
+    sl.clear(), wb.clear();
+    for (int s = 0; s < S; ++s) {
+      const int tk = (begin + c0 + s) * non->sample + T1;
+      const int base = s * (3 * T1 + 3);
+      for (int x = 0; x < 3; ++x)
+        for (int t1 = 0; t1 < T1; ++t1)
+          sl.push_back(SlotDesc{tk, T3, T3 - 1, x, 1, (s * 3 + x) * T1 + t1, 0, (base + x * T1 + t1) * T3});
+      for (int x = 0; x < 3; ++x) sl.push_back(SlotDesc{tk, T3, T3 - 1, x, 0, -1, 0, (base + 3 * T1 + x) * T3});
+      for (int b = 0; b < 3 * T1 + 3; b += vp) wb.push_back(base + b);
+    }
+    wb.push_back((int)sl.size());
+    if ((rc = upl(slots, sl.data(), sl.size() * sizeof(SlotDesc))) || (rc = upl(wgb, wb.data(), wb.size() * 4)))
+      return rc;
+    a.slots = slots.as<SlotDesc>(), a.wg_slot_begin = wgb.as<int>(), a.nwg = (int)wb.size() - 1;
+    a.state = L0.as<float2>(), a.writeback = 0, a.hist = hist.as<float2>();
+    if ((rc = launch_conveyor(a, vp, block, st))) return rc;
+    // (3) GB and SE terms from the history
+    hipLaunchKernelGGL(gbse_kernel, dim3(S, T3), dim3(64), 27 * sizeof(float2), st, A);
+    NISE_HIP_CHECK(hipGetLastError());
+    // (4) EA: doubles vectors at tk, then per t3 the responses and one
+    //     doubles step with frame tk + t3
+    if (do_ea) {
+      hipLaunchKernelGGL(dd_kernel, dim3((N2 + 127) / 128, S, nvec), dim3(128), 0, st, A);
+      NISE_HIP_CHECK(hipGetLastError());
+      std::vector<int> vbh((size_t)S * gps + 1);
+      for (int s = 0; s < S; ++s)
+        for (int g = 0; g < gps; ++g) vbh[(size_t)s * gps + g] = s * nvec + g * vw;
+      vbh.back() = S * nvec;
+      std::vector<int64_t> rowh((size_t)T3 * S * gps);
+      for (int t3 = 0; t3 < T3; ++t3)
+        for (int s = 0; s < S; ++s)
+          for (int g = 0; g < gps; ++g)
+            rowh[((size_t)t3 * S + s) * gps + g] = (int64_t)(begin + c0 + s) * non->sample + T1 + t3 - f_lo;
+      if ((rc = upl(vb, vbh.data(), vbh.size() * 4)) || (rc = upl(rows, rowh.data(), rowh.size() * 8))) return rc;
+      DoublesArgs d;
+      d.N = N, d.K2 = (int)lex.size(), d.m = non->ts, d.ab2 = ab2.as<uint32_t>();
+      d.D2 = D2.as<float2>(), d.cs2 = cs2.as<float4>(), d.state = dbl.as<float2>();
+      d.vec_begin = vb.as<int>();
+      const size_t ea_smem = (size_t)(2 * N + (N & 1)) * 4 + (size_t)3 * N * 8;
+      for (int t3 = 0; t3 < T3; ++t3) {
+        hipLaunchKernelGGL(ea_kernel, dim3(S, 9), dim3(64), ea_smem, st, A, t3);
+        NISE_HIP_CHECK(hipGetLastError());
+        if (t3 + 1 < T3) {  // the state after the last t3 is not used
+          d.rows = rows.as<int64_t>() + (size_t)t3 * S * gps;
+          if ((rc = launch_doubles_step(d, S * gps, st))) return rc;
+        }
+      }
+    }
+    // (5) fixed-order reduction of this chunk into the 12 planes
+    hipLaunchKernelGGL(reduce2d_kernel, dim3((unsigned)((12 * plane + 127) / 128)), dim3(128), 0, st,
+                       E.as<float>(), S, T3, T1, wpl.as<float>(), ltb.as<float>(), Rd.as<double>());
+    NISE_HIP_CHECK(hipGetLastError());
+    NISE_HIP_CHECK(hipStreamSynchronize(st));  // host slot vectors are rebuilt next chunk
+  }
+  std::vector<double> Rh((size_t)12 * plane);
+  NISE_HIP_CHECK(hipMemcpyAsync(Rh.data(), Rd.p, Rh.size() * 8, hipMemcpyDeviceToHost, st));
+  NISE_HIP_CHECK(hipStreamSynchronize(st));
+  for (size_t i = 0; i < Rh.size(); ++i) R12[i] = (float)Rh[i];
+  return NISE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// File-level drop-in
+// ---------------------------------------------------------------------------
+namespace {
+[[noreturn]] void die2d(int code, const char *msg) {
+  printf("%s", msg);
+  fflush(stdout);
+  exit(code);
+}
+
+// src/MPI_subs.c:20-34
+void print2D_file(const char *name, float *R, float *I, const t_non *non, int sampleCount) {
+  FILE *out = fopen(name, "w");
+  if (!out) return;
+  const int T1 = non->tmax1;
+  for (int t1 = 0; t1 < non->tmax1; t1 += non->dt1) {
+    const int t2 = non->tmax2;
+    for (int t3 = 0; t3 < non->tmax3; t3 += non->dt3) {
+      float &r = R[(size_t)t3 * T1 + t1], &i = I[(size_t)t3 * T1 + t1];
+      r /= sampleCount;
+      i /= sampleCount;
+      if (t3 == 0) r /= 2, i /= 2;
+      fprintf(out, "%f %f %f %e %e\n", t1 * non->deltat, t2 * non->deltat, t3 * non->deltat, r, i);
+    }
+  }
+  fclose(out);
+}
+}  // namespace
+
+// src/calc_2DIR.c:22-629.  Rank 0 runs every work item on its GPU; other
+// ranks have nothing to add to the sums and return.
+extern "C" void calc_2DIR(t_non *non, int parentRank, int parentSize, int subRank, int subSize,
+                          MPI_Comm subComm, MPI_Comm rootComm) {
+  (void)parentSize, (void)subRank, (void)subSize, (void)subComm, (void)rootComm;
+  if (parentRank != 0) return;
+  if (non->cluster != -1) die2d(1, "calc_2DIR: the GPU path does not implement Cluster\n");
+  const int N = non->singles;
+  const int totalSampleCount = (non->length - non->tmax1 - non->tmax2 - non->tmax3 - 1) / non->sample + 1;
+  if (totalSampleCount > 0) printf("Making %d samples!\n", totalSampleCount);
+  else die2d(1, "Insufficient data to calculate spectrum.\nPlease, lower max times or provide longer\ntrajectory.\n");
+  if (non->end == 0) non->end = totalSampleCount;
+  int sampleCount = non->end - non->begin;
+  if (sampleCount == 0) sampleCount = 1;
+  const float shift1 = (non->max1 + non->min1) / 2;
+  printf("Rotating Wave Frequency shift %f.\n", shift1);
+  non->shifte = shift1;
+  non->shiftf = 2 * shift1;
+  const bool ctype = strcmp(non->hamiltonian, "Coupling") == 0;
+  const bool ea = tech_is(non, "EAIR") || tech_is(non, "2DIR");
+  const int64_t D = non->doubles;
+  const int64_t hw = ctype ? 1 + N : 1 + (int64_t)N * (N + 1) / 2 + D * (D + 1) / 2;
+  const int64_t mw = 1 + 3 * (int64_t)N + 3 * (int64_t)N * D;
+  const int64_t f_lo = (int64_t)non->begin * non->sample;
+  const int64_t f_hi = (int64_t)(non->end - 1) * non->sample + non->tmax1 + non->tmax2 + non->tmax3;
+  if (non->end <= non->begin) die2d(1, "calc_2DIR: no samples between Begin and End\n");
+  const int64_t nfr = f_hi - f_lo;
+  FILE *probe = fopen(non->energyFName, "rb");
+  if (!probe) die2d(1, "Hamiltonian file not found!\n");
+  fclose(probe);
+  probe = fopen(non->dipoleFName, "rb");
+  if (!probe) {
+    printf("Dipole file %s not found!\n", non->dipoleFName);
+    die2d(1, "");
+  }
+  fclose(probe);
+  const bool anh_file = non->anharmonicity == 0;  // src/calc_2DIR.c:132-146
+  if (anh_file) {
+    if (!(probe = fopen(non->anharFName, "rb"))) {
+      printf("Anharmonicity file %s not found!\n", non->anharFName);
+      die2d(1, "");
+    }
+    fclose(probe);
+    if (!(probe = fopen(non->overdipFName, "rb"))) {
+      printf("Overtone dipole file %s not found!\n", non->overdipFName);
+      die2d(1, "");
+    }
+    fclose(probe);
+  }
+  Pinned H, mu, Cc, Ah, Oh;
+  if (!H.alloc((size_t)(nfr * hw)) || !mu.alloc((size_t)((ctype ? 1 : nfr) * mw)))
+    die2d(1, "calc_2DIR: could not allocate pinned host memory\n");
+  if (!read_records(non->energyFName, hw, f_lo, nfr, H.p))
+    die2d(1, "Hamiltonian trajectory file to short, could not fill buffer!!!\n");
+  if (!read_records(non->dipoleFName, mw, ctype ? 0 : f_lo, ctype ? 1 : nfr, mu.p))
+    die2d(1, "Dipole trajectory file to short, could not fill buffer!!!\n");
+  if (ctype) {
+    const int64_t cw = 1 + (int64_t)N * (N + 1) / 2 + D * (D + 1) / 2;
+    if (!(probe = fopen(non->couplingFName, "rb"))) die2d(1, "Coupling file not found!\n");
+    fclose(probe);
+    if (!Cc.alloc((size_t)cw) || !read_records(non->couplingFName, cw, 0, 1, Cc.p))
+      die2d(1, "Coupling trajectory file to short, could not fill buffer!!!\n");
+  }
+  if (anh_file && ea) {
+    if (!Ah.alloc((size_t)(nfr * (1 + N))) || !Oh.alloc((size_t)(nfr * (1 + 3 * N))))
+      die2d(1, "calc_2DIR: could not allocate pinned host memory\n");
+    if (!read_records(non->anharFName, 1 + N, f_lo, nfr, Ah.p) ||
+        !read_records(non->overdipFName, 1 + 3 * (int64_t)N, f_lo, nfr, Oh.p))
+      die2d(1, "Anharmonicity or overtone dipole trajectory file to short, could not fill buffer!!!\n");
+  }
+  // the staged records start at frame f_lo: rebase the samples to begin = 0
+  t_non loc = *non;
+  loc.begin = 0, loc.end = non->end - non->begin;
+  std::vector<float> R((size_t)12 * non->tmax3 * non->tmax1);
+  const int rc = nise_2dir_batched(&loc, H.p, hw, mu.p, mw, nfr, ctype ? Cc.p : nullptr,
+                                   Ah.p, Oh.p, NISE_HOST, R.data());
+  if (rc) {
+    printf("calc_2DIR: GPU run failed: %s\n", nise_last_error());
+    die2d(1, "");
+  }
+  printf("Samples %d\n", sampleCount);
+  const size_t pl = (size_t)non->tmax3 * non->tmax1;
+  static const char *names[6] = {"RparI.dat", "RparII.dat", "RperI.dat", "RperII.dat", "RcroI.dat", "RcroII.dat"};
+  for (int p = 0; p < 3; ++p)
+    for (int k = 0; k < 2; ++k)
+      print2D_file(names[2 * p + k], R.data() + (4 * p + 2 * k) * pl, R.data() + (4 * p + 2 * k + 1) * pl, non,
+                   sampleCount);
+  printf("----------------------------------------\n");
+  printf(" 2DIR calculation succesfully completed\n");
+  printf("----------------------------------------\n\n");
+}

@@ -148,3 +148,36 @@ def shifted_center(H_full, N):
     d = H_full[:, 1:][:, a == b]
     c = float(np.mean(d))
     return c - 100.0, c + 100.0
+
+
+def twodir_system(N=6, nframes=200, seed=SEED0 + 2, dt=2.0, mean=1650.0, sd=10.0, tau=100.0,
+                  coupled=True, jmean=-5.0, jsd=2.0, dip_fluct=0.2, anh_mean=16.0, anh_sd=2.0):
+    """Records for a 2DIR run: Full Hamiltonian (chain couplings, or none
+    when coupled is False), fluctuating dipoles (unit vectors plus AR(1)
+    noise of sd dip_fluct per component), per-frame anharmonicities
+    (1 + N words) and overtone dipoles (1 + 3N words, sqrt(2) mu plus
+    noise) for runs with anharmonicity 0."""
+    rng = np.random.default_rng(seed)
+    diag = ar1(rng, nframes, N, mean, sd, tau, dt)
+    a, b = np.triu_indices(N, 1)
+    coup = np.zeros((nframes, a.size), np.float32)
+    if coupled and N > 1:
+        nn = np.nonzero(b == a + 1)[0]
+        j0 = rng.normal(jmean, jsd, N - 1)
+        jf = ar1(rng, nframes, N - 1, 0.0, 1.0, tau, dt)
+        coup[:, nn] = (j0[None, :] + jf).astype(np.float32)
+    H = full_records(diag, coup)
+    u = unit_vectors(rng, N)
+    noise = ar1(rng, nframes, 3 * N, 0.0, dip_fluct, tau, dt).reshape(nframes, 3, N)
+    mu_xyz = (u.T[None, :, :] + noise).astype(np.float32)  # (T, 3, N)
+    mu = np.empty((nframes, 1 + 3 * N), np.float32)
+    mu[:, 0] = headers(nframes)
+    mu[:, 1:] = mu_xyz.reshape(nframes, 3 * N)
+    anh = np.empty((nframes, 1 + N), np.float32)
+    anh[:, 0] = headers(nframes)
+    anh[:, 1:] = ar1(rng, nframes, N, anh_mean, anh_sd, tau, dt)
+    over = np.empty((nframes, 1 + 3 * N), np.float32)
+    over[:, 0] = headers(nframes)
+    on = ar1(rng, nframes, 3 * N, 0.0, dip_fluct, tau, dt)
+    over[:, 1:] = (np.sqrt(2.0) * mu_xyz.reshape(nframes, 3 * N) + on).astype(np.float32)
+    return H, mu, anh, over

@@ -131,3 +131,26 @@ def fft1d(non, re_S, im_S, samples):
                       np.ascontiguousarray(im_S, np.float32), samples, fr, sr, si, C.byref(cnt))
     k = cnt.value
     return fr[:k], sr[:k], si[:k]
+
+
+def twodir(non, H, mu, C_rec=None, A_rec=None, over_rec=None):
+    """Oracle 2DIR: the 12 response planes (12, tmax3, tmax1) float32 summed
+    over samples begin..end-1 and the 21 polarisation items in the
+    reference's work order (src/MPI_subs.c:74-95), un-normalised.  Sets
+    non.shifte = (min1 + max1) / 2 as src/calc_2DIR.c:79-81 does."""
+    H = np.ascontiguousarray(H, np.float32)
+    mu = np.ascontiguousarray(mu, np.float32)
+    Cr = None if C_rec is None else np.ascontiguousarray(C_rec, np.float32)
+    Ar = None if A_rec is None else np.ascontiguousarray(A_rec, np.float32)
+    Or = None if over_rec is None else np.ascontiguousarray(over_rec, np.float32)
+    non.shifte = np.float32((np.float32(non.max1) + np.float32(non.min1)) / np.float32(2))
+    R = np.zeros((12, non.tmax3, non.tmax1), np.float32)
+    end = non.end
+    if end == 0:
+        end = (H.shape[0] - non.tmax1 - non.tmax2 - non.tmax3 - 1) // non.sample + 1
+    for s in range(non.begin, end):
+        for m in range(21):
+            rc = oracle().or_2dir_item(C.byref(non), ptr(H), H.shape[1], ptr(mu), mu.shape[1],
+                                       ptr(Cr), ptr(Ar), ptr(Or), s, m, R)
+            assert rc == 0
+    return R
