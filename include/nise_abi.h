@@ -205,6 +205,61 @@ int nise_absorption_batched(const t_non *non, const float *H_rec,
                             const float *coupling_rec, int where,
                             int kernel, float *re_S, float *im_S);
 
+/* Absorption plan: the state of nise_absorption_batched kept on the device
+ * so that a run can be repeated (timed, replayed as a hipGraph, re-shaped)
+ * and differentiated.  Arguments as nise_absorption_batched; host records
+ * are copied at creation, device records must outlive the plan. */
+typedef struct nise_abs_plan nise_abs_plan;
+int nise_abs_plan_create(nise_abs_plan **plan, const t_non *non,
+                         const float *H_rec, int64_t H_stride,
+                         const float *mu_rec, int64_t mu_stride,
+                         int64_t nframes, const float *coupling_rec, int where,
+                         int kernel);
+/* Conveyor shape: vp vector slots per workgroup, samples_per_wg samples
+ * (3 polarisations each, 3*samples_per_wg <= vp), block 256 or 512
+ * (multi-wave), 64 (one wave) or -2 / -4 (one wave, 2 or 4 slots per lane). */
+int nise_abs_plan_set_shape(nise_abs_plan *plan, int vp, int samples_per_wg,
+                            int block);
+/* Run the plan; re_S / im_S (host, tmax1 each) may be NULL to leave the
+ * result on the device.  flags bit 0: time the stages with HIP events;
+ * bit 1: launch the stages as one captured hipGraph. */
+int nise_abs_plan_execute(nise_abs_plan *plan, float *re_S, float *im_S,
+                          int flags);
+int nise_abs_plan_sync(nise_abs_plan *plan);
+/* info[0..9] = N, rotations per sweep, layers, workgroups, vp, samples per
+ * workgroup, first frame, frames, block, steps; returns the count. */
+int nise_abs_plan_info(const nise_abs_plan *plan, int64_t *info, int n);
+/* ms of the last timed execute (tables, conveyor or plain kernel, reduce)
+ * and of the last adjoint run; returns the count (4). */
+int nise_abs_plan_last_timing(const nise_abs_plan *plan, float *ms, int n);
+void nise_abs_plan_destroy(nise_abs_plan *plan);
+
+/* Parameters of gradient runs (no counterpart in the reference): delta
+ * (host, N values or NULL for none) is added to every frame's shifted site
+ * energy, cscale multiplies every kept coupling.  delta = NULL and
+ * cscale = 1 give the reference's result bit for bit. */
+int nise_abs_plan_set_params(nise_abs_plan *plan, const float *delta,
+                             float cscale);
+/* Record forward checkpoints psi(t) at t = 0, every, 2*every, ... and
+ * tmax1-1 for every (sample, polarisation) vector on each execute. */
+int nise_abs_plan_enable_adjoint(nise_abs_plan *plan, int every);
+/* Adjoint of the last execute: given g_re[t] = dL/dRe S(t) and
+ * g_im[t] = dL/dIm S(t) (host, tmax1 each), writes dL/d delta (host, N
+ * doubles) and dL/d cscale.  Propagates the adjoint backwards with the
+ * inverse Trotter steps, rebuilding the forward states from the
+ * checkpoints. */
+int nise_abs_plan_adjoint(nise_abs_plan *plan, const float *g_re,
+                          const float *g_im, double *grad_delta,
+                          double *grad_c);
+
+/* One frame's propagation tables as the plan computes them (inspection):
+ * diag_out = N (cos, -sin) half-step phases, cs_out = (cos, sin) of every
+ * coupling of the strict upper triangle in lexicographic order, *K_out =
+ * their count. */
+int nise_frame_tables(const t_non *non, const float *H_rec, int64_t H_stride,
+                      const float *coupling_rec, int64_t pos, int sign,
+                      float *diag_out, float *cs_out, int *K_out);
+
 /* Batched 2DIR (src/calc_2DIR.c:184-506 over all samples in
  * [non->begin, non->end) and the 21 molecular polarisation items).  Uses
  * non->singles, tmax1, tmax3 (tmax2 must be 0), sample, begin, end (0: as

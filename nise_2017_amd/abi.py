@@ -114,6 +114,9 @@ def lib():
     L.nise_abs_plan_sync.argtypes = [_vp]
     L.nise_abs_plan_destroy.argtypes = [_vp]
     L.nise_abs_plan_destroy.restype = None
+    L.nise_abs_plan_set_params.argtypes = [_vp, _vp, C.c_float]
+    L.nise_abs_plan_enable_adjoint.argtypes = [_vp, C.c_int]
+    L.nise_abs_plan_adjoint.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double)]
     L.nise_2dir_batched.argtypes = [P, _vp, C.c_int64, _vp, C.c_int64, C.c_int64, _vp, _vp, _vp,
                                     C.c_int, _fp]
     for name in ("propagate_vec_coupling_S",):
@@ -161,6 +164,7 @@ class AbsPlan:
                                      mu_stride, nframes, ptr(coupling_rec), where, kernel))
         self.h = h
         self.tmax = non.tmax1
+        self.N = non.singles
 
     def set_shape(self, vp, spw, block=256):
         check(lib().nise_abs_plan_set_shape(self.h, vp, spw, block))
@@ -184,9 +188,33 @@ class AbsPlan:
         return dict(zip(keys, list(a)))
 
     def timing(self):
-        a = (C.c_float * 3)()
-        lib().nise_abs_plan_last_timing(self.h, a, 3)
-        return {"tables_ms": a[0], "conveyor_ms": a[1], "reduce_ms": a[2]}
+        a = (C.c_float * 4)()
+        lib().nise_abs_plan_last_timing(self.h, a, 4)
+        return {"tables_ms": a[0], "conveyor_ms": a[1], "reduce_ms": a[2], "adjoint_ms": a[3]}
+
+    def set_params(self, delta=None, cscale=1.0):
+        """Per-site energy offsets (N floats or None) and coupling scale."""
+        d = None
+        if delta is not None:
+            d = np.ascontiguousarray(delta, np.float32)
+            if d.shape != (self.N,):
+                raise ValueError(f"delta must have shape ({self.N},)")
+        check(lib().nise_abs_plan_set_params(self.h, ptr(d), float(cscale)))
+
+    def enable_adjoint(self, every):
+        check(lib().nise_abs_plan_enable_adjoint(self.h, int(every)))
+
+    def adjoint(self, g_re, g_im):
+        """(dL/d delta (N,), dL/d cscale) of the last execute for
+        g = dL/dRe S + i dL/dIm S."""
+        gr = np.ascontiguousarray(g_re, np.float32)
+        gi = np.ascontiguousarray(g_im, np.float32)
+        if gr.shape != (self.tmax,) or gi.shape != (self.tmax,):
+            raise ValueError(f"gradient inputs must have shape ({self.tmax},)")
+        gd = np.zeros(self.N, np.float64)
+        gc = C.c_double()
+        check(lib().nise_abs_plan_adjoint(self.h, ptr(gr), ptr(gi), ptr(gd), C.byref(gc)))
+        return gd, gc.value
 
     def close(self):
         if getattr(self, "h", None) and _lib is not None:

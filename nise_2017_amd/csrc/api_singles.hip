@@ -37,9 +37,12 @@ struct nise_abs_plan {
   int nsteps = 0;
   DevBuf ws_cs, ws_ab, resp_plain;
   int vp = 16, spw = 5, nwg = 0, block = 256;
+  // gradient runs: parameters and the adjoint's buffers (every == 0: off)
+  DevBuf delta, jf, ck, gtab, part_d, part_c, grad, adj_slots, adj_wg, adj_steps;
+  int every = 0, nck = 0, adj_vp = 0, adj_nwg = 0, adj_nsteps = 0;
   hipStream_t st = nullptr;
   hipEvent_t ev[4] = {};
-  float last_ms[3] = {0, 0, 0};
+  float last_ms[4] = {0, 0, 0, 0};
   bool graph_ready = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -76,7 +79,10 @@ int build_slots(nise_abs_plan *P) {
   for (int s = 0; s < ns; ++s) {
     if (s % P->spw == 0) wb.push_back((int)sl.size());
     int fs = (P->begin + s) * P->sample;
-    for (int x = 0; x < 3; ++x) sl.push_back(SlotDesc{fs, P->tmax, P->tmax - 1, x, 0, -1, x == 0 ? 3 : 0, -1});
+    for (int x = 0; x < 3; ++x) {
+      const int hist = P->every ? (int)(3 * (int64_t)s + x) * P->nck : -1;  // checkpoint rows
+      sl.push_back(SlotDesc{fs, P->tmax, P->tmax - 1, x, 0, -1, x == 0 ? 3 : 0, hist});
+    }
   }
   wb.push_back((int)sl.size());
   P->nwg = (int)wb.size() - 1;
@@ -94,7 +100,8 @@ int build_slots(nise_abs_plan *P) {
 int run_tables(nise_abs_plan *P) {
   float fac = trotter_factor(P->non.deltat, 1, P->m);
   return launch_frame_tables(P->src, P->N, P->sch, P->ab.as<uint32_t>(), fac, P->f_lo,
-                             P->f_hi - P->f_lo, P->diag.as<float2>(), P->cs.as<float2>(), P->st);
+                             P->f_hi - P->f_lo, P->diag.as<float2>(), P->cs.as<float2>(), P->st,
+                             P->every ? P->jf.as<float>() : nullptr);
 }
 
 int run_conveyor(nise_abs_plan *P) {
@@ -107,6 +114,7 @@ int run_conveyor(nise_abs_plan *P) {
   a.slots = P->slots.as<SlotDesc>(), a.wg_slot_begin = P->wg_begin.as<int>(), a.nwg = P->nwg;
   a.tmax_acc = P->tmax, a.partial = P->partial.as<double2>();
   a.steps = P->steps.as<int2>(), a.nsteps = P->nsteps;
+  if (P->every) a.hist = P->ck.as<float2>(), a.hist_every = P->every;
   return launch_conveyor(a, P->vp, P->block, P->st);
 }
 
@@ -322,10 +330,104 @@ extern "C" int nise_abs_plan_info(const nise_abs_plan *P, int64_t *info, int n) 
   return k;
 }
 
-// ms of the last timed execute: frame tables, conveyor (or plain), reduce
+// ms of the last timed execute: frame tables, conveyor (or plain), reduce;
+// then the last adjoint run
 extern "C" int nise_abs_plan_last_timing(const nise_abs_plan *P, float *ms, int n) {
-  for (int i = 0; i < 3 && i < n; ++i) ms[i] = P->last_ms[i];
-  return 3;
+  for (int i = 0; i < 4 && i < n; ++i) ms[i] = P->last_ms[i];
+  return 4;
+}
+
+namespace {
+void drop_graph(nise_abs_plan *P) {
+  if (P->graph_ready) {
+    (void)hipGraphExecDestroy(P->exec), (void)hipGraphDestroy(P->graph);
+    P->exec = nullptr, P->graph = nullptr, P->graph_ready = false;
+  }
+}
+}  // namespace
+
+extern "C" int nise_abs_plan_set_params(nise_abs_plan *P, const float *delta, float cscale) {
+  if (!P || P->kernel != 0) return set_error("no conveyor plan"), NISE_ERR_ARG;
+  if (!std::isfinite(cscale)) return set_error("coupling scale must be finite"), NISE_ERR_ARG;
+  int rc;
+  if (delta) {
+    if ((rc = upload(P->delta, delta, (size_t)P->N * 4, P->st))) return rc;
+    P->src.delta = P->delta.as<float>();
+  } else {
+    P->src.delta = nullptr;
+  }
+  P->src.cscale = cscale;
+  drop_graph(P);  // the captured table launch holds the old parameters
+  return NISE_OK;
+}
+
+extern "C" int nise_abs_plan_enable_adjoint(nise_abs_plan *P, int every) {
+  if (!P || P->kernel != 0) return set_error("no conveyor plan"), NISE_ERR_ARG;
+  if (every < 1) return set_error("checkpoint interval must be >= 1"), NISE_ERR_ARG;
+  const int T = P->tmax, N = P->N;
+  const int64_t nslot = 3 * (int64_t)(P->end - P->begin);
+  const int nck = (T - 1) / every + 1 + ((T - 1) % every ? 1 : 0);
+  if (nslot * nck > INT32_MAX) return set_error("too many checkpoints"), NISE_ERR_RANGE;
+  int rc;
+  if ((rc = P->ck.alloc((size_t)nslot * nck * N * sizeof(float2)))) return rc;
+  if ((rc = P->jf.alloc((size_t)(P->f_hi - P->f_lo) * std::max(P->sch.K, 1) * 4))) return rc;
+  P->every = every, P->nck = nck;
+  // backward grouping: consecutive slots, adj_vp per single-wave workgroup
+  P->adj_vp = adjoint_vp(P->sch, N, T);
+  std::vector<SlotDesc> sl;
+  std::vector<int> wb;
+  for (int64_t j = 0; j < nslot; ++j) {
+    if (j % P->adj_vp == 0) wb.push_back((int)j);
+    const int s = (int)(j / 3), x = (int)(j % 3);
+    sl.push_back(SlotDesc{(P->begin + s) * P->sample, T, T - 1, x, 0, -1, 0, (int)(j * nck)});
+  }
+  wb.push_back((int)nslot);
+  P->adj_nwg = (int)wb.size() - 1;
+  std::vector<int2> stp;
+  adjoint_steps(P->sch, P->adj_vp, stp);
+  P->adj_nsteps = (int)stp.size();
+  if ((rc = upload(P->adj_slots, sl.data(), sl.size() * sizeof(SlotDesc), P->st)) ||
+      (rc = upload(P->adj_wg, wb.data(), wb.size() * 4, P->st)) ||
+      (rc = upload(P->adj_steps, stp.data(), stp.size() * sizeof(int2), P->st)) ||
+      (rc = P->gtab.alloc((size_t)T * sizeof(float2))) ||
+      (rc = P->part_d.alloc((size_t)std::max(P->adj_nwg, 1) * N * 8)) ||
+      (rc = P->part_c.alloc((size_t)std::max(P->adj_nwg, 1) * 8)) ||
+      (rc = P->grad.alloc((size_t)(N + 1) * 8)))
+    return rc;
+  drop_graph(P);
+  return build_slots(P);
+}
+
+// Gradient of L with respect to the per-site offsets and the coupling scale
+// of the last forward execute, given g = dL/dRe S + i dL/dIm S per step.
+extern "C" int nise_abs_plan_adjoint(nise_abs_plan *P, const float *g_re, const float *g_im,
+                                     double *grad_delta, double *grad_c) {
+  if (!P || !P->every) return set_error("adjoint not enabled on this plan"), NISE_ERR_ARG;
+  if (!g_re || !g_im) return set_error("null gradient input"), NISE_ERR_ARG;
+  const int T = P->tmax, N = P->N;
+  std::vector<float2> gh(T);
+  for (int i = 0; i < T; ++i) gh[i] = make_float2(g_re[i], g_im[i]);
+  NISE_HIP_CHECK(hipMemcpyAsync(P->gtab.p, gh.data(), T * sizeof(float2), hipMemcpyHostToDevice, P->st));
+  AdjArgs a;
+  a.N = N, a.K = P->sch.K, a.m = P->m, a.T = T, a.every = P->every, a.nck = P->nck;
+  a.fac = trotter_factor(P->non.deltat, 1, P->m);
+  a.f_lo = P->f_lo, a.ab = P->ab.as<uint32_t>(), a.cs = P->cs.as<float2>(), a.jf = P->jf.as<float>();
+  a.diag = P->diag.as<float2>(), a.mu_rec = P->mu_dev, a.mu_stride = P->mu_stride;
+  a.slots = P->adj_slots.as<SlotDesc>(), a.wg_begin = P->adj_wg.as<int>(), a.nwg = P->adj_nwg;
+  a.steps = P->adj_steps.as<int2>(), a.nsteps = P->adj_nsteps;
+  a.ck = P->ck.as<float2>(), a.g = P->gtab.as<float2>();
+  a.part_delta = P->part_d.as<double>(), a.part_c = P->part_c.as<double>();
+  NISE_HIP_CHECK(hipEventRecord(P->ev[0], P->st));
+  int rc = launch_adjoint(a, P->adj_vp, P->grad.as<double>(), P->st);
+  if (rc) return rc;
+  NISE_HIP_CHECK(hipEventRecord(P->ev[1], P->st));
+  std::vector<double> out(N + 1);
+  NISE_HIP_CHECK(hipMemcpyAsync(out.data(), P->grad.p, (N + 1) * 8, hipMemcpyDeviceToHost, P->st));
+  NISE_HIP_CHECK(hipStreamSynchronize(P->st));
+  NISE_HIP_CHECK(hipEventElapsedTime(&P->last_ms[3], P->ev[0], P->ev[1]));
+  if (grad_delta) std::memcpy(grad_delta, out.data(), N * 8);
+  if (grad_c) *grad_c = out[N];
+  return NISE_OK;
 }
 
 extern "C" void nise_abs_plan_destroy(nise_abs_plan *P) { delete P; }
@@ -537,15 +639,6 @@ extern "C" int nise_frame_tables(const t_non *non, const float *H_rec, int64_t H
   if ((rc = dg.alloc(N * sizeof(float2))) || (rc = cs.alloc((lex.size() + 1) * sizeof(float2))))
     return rc;
   float fac = trotter_factor(non->deltat, sign, non->ts);
-  if (getenv("NISE_DEBUG")) {
-    std::vector<float> back(H_stride);
-    NISE_HIP_CHECK(hipMemcpy(back.data(), H.p, H_stride * 4, hipMemcpyDeviceToHost));
-    fprintf(stderr, "nise_frame_tables: ctype=%d stride=%lld pos=%lld H_rec=%p Hdev=%p src.H_rec=%p\n",
-            (int)ctype, (long long)H_stride, (long long)pos, (const void *)H_rec, H.p,
-            (const void *)src.H_rec);
-    for (int64_t i = 0; i < H_stride && i < 12; ++i)
-      fprintf(stderr, "  host %g dev %g\n", H_rec[pos * H_stride + i], back[i]);
-  }
   if ((rc = launch_frame_tables(src, N, sch, ab.as<uint32_t>(), fac, pos, 1, dg.as<float2>(),
                                 cs.as<float2>(), st)))
     return rc;

@@ -82,6 +82,9 @@ struct FrameSource {
   const int *shift_site = nullptr;   // device
   const float *shift_ab = nullptr;   // device, (a, b) pairs
   int apply_shift = 1;
+  const float *delta = nullptr;  // device, per-site static offsets added to
+                                 // the shifted diagonal (gradient runs)
+  float cscale = 1.f;            // scale of the kept couplings
 };
 
 // ---------------------------------------------------------------------------
@@ -129,10 +132,12 @@ inline float trotter_factor(float deltat, int sign, int m) {
 // Launch wrappers (trotter.hip).
 int launch_union_mask(const FrameSource &src, int N, int64_t f_lo, int64_t nfr,
                       uint8_t *mask_dev, hipStream_t st);
+// jf_dev (optional, [rows][K]): fac * J of each kept coupling before the
+// coupling scale, 0 for absent ones (angle derivative for gradients).
 int launch_frame_tables(const FrameSource &src, int N, const Schedule &sch,
                         const uint32_t *ab_dev, float fac, int64_t f_lo,
                         int64_t nfr, float2 *diag_dev, float2 *cs_dev,
-                        hipStream_t st);
+                        hipStream_t st, float *jf_dev = nullptr);
 
 // One vector slot of the conveyor kernel.
 struct SlotDesc {
@@ -146,7 +151,9 @@ struct SlotDesc {
               // whose responses it sums; 0 for members
   int hist;   // history row: psi at frames fs .. fs+nresp-1 (after the
               // pending half-phase, before the next) goes to rows hist+rel
-              // of ConveyorArgs::hist; -1 none
+              // of ConveyorArgs::hist (hist_every 1), or only the
+              // checkpoints rel = 0, K, 2K, ... and nresp-1 to rows
+              // hist + 0, 1, 2, ... (hist_every K); -1 none
 };
 
 struct ConveyorArgs {
@@ -168,6 +175,7 @@ struct ConveyorArgs {
   const int2 *steps = nullptr;   // rotation steps (build_steps)
   int nsteps = 0;
   float2 *hist = nullptr;        // [rows][N] recorded states
+  int hist_every = 1;            // record every step, or checkpoints every K
 };
 
 int launch_conveyor(const ConveyorArgs &a, int vp, int block, hipStream_t st);
@@ -205,6 +213,31 @@ int launch_doubles_tables(const FrameSource &src, int N, int K2, const uint32_t 
                           float2 *D2, float4 *cs2, hipStream_t st);
 int launch_doubles_step(const DoublesArgs &a, int ngroups, hipStream_t st);
 int doubles_np(int N);
+
+// Adjoint of the batched absorption (adjoint.hip).
+struct AdjArgs {
+  int N = 0, K = 0, m = 1, T = 0, every = 1, nck = 0;
+  float fac = 0.f;
+  int64_t f_lo = 0;
+  const uint32_t *ab = nullptr;
+  const float2 *cs = nullptr;   // [rows][K] (co, si) with the coupling scale
+  const float *jf = nullptr;    // [rows][K] f * J before the scale
+  const float2 *diag = nullptr; // [rows][N] half-step phases with the offsets
+  const float *mu_rec = nullptr;
+  int64_t mu_stride = 0;
+  const SlotDesc *slots = nullptr;  // fs, x, hist = checkpoint row base
+  const int *wg_begin = nullptr;
+  int nwg = 0;
+  const int2 *steps = nullptr;  // rotation steps in forward order
+  int nsteps = 0;
+  const float2 *ck = nullptr;   // forward checkpoints
+  const float2 *g = nullptr;    // [T]
+  double *part_delta = nullptr; // [nwg][N]
+  double *part_c = nullptr;     // [nwg]
+};
+int launch_adjoint(const AdjArgs &a, int vp, double *out, hipStream_t st);
+int adjoint_vp(const Schedule &s, int N, int T);
+void adjoint_steps(const Schedule &s, int vp, std::vector<int2> &steps);
 
 // Library-wide stream and device setup.
 int ensure_device();

@@ -61,13 +61,14 @@ int launch_union_mask(const FrameSource &src, int N, int64_t f_lo, int64_t nfr, 
 // Per-frame tables: D phases and rotation (cos, sin) in schedule order.
 // ---------------------------------------------------------------------------
 __global__ void frame_tables_kernel(FrameSource s, int N, int K, const uint32_t *ab, float fac,
-                                    int64_t f_lo, float2 *diag, float2 *cs) {
+                                    int64_t f_lo, float2 *diag, float2 *cs, float *jf) {
   int64_t row = blockIdx.y;
   int64_t f = f_lo + row;
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < N) {
     // src/propagate.c:863-866: cos/sin of (0.5 * H0 * f) in double
     float h = site_energy(s, N, f, j);
+    if (s.delta) h = __fadd_rn(h, s.delta[j]);
     double arg = 0.5 * (double)h * (double)fac;
     diag[row * N + j] = make_float2((float)cos(arg), (float)(-sin(arg)));
   } else if (j < N + K) {
@@ -78,24 +79,26 @@ __global__ void frame_tables_kernel(FrameSource s, int N, int K, const uint32_t 
     float co = 1.f, si = 0.f;  // absent coupling: exact identity rotation
     if (J != 0.f) {
       // src/propagate.c:879-882: J = J*f; si = -sin(J); co = sqrt(1 - si*si)
-      float Jf = __fmul_rn(J, fac);
+      float Jf = __fmul_rn(__fmul_rn(J, s.cscale), fac);
       si = (float)(-sin((double)Jf));
       co = (float)sqrt((double)__fsub_rn(1.0f, __fmul_rn(si, si)));
     }
     cs[row * K + r] = make_float2(co, si);
+    if (jf) jf[row * K + r] = __fmul_rn(J, fac);
   }
 }
 
 int launch_frame_tables(const FrameSource &src, int N, const Schedule &sch, const uint32_t *ab_dev,
                         float fac, int64_t f_lo, int64_t nfr, float2 *diag_dev, float2 *cs_dev,
-                        hipStream_t st) {
+                        hipStream_t st, float *jf_dev) {
   int n = N + sch.K;
   int64_t done = 0;
   while (done < nfr) {  // grid.y is limited to 65535
     int64_t chunk = nfr - done < 65535 ? nfr - done : 65535;
     dim3 grid((unsigned)((n + 255) / 256), (unsigned)chunk);
     hipLaunchKernelGGL(frame_tables_kernel, grid, dim3(256), 0, st, src, N, sch.K, ab_dev, fac,
-                       f_lo + done, diag_dev + done * N, cs_dev + done * sch.K);
+                       f_lo + done, diag_dev + done * N, cs_dev + done * sch.K,
+                       jf_dev ? jf_dev + done * sch.K : nullptr);
     NISE_HIP_CHECK(hipGetLastError());
     done += chunk;
   }
@@ -138,6 +141,16 @@ __device__ __forceinline__ void rotate_pair(float2 &pa, float2 &pb, float co, fl
 // frame's rotations run.
 // ---------------------------------------------------------------------------
 constexpr int kRing = 8;
+
+// History row recorded at response step rel of a slot, or -1: every step
+// when every == 1; else the checkpoints rel = 0, every, 2*every, ... and
+// the last response step.
+__device__ __forceinline__ int64_t hist_row(const SlotDesc &sd, int64_t rel, bool resp, int every) {
+  if (!resp || sd.hist < 0) return -1;
+  if (rel % every == 0) return (int64_t)sd.hist + rel / every;
+  if (rel == sd.nresp - 1) return (int64_t)sd.hist + rel / every + 1;
+  return -1;
+}
 
 __device__ __forceinline__ void lds_barrier() {
   // LDS traffic of this wave complete, then the workgroup barrier; global
@@ -245,6 +258,7 @@ __global__ void __launch_bounds__(BLOCK) conveyor_kernel(ConveyorArgs A) {
     // mu(f).psi (src/absorption.c:130, calc_S1 :205-212), first half-phase
     // of frame f.
     float rr = 0.f, ri = 0.f;
+    const int64_t hr = hist_row(sd, rel, resp, A.hist_every);
     if (starts || prev_prop || resp || prop) {
       for (int i = q; i < N; i += kRows) {
         float2 p;
@@ -258,7 +272,7 @@ __global__ void __launch_bounds__(BLOCK) conveyor_kernel(ConveyorArgs A) {
           const float m = mu[i];
           rr = __fadd_rn(rr, __fmul_rn(m, p.x));
           ri = __fadd_rn(ri, __fmul_rn(m, p.y));
-          if (sd.hist >= 0) A.hist[((int64_t)sd.hist + rel) * N + i] = p;
+          if (hr >= 0) A.hist[hr * N + i] = p;
         }
         if (prop) p = cmul_diag(p, Dcur[i]);
         psi[i * VP + v] = p;
@@ -460,6 +474,7 @@ __global__ void __launch_bounds__(64) conveyor_w1_kernel(ConveyorArgs A) {
     const float2 *Dprev = Dsh + ((f + 1) & 1) * N;
     const float *mu = mush + sd.x * N;
     float rr = 0.f, ri = 0.f;
+    const int64_t hr = hist_row(sd, rel, resp, A.hist_every);
     if (starts || prev_prop || resp || prop) {
       for (int i = q; i < N; i += kRows) {
         float2 p;
@@ -473,7 +488,7 @@ __global__ void __launch_bounds__(64) conveyor_w1_kernel(ConveyorArgs A) {
           const float m = mu[i];
           rr = __fadd_rn(rr, __fmul_rn(m, p.x));
           ri = __fadd_rn(ri, __fmul_rn(m, p.y));
-          if (sd.hist >= 0) A.hist[((int64_t)sd.hist + rel) * N + i] = p;
+          if (hr >= 0) A.hist[hr * N + i] = p;
         }
         if (prop) p = cmul_diag(p, Dcur[i]);
         psi[i * VP + v] = p;
@@ -691,6 +706,7 @@ __global__ void __launch_bounds__(64) conveyor_s_kernel(ConveyorArgs A) {
   for (int64_t f = F0; f < F1; ++f) {
     const int64_t row = f - A.f_lo;
     bool starts[S], pprop[S], resp[S], prop[S];
+    int64_t hrk[S];
     bool any_site = false, any_prop = false;
 #pragma unroll
     for (int k = 0; k < S; ++k) {
@@ -700,6 +716,7 @@ __global__ void __launch_bounds__(64) conveyor_s_kernel(ConveyorArgs A) {
       pprop[k] = has && rel >= 1 && rel - 1 < sd[k].nprop;
       resp[k] = has && rel >= 0 && rel < sd[k].nresp;
       prop[k] = has && rel >= 0 && rel < sd[k].nprop;
+      hrk[k] = hist_row(sd[k], rel, resp[k], A.hist_every);
       any_site |= starts[k] || pprop[k] || resp[k] || prop[k];
       any_prop |= prop[k];
     }
@@ -721,7 +738,7 @@ __global__ void __launch_bounds__(64) conveyor_s_kernel(ConveyorArgs A) {
           if (resp[k]) {
             rr[k] = __fadd_rn(rr[k], __fmul_rn(m, p[k].x));
             ri[k] = __fadd_rn(ri[k], __fmul_rn(m, p[k].y));
-            if (sd[k].hist >= 0) A.hist[((int64_t)sd[k].hist + (f - sd[k].fs)) * N + i] = p[k];
+            if (hrk[k] >= 0) A.hist[hrk[k] * N + i] = p[k];
           }
           if (prop[k]) p[k] = cmul_diag(p[k], dc);
         }

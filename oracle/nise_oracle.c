@@ -90,7 +90,8 @@ void or_coupling_S_f32(int N, const float *H, float *cr, float *ci, int m,
 /* Same algorithm, double state and double trig (error reference). */
 static void coupling_S_f64_impl(int N, const float *H, double *cr, double *ci,
                                 int m, int sign, float deltat,
-                                float couplingcut, int inverse) {
+                                float couplingcut, int inverse,
+                                const double *delta, double cscale) {
   float ff = deltat * OR_icm2ifs * OR_twoPi * sign / m;
   double f = ff;
   int N2 = (N * (N - 1)) / 2;
@@ -103,15 +104,16 @@ static void coupling_S_f64_impl(int N, const float *H, double *cr, double *ci,
   int k = 0;
   for (int a = 0; a < N; a++) {
     double h0 = H[or_sindex(a, a, N)];
+    if (delta) h0 += delta[a];
     reU[a] = cos(0.5 * h0 * f);
     imU[a] = -sin(0.5 * h0 * f);
     if (inverse) imU[a] = -imU[a];
     for (int b = a + 1; b < N; b++) {
       float J = H[or_sindex(a, b, N)];
       if (fabs(J) > couplingcut) {
-        double s = -sin((double)J * f);
+        double s = -sin((double)J * cscale * f);
         si[k] = inverse ? -s : s;
-        co[k] = cos((double)J * f);
+        co[k] = cos((double)J * cscale * f);
         col[k] = a, row[k] = b;
         k++;
       }
@@ -145,12 +147,12 @@ static void coupling_S_f64_impl(int N, const float *H, double *cr, double *ci,
 
 void or_coupling_S_f64(int N, const float *H, double *cr, double *ci, int m,
                        int sign, float deltat, float couplingcut) {
-  coupling_S_f64_impl(N, H, cr, ci, m, sign, deltat, couplingcut, 0);
+  coupling_S_f64_impl(N, H, cr, ci, m, sign, deltat, couplingcut, 0, NULL, 1.0);
 }
 
 void or_coupling_S_inv_f64(int N, const float *H, double *cr, double *ci,
                            int m, int sign, float deltat, float couplingcut) {
-  coupling_S_f64_impl(N, H, cr, ci, m, sign, deltat, couplingcut, 1);
+  coupling_S_f64_impl(N, H, cr, ci, m, sign, deltat, couplingcut, 1, NULL, 1.0);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -587,6 +589,64 @@ int or_absorption(const t_non *non, const float *H_rec, long long H_stride,
       if (re_S64) re_S64[t1] += pr64[(size_t)t * tmax + t1], im_S64[t1] += pi64[(size_t)t * tmax + t1];
     }
   free(pr), free(pi), free(pr64), free(pi64);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* FP64 absorption with gradient-run parameters: delta[a] added to every    */
+/* frame's shifted site energy, kept couplings scaled by cscale (the        */
+/* parameterisation of nise_abs_plan_set_params).  Reference for central   */
+/* finite differences of the adjoint; propagation 1 only.                  */
+/* ---------------------------------------------------------------------- */
+int or_absorption_param_f64(const t_non *non, const float *H_rec, long long H_stride,
+                            const float *mu_rec, long long mu_stride,
+                            const float *C_rec, const double *delta, double cscale,
+                            double *re_S, double *im_S, int nthreads) {
+  int N = non->singles, tmax = non->tmax;
+  int nn2 = N * (N + 1) / 2;
+  if (non->propagation != 1) return NISE_ERR_ARG;
+  memset(re_S, 0, tmax * sizeof(double)), memset(im_S, 0, tmax * sizeof(double));
+  int nsamp = non->end - non->begin;
+  if (nthreads < 1) nthreads = 1;
+  double *pr = calloc((size_t)nthreads * tmax, sizeof(double));
+  double *pi = calloc((size_t)nthreads * tmax, sizeof(double));
+#pragma omp parallel num_threads(nthreads)
+  {
+    int tid = 0, nt = 1;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+    nt = omp_get_num_threads();
+#endif
+    int s0 = non->begin + (int)((long long)nsamp * tid / nt);
+    int s1 = non->begin + (int)((long long)nsamp * (tid + 1) / nt);
+    float *He = malloc(nn2 * sizeof(float));
+    float *mu = malloc(N * sizeof(float));
+    double *dr = malloc(N * sizeof(double)), *di = malloc(N * sizeof(double));
+    double *myr = pr + (size_t)tid * tmax, *myi = pi + (size_t)tid * tmax;
+    for (int samples = s0; samples < s1; samples++) {
+      int ti = samples * non->sample;
+      for (int x = 0; x < 3; x++) {
+        read_mu(non, mu_rec, mu_stride, ti, x, mu);
+        for (int i = 0; i < N; i++) dr[i] = mu[i], di[i] = 0;
+        for (int t1 = 0; t1 < tmax; t1++) {
+          int tj = ti + t1;
+          or_read_H(non, H_rec, H_stride, C_rec, tj, He, 1);
+          read_mu(non, mu_rec, mu_stride, tj, x, mu);
+          for (int i = 0; i < N; i++) {
+            myr[t1] += (double)mu[i] * dr[i];
+            myi[t1] += (double)mu[i] * di[i];
+          }
+          coupling_S_f64_impl(N, He, dr, di, non->ts, 1, non->deltat, non->couplingcut, 0,
+                              delta, cscale);
+        }
+      }
+    }
+    free(He), free(mu), free(dr), free(di);
+  }
+  for (int t = 0; t < nthreads; t++)
+    for (int t1 = 0; t1 < tmax; t1++)
+      re_S[t1] += pr[(size_t)t * tmax + t1], im_S[t1] += pi[(size_t)t * tmax + t1];
+  free(pr), free(pi);
   return 0;
 }
 

@@ -24,10 +24,15 @@
  *   or_absorption : per-step composition of the pinned primitives; compared
  *     with the reference-primitive driver in oracle/_ref.
  *   or_1dfft : numpy.fft with the reference's re/im swap and scaling.
- *   or_2dir_item : composition of pinned primitives; N=1 three-level closed
- *     form in tests.  The order and signs of the 12 accumulators follow
- *     src/calc_2DIR.c line by line; calc_2DIR.c itself needs MPI and is not
- *     built here, so that part is "parity unpinned" beyond the closed form.
+ *   or_absorption_param_f64 : or_absorption's FP64 path with the gradient
+ *     parameters; at delta = 0, cscale = 1 it equals or_absorption's FP64
+ *     result exactly (tests), and an energy offset on an uncoupled site is a
+ *     pure phase e^{-i f delta t} (closed form in tests).
+ *   or_2dir_item : composition of pinned primitives, checked against the
+ *     closed form of uncoupled anharmonic sites (each pathway alone, the
+ *     two-level sum, per-frame anharmonicities and overtone dipoles).  The
+ *     order and signs of the 12 accumulators follow src/calc_2DIR.c line by
+ *     line; calc_2DIR.c itself needs MPI and is not built here.
  */
 #ifndef NISE_ORACLE_H
 #define NISE_ORACLE_H
@@ -81,6 +86,12 @@ int or_absorption(const t_non *non, const float *H_rec, long long H_stride,
 int or_1dfft(const t_non *non, const float *re_S, const float *im_S,
              int samples, double *freq, double *spec_r, double *spec_i,
              int *nout);
+
+int or_absorption_param_f64(const t_non *non, const float *H_rec,
+                            long long H_stride, const float *mu_rec,
+                            long long mu_stride, const float *C_rec,
+                            const double *delta, double cscale,
+                            double *re_S, double *im_S, int nthreads);
 
 int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
                  const float *mu_rec, long long mu_stride, const float *C_rec,

@@ -61,6 +61,8 @@ def oracle():
     L.or_absorption.argtypes = [P, _vp, C.c_longlong, _vp, C.c_longlong, _vp, _vp, _vp, _vp, _vp,
                                 C.c_int]
     L.or_1dfft.argtypes = [P, _f32, _f32, C.c_int, _f64, _f64, _f64, C.POINTER(C.c_int)]
+    L.or_absorption_param_f64.argtypes = [P, _vp, C.c_longlong, _vp, C.c_longlong, _vp, _vp,
+                                          C.c_double, _f64, _f64, C.c_int]
     L.or_2dir_item.argtypes = [P, _vp, C.c_longlong, _vp, C.c_longlong, _vp, _vp, _vp, C.c_int,
                                C.c_int, _f32]
     _o = L
@@ -154,3 +156,19 @@ def twodir(non, H, mu, C_rec=None, A_rec=None, over_rec=None):
                                        ptr(Cr), ptr(Ar), ptr(Or), s, m, R)
             assert rc == 0
     return R
+
+
+def absorption_param_f64(non, H, mu, delta=None, cscale=1.0, C_rec=None, nthreads=1):
+    """FP64 oracle absorption with per-site offsets delta (N) and coupling
+    scale cscale: returns (re_S, im_S) float64."""
+    T = non.tmax1
+    H = np.ascontiguousarray(H, np.float32)
+    mu = np.ascontiguousarray(mu, np.float32)
+    Cr = None if C_rec is None else np.ascontiguousarray(C_rec, np.float32)
+    d = None if delta is None else np.ascontiguousarray(delta, np.float64)
+    re = np.zeros(T, np.float64)
+    im = np.zeros(T, np.float64)
+    rc = oracle().or_absorption_param_f64(C.byref(non), ptr(H), H.shape[1], ptr(mu), mu.shape[1],
+                                          ptr(Cr), ptr(d), float(cscale), re, im, nthreads)
+    assert rc == 0
+    return re, im

@@ -21,7 +21,8 @@ def test_header_declares_the_drop_in_symbols():
     for must in ("propagate_vector", "propagate_matrix", "propagate_vec_coupling_S",
                  "propagate_vec_coupling_S_doubles", "propagate_vec_DIA", "diagonalizeLPD",
                  "propagate_double_2DIR_control", "calc_S1", "absorption", "do_1DFFT",
-                 "calc_2DIR", "nise_absorption_batched"):
+                 "calc_2DIR", "nise_absorption_batched", "nise_2dir_batched",
+                 "nise_abs_plan_create", "nise_abs_plan_adjoint"):
         assert must in names, must
 
 
