@@ -72,6 +72,83 @@ def host_records(diag, row, mus, f_hi):
     return H, synth.dipole_records(mus, f_hi)
 
 
+def adjoint_leg(args, torch):
+    """Gradient of the absorption spectrum for the N=128 chain: 20,000
+    frames, tmax 256, 1000 samples every 19 frames, Trotter 1; parameters
+    128 site offsets (0) and the coupling scale (1); loss |spec - target|^2
+    on the 2048-point grid, target from offsets N(0, 5) and scale 1.1.
+    Reports forward and adjoint times per checkpoint interval K, memory
+    against storing every state, and the gradient against central finite
+    differences of the FP64 oracle (h = 0.01 cm^-1, 1e-4 for the scale)."""
+    import numpy as np
+
+    from nise_2017_amd import abi, adjoint, synth
+    N, T, sr, ns, nfr = 128, 256, 19, 1000, 20000
+    log("adjoint leg: N=128 chain, 1000 samples")
+    H, mu = synth.chain_absorption(N=N, nframes=nfr, seed=20261202 + 3)
+    need = (ns - 1) * sr + T
+    lo, hi = synth.shifted_center(H[:need], N)
+    non = abi.make_non(singles=N, tmax1=T, sample=sr, begin=0, end=ns, ts=1, propagation=1,
+                       deltat=2.0, min1=lo, max1=hi, fft=2048, lifetime=1300.0)
+    non.shifte = np.float32((np.float32(hi) + np.float32(lo)) / np.float32(2))
+    Hd, mud = torch.from_numpy(H).cuda(), torch.from_numpy(mu).cuda()
+    freq, A_re, A_im = adjoint.spectrum_matrix(non, ns)
+    rng = np.random.default_rng(20261202 + 3)
+    d_t = rng.normal(0, 5, N).astype(np.float32)
+    out = {"workload": "N=128 chain, 20000 frames, tmax 256, 1000 samples every 19 frames, 3 pols",
+           "params": N + 1, "grid_points": int(len(freq)), "runs": []}
+    gref = None
+    for K in [int(k) for k in args.adjoint_every.split(",")] + [1]:
+        model = adjoint.AdjointAbsorption(non, Hd, mud, every=K)
+        if gref is None:
+            St = model.forward(d_t, 1.1)
+            target = A_re @ St[0] + A_im @ St[1]
+        model.forward(None, 1.0, timed=True)  # warm
+        S = model.forward(None, 1.0, timed=True)
+        t = model.timing()
+        fwd = t["tables_ms"] + t["conveyor_ms"] + t["reduce_ms"]
+        r = A_re @ S[0] + A_im @ S[1] - target
+        g_re, g_im = 2 * A_re.T @ r, 2 * A_im.T @ r
+        model.backward(g_re, g_im)
+        gd, gc = model.backward(g_re, g_im)
+        adj = model.timing()["adjoint_ms"]
+        nck = (T - 1) // K + 1 + (1 if (T - 1) % K else 0)
+        out["runs"].append({"K": K, "forward_ms": round(fwd, 3), "adjoint_ms": round(adj, 3),
+                            "adjoint_over_forward": round(adj / fwd, 3),
+                            "checkpoint_bytes": 3 * ns * nck * N * 8,
+                            "all_states_bytes": 3 * ns * T * N * 8})
+        log(f"adjoint K={K}: forward {fwd:.2f} ms, adjoint {adj:.2f} ms")
+        if gref is None:
+            gref = (gd, gc, float(r @ r))
+        model.close()
+    del Hd, mud
+    if args.fd_params > 0 and not args.no_cpu:
+        from oracle import oracle
+        threads = max(1, min(16, os.cpu_count() or 1))
+        gd, gc, L0 = gref
+
+        def L(d, c):
+            re, im = oracle.absorption_param_f64(non, H, mu, d, c, nthreads=threads)
+            rr = A_re @ re + A_im @ im - target
+            return float(rr @ rr)
+        idx = list(np.argsort(-np.abs(gd))[:args.fd_params])
+        fd = []
+        d0 = np.zeros(N)
+        for a in idx:
+            e = np.zeros(N)
+            e[a] = 0.01
+            fd.append((L(d0 + e, 1.0) - L(d0 - e, 1.0)) / 0.02)
+        fdc = (L(d0, 1.0 + 1e-4) - L(d0, 1.0 - 1e-4)) / 2e-4
+        g_sel = np.array([gd[a] for a in idx] + [gc])
+        g_fd = np.array(fd + [fdc])
+        out["fd_check"] = {"params": [f"delta[{a}]" for a in idx] + ["cscale"],
+                           "gpu": [float(x) for x in g_sel], "fd_fp64_oracle": [float(x) for x in g_fd],
+                           "rel_err": float(np.linalg.norm(g_sel - g_fd) / np.linalg.norm(g_fd)),
+                           "h": {"delta": 0.01, "cscale": 1e-4}}
+        log(f"adjoint FD check rel err {out['fd_check']['rel_err']:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +164,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--shape", default="", help="vp,spw,block conveyor shape override")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the N=128 adjoint leg")
+    ap.add_argument("--adjoint-every", default="16,32,64", help="checkpoint intervals K")
+    ap.add_argument("--fd-params", type=int, default=3, help="offsets checked by finite differences")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -217,6 +297,9 @@ def main():
         err = float(np.linalg.norm((g_re - o_re) + 1j * (g_im - o_im)) /
                     np.linalg.norm(o_re + 1j * o_im))
         extra["parity_rel_l2_vs_fp32_oracle"] = err
+
+    if not args.no_adjoint:
+        extra["adjoint"] = adjoint_leg(args, torch)
 
     line = {
         "metric": "site-steps/s propagated (N=256 absorption) vs plain GPU; adjoint/forward cost",

@@ -220,14 +220,19 @@ int nise_abs_plan_create(nise_abs_plan **plan, const t_non *non,
  * (multi-wave), 64 (one wave) or -2 / -4 (one wave, 2 or 4 slots per lane). */
 int nise_abs_plan_set_shape(nise_abs_plan *plan, int vp, int samples_per_wg,
                             int block);
+/* Row conveyor (the default): one wave per workgroup holding 64/rows
+ * vectors, rows rotations of a layer at a time (1, 2, 4, 8 or 16; 0 picks
+ * the count from the coupling structure). */
+int nise_abs_plan_set_rows(nise_abs_plan *plan, int rows);
 /* Run the plan; re_S / im_S (host, tmax1 each) may be NULL to leave the
  * result on the device.  flags bit 0: time the stages with HIP events;
  * bit 1: launch the stages as one captured hipGraph. */
 int nise_abs_plan_execute(nise_abs_plan *plan, float *re_S, float *im_S,
                           int flags);
 int nise_abs_plan_sync(nise_abs_plan *plan);
-/* info[0..9] = N, rotations per sweep, layers, workgroups, vp, samples per
- * workgroup, first frame, frames, block, steps; returns the count. */
+/* info[0..11] = N, schedule entries per sweep, layers, workgroups, vp,
+ * samples per workgroup, first frame, frames, block, steps, rows (0: slot
+ * conveyor), rotations per sweep; returns the count. */
 int nise_abs_plan_info(const nise_abs_plan *plan, int64_t *info, int n);
 /* ms of the last timed execute (tables, conveyor or plain kernel, reduce)
  * and of the last adjoint run; returns the count (4). */

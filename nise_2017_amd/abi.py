@@ -99,6 +99,13 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with python -m nise_2017_amd.build")
+    # torch bundles its own libamdhip64.so.7 under the same soname as ROCm's.
+    # Whichever is loaded first serves the whole process, and torch's own
+    # device code needs its copy: load torch (when installed) first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     P = C.POINTER(TNon)
     L.nise_last_error.restype = C.c_char_p
@@ -109,6 +116,7 @@ def lib():
                                        C.c_int64, _vp, C.c_int, C.c_int]
     L.nise_abs_plan_execute.argtypes = [_vp, _vp, _vp, C.c_int]
     L.nise_abs_plan_set_shape.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+    L.nise_abs_plan_set_rows.argtypes = [_vp, C.c_int]
     L.nise_abs_plan_info.argtypes = [_vp, C.POINTER(C.c_int64), C.c_int]
     L.nise_abs_plan_last_timing.argtypes = [_vp, C.POINTER(C.c_float), C.c_int]
     L.nise_abs_plan_sync.argtypes = [_vp]
@@ -181,10 +189,15 @@ class AbsPlan:
     def sync(self):
         check(lib().nise_abs_plan_sync(self.h))
 
+    def set_rows(self, rows=0):
+        """Row conveyor with the given row count (0: from the structure)."""
+        check(lib().nise_abs_plan_set_rows(self.h, int(rows)))
+
     def info(self):
-        a = (C.c_int64 * 10)()
-        lib().nise_abs_plan_info(self.h, a, 10)
-        keys = ("N", "K", "nlayers", "nwg", "vp", "spw", "f_lo", "nframes", "block", "nsteps")
+        a = (C.c_int64 * 12)()
+        lib().nise_abs_plan_info(self.h, a, 12)
+        keys = ("N", "K", "nlayers", "nwg", "vp", "spw", "f_lo", "nframes", "block", "nsteps",
+                "rows", "rotations")
         return dict(zip(keys, list(a)))
 
     def timing(self):

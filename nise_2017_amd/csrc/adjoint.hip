@@ -119,9 +119,9 @@ __global__ void __launch_bounds__(64) adjoint_kernel(AdjArgs A) {
       for (int s = A.nsteps - 1; s >= 0; --s) {
         const int2 st = A.steps[s];
         const int r = st.x + q;
-        if (step && r < st.y) {
-          const uint32_t pk = A.ab[r];
-          const int a = (int)(pk & 0xffffu), b = (int)(pk >> 16);
+        const uint32_t pk = (step && r < st.y) ? A.ab[r] : 0u;
+        const int a = (int)(pk & 0xffffu), b = (int)(pk >> 16);
+        if (step && r < st.y && a < N) {  // a >= N: row-schedule padding
           const float2 c = A.cs[row * A.K + r];
           const float j = A.jf[row * A.K + r];
           float2 pa = phi[a * VP + v], pb = phi[b * VP + v];

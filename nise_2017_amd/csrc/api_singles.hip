@@ -32,14 +32,17 @@ struct nise_abs_plan {
   const float *mu_dev = nullptr;
   int64_t mu_stride = 0;
   DevBuf H_own, mu_own, C_own, shift_site, shift_ab;
-  Schedule sch;
-  DevBuf ab, layer_off, diag, cs, slots, wg_begin, partial, out, steps;
+  Schedule sch;                // active schedule (one of the two below)
+  Schedule sch_asap, sch_rows; // layered (slot conveyor) / row schedule
+  int rows = 0;                // rows of the row conveyor, 0: slot conveyor
+  DevBuf ab, layer_off, diag, cs, slots, wg_begin, partial, out, steps, row_steps;
   int nsteps = 0;
   DevBuf ws_cs, ws_ab, resp_plain;
   int vp = 16, spw = 5, nwg = 0, block = 256;
   // gradient runs: parameters and the adjoint's buffers (every == 0: off)
   DevBuf delta, jf, ck, gtab, part_d, part_c, grad, adj_slots, adj_wg, adj_steps;
   int every = 0, nck = 0, adj_vp = 0, adj_nwg = 0, adj_nsteps = 0;
+  std::vector<uint8_t> mask;   // union coupling structure
   hipStream_t st = nullptr;
   hipEvent_t ev[4] = {};
   float last_ms[4] = {0, 0, 0, 0};
@@ -70,9 +73,31 @@ int upload(DevBuf &dst, const void *src, size_t bytes, hipStream_t st) {
 
 bool is_coupling_type(const t_non *non) { return std::strcmp(non->hamiltonian, "Coupling") == 0; }
 
+// Row conveyor set-up: schedule, step table, tables and partials.
+int build_rows(nise_abs_plan *P) {
+  int rc;
+  P->sch = P->sch_rows;
+  std::vector<uint2> tab;
+  rows_step_table(P->sch, P->rows, tab);
+  if ((rc = upload(P->row_steps, tab.data(), tab.size() * sizeof(uint2), P->st))) return rc;
+  if ((rc = upload(P->ab, P->sch.ab.data(), P->sch.ab.size() * 4, P->st))) return rc;
+  if ((rc = upload(P->layer_off, P->sch.layer_off.data(), P->sch.layer_off.size() * 4, P->st))) return rc;
+  const int64_t nfr = P->f_hi - P->f_lo;
+  if ((rc = P->cs.alloc((size_t)nfr * std::max(P->sch.K, 1) * sizeof(float2)))) return rc;
+  if (P->every && (rc = P->jf.alloc((size_t)nfr * std::max(P->sch.K, 1) * 4))) return rc;
+  const int G = 64 / P->rows;
+  P->nwg = (3 * (P->end - P->begin) + G - 1) / G;
+  return P->partial.alloc((size_t)std::max(P->nwg, 1) * P->tmax * sizeof(double2));
+}
+
 // Slot assignment: consecutive samples, three polarisations each, spw
 // samples per workgroup (3*spw <= vp slots).
 int build_slots(nise_abs_plan *P) {
+  if (P->rows) return build_rows(P);
+  int rc0;
+  P->sch = P->sch_asap;
+  if ((rc0 = upload(P->ab, P->sch.ab.data(), P->sch.ab.size() * 4, P->st))) return rc0;
+  if ((rc0 = upload(P->layer_off, P->sch.layer_off.data(), P->sch.layer_off.size() * 4, P->st))) return rc0;
   std::vector<SlotDesc> sl;
   std::vector<int> wb;
   int ns = P->end - P->begin;
@@ -105,6 +130,15 @@ int run_tables(nise_abs_plan *P) {
 }
 
 int run_conveyor(nise_abs_plan *P) {
+  if (P->rows) {
+    RowsArgs r;
+    r.N = P->N, r.tmax = P->tmax, r.m = P->m, r.nlayers = P->sch.nlayers(), r.K = P->sch.K;
+    r.f_lo = P->f_lo, r.nvec = 3 * (P->end - P->begin), r.begin = P->begin, r.sample = P->sample;
+    r.steps = P->row_steps.as<uint2>(), r.cs = P->cs.as<float2>(), r.diag = P->diag.as<float2>();
+    r.mu_rec = P->mu_dev, r.mu_stride = P->mu_stride, r.partial = P->partial.as<double2>();
+    if (P->every) r.hist = P->ck.as<float2>(), r.hist_every = P->every, r.nck = P->nck;
+    return launch_rows(r, P->rows, P->st);
+  }
   ConveyorArgs a;
   a.N = P->N, a.K = P->sch.K, a.nlayers = P->sch.nlayers(), a.m = P->m;
   a.f_lo = P->f_lo;
@@ -238,21 +272,24 @@ extern "C" int nise_abs_plan_create(nise_abs_plan **out, const t_non *non, const
       NISE_HIP_CHECK(hipMemcpyAsync(mask.data(), mk.p, npair, hipMemcpyDeviceToHost, P->st));
       NISE_HIP_CHECK(hipStreamSynchronize(P->st));
     }
-    build_schedule(N, mask, P->sch);
-    if ((rc = upload(P->ab, P->sch.ab.data(), P->sch.ab.size() * 4, P->st))) return rc;
-    if ((rc = upload(P->layer_off, P->sch.layer_off.data(), P->sch.layer_off.size() * 4, P->st))) return rc;
+    build_schedule(N, mask, P->sch_asap);
     int64_t nfr = P->f_hi - P->f_lo;
     if ((rc = P->diag.alloc((size_t)nfr * N * sizeof(float2)))) return rc;
-    if ((rc = P->cs.alloc((size_t)nfr * std::max(P->sch.K, 1) * sizeof(float2)))) return rc;
-    // default shape: as many samples per workgroup as fit 16 slots, wider
-    // when LDS allows and the run is long
+    if ((rc = P->cs.alloc((size_t)nfr * std::max(P->sch_asap.K, 1) * sizeof(float2)))) return rc;
+    // default: the row conveyor with rows matched to the layer width; the
+    // slot conveyor's shape (set_shape) stays available
     P->vp = 16, P->spw = 5;
+    P->rows = rows_for(N, P->sch_asap.Kreal, P->sch_asap.nlayers());
+    build_schedule_rows(N, mask, P->rows, P->sch_rows);
+    P->mask = mask;
     if ((rc = build_slots(P))) return rc;
   }
   NISE_HIP_CHECK(hipStreamSynchronize(P->st));
   *out = guard.release();
   return NISE_OK;
 }
+
+extern "C" int nise_abs_plan_enable_adjoint(nise_abs_plan *P, int every);
 
 extern "C" int nise_abs_plan_set_shape(nise_abs_plan *P, int vp, int samples_per_wg, int block) {
   if (!P || P->kernel != 0) return set_error("no conveyor plan"), NISE_ERR_ARG;
@@ -261,12 +298,33 @@ extern "C" int nise_abs_plan_set_shape(nise_abs_plan *P, int vp, int samples_per
       !(block == 64 || block == 256 || block == 512 || block == -2 || block == -4) ||
       (block > 0 && vp > block))
     return set_error("bad shape vp=%d spw=%d block=%d", vp, samples_per_wg, block), NISE_ERR_ARG;
-  P->vp = vp, P->spw = samples_per_wg, P->block = block;
+  P->vp = vp, P->spw = samples_per_wg, P->block = block, P->rows = 0;
   if (P->graph_ready) {
     (void)hipGraphExecDestroy(P->exec), (void)hipGraphDestroy(P->graph);
     P->exec = nullptr, P->graph = nullptr, P->graph_ready = false;
   }
-  return build_slots(P);
+  int rc = build_slots(P);
+  if (rc) return rc;
+  return P->every ? nise_abs_plan_enable_adjoint(P, P->every) : NISE_OK;
+}
+
+// Row conveyor with R rows (1, 2, 4, 8 or 16; 0: chosen from the schedule).
+extern "C" int nise_abs_plan_set_rows(nise_abs_plan *P, int R) {
+  if (!P || P->kernel != 0) return set_error("no conveyor plan"), NISE_ERR_ARG;
+  if (R == 0) R = rows_for(P->N, P->sch_asap.Kreal, P->sch_asap.nlayers());
+  if (!(R == 1 || R == 2 || R == 4 || R == 8 || R == 16))
+    return set_error("rows must be 1, 2, 4, 8 or 16, got %d", R), NISE_ERR_ARG;
+  if (rows_smem(P->N, R) > 160 * 1024)
+    return set_error("N=%d does not fit the row conveyor with %d rows", P->N, R), NISE_ERR_ARG;
+  if (R != P->rows || P->sch_rows.ab.empty()) build_schedule_rows(P->N, P->mask, R, P->sch_rows);
+  P->rows = R;
+  if (P->graph_ready) {
+    (void)hipGraphExecDestroy(P->exec), (void)hipGraphDestroy(P->graph);
+    P->exec = nullptr, P->graph = nullptr, P->graph_ready = false;
+  }
+  int rc = build_slots(P);
+  if (rc) return rc;
+  return P->every ? nise_abs_plan_enable_adjoint(P, P->every) : NISE_OK;
 }
 
 // flags bit 0: time the stages with HIP events (no graph)
@@ -321,10 +379,11 @@ extern "C" int nise_abs_plan_sync(nise_abs_plan *P) {
   return NISE_OK;
 }
 
-// info: N, K, nlayers, nwg, vp, samples per workgroup, first frame, frames
+// info: N, entries per sweep, layers, workgroups, vp, samples per workgroup,
+// first frame, frames, block, steps, rows (0: slot conveyor), rotations
 extern "C" int nise_abs_plan_info(const nise_abs_plan *P, int64_t *info, int n) {
   int64_t v[] = {P->N, P->sch.K, P->sch.nlayers(), P->nwg, P->vp, P->spw, P->f_lo, P->f_hi - P->f_lo,
-                  P->block, P->nsteps};
+                  P->block, P->nsteps, P->rows, P->sch.Kreal};
   int k = (int)(sizeof(v) / sizeof(v[0]));
   for (int i = 0; i < k && i < n; ++i) info[i] = v[i];
   return k;

@@ -60,7 +60,8 @@ struct DevBuf {
 // in the same order as the reference's sequential sweep.
 struct Schedule {
   int N = 0;
-  int K = 0;  // rotations per sweep (union of kept couplings)
+  int K = 0;      // entries per sweep (rotations, plus padding in row schedules)
+  int Kreal = 0;  // rotations of the union coupling structure
   std::vector<uint32_t> ab;
   std::vector<int> layer_off;
   int nlayers() const { return (int)layer_off.size() - 1; }
@@ -69,6 +70,9 @@ struct Schedule {
 // Build the layered schedule from a kept-coupling mask over the strict upper
 // triangle, enumerated in lexicographic (a<b) order.
 void build_schedule(int N, const std::vector<uint8_t> &mask, Schedule &s);
+// Row schedule: layers of exactly R entries (rotations, then dummy pairs
+// (N+q, N+q) in the unused rows q), for the row conveyor.
+void build_schedule_rows(int N, const std::vector<uint8_t> &mask, int R, Schedule &s);
 
 // Singles parameters shared by the frame-table kernels.
 struct FrameSource {
@@ -118,6 +122,23 @@ __device__ inline float kept_coupling(const FrameSource &s, int N, int64_t f, in
   float J = raw_coupling(s, N, f, a, b);
   if (fabsf(J) < s.cut) J = 0.f;
   return (fabsf(J) > s.cut) ? J : 0.f;
+}
+
+// Complex helpers with the reference's operation order.
+// src/propagate.c:871-872 / :892-893: (cr*re - ci*im, ci*re + cr*im)
+__device__ __forceinline__ float2 cmul_diag(float2 c, float2 u) {
+  return make_float2(__fsub_rn(__fmul_rn(c.x, u.x), __fmul_rn(c.y, u.y)),
+                     __fadd_rn(__fmul_rn(c.y, u.x), __fmul_rn(c.x, u.y)));
+}
+
+// src/propagate.c:883-887
+__device__ __forceinline__ void rotate_pair(float2 &pa, float2 &pb, float co, float si) {
+  float cr1 = __fsub_rn(__fmul_rn(co, pa.x), __fmul_rn(si, pb.y));
+  float ci1 = __fadd_rn(__fmul_rn(co, pa.y), __fmul_rn(si, pb.x));
+  float cr2 = __fsub_rn(__fmul_rn(co, pb.x), __fmul_rn(si, pa.y));
+  float ci2 = __fadd_rn(__fmul_rn(co, pb.y), __fmul_rn(si, pa.x));
+  pa = make_float2(cr1, ci1);
+  pb = make_float2(cr2, ci2);
 }
 
 // Reference time factor f = deltat*icm2ifs*twoPi*sign/m (src/propagate.c:836).
@@ -213,6 +234,27 @@ int launch_doubles_tables(const FrameSource &src, int N, int K2, const uint32_t 
                           float2 *D2, float4 *cs2, hipStream_t st);
 int launch_doubles_step(const DoublesArgs &a, int ngroups, hipStream_t st);
 int doubles_np(int N);
+
+// Row conveyor (rows.hip).  Vector v = 3 s + x is polarisation x of sample
+// begin + s, starting at frame (begin + s) * sample.
+struct RowsArgs {
+  int N = 0, tmax = 0, m = 1, nlayers = 0;  // nlayers: multiple of 8
+  int64_t K = 0;                 // entries per sweep (= nlayers * R)
+  int64_t f_lo = 0;
+  int nvec = 0, begin = 0, sample = 1;
+  const uint2 *steps = nullptr;  // [nlayers * R] LDS byte offsets (a, b)
+  const float2 *cs = nullptr;    // [rows][K] (cos, sin) in row-schedule order
+  const float2 *diag = nullptr;  // [rows][N]
+  const float *mu_rec = nullptr;
+  int64_t mu_stride = 0;
+  double2 *partial = nullptr;    // [waves][tmax]
+  float2 *hist = nullptr;        // checkpoints (hist_every, nck per vector)
+  int hist_every = 1, nck = 0;
+};
+int launch_rows(const RowsArgs &a, int R, hipStream_t st);
+size_t rows_smem(int N, int R);
+int rows_for(int N, int Kreal, int nlayers_asap);
+void rows_step_table(const Schedule &s, int R, std::vector<uint2> &tab);
 
 // Adjoint of the batched absorption (adjoint.hip).
 struct AdjArgs {

@@ -1,5 +1,6 @@
 // Host runtime of libnise_hip: error state, device buffers, the layered
 // rotation schedule, device and stream setup.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
@@ -46,6 +47,7 @@ void DevBuf::release() {
 // one layer share no site.
 void build_schedule(int N, const std::vector<uint8_t> &mask, Schedule &s) {
   s.N = N;
+  s.ab.clear();
   std::vector<int> la, lb, lay;
   std::vector<int> last(N, -1);
   int nl = 0;
@@ -58,7 +60,7 @@ void build_schedule(int N, const std::vector<uint8_t> &mask, Schedule &s) {
       la.push_back(a), lb.push_back(b), lay.push_back(l);
       if (l + 1 > nl) nl = l + 1;
     }
-  s.K = (int)la.size();
+  s.K = s.Kreal = (int)la.size();
   s.layer_off.assign(nl + 1, 0);
   for (int l : lay) s.layer_off[l + 1]++;
   for (int l = 0; l < nl; ++l) s.layer_off[l + 1] += s.layer_off[l];
@@ -66,6 +68,44 @@ void build_schedule(int N, const std::vector<uint8_t> &mask, Schedule &s) {
   s.ab.assign(s.K, 0);
   for (int k = 0; k < s.K; ++k)  // stable within a layer
     s.ab[fill[lay[k]]++] = (uint32_t)la[k] | ((uint32_t)lb[k] << 16);
+}
+
+// Layers of at most R rotations: each rotation of the lexicographic list
+// goes to the first layer after the last ones touching its sites that still
+// has room, so every site keeps its order.  Each layer is padded to exactly
+// R entries; the padding entry of row q is the dummy pair (N + q, N + q),
+// which the table kernel turns into an identity and the kernels apply to a
+// scratch site.  The layer count is padded to a multiple of kMinLayers.
+void build_schedule_rows(int N, const std::vector<uint8_t> &mask, int R, Schedule &s) {
+  constexpr int kMinLayers = 8;
+  s.N = N;
+  std::vector<int> last(N, -1);
+  std::vector<std::vector<uint32_t>> layers;
+  int64_t p = 0;
+  int K = 0;
+  for (int a = 0; a < N; ++a)
+    for (int b = a + 1; b < N; ++b, ++p) {
+      if (!mask[p]) continue;
+      size_t l = (size_t)(1 + (last[a] > last[b] ? last[a] : last[b]));
+      while (l < layers.size() && (int)layers[l].size() >= R) ++l;
+      if (l >= layers.size()) layers.resize(l + 1);
+      layers[l].push_back((uint32_t)a | ((uint32_t)b << 16));
+      last[a] = last[b] = (int)l;
+      ++K;
+    }
+  // at least kMinLayers layers, and a multiple of them (the kernel's
+  // operand ring runs across sweeps)
+  const size_t nl = std::max<size_t>(kMinLayers, (layers.size() + kMinLayers - 1) / kMinLayers * kMinLayers);
+  layers.resize(nl);
+  s.Kreal = K;
+  s.K = (int)layers.size() * R;
+  s.ab.clear();
+  s.layer_off.assign(1, 0);
+  for (auto &L : layers) {
+    for (int q = 0; q < R; ++q)
+      s.ab.push_back(q < (int)L.size() ? L[q] : (uint32_t)(N + q) | ((uint32_t)(N + q) << 16));
+    s.layer_off.push_back((int)s.ab.size());
+  }
 }
 
 static std::once_flag g_dev_once;

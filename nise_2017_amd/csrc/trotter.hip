@@ -75,7 +75,7 @@ __global__ void frame_tables_kernel(FrameSource s, int N, int K, const uint32_t 
     int r = j - N;
     uint32_t p = ab[r];
     int a = (int)(p & 0xffffu), b = (int)(p >> 16);
-    float J = kept_coupling(s, N, f, a, b);
+    float J = a < N ? kept_coupling(s, N, f, a, b) : 0.f;  // a >= N: padding
     float co = 1.f, si = 0.f;  // absent coupling: exact identity rotation
     if (J != 0.f) {
       // src/propagate.c:879-882: J = J*f; si = -sin(J); co = sqrt(1 - si*si)
@@ -105,24 +105,6 @@ int launch_frame_tables(const FrameSource &src, int N, const Schedule &sch, cons
   return NISE_OK;
 }
 
-// ---------------------------------------------------------------------------
-// Complex helpers with the reference's operation order.
-// ---------------------------------------------------------------------------
-// src/propagate.c:871-872 / :892-893: (cr*re - ci*im, ci*re + cr*im)
-__device__ __forceinline__ float2 cmul_diag(float2 c, float2 u) {
-  return make_float2(__fsub_rn(__fmul_rn(c.x, u.x), __fmul_rn(c.y, u.y)),
-                     __fadd_rn(__fmul_rn(c.y, u.x), __fmul_rn(c.x, u.y)));
-}
-
-// src/propagate.c:883-887
-__device__ __forceinline__ void rotate_pair(float2 &pa, float2 &pb, float co, float si) {
-  float cr1 = __fsub_rn(__fmul_rn(co, pa.x), __fmul_rn(si, pb.y));
-  float ci1 = __fadd_rn(__fmul_rn(co, pa.y), __fmul_rn(si, pb.x));
-  float cr2 = __fsub_rn(__fmul_rn(co, pb.x), __fmul_rn(si, pa.y));
-  float ci2 = __fadd_rn(__fmul_rn(co, pb.y), __fmul_rn(si, pa.x));
-  pa = make_float2(cr1, ci1);
-  pb = make_float2(cr2, ci2);
-}
 
 // ---------------------------------------------------------------------------
 // Conveyor kernel.

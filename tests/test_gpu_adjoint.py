@@ -63,12 +63,16 @@ def fd_grad(orc, non, H, mu, Cr, d0, c0, A_re, A_im, target, h=0.01, hc=1e-4):
     return g, gc
 
 
-def run_case(orc, every, shape=None, **kw):
+def run_case(orc, every, shape=None, tol=1e-4, **kw):
+    """dL/dS from the FP64 oracle's S isolates the adjoint from the FP32
+    forward error: the residual spec - target is a few 1e-3 of spec here, so
+    the forward's ~1e-6 relative error would otherwise move dL/dS by ~1e-3
+    (test_end_to_end covers the GPU's own dL/dS)."""
     non, H, mu, Cr = make_case(**kw)
     rng = np.random.default_rng(kw.get("seed", 61) + 1)
     N = non.singles
     d0 = rng.normal(0, 2, N).astype(np.float32).astype(np.float64)
-    c0 = 1.05
+    c0 = float(np.float32(1.05))
     freq, A_re, A_im = adjoint.spectrum_matrix(non, non.end)
     St = orc.absorption_param_f64(non, H, mu, rng.normal(0, 5, N), 1.1, Cr)
     target = A_re @ St[0] + A_im @ St[1]
@@ -77,15 +81,36 @@ def run_case(orc, every, shape=None, **kw):
     So = orc.absorption_param_f64(non, H, mu, d0, c0, Cr)
     err_fwd = np.linalg.norm((S[0] - So[0]) + 1j * (S[1] - So[1])) / np.linalg.norm(So[0] + 1j * So[1])
     assert err_fwd < 1e-4
-    _, g_re, g_im = loss_grad(S, A_re, A_im, target)
+    _, g_re, g_im = loss_grad(So, A_re, A_im, target)
     gd, gc = model.backward(g_re, g_im)
     fd, fdc = fd_grad(orc, non, H, mu, Cr, d0, c0, A_re, A_im, target)
     model.close()
     e_d = np.linalg.norm(gd - fd) / np.linalg.norm(fd)
     e_c = abs(gc - fdc) / abs(fdc)
     say(f"every={every} shape={shape} kw={kw}: fwd {err_fwd:.1e}, d-grad {e_d:.1e}, c-grad {e_c:.1e}")
-    assert e_d < 1e-3 and e_c < 1e-3
+    assert e_d < tol and e_c < tol
     return gd, gc
+
+
+def test_end_to_end(orc):
+    """The GPU's own forward gives dL/dS; target far from the start."""
+    non, H, mu, Cr = make_case(N=8, T=48, seed=67)
+    N = non.singles
+    rng = np.random.default_rng(68)
+    freq, A_re, A_im = adjoint.spectrum_matrix(non, non.end)
+    St = orc.absorption_param_f64(non, H, mu, rng.normal(0, 30, N), 1.6, Cr)
+    target = A_re @ St[0] + A_im @ St[1]
+    model = adjoint.AdjointAbsorption(non, H, mu, every=16)
+    S = model.forward(None, 1.0)
+    L0, g_re, g_im = loss_grad(S, A_re, A_im, target)
+    gd, gc = model.backward(g_re, g_im)
+    model.close()
+    fd, fdc = fd_grad(orc, non, H, mu, Cr, np.zeros(N), 1.0, A_re, A_im, target)
+    g = np.append(gd, gc)
+    f = np.append(fd, fdc)
+    err = np.linalg.norm(g - f) / np.linalg.norm(f)
+    say(f"end to end: gradient rel err {err:.1e}, c-grad {abs(gc - fdc) / abs(fdc):.1e}")
+    assert err < 1e-3
 
 
 @pytest.mark.parametrize("every", [1, 8, 47, 64])

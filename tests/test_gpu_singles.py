@@ -188,6 +188,27 @@ def test_absorption_shapes_agree(orc, nise_lib):
         if base is None:
             base = re + 1j * im
         assert rel_l2(re + 1j * im, base) < 1e-6
+    for rows in (1, 2, 4, 8, 16, 0):
+        p = abi.AbsPlan(non, H, H.shape[1], mu, mu.shape[1], H.shape[0])
+        p.set_rows(rows)
+        re, im = p.execute()
+        p.close()
+        assert rel_l2(re + 1j * im, base) < 1e-6, rows
+
+
+@pytest.mark.parametrize("rows", [1, 4, 16])
+def test_row_conveyor_vs_oracle(orc, nise_lib, rows):
+    """Row conveyor on a lattice (wide layers) and a chain (one rotation per
+    layer), Trotter 1 and 3, against the FP32 oracle."""
+    for kind, N, T, sr, ns, ts in (("lattice", 36, 40, 9, 13, 1), ("chain", 10, 33, 5, 11, 3)):
+        non, H, mu, _ = absorption_case(kind, N, T, sr, ns)
+        non.ts = ts
+        p = abi.AbsPlan(non, H, H.shape[1], mu, mu.shape[1], H.shape[0])
+        p.set_rows(rows)
+        re, im = p.execute()
+        p.close()
+        o_re, o_im = orc.absorption(non, H, mu)
+        assert rel_l2(re + 1j * im, o_re + 1j * o_im) < 1e-5, (kind, rows)
 
 
 def test_graph_replay_matches_eager(nise_lib):
