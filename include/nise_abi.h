@@ -198,7 +198,8 @@ enum { NISE_HOST = 0, NISE_DEVICE = 1 };
  * re_S/im_S (host, length tmax1) receive the sums over samples and
  * polarisations, un-normalised, exactly what the reference accumulates.
  * kernel: 0 = frame-major batched kernel (default), 1 = plain
- * one-sample-per-launch kernel (baseline). */
+ * one-sample-per-launch kernel (baseline).  propagation 0 (exact
+ * exponential, N <= 64) runs the DIA conveyor of nise_absorption_multi. */
 int nise_absorption_batched(const t_non *non, const float *H_rec,
                             int64_t H_stride, const float *mu_rec,
                             int64_t mu_stride, int64_t nframes,
@@ -281,6 +282,29 @@ int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H_stride,
                       const float *mu_rec, int64_t mu_stride, int64_t nframes,
                       const float *coupling_rec, const float *anh_rec,
                       const float *over_rec, int where, float *R12);
+
+/* One system of a multi-system batch. */
+typedef struct {
+  const t_non *non;          /* per-system run parameters */
+  const float *H_rec;        /* Hamiltonian records (host or device) */
+  int64_t H_stride;
+  const float *mu_rec;       /* dipole records */
+  int64_t mu_stride;
+  int64_t nframes;
+  const float *coupling_rec; /* "Coupling" type: one Full record, else NULL */
+} nise_system;
+
+/* Absorption of nsys independent systems of uneven size (no counterpart in
+ * the reference; each system is what nise_absorption_batched computes).
+ * Systems with propagation 1 share one row-conveyor launch; systems with
+ * propagation 0 (exact exponential) and N <= 64 share one DIA-conveyor
+ * launch fed by per-frame propagators from the batched Jacobi solver.
+ * re_S / im_S (host) receive each system's tmax1 values, systems one after
+ * another.  done_ms (host, nsys, optional) receives each system's
+ * completion time in ms after the first workgroup of the run started
+ * (device wall clock). */
+int nise_absorption_multi(const nise_system *systems, int nsys, int where,
+                          float *re_S, float *im_S, double *done_ms);
 
 /* Last error message of the calling thread (static storage). */
 const char *nise_last_error(void);

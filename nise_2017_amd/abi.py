@@ -125,6 +125,7 @@ def lib():
     L.nise_abs_plan_set_params.argtypes = [_vp, _vp, C.c_float]
     L.nise_abs_plan_enable_adjoint.argtypes = [_vp, C.c_int]
     L.nise_abs_plan_adjoint.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double)]
+    L.nise_absorption_multi.argtypes = [_vp, C.c_int, C.c_int, _fp, _fp, _vp]
     L.nise_2dir_batched.argtypes = [P, _vp, C.c_int64, _vp, C.c_int64, C.c_int64, _vp, _vp, _vp,
                                     C.c_int, _fp]
     for name in ("propagate_vec_coupling_S",):
@@ -236,6 +237,41 @@ class AbsPlan:
 
     def __del__(self):
         self.close()
+
+
+class NiseSystem(C.Structure):
+    """Mirror of nise_system (include/nise_abi.h)."""
+    _fields_ = [("non", C.POINTER(TNon)), ("H_rec", C.c_void_p), ("H_stride", C.c_int64),
+                ("mu_rec", C.c_void_p), ("mu_stride", C.c_int64), ("nframes", C.c_int64),
+                ("coupling_rec", C.c_void_p)]
+
+
+def absorption_multi(systems, where=None):
+    """nise_absorption_multi over [(non, H, mu, coupling_rec or None), ...]:
+    returns ([(re_S, im_S) per system], done_ms array)."""
+    if where is None:
+        H0 = systems[0][1]
+        where = NISE_DEVICE if hasattr(H0, "data_ptr") and H0.is_cuda else NISE_HOST
+    arr = (NiseSystem * len(systems))()
+    keep = []
+    for k, (non, H, mu, Cr) in enumerate(systems):
+        for a in (H, mu, Cr):
+            if a is not None and not hasattr(a, "data_ptr") and a.dtype != np.float32:
+                raise ValueError("records must be float32")
+        arr[k] = NiseSystem(C.pointer(non), ptr(H), H.shape[1], ptr(mu), mu.shape[1], H.shape[0],
+                            ptr(Cr))
+        keep.append(non)
+    T = [non.tmax1 for non, _, _, _ in systems]
+    re = np.zeros(sum(T), np.float32)
+    im = np.zeros(sum(T), np.float32)
+    done = np.zeros(len(systems), np.float64)
+    check(lib().nise_absorption_multi(C.cast(arr, C.c_void_p), len(systems), where, re, im,
+                                      done.ctypes.data_as(C.c_void_p)))
+    out, o = [], 0
+    for t in T:
+        out.append((re[o:o + t].copy(), im[o:o + t].copy()))
+        o += t
+    return out, done
 
 
 def twodir_batched(non, H, mu, nframes=None, coupling_rec=None, anh_rec=None, over_rec=None,

@@ -35,7 +35,8 @@ struct nise_abs_plan {
   Schedule sch;                // active schedule (one of the two below)
   Schedule sch_asap, sch_rows; // layered (slot conveyor) / row schedule
   int rows = 0;                // rows of the row conveyor, 0: slot conveyor
-  DevBuf ab, layer_off, diag, cs, slots, wg_begin, partial, out, steps, row_steps;
+  DevBuf ab, layer_off, diag, cs, slots, wg_begin, partial, out, steps, row_steps, row_sys, row_wmap;
+  size_t row_smem = 0;
   int nsteps = 0;
   DevBuf ws_cs, ws_ab, resp_plain;
   int vp = 16, spw = 5, nwg = 0, block = 256;
@@ -73,6 +74,22 @@ int upload(DevBuf &dst, const void *src, size_t bytes, hipStream_t st) {
 
 bool is_coupling_type(const t_non *non) { return std::strcmp(non->hamiltonian, "Coupling") == 0; }
 
+// Descriptor of the plan's single system for the row conveyor (device
+// copy and workgroup map; refreshed when checkpoints are switched on).
+int upload_rows_desc(nise_abs_plan *P) {
+  RowsArgs r;
+  r.N = P->N, r.tmax = P->tmax, r.m = P->m, r.nlayers = P->sch.nlayers(), r.K = P->sch.K;
+  r.f_lo = P->f_lo, r.nvec = 3 * (P->end - P->begin), r.begin = P->begin, r.sample = P->sample;
+  r.steps = P->row_steps.as<uint2>(), r.cs = P->cs.as<float2>(), r.diag = P->diag.as<float2>();
+  r.mu_rec = P->mu_dev, r.mu_stride = P->mu_stride, r.partial = P->partial.as<double2>();
+  if (P->every) r.hist = P->ck.as<float2>(), r.hist_every = P->every, r.nck = P->nck;
+  std::vector<int2> wmap;
+  int rc = rows_prepare(&r, 1, P->rows, wmap, P->row_smem);
+  if (rc) return rc;
+  if ((rc = upload(P->row_sys, &r, sizeof(r), P->st))) return rc;
+  return upload(P->row_wmap, wmap.data(), wmap.size() * sizeof(int2), P->st);
+}
+
 // Row conveyor set-up: schedule, step table, tables and partials.
 int build_rows(nise_abs_plan *P) {
   int rc;
@@ -87,7 +104,8 @@ int build_rows(nise_abs_plan *P) {
   if (P->every && (rc = P->jf.alloc((size_t)nfr * std::max(P->sch.K, 1) * 4))) return rc;
   const int G = 64 / P->rows;
   P->nwg = (3 * (P->end - P->begin) + G - 1) / G;
-  return P->partial.alloc((size_t)std::max(P->nwg, 1) * P->tmax * sizeof(double2));
+  if ((rc = P->partial.alloc((size_t)std::max(P->nwg, 1) * P->tmax * sizeof(double2)))) return rc;
+  return upload_rows_desc(P);
 }
 
 // Slot assignment: consecutive samples, three polarisations each, spw
@@ -130,15 +148,9 @@ int run_tables(nise_abs_plan *P) {
 }
 
 int run_conveyor(nise_abs_plan *P) {
-  if (P->rows) {
-    RowsArgs r;
-    r.N = P->N, r.tmax = P->tmax, r.m = P->m, r.nlayers = P->sch.nlayers(), r.K = P->sch.K;
-    r.f_lo = P->f_lo, r.nvec = 3 * (P->end - P->begin), r.begin = P->begin, r.sample = P->sample;
-    r.steps = P->row_steps.as<uint2>(), r.cs = P->cs.as<float2>(), r.diag = P->diag.as<float2>();
-    r.mu_rec = P->mu_dev, r.mu_stride = P->mu_stride, r.partial = P->partial.as<double2>();
-    if (P->every) r.hist = P->ck.as<float2>(), r.hist_every = P->every, r.nck = P->nck;
-    return launch_rows(r, P->rows, P->st);
-  }
+  if (P->rows)
+    return launch_rows(P->row_sys.as<RowsArgs>(), P->row_wmap.as<int2>(), P->nwg, P->rows, P->row_smem,
+                       P->st);
   ConveyorArgs a;
   a.N = P->N, a.K = P->sch.K, a.nlayers = P->sch.nlayers(), a.m = P->m;
   a.f_lo = P->f_lo;
@@ -495,6 +507,10 @@ extern "C" int nise_absorption_batched(const t_non *non, const float *H_rec, int
                                        const float *mu_rec, int64_t mu_stride, int64_t nframes,
                                        const float *coupling_rec, int where, int kernel,
                                        float *re_S, float *im_S) {
+  if (non && non->propagation == 0) {  // exact exponential: the DIA conveyor
+    nise_system y{non, H_rec, H_stride, mu_rec, mu_stride, nframes, coupling_rec};
+    return nise_absorption_multi(&y, 1, where, re_S, im_S, nullptr);
+  }
   nise_abs_plan *P = nullptr;
   int rc = nise_abs_plan_create(&P, non, H_rec, H_stride, mu_rec, mu_stride, nframes, coupling_rec,
                                 where, kernel);

@@ -235,8 +235,9 @@ int launch_doubles_tables(const FrameSource &src, int N, int K2, const uint32_t 
 int launch_doubles_step(const DoublesArgs &a, int ngroups, hipStream_t st);
 int doubles_np(int N);
 
-// Row conveyor (rows.hip).  Vector v = 3 s + x is polarisation x of sample
-// begin + s, starting at frame (begin + s) * sample.
+// Row conveyor (rows.hip), one descriptor per system.  Vector v = 3 s + x
+// is polarisation x of sample begin + s, starting at frame (begin + s) *
+// sample.  partial holds one row of tmax per workgroup of the system.
 struct RowsArgs {
   int N = 0, tmax = 0, m = 1, nlayers = 0;  // nlayers: multiple of 8
   int64_t K = 0;                 // entries per sweep (= nlayers * R)
@@ -251,7 +252,21 @@ struct RowsArgs {
   float2 *hist = nullptr;        // checkpoints (hist_every, nck per vector)
   int hist_every = 1, nck = 0;
 };
-int launch_rows(const RowsArgs &a, int R, hipStream_t st);
+int rows_prepare(const RowsArgs *sys, int nsys, int R, std::vector<int2> &wmap, size_t &smem);
+// times (optional): per workgroup (start, end) on the device wall clock.
+int launch_rows(const RowsArgs *sys, const int2 *wmap, int nwg, int R, size_t smem, hipStream_t st,
+                unsigned long long *times = nullptr);
+
+// DIA conveyor (multi.hip): one descriptor per system, vp consecutive
+// vectors per 256-thread workgroup, Np = N rounded up to a power of two.
+struct DiaSys {
+  int N = 0, Np = 1, vp = 1, tmax = 0, begin = 0, sample = 1, nvec = 0;
+  int64_t f_lo = 0;
+  const float2 *U = nullptr;     // [rows][N][N] per-frame propagators
+  const float *mu_rec = nullptr;
+  int64_t mu_stride = 0;
+  double2 *partial = nullptr;    // [workgroups][tmax]
+};
 size_t rows_smem(int N, int R);
 int rows_for(int N, int Kreal, int nlayers_asap);
 void rows_step_table(const Schedule &s, int R, std::vector<uint2> &tab);

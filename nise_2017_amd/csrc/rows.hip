@@ -41,14 +41,20 @@ __device__ __forceinline__ void rot_step(unsigned char *smem, uint32_t gofs, uin
   *pb_ = pb;
 }
 
+// Workgroup w runs the vectors wmap[w].y .. wmap[w].y + G - 1 of system
+// wmap[w].x (systems of different sizes share one launch).
 template <int R>
-__global__ void __launch_bounds__(64) rows_kernel(RowsArgs A) {
+__global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *sys, const int2 *wmap,
+                                                  unsigned long long *times) {
   constexpr int G = 64 / R;
   extern __shared__ __align__(16) unsigned char smem[];
+  const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
+  const int2 wm = wmap[blockIdx.x];
+  const RowsArgs &A = sys[wm.x];
   const int N = A.N, T = A.tmax, nl = A.nlayers;
   const int lane = threadIdx.x, g = lane / R, q = lane % R;
-  const int w = blockIdx.x;
-  const int v = w * G + g;
+  const int w = wm.y / G;  // workgroup index within the system
+  const int v = wm.y + g;
   const bool has = v < A.nvec;
   const int vv = has ? v : w * G;  // inactive lanes shadow the wave's first vector
   const int s = vv / 3, x = vv % 3;
@@ -108,7 +114,13 @@ __global__ void __launch_bounds__(64) rows_kernel(RowsArgs A) {
       si += (double)__shfl(ri, k * R);
     }
     if (lane == 0) A.partial[(int64_t)w * T + t] = make_double2(sr, si);
-    if (last) break;
+    if (last) {
+      if (lane == 0 && times) {  // device wall clock at start and end
+        times[2 * blockIdx.x] = t_start;
+        times[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
+      }
+      break;
+    }
 
     for (int sub = 0; sub < A.m; ++sub) {
       if (sub > 0)
@@ -144,24 +156,42 @@ __global__ void __launch_bounds__(64) rows_kernel(RowsArgs A) {
 
 size_t rows_smem(int N, int R) { return (size_t)(N + R) * (64 / R) * 8; }
 
-int launch_rows(const RowsArgs &a, int R, hipStream_t st) {
-  const size_t smem = rows_smem(a.N, R);
-  if (smem > 160 * 1024)
-    return set_error("row conveyor: N=%d with %d rows needs %zu B of LDS", a.N, R, smem), NISE_ERR_ARG;
-  if (a.nlayers < kRingR || a.nlayers % kRingR)
-    return set_error("row conveyor: layer count %d is not a multiple of %d", a.nlayers, kRingR), NISE_ERR_ARG;
-  const void *fn = R == 1 ? (const void *)rows_kernel<1>
-                 : R == 2 ? (const void *)rows_kernel<2>
-                 : R == 4 ? (const void *)rows_kernel<4>
-                 : R == 8 ? (const void *)rows_kernel<8>
-                 : R == 16 ? (const void *)rows_kernel<16> : nullptr;
+static const void *rows_fn(int R) {
+  return R == 1 ? (const void *)rows_kernel<1>
+       : R == 2 ? (const void *)rows_kernel<2>
+       : R == 4 ? (const void *)rows_kernel<4>
+       : R == 8 ? (const void *)rows_kernel<8>
+       : R == 16 ? (const void *)rows_kernel<16> : nullptr;
+}
+
+// Checks the systems against the row count and builds the workgroup map
+// (G consecutive vectors of one system per workgroup) and the LDS size.
+int rows_prepare(const RowsArgs *sys, int nsys, int R, std::vector<int2> &wmap, size_t &smem) {
+  if (!rows_fn(R)) return set_error("row conveyor: unsupported row count %d", R), NISE_ERR_ARG;
+  smem = 0;
+  wmap.clear();
+  const int G = 64 / R;
+  for (int k = 0; k < nsys; ++k) {
+    const RowsArgs &a = sys[k];
+    if (a.nlayers < kRingR || a.nlayers % kRingR)
+      return set_error("row conveyor: layer count %d is not a multiple of %d", a.nlayers, kRingR), NISE_ERR_ARG;
+    if (a.K != (int64_t)a.nlayers * R) return set_error("row conveyor: schedule built for other rows"), NISE_ERR_ARG;
+    smem = std::max(smem, rows_smem(a.N, R));
+    for (int v = 0; v < a.nvec; v += G) wmap.push_back(make_int2(k, v));
+  }
+  if (smem > 160 * 1024) return set_error("row conveyor: %zu B of LDS needed", smem), NISE_ERR_ARG;
+  return NISE_OK;
+}
+
+// sys / wmap: device copies (rows_prepare), nwg workgroups.
+int launch_rows(const RowsArgs *sys, const int2 *wmap, int nwg, int R, size_t smem, hipStream_t st,
+                unsigned long long *times) {
+  const void *fn = rows_fn(R);
   if (!fn) return set_error("row conveyor: unsupported row count %d", R), NISE_ERR_ARG;
-  const int nwg = (a.nvec + 64 / R - 1) / (64 / R);
   if (nwg == 0) return NISE_OK;
   NISE_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-  RowsArgs args = a;
-  void *kargs[] = {&args};
-  NISE_HIP_CHECK(hipLaunchKernel(fn, dim3(nwg), dim3(64), kargs, smem, st));
+  void *kargs[] = {&sys, &wmap, &times};
+  NISE_HIP_CHECK(hipLaunchKernel(fn, dim3((unsigned)nwg), dim3(64), kargs, smem, st));
   return NISE_OK;
 }
 

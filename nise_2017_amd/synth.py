@@ -181,3 +181,36 @@ def twodir_system(N=6, nframes=200, seed=SEED0 + 2, dt=2.0, mean=1650.0, sd=10.0
     on = ar1(rng, nframes, 3 * N, 0.0, dip_fluct, tau, dt)
     over[:, 1:] = (np.sqrt(2.0) * mu_xyz.reshape(nframes, 3 * N) + on).astype(np.float32)
     return H, mu, anh, over
+
+
+def geometric_system(N, nframes, seed, dt=1.0, mean=15000.0, sd=100.0, tau=50.0, spacing=5.0,
+                     prefactor=5034.0, cut=0.5):
+    """Coupling-type records for N sites uniform in a cube at density one per
+    spacing^3 (Angstrom), unit transition dipoles with random orientation,
+    point-dipole couplings J = prefactor * kappa / r^3 (cm^-1, r in
+    Angstrom), |J| < cut dropped; AR(1) site energies (mean, sd, tau fs).
+    Returns (H diagonal records (T, 1 + N), coupling record (1 + N(N+1)/2,),
+    dipole record (1, 1 + 3N))."""
+    rng = np.random.default_rng(seed)
+    side = spacing * N ** (1.0 / 3.0)
+    pos = rng.uniform(0.0, side, (N, 3))
+    mu = unit_vectors(rng, N)
+    J = np.zeros(N * (N - 1) // 2, np.float32)
+    if N > 1:
+        a, b = np.triu_indices(N, 1)
+        r = pos[b] - pos[a]
+        d = np.maximum(np.linalg.norm(r, axis=1), 1.0)
+        rh = r / d[:, None]
+        kappa = np.einsum("ij,ij->i", mu[a], mu[b]) - 3 * np.einsum("ij,ij->i", mu[a], rh) * \
+            np.einsum("ij,ij->i", mu[b], rh)
+        J = (prefactor * kappa / d ** 3).astype(np.float32)
+        J[np.abs(J) < cut] = 0.0
+    diag = ar1(rng, nframes, N, mean, sd, tau, dt)
+    H = np.empty((nframes, 1 + N), np.float32)
+    H[:, 0] = headers(nframes)
+    H[:, 1:] = diag
+    nn2 = N * (N + 1) // 2
+    Crec = np.zeros(1 + nn2, np.float32)
+    ai, bi = np.triu_indices(N)
+    Crec[1 + np.nonzero(ai != bi)[0]] = J
+    return H, Crec, dipole_records(mu, 1)
