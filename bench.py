@@ -149,6 +149,54 @@ def adjoint_leg(args, torch):
     return out
 
 
+def multi_leg(args, torch):
+    """Uneven-size batch: 96 systems, N = round(exp U[ln 8, ln 1024]), sites
+    uniform in a cube at one per (5 A)^3, point-dipole couplings 8 cm^-1 at
+    5 A (kappa = 1) dropped below 0.5 cm^-1, AR(1) energies (15000, 100
+    cm^-1, 50 fs), Coupling type, dt 1 fs, 1503 frames; tmax 512, 100
+    samples every 10 frames, 3 polarisations; N <= 64 on the exact
+    exponential (DIA) path, the rest on the Trotter path.  One call runs
+    the per-frame tables, the two launches and the reductions."""
+    import numpy as np
+
+    from nise_2017_amd import abi, synth
+    rng = np.random.default_rng(20261202 + 4)
+    Ns = np.round(np.exp(rng.uniform(np.log(8), np.log(1024), args.multi_systems))).astype(int)
+    T, sr, ns = 512, 10, 100
+    nfr = (ns - 1) * sr + T + 1 + 2
+    log(f"multi leg: {len(Ns)} systems, sum N = {int(Ns.sum())}, generating")
+    systems = []
+    for k, N in enumerate(Ns):
+        H, Cr, mu = synth.geometric_system(int(N), nfr, seed=20261202 + 4000 + k)
+        mean = float(H[:, 1:].mean())
+        non = abi.make_non(singles=int(N), tmax1=T, sample=sr, begin=0, end=ns, ts=1,
+                           propagation=0 if N <= 64 else 1, deltat=1.0, min1=mean - 600.0,
+                           max1=mean + 600.0, hamiltonian="Coupling", couplingcut=0.5)
+        non.shifte = np.float32((np.float32(non.max1) + np.float32(non.min1)) / np.float32(2))
+        systems.append((non, torch.from_numpy(H).cuda(), torch.from_numpy(mu).cuda(),
+                        torch.from_numpy(Cr).cuda()))
+    torch.cuda.synchronize()
+    abi.absorption_multi(systems)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out, done = abi.absorption_multi(systems)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    site_steps = float(sum(3 * ns * T * int(N) for N in Ns))
+    res = {"workload": f"{len(Ns)} systems, N log-uniform in [8, 1024] (sum {int(Ns.sum())}), "
+                       f"Coupling type, tmax {T}, {ns} samples, 3 pols, dt 1 fs",
+           "systems_dia": int((Ns <= 64).sum()), "systems_trotter": int((Ns > 64).sum()),
+           "site_steps": site_steps, "wall_ms": round(wall * 1e3, 3),
+           "site_steps_per_s": site_steps / wall,
+           "completion_ms_p50": float(np.percentile(done, 50)),
+           "completion_ms_p99": float(np.percentile(done, 99)),
+           "note": "wall time includes the per-frame tables (Jacobi for DIA systems); completion "
+                   "times are device-clock times of each system's last workgroup"}
+    log(f"multi leg: {res['site_steps_per_s']:.3e} site-steps/s, p50 {res['completion_ms_p50']:.2f} ms, "
+        f"p99 {res['completion_ms_p99']:.2f} ms")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,7 +212,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--shape", default="", help="vp,spw,block conveyor shape override")
+    ap.add_argument("--rows", type=int, default=0, help="row conveyor with this many rows")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the N=128 adjoint leg")
+    ap.add_argument("--no-multi", action="store_true", help="skip the uneven-size batch leg")
+    ap.add_argument("--multi-systems", type=int, default=96)
     ap.add_argument("--adjoint-every", default="16,32,64", help="checkpoint intervals K")
     ap.add_argument("--fd-params", type=int, default=3, help="offsets checked by finite differences")
     args = ap.parse_args()
@@ -198,9 +249,15 @@ def main():
 
     log("building the plan (coupling structure, rotation schedule)")
     plan = abi.AbsPlan(non, H, H.shape[1], mu, mu.shape[1], nfr, where=abi.NISE_DEVICE, kernel=0)
+    tune = None
     if args.shape:
         vp, spw, blk = (int(x) for x in args.shape.split(","))
         plan.set_shape(vp, spw, blk)
+    elif args.rows:
+        plan.set_rows(args.rows)
+    else:
+        tune = plan.autotune()
+        log("autotune (code, conveyor ms): " + ", ".join(f"({c}, {t:.2f})" for c, t in tune))
     info = plan.info()
     log(f"plan: {info}")
 
@@ -233,7 +290,7 @@ def main():
     achieved = site_steps * B / (conv * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "conveyor_kernel", "kernel_ms": round(conv, 4),
+                "kernel": "rows_kernel" if info["rows"] else "conveyor_kernel", "kernel_ms": round(conv, 4),
                 "bytes_per_site_step": round(B, 3)}
     tf = os.path.join(ROOT, "profiles", "conveyor_lattice256_traffic.json")
     if os.path.exists(tf):
@@ -300,6 +357,8 @@ def main():
 
     if not args.no_adjoint:
         extra["adjoint"] = adjoint_leg(args, torch)
+    if not args.no_multi:
+        extra["multi"] = multi_leg(args, torch)
 
     line = {
         "metric": "site-steps/s propagated (N=256 absorption) vs plain GPU; adjoint/forward cost",
@@ -309,8 +368,12 @@ def main():
         "config": {"workload": f"absorption N={N} {L}x{L} dipole lattice, Full FP32 frames, "
                                f"{nfr} frames, tmax {T}, {ns} samples every {sr} frames, 3 pols, "
                                f"Trotter 1", "N": N, "frames": nfr, "tmax": T,
-                   "samples": ns, "samplerate": sr, "shape": {"vp": info["vp"], "spw": info["spw"], "block": info["block"]},
-                   "K": info["K"], "layers": info["nlayers"]},
+                   "samples": ns, "samplerate": sr,
+                   "conveyor": ({"kernel": "rows_kernel", "rows": info["rows"]} if info["rows"] else
+                                {"kernel": "conveyor_kernel", "vp": info["vp"], "spw": info["spw"],
+                                 "block": info["block"]}),
+                   "rotations": info["rotations"], "layers": info["nlayers"],
+                   "autotune": [{"code": c, "ms": round(t, 3)} for c, t in tune] if tune else None},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     line.update(extra)

@@ -225,6 +225,13 @@ int nise_abs_plan_set_shape(nise_abs_plan *plan, int vp, int samples_per_wg,
  * vectors, rows rotations of a layer at a time (1, 2, 4, 8 or 16; 0 picks
  * the count from the coupling structure). */
 int nise_abs_plan_set_rows(nise_abs_plan *plan, int rows);
+/* Autotuner: times each candidate configuration (row conveyor with 1..16
+ * rows, slot-conveyor shapes) on the plan's own workload and keeps the
+ * fastest.  ms / codes (host, n entries each, optional) receive each timed
+ * candidate's conveyor time and code (rows R as R; slot shapes as
+ * -(10000 vp + 1000 spw + block), block -2/-4 as 902/904).  Returns the
+ * number of candidates timed, or minus an error code. */
+int nise_abs_plan_autotune(nise_abs_plan *plan, float *ms, int *codes, int n);
 /* Run the plan; re_S / im_S (host, tmax1 each) may be NULL to leave the
  * result on the device.  flags bit 0: time the stages with HIP events;
  * bit 1: launch the stages as one captured hipGraph. */

@@ -117,6 +117,7 @@ def lib():
     L.nise_abs_plan_execute.argtypes = [_vp, _vp, _vp, C.c_int]
     L.nise_abs_plan_set_shape.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.nise_abs_plan_set_rows.argtypes = [_vp, C.c_int]
+    L.nise_abs_plan_autotune.argtypes = [_vp, _vp, _vp, C.c_int]
     L.nise_abs_plan_info.argtypes = [_vp, C.POINTER(C.c_int64), C.c_int]
     L.nise_abs_plan_last_timing.argtypes = [_vp, C.POINTER(C.c_float), C.c_int]
     L.nise_abs_plan_sync.argtypes = [_vp]
@@ -189,6 +190,17 @@ class AbsPlan:
 
     def sync(self):
         check(lib().nise_abs_plan_sync(self.h))
+
+    def autotune(self):
+        """Time the candidate configurations and keep the fastest; returns
+        [(code, conveyor ms), ...] (code: rows R, or -(10000 vp + 1000 spw +
+        block))."""
+        ms = (C.c_float * 16)()
+        codes = (C.c_int * 16)()
+        k = lib().nise_abs_plan_autotune(self.h, ms, codes, 16)
+        if k < 0:
+            check(-k)
+        return [(codes[i], ms[i]) for i in range(k)]
 
     def set_rows(self, rows=0):
         """Row conveyor with the given row count (0: from the structure)."""

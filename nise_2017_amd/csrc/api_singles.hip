@@ -339,6 +339,47 @@ extern "C" int nise_abs_plan_set_rows(nise_abs_plan *P, int R) {
   return P->every ? nise_abs_plan_enable_adjoint(P, P->every) : NISE_OK;
 }
 
+extern "C" int nise_abs_plan_execute(nise_abs_plan *P, float *re_S, float *im_S, int flags);
+
+// Autotuner: times every candidate conveyor configuration on the plan's own
+// workload (two event-timed runs each, the faster counts) and keeps the
+// fastest.  Candidates: the row conveyor with 1..16 rows (those whose LDS
+// image fits) and slot-conveyor shapes.  ms[i] (optional, n entries)
+// receives the candidates' times in the order of codes[i]: rows R as R,
+// slot shapes as -(1000*vp + 10*spw) - ... see nise_abs_plan_info for the
+// chosen one.  Returns the number of candidates timed.
+extern "C" int nise_abs_plan_autotune(nise_abs_plan *P, float *ms, int *codes, int n) {
+  if (!P || P->kernel != 0) return set_error("no conveyor plan"), -NISE_ERR_ARG;
+  struct Cand { int rows, vp, spw, block; };
+  std::vector<Cand> cands;
+  for (int R : {1, 2, 4, 8, 16})
+    if (rows_smem(P->N, R) <= 160 * 1024) cands.push_back({R, 0, 0, 0});
+  const Cand slot[] = {{0, 8, 2, 256}, {0, 16, 5, 256}, {0, 8, 2, 64}};
+  for (const Cand &c : slot) cands.push_back(c);
+  float best = 1e30f;
+  int bi = -1, k = 0;
+  for (size_t i = 0; i < cands.size(); ++i) {
+    const Cand &c = cands[i];
+    int rc = c.rows ? nise_abs_plan_set_rows(P, c.rows) : nise_abs_plan_set_shape(P, c.vp, c.spw, c.block);
+    if (rc) continue;  // shape does not fit this system
+    float t = 1e30f;
+    for (int r = 0; r < 2; ++r) {
+      if ((rc = nise_abs_plan_execute(P, nullptr, nullptr, 1))) return -rc;
+      t = std::min(t, P->last_ms[1]);
+    }
+    if (k < n) {
+      if (ms) ms[k] = t;
+      if (codes) codes[k] = c.rows ? c.rows : -(c.vp * 10000 + c.spw * 1000 + (c.block < 0 ? 900 - c.block : c.block));
+    }
+    ++k;
+    if (t < best) best = t, bi = (int)i;
+  }
+  if (bi < 0) return set_error("autotune: no configuration ran"), -NISE_ERR_ARG;
+  const Cand &c = cands[bi];
+  int rc = c.rows ? nise_abs_plan_set_rows(P, c.rows) : nise_abs_plan_set_shape(P, c.vp, c.spw, c.block);
+  return rc ? -rc : k;
+}
+
 // flags bit 0: time the stages with HIP events (no graph)
 //       bit 1: replay a captured hipGraph of the stages
 extern "C" int nise_abs_plan_execute(nise_abs_plan *P, float *re_S, float *im_S, int flags) {

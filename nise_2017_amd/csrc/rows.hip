@@ -1,7 +1,7 @@
 // Row conveyor: batched absorption propagation with one wave per workgroup.
 //
 // A wave holds G = 64/R vectors (consecutive (sample, polarisation) pairs)
-// in LDS as psi[site][G].  Lane (g, q) = (vector, row): in the rotation
+// in LDS as psi[site][G].  Lane q*G + g = (row q, vector g): in the rotation
 // sweep row q applies entry q of every layer of the row schedule
 // (build_schedule_rows: layers of exactly R site-disjoint entries, padding
 // entries act on the scratch site N + q), so a sweep is nlayers steps with
@@ -52,7 +52,9 @@ __global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *sys, const int
   const int2 wm = wmap[blockIdx.x];
   const RowsArgs &A = sys[wm.x];
   const int N = A.N, T = A.tmax, nl = A.nlayers;
-  const int lane = threadIdx.x, g = lane / R, q = lane % R;
+  // lane = q * G + g: the G vectors of a row are adjacent, so a row's LDS
+  // accesses cover one contiguous 8*G-byte run (fewer bank conflicts)
+  const int lane = threadIdx.x, q = lane / G, g = lane % G;
   const int w = wm.y / G;  // workgroup index within the system
   const int v = wm.y + g;
   const bool has = v < A.nvec;
@@ -103,15 +105,15 @@ __global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *sys, const int
     // response of step t: rows of a vector, then the wave's vectors in order
     if (!has) rr = 0.f, ri = 0.f;
 #pragma unroll
-    for (int off = 1; off < R; off <<= 1) {
+    for (int off = G; off < 64; off <<= 1) {  // over the rows of a vector
       rr += __shfl_xor(rr, off);
       ri += __shfl_xor(ri, off);
     }
     double sr = 0.0, si = 0.0;
 #pragma unroll
-    for (int k = 0; k < G; ++k) {
-      sr += (double)__shfl(rr, k * R);
-      si += (double)__shfl(ri, k * R);
+    for (int k = 0; k < G; ++k) {  // the wave's vectors in order
+      sr += (double)__shfl(rr, k);
+      si += (double)__shfl(ri, k);
     }
     if (lane == 0) A.partial[(int64_t)w * T + t] = make_double2(sr, si);
     if (last) {

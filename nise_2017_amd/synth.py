@@ -184,11 +184,12 @@ def twodir_system(N=6, nframes=200, seed=SEED0 + 2, dt=2.0, mean=1650.0, sd=10.0
 
 
 def geometric_system(N, nframes, seed, dt=1.0, mean=15000.0, sd=100.0, tau=50.0, spacing=5.0,
-                     prefactor=5034.0, cut=0.5):
+                     prefactor=1000.0, cut=0.5):
     """Coupling-type records for N sites uniform in a cube at density one per
     spacing^3 (Angstrom), unit transition dipoles with random orientation,
     point-dipole couplings J = prefactor * kappa / r^3 (cm^-1, r in
-    Angstrom), |J| < cut dropped; AR(1) site energies (mean, sd, tau fs).
+    Angstrom; the default gives 8 cm^-1 at 5 Angstrom for kappa = 1),
+    |J| < cut dropped; AR(1) site energies (mean, sd, tau fs).
     Returns (H diagonal records (T, 1 + N), coupling record (1 + N(N+1)/2,),
     dipole record (1, 1 + 3N))."""
     rng = np.random.default_rng(seed)
