@@ -197,6 +197,47 @@ def multi_leg(args, torch):
     return res
 
 
+def twodir_leg(args, torch):
+    """2DIR with the two-exciton manifold: N=32 ring, Full type, AR(1)
+    energies (1650, sd 12 cm^-1), static nearest-neighbour couplings
+    N(-6, 2), anharmonicity 16 cm^-1, t1 = t3 = 64 steps, t2 = 0, dt 2 fs,
+    400 samples every 10 frames, Trotter 1, all 21 polarisation items."""
+    import numpy as np
+
+    from nise_2017_amd import abi, synth
+    N, T, sr, ns = 32, 64, 10, 400
+    nfr = (ns - 1) * sr + 2 * T + 2
+    H, mu = synth.ring_2dir(N=N, nframes=nfr, seed=20261202 + 2)
+    lo, hi = synth.shifted_center(H, N)
+    mk = lambda end: abi.make_non(singles=N, tmax1=T, tmax2=0, tmax3=T, sample=sr, begin=0, end=end,  # noqa: E731
+                                  ts=1, propagation=1, deltat=2.0, min1=lo, max1=hi, technique="2DIR",
+                                  anharmonicity=16.0)
+    Hd, mud = torch.from_numpy(H).cuda(), torch.from_numpy(mu).cuda()
+    abi.twodir_batched(mk(2), Hd, mud)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    R = abi.twodir_batched(mk(ns), Hd, mud)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    items = ns * 21
+    res = {"workload": f"N={N} ring, t1=t3={T}, {ns} samples x 21 items, anharmonicity 16",
+           "wall_ms": round(wall * 1e3, 3), "items_per_s": items / wall}
+    log(f"2DIR leg: {wall * 1e3:.1f} ms for {items} work items")
+    if not args.no_cpu:
+        from oracle import oracle
+        k = 2
+        t0 = time.perf_counter()
+        ref = oracle.twodir(mk(k), H, mu)
+        dt = time.perf_counter() - t0
+        Rk = abi.twodir_batched(mk(k), Hd, mud)
+        err = float(np.linalg.norm(Rk - ref) / np.linalg.norm(ref))
+        res["parity_rel_l2_vs_oracle"] = err
+        res["cpu_baseline"] = {"value": k * 21 / dt, "unit": "items/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} samples x 21 items, oracle item loop, {dt:.2f} s"}
+        log(f"2DIR parity {err:.1e}, oracle {k * 21 / dt:.2f} items/s")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +256,7 @@ def main():
     ap.add_argument("--rows", type=int, default=0, help="row conveyor with this many rows")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the N=128 adjoint leg")
     ap.add_argument("--no-multi", action="store_true", help="skip the uneven-size batch leg")
+    ap.add_argument("--no-2dir", action="store_true", help="skip the 2DIR leg")
     ap.add_argument("--multi-systems", type=int, default=96)
     ap.add_argument("--adjoint-every", default="16,32,64", help="checkpoint intervals K")
     ap.add_argument("--fd-params", type=int, default=3, help="offsets checked by finite differences")
@@ -359,6 +401,8 @@ def main():
         extra["adjoint"] = adjoint_leg(args, torch)
     if not args.no_multi:
         extra["multi"] = multi_leg(args, torch)
+    if not args.no_2dir:
+        extra["twodir"] = twodir_leg(args, torch)
 
     line = {
         "metric": "site-steps/s propagated (N=256 absorption) vs plain GPU; adjoint/forward cost",

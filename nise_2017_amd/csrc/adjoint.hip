@@ -177,6 +177,125 @@ __global__ void adjoint_reduce_kernel(const double *pd, const double *pc, int nw
   }
 }
 
+// Row variant (used with the row conveyor's schedule and tables): one wave
+// holds G = 64/R vectors, lane q*G + g = (row q, vector g), each vector at
+// its own frame (time skew, as in rows_kernel), walking t = tmax-1 .. 0 in
+// lock step.  Rotation entries are the row schedule's (padding pairs act on
+// the zeroed scratch sites N..N+R-1 with jf = 0).  Per-site offset
+// gradients are summed over the wave's vectors with shuffles and added to an
+// FP64 LDS accumulator by the g == 0 lane of each row.
+template <int R>
+__global__ void __launch_bounds__(64) adjoint_rows_kernel(AdjRowsArgs A) {
+  constexpr int G = 64 / R;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = A.N, T = A.T, nl = A.nlayers, NS = N + R;
+  double *gd = reinterpret_cast<double *>(smem);                  // [N]
+  float2 *phi = reinterpret_cast<float2 *>(gd + N);                // [N+R][G]
+  float2 *chi = phi + (size_t)NS * G;                              // [N+R][G]
+  const int lane = threadIdx.x, q = lane / G, g = lane % G;
+  const int v = blockIdx.x * G + g;
+  const bool has = v < A.nvec;
+  const int vv = has ? v : blockIdx.x * G;
+  const int s = vv / 3, x = vv % 3;
+  const int64_t fs = (int64_t)(A.begin + s) * A.sample;
+  const int64_t ck0 = (int64_t)vv * A.nck;
+  const float hf = 0.5f * A.fac;
+  for (int i = lane; i < N; i += 64) gd[i] = 0.0;
+  for (int i = lane; i < NS * G; i += 64) phi[i] = chi[i] = make_float2(0.f, 0.f);
+  double cg = 0.0;
+  auto site_pass = [&](const float2 *D, int t, bool fin) {
+    // half-phase conj(D) on phi and chi with the offset gradient; on the
+    // last substep also the response term g(t) mu(t) and the checkpoint
+    const float2 gv = A.g[t];
+    const float *mu = A.mu_rec + (A.mu_stride ? (fs + t) * A.mu_stride : 0) + 1 + (int64_t)x * N;
+    const bool reset = fin && (t % A.every == 0);
+    const int64_t ckr = ck0 + t / A.every;
+    for (int i = q; i < N; i += R) {
+      float2 p = phi[i * G + g], c = chi[i * G + g];
+      float cc = hf * im_cj(c, p);
+      const float2 d = D[i];
+      p = cmul_conj_d(p, d);
+      c = cmul_conj_d(c, d);
+      if (fin) {
+        const float m = mu[i];
+        c.x = __fadd_rn(c.x, __fmul_rn(gv.x, m));
+        c.y = __fadd_rn(c.y, __fmul_rn(gv.y, m));
+        if (reset) p = A.ck[ckr * N + i];
+      }
+      phi[i * G + g] = p;
+      chi[i * G + g] = c;
+      if (!has) cc = 0.f;
+#pragma unroll
+      for (int off = 1; off < G; off <<= 1) cc += __shfl_xor(cc, off);
+      if (g == 0) gd[i] += (double)cc;
+    }
+  };
+  for (int t = T - 1; t >= 0; --t) {
+    const int64_t row = fs + t - A.f_lo;
+    if (t == T - 1) {  // start: psi(T-1) and lambda(T-1) = g(T-1) mu(T-1)
+      const float2 gv = A.g[t];
+      const float *mu = A.mu_rec + (A.mu_stride ? (fs + t) * A.mu_stride : 0) + 1 + (int64_t)x * N;
+      for (int i = q; i < N; i += R) {
+        phi[i * G + g] = A.ck[(ck0 + A.nck - 1) * N + i];
+        chi[i * G + g] = make_float2(__fmul_rn(gv.x, mu[i]), __fmul_rn(gv.y, mu[i]));
+      }
+      continue;
+    }
+    const float2 *D = A.diag + row * N;
+    const float2 *cs = A.cs + row * A.K + q;
+    const float *jf = A.jf + row * A.K + q;
+    const uint2 *stq = A.steps + q;
+    for (int sub = 0; sub < A.m; ++sub) {
+      site_pass(D, t, false);
+      float cf = 0.f;
+      for (int l = nl - 1; l >= 0; --l) {
+        const uint2 e = stq[l * R];
+        const float2 c = cs[l * R];
+        const float j = jf[l * R];
+        float2 *pa_ = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(phi) + e.x + g * 8);
+        float2 *pb_ = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(phi) + e.y + g * 8);
+        float2 *xa_ = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(chi) + e.x + g * 8);
+        float2 *xb_ = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(chi) + e.y + g * 8);
+        float2 pa = *pa_, pb = *pb_, xa = *xa_, xb = *xb_;
+        cf += j * (im_cj(xa, pb) + im_cj(xb, pa));
+        rotate_back(pa, pb, c.x, c.y);
+        rotate_back(xa, xb, c.x, c.y);
+        *pa_ = pa, *pb_ = pb, *xa_ = xa, *xb_ = xb;
+      }
+      cg += has ? (double)cf : 0.0;
+      site_pass(D, t, sub == A.m - 1);
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) cg += __shfl_xor(cg, off);
+  for (int i = lane; i < N; i += 64) A.part_delta[(int64_t)blockIdx.x * N + i] = gd[i];
+  if (lane == 0) A.part_c[blockIdx.x] = cg;
+}
+
+size_t adjoint_rows_smem(int N, int R) { return (size_t)N * 8 + (size_t)2 * (N + R) * (64 / R) * 8; }
+
+int launch_adjoint_rows(const AdjRowsArgs &a, int R, double *out, hipStream_t st) {
+  const size_t smem = adjoint_rows_smem(a.N, R);
+  if (smem > 160 * 1024) return set_error("adjoint: N=%d needs %zu B of LDS", a.N, smem), NISE_ERR_ARG;
+  const void *fn = R == 1 ? (const void *)adjoint_rows_kernel<1>
+                 : R == 2 ? (const void *)adjoint_rows_kernel<2>
+                 : R == 4 ? (const void *)adjoint_rows_kernel<4>
+                 : R == 8 ? (const void *)adjoint_rows_kernel<8>
+                 : R == 16 ? (const void *)adjoint_rows_kernel<16> : nullptr;
+  if (!fn) return set_error("adjoint: unsupported rows %d", R), NISE_ERR_ARG;
+  const int nwg = (a.nvec + 64 / R - 1) / (64 / R);
+  if (nwg > 0) {
+    NISE_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    AdjRowsArgs args = a;
+    void *kargs[] = {&args};
+    NISE_HIP_CHECK(hipLaunchKernel(fn, dim3(nwg), dim3(64), kargs, smem, st));
+  }
+  hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((a.N + 1 + 127) / 128), dim3(128), 0, st,
+                     a.part_delta, a.part_c, nwg, a.N, out);
+  NISE_HIP_CHECK(hipGetLastError());
+  return NISE_OK;
+}
+
 size_t adjoint_smem(int N, int vp, int T) {
   return (size_t)N * 8 + (size_t)2 * N * vp * 8 + (size_t)N * 8 + (size_t)T * 8 + (size_t)3 * N * 4;
 }

@@ -530,7 +530,24 @@ extern "C" int nise_abs_plan_adjoint(nise_abs_plan *P, const float *g_re, const 
   a.ck = P->ck.as<float2>(), a.g = P->gtab.as<float2>();
   a.part_delta = P->part_d.as<double>(), a.part_c = P->part_c.as<double>();
   NISE_HIP_CHECK(hipEventRecord(P->ev[0], P->st));
-  int rc = launch_adjoint(a, P->adj_vp, P->grad.as<double>(), P->st);
+  int rc;
+  if (P->rows) {  // time-skewed row variant on the row conveyor's tables
+    AdjRowsArgs r;
+    r.N = N, r.T = T, r.m = P->m, r.nlayers = P->sch.nlayers(), r.every = P->every, r.nck = P->nck;
+    r.nvec = 3 * (P->end - P->begin), r.begin = P->begin, r.sample = P->sample;
+    r.K = P->sch.K, r.f_lo = P->f_lo, r.fac = a.fac;
+    r.steps = P->row_steps.as<uint2>(), r.cs = P->cs.as<float2>(), r.jf = P->jf.as<float>();
+    r.diag = P->diag.as<float2>(), r.mu_rec = P->mu_dev, r.mu_stride = P->mu_stride;
+    r.ck = P->ck.as<float2>(), r.g = P->gtab.as<float2>();
+    const int nwg = (r.nvec + 64 / P->rows - 1) / (64 / P->rows);
+    if ((rc = P->part_d.alloc((size_t)std::max(nwg, 1) * N * 8)) ||
+        (rc = P->part_c.alloc((size_t)std::max(nwg, 1) * 8)))
+      return rc;
+    r.part_delta = P->part_d.as<double>(), r.part_c = P->part_c.as<double>();
+    rc = launch_adjoint_rows(r, P->rows, P->grad.as<double>(), P->st);
+  } else {
+    rc = launch_adjoint(a, P->adj_vp, P->grad.as<double>(), P->st);
+  }
   if (rc) return rc;
   NISE_HIP_CHECK(hipEventRecord(P->ev[1], P->st));
   std::vector<double> out(N + 1);

@@ -293,6 +293,24 @@ struct AdjArgs {
   double *part_c = nullptr;     // [nwg]
 };
 int launch_adjoint(const AdjArgs &a, int vp, double *out, hipStream_t st);
+
+// Row variant of the adjoint (the row conveyor's schedule and tables).
+struct AdjRowsArgs {
+  int N = 0, T = 0, m = 1, nlayers = 0, every = 1, nck = 0, nvec = 0, begin = 0, sample = 1;
+  int64_t K = 0, f_lo = 0;
+  float fac = 0.f;
+  const uint2 *steps = nullptr;  // row-schedule LDS byte offsets (a, b) for 64/R vectors
+  const float2 *cs = nullptr;
+  const float *jf = nullptr;
+  const float2 *diag = nullptr;
+  const float *mu_rec = nullptr;
+  int64_t mu_stride = 0;
+  const float2 *ck = nullptr;
+  const float2 *g = nullptr;
+  double *part_delta = nullptr;
+  double *part_c = nullptr;
+};
+int launch_adjoint_rows(const AdjRowsArgs &a, int R, double *out, hipStream_t st);
 int adjoint_vp(const Schedule &s, int N, int T);
 void adjoint_steps(const Schedule &s, int vp, std::vector<int2> &steps);
 

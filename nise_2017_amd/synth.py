@@ -215,3 +215,22 @@ def geometric_system(N, nframes, seed, dt=1.0, mean=15000.0, sd=100.0, tau=50.0,
     ai, bi = np.triu_indices(N)
     Crec[1 + np.nonzero(ai != bi)[0]] = J
     return H, Crec, dipole_records(mu, 1)
+
+
+def ring_2dir(N=32, nframes=4200, seed=SEED0 + 2, dt=2.0, mean=1650.0, sd=12.0, tau=100.0,
+              jmean=-6.0, jsd=2.0):
+    """N-site ring, Full type: AR(1) site energies, static nearest-neighbour
+    couplings N(jmean, jsd) including the (0, N-1) bond, fixed random unit
+    dipoles.  Returns (H, mu) records."""
+    rng = np.random.default_rng(seed)
+    diag = ar1(rng, nframes, N, mean, sd, tau, dt)
+    a, b = np.triu_indices(N, 1)
+    coup = np.zeros(a.size, np.float32)
+    J = rng.normal(jmean, jsd, N).astype(np.float32)
+    for i in range(N):
+        x, y = sorted((i, (i + 1) % N))
+        if x != y:
+            coup[np.nonzero((a == x) & (b == y))[0]] = J[i]
+    H = full_records(diag, coup)
+    mu = dipole_records(unit_vectors(rng, N), nframes)
+    return H, mu
