@@ -273,6 +273,14 @@ int nise_frame_tables(const t_non *non, const float *H_rec, int64_t H_stride,
                       const float *coupling_rec, int64_t pos, int sign,
                       float *diag_out, float *cs_out, int *K_out);
 
+/* The row schedule the row conveyor uses (inspection, host only): for the
+ * kept-coupling mask over the strict upper triangle in lexicographic order
+ * (N(N-1)/2 bytes), ab_out receives *nlayers layers of R entries
+ * (a | b << 16); entry q of a padded layer is (N+q, N+q).  Returns the
+ * entry count, or -1 if cap is too small. */
+int nise_row_schedule(int N, const uint8_t *mask, int R, uint32_t *ab_out,
+                      int cap, int *nlayers);
+
 /* Batched 2DIR (src/calc_2DIR.c:184-506 over all samples in
  * [non->begin, non->end) and the 21 molecular polarisation items).  Uses
  * non->singles, tmax1, tmax3 (tmax2 must be 0), sample, begin, end (0: as

@@ -60,13 +60,15 @@ class AdjointAbsorption:
     """
 
     def __init__(self, non, H, mu, nframes=None, coupling_rec=None, every=16, where=None,
-                 shape=None):
+                 shape=None, rows=None):
         if where is None:
             where = abi.NISE_DEVICE if hasattr(H, "data_ptr") and H.is_cuda else abi.NISE_HOST
         nfr = H.shape[0] if nframes is None else nframes
         self.plan = abi.AbsPlan(non, H, H.shape[1], mu, mu.shape[1], nfr, coupling_rec, where=where)
         if shape is not None:
             self.plan.set_shape(*shape)
+        elif rows is not None:
+            self.plan.set_rows(rows)
         self.plan.enable_adjoint(every)
         self.N, self.T = non.singles, non.tmax1
 

@@ -218,3 +218,19 @@ void rows_step_table(const Schedule &s, int R, std::vector<uint2> &tab) {
 }
 
 }  // namespace nise
+
+// Inspection of the row schedule (host only): entries (a | b << 16) of
+// nlayers layers of R, for a kept-coupling mask over the strict upper
+// triangle in lexicographic order.  Returns the entry count, or -1 if cap
+// is too small.
+extern "C" int nise_row_schedule(int N, const uint8_t *mask, int R, uint32_t *ab_out, int cap,
+                                 int *nlayers) {
+  if (N < 1 || R < 1 || !mask) return -1;
+  std::vector<uint8_t> m(mask, mask + (size_t)N * (N - 1) / 2);
+  nise::Schedule s;
+  nise::build_schedule_rows(N, m, R, s);
+  if ((int)s.ab.size() > cap) return -1;
+  std::copy(s.ab.begin(), s.ab.end(), ab_out);
+  if (nlayers) *nlayers = s.nlayers();
+  return (int)s.ab.size();
+}

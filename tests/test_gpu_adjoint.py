@@ -63,7 +63,7 @@ def fd_grad(orc, non, H, mu, Cr, d0, c0, A_re, A_im, target, h=0.01, hc=1e-4):
     return g, gc
 
 
-def run_case(orc, every, shape=None, tol=1e-4, **kw):
+def run_case(orc, every, shape=None, tol=1e-4, rows=None, **kw):
     """dL/dS from the FP64 oracle's S isolates the adjoint from the FP32
     forward error: the residual spec - target is a few 1e-3 of spec here, so
     the forward's ~1e-6 relative error would otherwise move dL/dS by ~1e-3
@@ -76,7 +76,8 @@ def run_case(orc, every, shape=None, tol=1e-4, **kw):
     freq, A_re, A_im = adjoint.spectrum_matrix(non, non.end)
     St = orc.absorption_param_f64(non, H, mu, rng.normal(0, 5, N), 1.1, Cr)
     target = A_re @ St[0] + A_im @ St[1]
-    model = adjoint.AdjointAbsorption(non, H, mu, coupling_rec=Cr, every=every, shape=shape)
+    model = adjoint.AdjointAbsorption(non, H, mu, coupling_rec=Cr, every=every, shape=shape,
+                                      rows=rows)
     S = model.forward(d0, c0)
     So = orc.absorption_param_f64(non, H, mu, d0, c0, Cr)
     err_fwd = np.linalg.norm((S[0] - So[0]) + 1j * (S[1] - So[1])) / np.linalg.norm(So[0] + 1j * So[1])
@@ -129,6 +130,12 @@ def test_gradient_coupling_type(orc):
 @pytest.mark.parametrize("shape", [(16, 5, 256), (8, 2, 64), (16, 5, -4), (8, 2, -2)])
 def test_checkpoints_in_every_conveyor_variant(orc, shape):
     run_case(orc, 4, shape=shape, N=8, T=40, seed=64)
+
+
+@pytest.mark.parametrize("rows", [1, 4, 16])
+def test_row_adjoint_variants(orc, rows):
+    run_case(orc, 8, rows=rows, N=9, T=40, seed=69)
+    run_case(orc, 5, rows=rows, N=9, T=24, seed=70, ctype=True)
 
 
 def test_forward_without_parameters_is_bit_exact():
