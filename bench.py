@@ -389,6 +389,26 @@ def main():
                          f"preloaded, OpenMP over samples, {dt:.1f} s",
                "single_thread_value": 3 * T * N / per_sample}
         log(f"cpu baseline: {cpu['value']:.3e} site-steps/s on {threads} threads")
+        if oracle.have_ref():
+            # the reference's own compiled absorption loop (propagate.c, read_He
+            # per step from files in the page cache), one thread, a few samples
+            import tempfile
+            kr = 2
+            Hr, mr = host_records(diag, row, mus, f_need(kr))
+            with tempfile.TemporaryDirectory() as d:
+                hf, df = os.path.join(d, "H.bin"), os.path.join(d, "mu.bin")
+                Hr.tofile(hf)
+                mr.tofile(df)
+                re_r = np.zeros(T, np.float32)
+                im_r = np.zeros(T, np.float32)
+                t_a = time.perf_counter()
+                oracle.ref().ref_absorption_files(N, hf.encode(), df.encode(), b"", b"", b"Full", lo, hi,
+                                                  2.0, 0.0, 1, T, sr, 0, kr, Hr.shape[0], re_r, im_r)
+                dtr = time.perf_counter() - t_a
+            cpu["reference_build"] = {"value": kr * 3 * T * N / dtr, "cores": 1, "kind": "reference",
+                                      "sample": f"{kr} samples, reference sources compiled here, "
+                                                f"frames read per step from files, {dtr:.1f} s"}
+            log(f"reference build: {cpu['reference_build']['value']:.3e} site-steps/s on 1 thread")
         # spectral check of the GPU run against the oracle on the same samples
         pc = abi.AbsPlan(nonk, H, H.shape[1], mu, mu.shape[1], nfr, where=abi.NISE_DEVICE)
         g_re, g_im = pc.execute()

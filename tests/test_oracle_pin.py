@@ -371,3 +371,29 @@ def test_1dfft_matches_numpy(orc):
     assert np.allclose(si, spec_i[k[sel]], rtol=1e-6, atol=1e-7 * np.abs(spec_i).max())
     # a planted mistake (no re/im swap) would fail: the two parts differ
     assert not np.allclose(sr, spec_i[k[sel]], rtol=1e-3)
+
+
+def test_oracle_absorption_equals_reference_build_cfg0(orc):
+    """The full cfg0 run (N=16 chain, 200 samples x 3 pols x 128 steps):
+    the oracle's absorption loop is bit-identical to the reference's own
+    compiled code reading the same GROBIN files."""
+    import os
+    import tempfile
+    if not orc.have_ref():
+        pytest.skip("reference build (oracle/_ref) not present")
+    N, T, sr, ns = 16, 128, 20, 200
+    H, mu = synth.chain_absorption(N=N, nframes=4200, seed=20261202)
+    lo, hi = synth.shifted_center(H, N)
+    with tempfile.TemporaryDirectory() as d:
+        hf, df = os.path.join(d, "H.bin"), os.path.join(d, "mu.bin")
+        H.tofile(hf)
+        mu.tofile(df)
+        re_r = np.zeros(T, np.float32)
+        im_r = np.zeros(T, np.float32)
+        orc.ref().ref_absorption_files(N, hf.encode(), df.encode(), b"", b"", b"Full", lo, hi, 2.0,
+                                       0.0, 1, T, sr, 0, ns, H.shape[0], re_r, im_r)
+    non = make_non(singles=N, tmax1=T, sample=sr, begin=0, end=ns, ts=1, propagation=1,
+                   deltat=2.0, min1=lo, max1=hi)
+    non.shifte = np.float32((np.float32(hi) + np.float32(lo)) / np.float32(2))
+    o_re, o_im = orc.absorption(non, H, mu)
+    assert np.array_equal(o_re, re_r) and np.array_equal(o_im, im_r)
