@@ -334,7 +334,9 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "rows_kernel" if info["rows"] else "conveyor_kernel", "kernel_ms": round(conv, 4),
                 "bytes_per_site_step": round(B, 3)}
-    tf = os.path.join(ROOT, "profiles", "conveyor_lattice256_traffic.json")
+    # HBM bytes per conveyor launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # passes over the same workload (scaled by a copy kernel of known size)
+    tf = os.path.join(ROOT, "profiles", f"{roofline['kernel']}_lattice256_traffic.json")
     if os.path.exists(tf):
         with open(tf) as fh:
             roofline["traffic"] = json.load(fh).get("bytes_per_launch")

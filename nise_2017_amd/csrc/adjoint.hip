@@ -242,16 +242,18 @@ __global__ void __launch_bounds__(64) adjoint_rows_kernel(AdjRowsArgs A) {
       continue;
     }
     const float2 *D = A.diag + row * N;
-    const float2 *cs = A.cs + row * A.K + q;
-    const float *jf = A.jf + row * A.K + q;
-    const uint2 *stq = A.steps + q;
+    // entry q of layer l in device order (rows_device_order)
+    const float2 *cs = A.cs + row * A.K + 4 * q;
+    const float *jf = A.jf + row * A.K + 4 * q;
+    const uint2 *stq = A.steps + 4 * q;
     for (int sub = 0; sub < A.m; ++sub) {
       site_pass(D, t, false);
       float cf = 0.f;
       for (int l = nl - 1; l >= 0; --l) {
-        const uint2 e = stq[l * R];
-        const float2 c = cs[l * R];
-        const float j = jf[l * R];
+        const int le = (l >> 2) * 4 * R + (l & 3);
+        const uint2 e = stq[le];
+        const float2 c = cs[le];
+        const float j = jf[le];
         float2 *pa_ = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(phi) + e.x + g * 8);
         float2 *pb_ = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(phi) + e.y + g * 8);
         float2 *xa_ = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(chi) + e.x + g * 8);

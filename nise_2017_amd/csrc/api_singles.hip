@@ -40,6 +40,7 @@ struct nise_abs_plan {
   int nsteps = 0;
   DevBuf ws_cs, ws_ab, resp_plain;
   int vp = 16, spw = 5, nwg = 0, block = 256;
+  int npart = 0;  // rows of partial responses (workgroups, or vectors for the row conveyor)
   // gradient runs: parameters and the adjoint's buffers (every == 0: off)
   DevBuf delta, jf, ck, gtab, part_d, part_c, grad, adj_slots, adj_wg, adj_steps;
   int every = 0, nck = 0, adj_vp = 0, adj_nwg = 0, adj_nsteps = 0;
@@ -97,14 +98,17 @@ int build_rows(nise_abs_plan *P) {
   std::vector<uint2> tab;
   rows_step_table(P->sch, P->rows, tab);
   if ((rc = upload(P->row_steps, tab.data(), tab.size() * sizeof(uint2), P->st))) return rc;
-  if ((rc = upload(P->ab, P->sch.ab.data(), P->sch.ab.size() * 4, P->st))) return rc;
+  std::vector<uint32_t> ab;  // the frame tables follow the step table's order
+  rows_device_order(P->sch, P->rows, ab);
+  if ((rc = upload(P->ab, ab.data(), ab.size() * 4, P->st))) return rc;
   if ((rc = upload(P->layer_off, P->sch.layer_off.data(), P->sch.layer_off.size() * 4, P->st))) return rc;
   const int64_t nfr = P->f_hi - P->f_lo;
   if ((rc = P->cs.alloc((size_t)nfr * std::max(P->sch.K, 1) * sizeof(float2)))) return rc;
   if (P->every && (rc = P->jf.alloc((size_t)nfr * std::max(P->sch.K, 1) * 4))) return rc;
   const int G = 64 / P->rows;
   P->nwg = (3 * (P->end - P->begin) + G - 1) / G;
-  if ((rc = P->partial.alloc((size_t)std::max(P->nwg, 1) * P->tmax * sizeof(double2)))) return rc;
+  P->npart = 3 * (P->end - P->begin);  // one row per vector
+  if ((rc = P->partial.alloc((size_t)std::max(P->npart, 1) * P->tmax * sizeof(double2)))) return rc;
   return upload_rows_desc(P);
 }
 
@@ -129,6 +133,7 @@ int build_slots(nise_abs_plan *P) {
   }
   wb.push_back((int)sl.size());
   P->nwg = (int)wb.size() - 1;
+  P->npart = P->nwg;
   int rc;
   std::vector<int2> stp;
   build_steps(P->sch, P->vp, P->block, stp);
@@ -166,7 +171,7 @@ int run_conveyor(nise_abs_plan *P) {
 
 int run_reduce(nise_abs_plan *P) {
   float *re = P->out.as<float>();
-  return launch_reduce_partials(P->partial.as<double2>(), P->nwg, P->tmax, re, re + P->tmax, P->st);
+  return launch_reduce_partials(P->partial.as<double2>(), P->npart, P->tmax, re, re + P->tmax, P->st);
 }
 
 __global__ void zero_d2(double2 *p, int n) {

@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(256) dia_conveyor_kernel(const DiaSys *sys, co
   for (int64_t f = F0; f < F1; ++f) {
     const int64_t row = f - S.f_lo;
     const float2 *Uf = S.U + row * N * N;
-    for (int i = t; i < N * N; i += 256) U[i] = Uf[i];
+    for (int i = t; i < N * N; i += 256) U[i] = gld(Uf + i);
     const float *mu = S.mu_rec + (S.mu_stride ? f * S.mu_stride : 0) + 1;
     __syncthreads();
     for (int v = sg; v < nv; v += groups) {
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(256) dia_conveyor_kernel(const DiaSys *sys, co
       float r = 0.f, q = 0.f;
       float2 p = make_float2(0.f, 0.f);
       if (act && a < N) {
-        const float m = mu[x * N + a];
+        const float m = gld(mu + x * N + a);
         p = rel == 0 ? make_float2(m, 0.f) : psA[v * Np + a];
         if (rel == 0) psA[v * Np + a] = p;
         r = __fmul_rn(m, p.x);
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(256) dia_conveyor_kernel(const DiaSys *sys, co
     psA = psB, psB = tmp;
   }
   const int w = v0 / S.vp;
-  for (int i = t; i < T; i += 256) S.partial[(int64_t)w * T + i] = acc[i];
+  for (int i = t; i < T; i += 256) gst(S.partial + (int64_t)w * T + i, acc[i]);
   if (t == 0 && times) {
     times[2 * blockIdx.x] = t_start;
     times[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
@@ -157,7 +157,7 @@ int up(DevBuf &d, const void *s, size_t n) {
 }
 
 struct SysState {
-  int N = 0, T = 0, dia = 0, nvec = 0, nwg = 0;
+  int N = 0, T = 0, dia = 0, nvec = 0, nwg = 0;  // nwg: rows of partial responses
   int64_t f_lo = 0, f_hi = 0;
   FrameSource src;
   DevBuf H, mu, C, shs, sha, ab, diag, cs, steps, U, partial;
@@ -275,15 +275,16 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
       build_schedule_rows(N, s.mask, R, s.sch);
       std::vector<uint2> tab;
       rows_step_table(s.sch, R, tab);
-      if ((rc = up(s.ab, s.sch.ab.data(), s.sch.ab.size() * 4)) ||
+      std::vector<uint32_t> abd;  // the frame tables follow the step table's order
+      rows_device_order(s.sch, R, abd);
+      if ((rc = up(s.ab, abd.data(), abd.size() * 4)) ||
           (rc = up(s.steps, tab.data(), tab.size() * sizeof(uint2))) ||
           (rc = s.diag.alloc((size_t)nfr * N * 8)) || (rc = s.cs.alloc((size_t)nfr * s.sch.K * 8)))
         return rc;
       if ((rc = launch_frame_tables(s.src, N, s.sch, s.ab.as<uint32_t>(), trotter_factor(non->deltat, 1, non->ts),
                                     s.f_lo, nfr, s.diag.as<float2>(), s.cs.as<float2>(), st)))
         return rc;
-      const int G = 64 / R;
-      s.nwg = (s.nvec + G - 1) / G;
+      s.nwg = s.nvec;  // partial rows: one per vector
       if ((rc = s.partial.alloc((size_t)s.nwg * s.T * sizeof(double2)))) return rc;
       RowsArgs r;
       r.N = N, r.tmax = s.T, r.m = non->ts, r.nlayers = s.sch.nlayers(), r.K = s.sch.K, r.f_lo = s.f_lo;

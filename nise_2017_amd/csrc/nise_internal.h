@@ -124,6 +124,32 @@ __device__ inline float kept_coupling(const FrameSource &s, int N, int64_t f, in
   return (fabsf(J) > s.cut) ? J : 0.f;
 }
 
+// Global-memory access through a pointer read from a descriptor in memory.
+// The compiler cannot place such a pointer in an address space and emits
+// flat instructions, which also count against the LDS wait counter: every
+// wait for an LDS result would then wait for outstanding HBM loads too.
+// The access goes through a plain integer (or integer vector) type of the
+// same size: the address-space qualifier does not survive on class types.
+template <int B> struct GWord;
+template <> struct GWord<4> { typedef unsigned type; };
+template <> struct GWord<8> { typedef unsigned long long type; };
+template <> struct GWord<16> { typedef unsigned type __attribute__((ext_vector_type(4))); };
+template <class T>
+__device__ __forceinline__ T gld(const T *p) {
+  typedef typename GWord<sizeof(T)>::type W;
+  const W w = *(const __attribute__((address_space(1))) W *)p;
+  T v;
+  __builtin_memcpy(&v, &w, sizeof(T));
+  return v;
+}
+template <class T>
+__device__ __forceinline__ void gst(T *p, T v) {
+  typedef typename GWord<sizeof(T)>::type W;
+  W w;
+  __builtin_memcpy(&w, &v, sizeof(T));
+  *(__attribute__((address_space(1))) W *)p = w;
+}
+
 // Complex helpers with the reference's operation order.
 // src/propagate.c:871-872 / :892-893: (cr*re - ci*im, ci*re + cr*im)
 __device__ __forceinline__ float2 cmul_diag(float2 c, float2 u) {
@@ -237,7 +263,7 @@ int doubles_np(int N);
 
 // Row conveyor (rows.hip), one descriptor per system.  Vector v = 3 s + x
 // is polarisation x of sample begin + s, starting at frame (begin + s) *
-// sample.  partial holds one row of tmax per workgroup of the system.
+// sample.  partial holds one row of tmax per vector of the system.
 struct RowsArgs {
   int N = 0, tmax = 0, m = 1, nlayers = 0;  // nlayers: multiple of 8
   int64_t K = 0;                 // entries per sweep (= nlayers * R)
@@ -248,7 +274,7 @@ struct RowsArgs {
   const float2 *diag = nullptr;  // [rows][N]
   const float *mu_rec = nullptr;
   int64_t mu_stride = 0;
-  double2 *partial = nullptr;    // [waves][tmax]
+  double2 *partial = nullptr;    // [vectors][tmax]
   float2 *hist = nullptr;        // checkpoints (hist_every, nck per vector)
   int hist_every = 1, nck = 0;
 };
@@ -269,6 +295,7 @@ struct DiaSys {
 };
 size_t rows_smem(int N, int R);
 int rows_for(int N, int Kreal, int nlayers_asap);
+void rows_device_order(const Schedule &s, int R, std::vector<uint32_t> &ab);
 void rows_step_table(const Schedule &s, int R, std::vector<uint2> &tab);
 
 // Adjoint of the batched absorption (adjoint.hip).

@@ -11,16 +11,23 @@
 // vector is propagated with the reference's FP32 operations in the
 // reference's order.
 //
-// The G vectors of a wave run in lock step over relative time t = 0..tmax-1
-// but each at its own frame fs + t; a lane reads its vector's frame tables
-// (half-step phases, rotation (cos, sin) in row-schedule order, dipoles).
-// Per step t: the per-site pass applies the pending half-phase of frame
-// fs+t-1, accumulates mu(fs+t).psi (src/absorption.c:130, calc_S1 :205-212),
-// records checkpoints, and applies the first half-phase of frame fs+t; then
-// the m Trotter sweeps of frame fs+t.  The wave's response of step t (all
-// its vectors) is summed over rows and vectors in a fixed order and written
-// as an FP64 partial.
+// The wave runs frame-synchronously: all its vectors see the same frame f
+// at the same time, vector g being at relative time f - fs_g (its sample's
+// start frame fs_g) and active while that lies in [0, tmax).  A frame's
+// rotation table is therefore the same for every lane of a row, so each
+// lane loads one entry of a block of 4 layers (lane & 3 picks the layer)
+// with a wave-uniform base address, and the quad broadcasts it (DPP
+// quad_perm, folded into the arithmetic that uses it): one 8-byte load per
+// lane covers 4 layers.  An 8-layer register ring prefetches the entries.
+//
+// Per frame: the per-site pass applies the pending half-phase of frame
+// f-1, accumulates mu(f).psi (src/absorption.c:130, calc_S1 :205-212),
+// records checkpoints, and applies the first half-phase of frame f; then
+// the m Trotter sweeps of frame f if any vector still propagates.  Each
+// vector's response at its relative time is summed over rows and written
+// as an FP64 partial of that vector.
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -28,7 +35,9 @@
 
 namespace nise {
 
-constexpr int kRingR = 8;  // rotation operands in flight per lane
+constexpr int kRingG = 2;           // ring slots per lane, 4 layers each
+constexpr int kRingL = 4 * kRingG;  // layers in flight (nlayers is a multiple)
+constexpr int kSiteU = 4;           // sites per batch of HBM loads in the per-site pass
 
 // One rotation (src/propagate.c:883-887) of the pair at LDS byte offsets
 // e.x, e.y (plus the lane's vector column) with (cos, sin) = c.
@@ -41,118 +50,190 @@ __device__ __forceinline__ void rot_step(unsigned char *smem, uint32_t gofs, uin
   *pb_ = pb;
 }
 
+// Value of quad lane j in every lane of the quad.
+template <int j>
+__device__ __forceinline__ uint32_t quad_bcast(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, j * 0x55, 0xF, 0xF, false);
+}
+template <int j>
+__device__ __forceinline__ float quad_bcast(float x) {
+  return __uint_as_float(quad_bcast<j>(__float_as_uint(x)));
+}
+
+// Layer j of a ring slot: the entry loaded by quad lane j.
+template <int j>
+__device__ __forceinline__ void rot_slot(unsigned char *smem, uint32_t gofs, uint2 e, float2 c) {
+  rot_step(smem, gofs, make_uint2(quad_bcast<j>(e.x), quad_bcast<j>(e.y)),
+           make_float2(quad_bcast<j>(c.x), quad_bcast<j>(c.y)));
+}
+
 // Workgroup w runs the vectors wmap[w].y .. wmap[w].y + G - 1 of system
 // wmap[w].x (systems of different sizes share one launch).
 template <int R>
-__global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *sys, const int2 *wmap,
+__global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ sys, const int2 *__restrict__ wmap,
                                                   unsigned long long *times) {
   constexpr int G = 64 / R;
+  static_assert(G >= 4, "a row needs a whole quad of lanes");
   extern __shared__ __align__(16) unsigned char smem[];
   const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
-  const int2 wm = wmap[blockIdx.x];
-  const RowsArgs &A = sys[wm.x];
-  const int N = A.N, T = A.tmax, nl = A.nlayers;
+  // wave-uniform: the descriptor is read with scalar loads and kept in
+  // registers (the kernel's own stores cannot alias it)
+  const int2 wm0 = wmap[blockIdx.x];
+  const int2 wm = make_int2(__builtin_amdgcn_readfirstlane(wm0.x), __builtin_amdgcn_readfirstlane(wm0.y));
+  const RowsArgs A = sys[wm.x];
+  const int N = A.N, T = A.tmax, nl = A.nlayers, m = A.m;
+  const int64_t K = A.K;
   // lane = q * G + g: the G vectors of a row are adjacent, so a row's LDS
-  // accesses cover one contiguous 8*G-byte run (fewer bank conflicts)
+  // accesses cover one contiguous 8*G-byte run
   const int lane = threadIdx.x, q = lane / G, g = lane % G;
-  const int w = wm.y / G;  // workgroup index within the system
   const int v = wm.y + g;
   const bool has = v < A.nvec;
-  const int vv = has ? v : w * G;  // inactive lanes shadow the wave's first vector
-  const int s = vv / 3, x = vv % 3;
-  const int64_t fs = (int64_t)(A.begin + s) * A.sample;
+  const int x = has ? v % 3 : 0;
+  const int64_t fs = has ? (int64_t)(A.begin + v / 3) * A.sample : LLONG_MAX / 4;
   const uint32_t gofs = (uint32_t)g * 8u;
   const int64_t hist0 = (A.hist && has) ? (int64_t)v * A.nck : -1;
-  const int64_t K = A.K;
+  // frames of the wave: first vector's start to last vector's end
+  const int vlast = min(wm.y + G, A.nvec) - 1;
+  const int64_t F0 = (int64_t)(A.begin + wm.y / 3) * A.sample;
+  const int64_t F1 = (int64_t)(A.begin + vlast / 3) * A.sample + T;
 
-  // ring of (LDS byte offsets of a and b, (cos, sin)) for the next kRingR steps
-  uint2 re[kRingR];
-  float2 rc[kRingR];
-  const uint2 *stq = A.steps + q;
-  {
-    const float2 *c0 = A.cs + (fs - A.f_lo) * K + q;
-#pragma unroll
-    for (int j = 0; j < kRingR; ++j) {
-      re[j] = stq[j * R];
-      rc[j] = c0[j * R];
+  for (int i = lane; i < (N + R) * G; i += 64) reinterpret_cast<float2 *>(smem)[i] = make_float2(0.f, 0.f);
+
+  // ring: slot k holds, in quad lane j, entry q of layer 4k + j of the
+  // block in flight; loff is the lane's entry within a block of 4 layers
+  // (device order, rows_device_order)
+  const uint32_t loff = (uint32_t)(4 * q + (lane & 3));
+  uint2 re[kRingG];
+  float2 rc[kRingG];
+  int64_t ring_key = -1;  // sweep (f * m + sub) whose first block the ring holds
+
+  for (int64_t f = F0; f < F1; ++f) {
+    const int64_t t = f - fs;  // this lane's relative time
+    const bool act = t >= 0 && t < T;
+    if (!__ballot(act)) {  // no vector of the wave is at this frame: jump
+      int64_t nf = F1;
+      for (int k = 0; k < G && wm.y + k <= vlast; ++k) {
+        const int64_t fk = (int64_t)(A.begin + (wm.y + k) / 3) * A.sample;
+        if (fk > f) { nf = fk; break; }
+      }
+      f = nf - 1;
+      continue;
     }
-  }
-
-  for (int t = 0; t < T; ++t) {
-    const int64_t f = fs + t;
     const int64_t row = f - A.f_lo;
-    const float *mu = A.mu_rec + (A.mu_stride ? f * A.mu_stride : 0) + 1 + (int64_t)x * N;
-    const float2 *Dc = A.diag + row * N;
-    const float2 *Dp = Dc - N;
     const bool last = t == T - 1;
-    int64_t hr = -1;
-    if (hist0 >= 0) {
-      if (t % A.hist_every == 0) hr = hist0 + t / A.hist_every;
-      else if (last) hr = hist0 + t / A.hist_every + 1;
-    }
     float rr = 0.f, ri = 0.f;
-    for (int i = q; i < N; i += R) {
-      const float m = mu[i];
-      float2 p;
-      if (t == 0) p = make_float2(m, 0.f);
-      else p = cmul_diag(*reinterpret_cast<const float2 *>(smem + ((uint32_t)i * G * 8u + gofs)), Dp[i]);
-      rr = __fadd_rn(rr, __fmul_rn(m, p.x));
-      ri = __fadd_rn(ri, __fmul_rn(m, p.y));
-      if (hr >= 0) A.hist[hr * N + i] = p;
-      if (!last) p = cmul_diag(p, Dc[i]);
-      *reinterpret_cast<float2 *>(smem + ((uint32_t)i * G * 8u + gofs)) = p;
-    }
-    // response of step t: rows of a vector, then the wave's vectors in order
-    if (!has) rr = 0.f, ri = 0.f;
+    if (act) {
+      const float *mu = A.mu_rec + (A.mu_stride ? f * A.mu_stride : 0) + 1 + (int64_t)x * N;
+      const float2 *Dc = A.diag + row * N;
+      const float2 *Dp = Dc - N;
+      int64_t hr = -1;
+      if (hist0 >= 0) {
+        if (t % A.hist_every == 0) hr = hist0 + t / A.hist_every;
+        else if (last) hr = hist0 + t / A.hist_every + 1;
+      }
+      // kSiteU sites per batch: all HBM loads of a batch are issued before
+      // the first is used (indexes past N load site 0 and are not applied)
+      for (int i0 = q; i0 < N; i0 += kSiteU * R) {
+        float mv[kSiteU];
+        float2 dp[kSiteU], dc[kSiteU];
 #pragma unroll
-    for (int off = G; off < 64; off <<= 1) {  // over the rows of a vector
+        for (int u = 0; u < kSiteU; ++u) {
+          const int i = i0 + u * R, ii = i < N ? i : 0;
+          mv[u] = gld(mu + ii);
+          dp[u] = t ? gld(Dp + ii) : make_float2(0.f, 0.f);
+          dc[u] = last ? make_float2(0.f, 0.f) : gld(Dc + ii);
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keep the batch of loads together
+#pragma unroll
+        for (int u = 0; u < kSiteU; ++u) {
+          const int i = i0 + u * R;
+          const bool ok = i < N;
+          float2 *pp = reinterpret_cast<float2 *>(smem + ((uint32_t)(ok ? i : q) * G * 8u + gofs));
+          const float mm = mv[u];
+          float2 p = t == 0 ? make_float2(mm, 0.f) : cmul_diag(*pp, dp[u]);
+          if (ok) {
+            rr = __fadd_rn(rr, __fmul_rn(mm, p.x));
+            ri = __fadd_rn(ri, __fmul_rn(mm, p.y));
+            if (hr >= 0) gst(A.hist + hr * N + i, p);
+            if (!last) p = cmul_diag(p, dc[u]);
+            *pp = p;
+          }
+        }
+      }
+    }
+    // response of each vector at its relative time: sum over its rows
+#pragma unroll
+    for (int off = G; off < 64; off <<= 1) {
       rr += __shfl_xor(rr, off);
       ri += __shfl_xor(ri, off);
     }
-    double sr = 0.0, si = 0.0;
-#pragma unroll
-    for (int k = 0; k < G; ++k) {  // the wave's vectors in order
-      sr += (double)__shfl(rr, k);
-      si += (double)__shfl(ri, k);
-    }
-    if (lane == 0) A.partial[(int64_t)w * T + t] = make_double2(sr, si);
-    if (last) {
-      if (lane == 0 && times) {  // device wall clock at start and end
-        times[2 * blockIdx.x] = t_start;
-        times[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
-      }
-      break;
-    }
+    if (act && q == 0) gst(A.partial + (int64_t)v * T + t, make_double2((double)rr, (double)ri));
+    if (!__ballot(act && !last)) continue;  // nothing propagates past this frame
 
-    for (int sub = 0; sub < A.m; ++sub) {
-      if (sub > 0)
-        for (int i = q; i < N; i += R) {
-          float2 *pp = reinterpret_cast<float2 *>(smem + ((uint32_t)i * G * 8u + gofs));
-          *pp = cmul_diag(cmul_diag(*pp, Dc[i]), Dc[i]);
+    for (int sub = 0; sub < m; ++sub) {
+      if (sub > 0) {
+        const float2 *Dc = A.diag + row * N;
+        for (int i0 = q; i0 < N; i0 += kSiteU * R) {
+          float2 dc[kSiteU];
+#pragma unroll
+          for (int u = 0; u < kSiteU; ++u) {
+            const int i = i0 + u * R;
+            dc[u] = gld(Dc + (i < N ? i : 0));
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int u = 0; u < kSiteU; ++u) {
+            const int i = i0 + u * R;
+            if (i < N) {
+              float2 *pp = reinterpret_cast<float2 *>(smem + ((uint32_t)i * G * 8u + gofs));
+              *pp = cmul_diag(cmul_diag(*pp, dc[u]), dc[u]);
+            }
+          }
         }
-      // Each ring slot is consumed and refilled with the operands kRingR
-      // steps ahead: from this sweep, and in the last group (nlayers is a
-      // multiple of kRingR) from the first steps of the next sweep, which
+      }
+      const float2 *csf = A.cs + row * K + loff;
+      if (ring_key != f * m + sub) {  // the previous sweep prefetched another one
+#pragma unroll
+        for (int k = 0; k < kRingG; ++k) {
+          re[k] = gld(A.steps + loff + 4 * k * R);
+          rc[k] = gld(csf + 4 * k * R);
+        }
+      }
+      // Each ring slot is consumed and refilled with the entries kRingL
+      // layers ahead: from this sweep, and in the last block (nlayers is a
+      // multiple of kRingL) from the first block of the next sweep, which
       // runs this frame again or the next frame.
-      const float2 *csc = A.cs + row * K + q;
-      const float2 *csn = A.cs + (row + (sub + 1 < A.m ? 0 : 1)) * K + q;
-      const uint2 *sa = stq + kRingR * R;
-      const float2 *ca = csc + kRingR * R;
-      for (int b0 = 0; b0 < nl - kRingR; b0 += kRingR, sa += kRingR * R, ca += kRingR * R) {
+      const uint2 *sa = A.steps + loff + kRingL * R;
+      const float2 *ca = csf + kRingL * R;
+      for (int b0 = 0; b0 < nl - kRingL; b0 += kRingL, sa += kRingL * R, ca += kRingL * R) {
 #pragma unroll
-        for (int j = 0; j < kRingR; ++j) {
-          rot_step(smem, gofs, re[j], rc[j]);
-          re[j] = sa[j * R];
-          rc[j] = ca[j * R];
+        for (int k = 0; k < kRingG; ++k) {
+          rot_slot<0>(smem, gofs, re[k], rc[k]);
+          rot_slot<1>(smem, gofs, re[k], rc[k]);
+          rot_slot<2>(smem, gofs, re[k], rc[k]);
+          rot_slot<3>(smem, gofs, re[k], rc[k]);
+          re[k] = gld(sa + 4 * k * R);
+          rc[k] = gld(ca + 4 * k * R);
+          __builtin_amdgcn_sched_barrier(0);  // the refill stays here
         }
       }
+      const float2 *cn = A.cs + (row + (sub + 1 < m ? 0 : 1)) * K + loff;
 #pragma unroll
-      for (int j = 0; j < kRingR; ++j) {
-        rot_step(smem, gofs, re[j], rc[j]);
-        re[j] = stq[j * R];
-        rc[j] = csn[j * R];
+      for (int k = 0; k < kRingG; ++k) {
+        rot_slot<0>(smem, gofs, re[k], rc[k]);
+        rot_slot<1>(smem, gofs, re[k], rc[k]);
+        rot_slot<2>(smem, gofs, re[k], rc[k]);
+        rot_slot<3>(smem, gofs, re[k], rc[k]);
+        re[k] = gld(A.steps + loff + 4 * k * R);
+        rc[k] = gld(cn + 4 * k * R);
+        __builtin_amdgcn_sched_barrier(0);
       }
+      ring_key = f * m + sub + 1;
     }
+  }
+  if (lane == 0 && times) {  // device wall clock at start and end
+    times[2 * blockIdx.x] = t_start;
+    times[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
   }
 }
 
@@ -175,8 +256,8 @@ int rows_prepare(const RowsArgs *sys, int nsys, int R, std::vector<int2> &wmap, 
   const int G = 64 / R;
   for (int k = 0; k < nsys; ++k) {
     const RowsArgs &a = sys[k];
-    if (a.nlayers < kRingR || a.nlayers % kRingR)
-      return set_error("row conveyor: layer count %d is not a multiple of %d", a.nlayers, kRingR), NISE_ERR_ARG;
+    if (a.nlayers < kRingL || a.nlayers % kRingL)
+      return set_error("row conveyor: layer count %d is not a multiple of %d", a.nlayers, kRingL), NISE_ERR_ARG;
     if (a.K != (int64_t)a.nlayers * R) return set_error("row conveyor: schedule built for other rows"), NISE_ERR_ARG;
     smem = std::max(smem, rows_smem(a.N, R));
     for (int v = 0; v < a.nvec; v += G) wmap.push_back(make_int2(k, v));
@@ -208,13 +289,25 @@ int rows_for(int N, int Kreal, int nlayers_asap) {
   return R;
 }
 
-// Static step table of a row schedule: LDS byte offsets of the two sites of
-// every entry for a wave of 64/R vectors.
+// Device order of a row schedule's entries (the order of the step table and
+// of every per-frame table built from it): entry q of layer l sits at
+// (l / 4) * 4R + 4q + l % 4, so the 4 layers a quad loads at once are one
+// contiguous 32-byte run and a wave's load of a 4-layer block is contiguous.
+void rows_device_order(const Schedule &s, int R, std::vector<uint32_t> &ab) {
+  ab.resize(s.ab.size());
+  const int nl = s.nlayers();
+  for (int l = 0; l < nl; ++l)
+    for (int q = 0; q < R; ++q) ab[(size_t)(l / 4) * 4 * R + 4 * q + l % 4] = s.ab[(size_t)l * R + q];
+}
+
+// Static step table of a row schedule in device order: LDS byte offsets of
+// the two sites of every entry for a wave of 64/R vectors.
 void rows_step_table(const Schedule &s, int R, std::vector<uint2> &tab) {
   const uint32_t G = 64 / R;
-  tab.resize(s.ab.size());
-  for (size_t i = 0; i < s.ab.size(); ++i)
-    tab[i] = make_uint2((s.ab[i] & 0xffffu) * G * 8u, (s.ab[i] >> 16) * G * 8u);
+  std::vector<uint32_t> ab;
+  rows_device_order(s, R, ab);
+  tab.resize(ab.size());
+  for (size_t i = 0; i < ab.size(); ++i) tab[i] = make_uint2((ab[i] & 0xffffu) * G * 8u, (ab[i] >> 16) * G * 8u);
 }
 
 }  // namespace nise
