@@ -139,14 +139,6 @@ size_t dia_conveyor_smem(int N, int Np, int vp, int T) {
   return (size_t)N * N * 8 + (size_t)2 * vp * Np * 8 + (size_t)T * 16 + (size_t)vp * 8;
 }
 
-// Per-system reduction of workgroup partials (fixed order, FP64).
-__global__ void reduce_sys_kernel(const double2 *partial, int nwg, int T, float *re, float *im) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= T) return;
-  double sr = 0.0, si = 0.0;
-  for (int g = 0; g < nwg; ++g) sr += partial[(int64_t)g * T + i].x, si += partial[(int64_t)g * T + i].y;
-  re[i] = (float)sr, im[i] = (float)si;
-}
 
 namespace {
 int up(DevBuf &d, const void *s, size_t n) {
@@ -378,9 +370,7 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
   for (int k = 0; k < nsys; ++k) {
     SysState &s = *S[k];
     float *o = out.as<float>() + 2 * off;
-    hipLaunchKernelGGL(reduce_sys_kernel, dim3((s.T + 127) / 128), dim3(128), 0, st, s.partial.as<double2>(),
-                       s.nwg, s.T, o, o + s.T);
-    NISE_HIP_CHECK(hipGetLastError());
+    if ((rc = launch_reduce_partials(s.partial.as<double2>(), s.nwg, s.T, o, o + s.T, st))) return rc;
     off += s.T;
   }
   std::vector<float> ho((size_t)2 * Tsum);

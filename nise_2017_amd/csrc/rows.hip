@@ -11,16 +11,18 @@
 // vector is propagated with the reference's FP32 operations in the
 // reference's order.
 //
-// The wave runs frame-synchronously: all its vectors see the same frame f
-// at the same time, vector g being at relative time f - fs_g (its sample's
-// start frame fs_g) and active while that lies in [0, tmax).  A frame's
-// rotation table is therefore the same for every lane of a row, so each
-// lane loads one entry of a block of 4 layers (lane & 3 picks the layer)
-// with a wave-uniform base address, and the quad broadcasts it (DPP
-// quad_perm, folded into the arithmetic that uses it): one 8-byte load per
-// lane covers 4 layers.  An 8-layer register ring prefetches the entries.
+// Each quad of vectors (g / 4) runs frame-synchronously: its 4 vectors see
+// the same frame f at the same time, vector g being at relative time
+// f - fs_g (its sample's start frame fs_g) and active while that lies in
+// [0, tmax); quads run time-skewed against each other.  A frame's rotation
+// table is therefore the same for the 4 lanes of a row within a quad, so
+// each lane loads one entry of a block of 4 layers (lane & 3 picks the
+// layer) and the quad broadcasts it (DPP quad_perm, folded into the
+// arithmetic that uses it): one 8-byte load per lane covers 4 layers.  With
+// R = 16 (4 vectors per wave) the table address is wave-uniform.  An
+// 8-layer register ring prefetches the entries.
 //
-// Per frame: the per-site pass applies the pending half-phase of frame
+// Per step: the per-site pass applies the pending half-phase of frame
 // f-1, accumulates mu(f).psi (src/absorption.c:130, calc_S1 :205-212),
 // records checkpoints, and applies the first half-phase of frame f; then
 // the m Trotter sweeps of frame f if any vector still propagates.  Each
@@ -92,10 +94,18 @@ __global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ s
   const int64_t fs = has ? (int64_t)(A.begin + v / 3) * A.sample : LLONG_MAX / 4;
   const uint32_t gofs = (uint32_t)g * 8u;
   const int64_t hist0 = (A.hist && has) ? (int64_t)v * A.nck : -1;
-  // frames of the wave: first vector's start to last vector's end
+  // Vector quad h = g / 4 (4 consecutive vectors) shares a frame: at wave
+  // time tau it is at frame F0(h) + tau, F0(h) being its first vector's
+  // start; quads run time-skewed, so the wave runs for the longest quad span
+  // (its last vector's start - F0(h) + tmax).
   const int vlast = min(wm.y + G, A.nvec) - 1;
-  const int64_t F0 = (int64_t)(A.begin + wm.y / 3) * A.sample;
-  const int64_t F1 = (int64_t)(A.begin + vlast / 3) * A.sample + T;
+  auto start_of = [&](int vec) { return (int64_t)(A.begin + vec / 3) * A.sample; };
+  int64_t span = 0;
+  for (int h = 0; 4 * h < G && wm.y + 4 * h <= vlast; ++h)
+    span = max(span, start_of(min(wm.y + 4 * h + 3, vlast)) - start_of(wm.y + 4 * h) + T);
+  const int vq = min(wm.y + (g & ~3), vlast);  // this lane's quad's first vector
+  const int64_t F0 = start_of(vq);
+  const int64_t Fend = start_of(min(vq + 3, vlast)) + T - 1;  // the quad's last frame
 
   for (int i = lane; i < (N + R) * G; i += 64) reinterpret_cast<float2 *>(smem)[i] = make_float2(0.f, 0.f);
 
@@ -105,21 +115,24 @@ __global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ s
   const uint32_t loff = (uint32_t)(4 * q + (lane & 3));
   uint2 re[kRingG];
   float2 rc[kRingG];
-  int64_t ring_key = -1;  // sweep (f * m + sub) whose first block the ring holds
+  int64_t ring_key = -1;  // sweep (tau * m + sub) whose first block the ring holds
 
-  for (int64_t f = F0; f < F1; ++f) {
-    const int64_t t = f - fs;  // this lane's relative time
+  for (int64_t tau = 0; tau < span; ++tau) {
+    const int64_t f = F0 + tau;  // this lane's frame
+    const int64_t t = f - fs;    // this lane's relative time
     const bool act = t >= 0 && t < T;
-    if (!__ballot(act)) {  // no vector of the wave is at this frame: jump
-      int64_t nf = F1;
+    if (!__ballot(act)) {  // no vector of the wave is at its frame: jump
+      int64_t nt = span;
       for (int k = 0; k < G && wm.y + k <= vlast; ++k) {
-        const int64_t fk = (int64_t)(A.begin + (wm.y + k) / 3) * A.sample;
-        if (fk > f) { nf = fk; break; }
+        const int64_t o = start_of(wm.y + k) - start_of(wm.y + (k & ~3));
+        if (o > tau) nt = min(nt, o);
       }
-      f = nf - 1;
+      tau = nt - 1;
       continue;
     }
-    const int64_t row = f - A.f_lo;
+    // table row of the frame, clamped to the quad's frames (a finished
+    // quad keeps loading valid rows)
+    const int64_t row = min(f, Fend) - A.f_lo;
     const bool last = t == T - 1;
     float rr = 0.f, ri = 0.f;
     if (act) {
@@ -192,7 +205,7 @@ __global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ s
         }
       }
       const float2 *csf = A.cs + row * K + loff;
-      if (ring_key != f * m + sub) {  // the previous sweep prefetched another one
+      if (ring_key != tau * m + sub) {  // the previous sweep prefetched another one
 #pragma unroll
         for (int k = 0; k < kRingG; ++k) {
           re[k] = gld(A.steps + loff + 4 * k * R);
@@ -217,7 +230,7 @@ __global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ s
           __builtin_amdgcn_sched_barrier(0);  // the refill stays here
         }
       }
-      const float2 *cn = A.cs + (row + (sub + 1 < m ? 0 : 1)) * K + loff;
+      const float2 *cn = A.cs + (sub + 1 < m ? row : min(f + 1, Fend) - A.f_lo) * K + loff;
 #pragma unroll
       for (int k = 0; k < kRingG; ++k) {
         rot_slot<0>(smem, gofs, re[k], rc[k]);
@@ -228,7 +241,7 @@ __global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ s
         rc[k] = gld(cn + 4 * k * R);
         __builtin_amdgcn_sched_barrier(0);
       }
-      ring_key = f * m + sub + 1;
+      ring_key = tau * m + sub + 1;
     }
   }
   if (lane == 0 && times) {  // device wall clock at start and end

@@ -934,22 +934,36 @@ int launch_conveyor(const ConveyorArgs &a, int vp, int block, hipStream_t st) {
 
 // Deterministic sum of the per-workgroup response partials (fixed order,
 // double accumulation), rounded to float at the end.
-__global__ void reduce_partials_kernel(const double2 *partial, int nwg, int tmax, float *re, float *im) {
-  int t1 = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t1 >= tmax) return;
+// Sum of the partial rows per time step in a fixed order: thread (x, y) of
+// a block sums rows y, y + kRedY, ... of time step blockIdx.x * kRedX + x,
+// then the kRedY sums are added in y order (FP64 throughout).
+constexpr int kRedX = 16, kRedY = 64;
+
+__global__ void __launch_bounds__(kRedX * kRedY) reduce_partials_kernel(const double2 *partial, int nwg, int tmax,
+                                                                        float *re, float *im) {
+  __shared__ double2 acc[kRedY][kRedX];
+  const int x = threadIdx.x % kRedX, y = threadIdx.x / kRedX;
+  const int t1 = blockIdx.x * kRedX + x;
   double sr = 0, si = 0;
-  for (int g = 0; g < nwg; ++g) {
-    double2 p = partial[(int64_t)g * tmax + t1];
-    sr += p.x, si += p.y;
+  if (t1 < tmax)
+    for (int g = y; g < nwg; g += kRedY) {
+      const double2 p = partial[(int64_t)g * tmax + t1];
+      sr += p.x, si += p.y;
+    }
+  acc[y][x] = make_double2(sr, si);
+  __syncthreads();
+  if (y == 0 && t1 < tmax) {
+    sr = 0, si = 0;
+    for (int k = 0; k < kRedY; ++k) sr += acc[k][x].x, si += acc[k][x].y;
+    re[t1] = (float)sr;
+    im[t1] = (float)si;
   }
-  re[t1] = (float)sr;
-  im[t1] = (float)si;
 }
 
 int launch_reduce_partials(const double2 *partial, int nwg, int tmax, float *re, float *im,
                            hipStream_t st) {
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3((tmax + 127) / 128), dim3(128), 0, st, partial,
-                     nwg, tmax, re, im);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3((tmax + kRedX - 1) / kRedX), dim3(kRedX * kRedY), 0, st,
+                     partial, nwg, tmax, re, im);
   NISE_HIP_CHECK(hipGetLastError());
   return NISE_OK;
 }
