@@ -17,6 +17,7 @@
 // and replaced by the forward checkpoint psi(rel) whenever rel is a multiple
 // of K, so the rounding drift of the inversion spans at most K steps.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -178,94 +179,202 @@ __global__ void adjoint_reduce_kernel(const double *pd, const double *pc, int nw
 }
 
 // Row variant (used with the row conveyor's schedule and tables): one wave
-// holds G = 64/R vectors, lane q*G + g = (row q, vector g), each vector at
-// its own frame (time skew, as in rows_kernel), walking t = tmax-1 .. 0 in
-// lock step.  Rotation entries are the row schedule's (padding pairs act on
-// the zeroed scratch sites N..N+R-1 with jf = 0).  Per-site offset
-// gradients are summed over the wave's vectors with shuffles and added to an
-// FP64 LDS accumulator by the g == 0 lane of each row.
+// holds G = 64/R vectors, lane q*G + g = (row q, vector g).  As in the
+// forward row conveyor, each quad of vectors (g / 4) is frame-synchronous
+// and quads are time-skewed: at wave time tau quad h is at frame
+// Fmax(h) - tau (Fmax: its last vector's last frame), vector g at relative
+// time t = frame - fs_g.  A vector starts (t = tmax-1) from its last
+// checkpoint and lambda = g mu, then steps back while t >= 0; lanes of
+// vectors outside their window are rotated too but contribute nothing.
+// Rotation entries (device order) are loaded one per lane per 4-layer block
+// with a quad broadcast, through a register ring walking the layers
+// backwards.  Padding pairs act on the scratch sites N..N+R-1 with jf = 0.
+// Per-site offset gradients are summed over the wave's vectors with
+// shuffles and added to an FP64 LDS accumulator by the g == 0 lane of each
+// row.
+constexpr int kAdjRingG = 2;              // ring slots per lane, 4 layers each
+constexpr int kAdjSiteU = 4;              // sites per batch of HBM loads
+
+// Layer j of a ring block, backwards (src/propagate.c:883-887 inverted):
+// the coupling-scale gradient term and R^dagger on phi and chi.
+template <int j>
+__device__ __forceinline__ void adj_slot(unsigned char *phib, unsigned char *chib, uint32_t gofs, uint2 e,
+                                         float2 c, float jv, float &cf) {
+  const uint32_t ea = quad_bcast<j>(e.x) + gofs, eb = quad_bcast<j>(e.y) + gofs;
+  const float co = quad_bcast<j>(c.x), si = quad_bcast<j>(c.y), jj = quad_bcast<j>(jv);
+  float2 *pa_ = reinterpret_cast<float2 *>(phib + ea), *pb_ = reinterpret_cast<float2 *>(phib + eb);
+  float2 *xa_ = reinterpret_cast<float2 *>(chib + ea), *xb_ = reinterpret_cast<float2 *>(chib + eb);
+  float2 pa = *pa_, pb = *pb_, xa = *xa_, xb = *xb_;
+  cf += jj * (im_cj(xa, pb) + im_cj(xb, pa));
+  rotate_back(pa, pb, co, si);
+  rotate_back(xa, xb, co, si);
+  *pa_ = pa, *pb_ = pb, *xa_ = xa, *xb_ = xb;
+}
+
 template <int R>
 __global__ void __launch_bounds__(64) adjoint_rows_kernel(AdjRowsArgs A) {
   constexpr int G = 64 / R;
+  static_assert(G >= 4, "a row needs a whole quad of lanes");
   extern __shared__ __align__(16) unsigned char smem[];
-  const int N = A.N, T = A.T, nl = A.nlayers, NS = N + R;
+  const int N = A.N, T = A.T, nl = A.nlayers, NS = N + R, m = A.m;
+  const int64_t K = A.K;
   double *gd = reinterpret_cast<double *>(smem);                  // [N]
   float2 *phi = reinterpret_cast<float2 *>(gd + N);                // [N+R][G]
   float2 *chi = phi + (size_t)NS * G;                              // [N+R][G]
+  unsigned char *phib = reinterpret_cast<unsigned char *>(phi), *chib = reinterpret_cast<unsigned char *>(chi);
   const int lane = threadIdx.x, q = lane / G, g = lane % G;
-  const int v = blockIdx.x * G + g;
+  const int v0 = blockIdx.x * G, v = v0 + g;
   const bool has = v < A.nvec;
-  const int vv = has ? v : blockIdx.x * G;
-  const int s = vv / 3, x = vv % 3;
-  const int64_t fs = (int64_t)(A.begin + s) * A.sample;
-  const int64_t ck0 = (int64_t)vv * A.nck;
+  const int x = has ? v % 3 : 0;
+  auto start_of = [&](int vec) { return (int64_t)(A.begin + vec / 3) * A.sample; };
+  const int64_t fs = has ? start_of(v) : LLONG_MAX / 4;
+  const int64_t ck0 = (int64_t)v * A.nck;
+  const uint32_t gofs = (uint32_t)g * 8u;
   const float hf = 0.5f * A.fac;
+  // quads: frames run from Fmax(h) down to F0(h)
+  const int vlast = min(v0 + G, A.nvec) - 1;
+  int64_t span = 0;
+  for (int h = 0; 4 * h < G && v0 + 4 * h <= vlast; ++h)
+    span = max(span, start_of(min(v0 + 4 * h + 3, vlast)) + T - start_of(v0 + 4 * h));
+  const int vq = min(v0 + (g & ~3), vlast);
+  const int64_t F0 = start_of(vq), Fmax = start_of(min(vq + 3, vlast)) + T - 1;
+
   for (int i = lane; i < N; i += 64) gd[i] = 0.0;
   for (int i = lane; i < NS * G; i += 64) phi[i] = chi[i] = make_float2(0.f, 0.f);
   double cg = 0.0;
-  auto site_pass = [&](const float2 *D, int t, bool fin) {
-    // half-phase conj(D) on phi and chi with the offset gradient; on the
-    // last substep also the response term g(t) mu(t) and the checkpoint
-    const float2 gv = A.g[t];
-    const float *mu = A.mu_rec + (A.mu_stride ? (fs + t) * A.mu_stride : 0) + 1 + (int64_t)x * N;
-    const bool reset = fin && (t % A.every == 0);
-    const int64_t ckr = ck0 + t / A.every;
-    for (int i = q; i < N; i += R) {
-      float2 p = phi[i * G + g], c = chi[i * G + g];
-      float cc = hf * im_cj(c, p);
-      const float2 d = D[i];
-      p = cmul_conj_d(p, d);
-      c = cmul_conj_d(c, d);
-      if (fin) {
-        const float m = mu[i];
-        c.x = __fadd_rn(c.x, __fmul_rn(gv.x, m));
-        c.y = __fadd_rn(c.y, __fmul_rn(gv.y, m));
-        if (reset) p = A.ck[ckr * N + i];
-      }
-      phi[i * G + g] = p;
-      chi[i * G + g] = c;
-      if (!has) cc = 0.f;
+
+  // Site pass before sweep (tau, sub), per element of the lane's vector:
+  //   B: the closing half-phase conj(D) of the previous sweep with its
+  //      offset gradient -- of frame f+1 (then also lambda += g mu and the
+  //      checkpoint reset) when sub == 0 and the vector stepped at t+1, of
+  //      frame f when sub > 0 and it steps at t;
+  //   A: the opening half-phase conj(D(f)) of this sweep with its offset
+  //      gradient, when the vector steps at t.
+  // Each phase's gradient term uses (phi, chi) just before the phase.  A
+  // vector's start (t = tmax-1) is written after the frame's sweeps, which
+  // rotate every column.
+  auto site_pass = [&](int64_t t, int sub, int64_t row, int64_t f) {
+    const bool step = t >= 0 && t < T - 1;
+    const bool bflag = sub == 0 ? (t + 1 >= 0 && t + 1 < T - 1) : step;
+    const bool bfin = sub == 0 && bflag;
+    const int64_t tb = t + 1;
+    const bool reset = bfin && (tb % A.every == 0);
+    // response and checkpoint of the closing phase of step t+1
+    const float2 gv = bfin ? gld(A.g + tb) : make_float2(0.f, 0.f);
+    const float *mu = A.mu_rec + (A.mu_stride ? (f + 1) * A.mu_stride : 0) + 1 + (int64_t)x * N;
+    const int64_t ckr = ck0 + (reset ? tb / A.every : 0);
+    const float2 *Da = A.diag + row * N;
+    const float2 *Db = sub == 0 ? A.diag + (max(min(f + 1, Fmax), F0) - A.f_lo) * N : Da;
+    for (int i0 = q; i0 < N; i0 += kAdjSiteU * R) {
+      float2 da[kAdjSiteU], db[kAdjSiteU], cv[kAdjSiteU];
+      float mv[kAdjSiteU];
 #pragma unroll
-      for (int off = 1; off < G; off <<= 1) cc += __shfl_xor(cc, off);
-      if (g == 0) gd[i] += (double)cc;
+      for (int u = 0; u < kAdjSiteU; ++u) {
+        const int i = i0 + u * R, ii = i < N ? i : 0;
+        da[u] = gld(Da + ii);
+        db[u] = gld(Db + ii);
+        mv[u] = bfin ? gld(mu + ii) : 0.f;
+        cv[u] = reset ? gld(A.ck + ckr * N + ii) : make_float2(0.f, 0.f);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < kAdjSiteU; ++u) {
+        const int i = i0 + u * R;
+        const bool ok = i < N;
+        const int is = ok ? i : q;
+        float2 p = phi[is * G + g], c = chi[is * G + g];
+        float cc = 0.f;
+        if (bflag) {
+          cc = hf * im_cj(c, p);
+          p = cmul_conj_d(p, db[u]);
+          c = cmul_conj_d(c, db[u]);
+          if (bfin) {
+            c.x = __fadd_rn(c.x, __fmul_rn(gv.x, mv[u]));
+            c.y = __fadd_rn(c.y, __fmul_rn(gv.y, mv[u]));
+            if (reset) p = cv[u];
+          }
+        }
+        if (step) {
+          cc += hf * im_cj(c, p);
+          p = cmul_conj_d(p, da[u]);
+          c = cmul_conj_d(c, da[u]);
+        }
+        if (ok) phi[is * G + g] = p, chi[is * G + g] = c;
+#pragma unroll
+        for (int off = 1; off < G; off <<= 1) cc += __shfl_xor(cc, off);
+        if (ok && g == 0) gd[i] += (double)cc;
+      }
     }
   };
-  for (int t = T - 1; t >= 0; --t) {
-    const int64_t row = fs + t - A.f_lo;
-    if (t == T - 1) {  // start: psi(T-1) and lambda(T-1) = g(T-1) mu(T-1)
-      const float2 gv = A.g[t];
-      const float *mu = A.mu_rec + (A.mu_stride ? (fs + t) * A.mu_stride : 0) + 1 + (int64_t)x * N;
-      for (int i = q; i < N; i += R) {
-        phi[i * G + g] = A.ck[(ck0 + A.nck - 1) * N + i];
-        chi[i * G + g] = make_float2(__fmul_rn(gv.x, mu[i]), __fmul_rn(gv.y, mu[i]));
+
+  const uint32_t loff = (uint32_t)(4 * q + (lane & 3));
+  const int nb = nl / 4;  // 4-layer blocks (a multiple of kAdjRingG)
+  uint2 re[kAdjRingG];
+  float2 rc[kAdjRingG];
+  float rj[kAdjRingG];
+  int64_t ring_key = -1;  // sweep (tau * m + sub) whose top blocks the ring holds
+
+  // tau runs one past the longest span: the last closing half-phase
+  for (int64_t tau = 0; tau <= span; ++tau) {
+    const int64_t f = Fmax - tau;  // this lane's frame
+    const int64_t t = f - fs;
+    const bool live = t >= -1 && t <= T - 1;  // start, step, or the closing B of step 0
+    if (!__ballot(live)) {  // no vector of the wave is in its window: jump
+      int64_t nt = span + 1;
+      for (int k = 0; k < G && v0 + k <= vlast; ++k) {
+        const int64_t fq = start_of(min(v0 + (k & ~3) + 3, vlast)) + T - 1;  // quad's Fmax
+        const int64_t o = fq - (start_of(v0 + k) + T - 1);
+        if (o > tau) nt = min(nt, o);
       }
+      tau = nt - 1;
       continue;
     }
-    const float2 *D = A.diag + row * N;
-    // entry q of layer l in device order (rows_device_order)
-    const float2 *cs = A.cs + row * A.K + 4 * q;
-    const float *jf = A.jf + row * A.K + 4 * q;
-    const uint2 *stq = A.steps + 4 * q;
-    for (int sub = 0; sub < A.m; ++sub) {
-      site_pass(D, t, false);
-      float cf = 0.f;
-      for (int l = nl - 1; l >= 0; --l) {
-        const int le = (l >> 2) * 4 * R + (l & 3);
-        const uint2 e = stq[le];
-        const float2 c = cs[le];
-        const float j = jf[le];
-        float2 *pa_ = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(phi) + e.x + g * 8);
-        float2 *pb_ = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(phi) + e.y + g * 8);
-        float2 *xa_ = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(chi) + e.x + g * 8);
-        float2 *xb_ = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(chi) + e.y + g * 8);
-        float2 pa = *pa_, pb = *pb_, xa = *xa_, xb = *xb_;
-        cf += j * (im_cj(xa, pb) + im_cj(xb, pa));
-        rotate_back(pa, pb, c.x, c.y);
-        rotate_back(xa, xb, c.x, c.y);
-        *pa_ = pa, *pb_ = pb, *xa_ = xa, *xb_ = xb;
+    const bool step = t >= 0 && t < T - 1;
+    const int64_t row = max(f, F0) - A.f_lo;  // clamped for finished quads
+    const bool sweep = __ballot(step) != 0;
+    for (int sub = 0; sub < (sweep ? m : 1); ++sub) {
+      site_pass(t, sub, row, f);
+      if (!sweep) break;
+      const float2 *csf = A.cs + row * K + loff;
+      const float *jff = A.jf + row * K + loff;
+      if (ring_key != tau * m + sub) {
+#pragma unroll
+        for (int k = 0; k < kAdjRingG; ++k) {
+          const int64_t o = (int64_t)(nb - 1 - k) * 4 * R;
+          re[k] = gld(A.steps + loff + o), rc[k] = gld(csf + o), rj[k] = gld(jff + o);
+        }
       }
-      cg += has ? (double)cf : 0.0;
-      site_pass(D, t, sub == A.m - 1);
+      float cf = 0.f;
+      // blocks nb-1 .. 0; the last group refills from the top blocks of
+      // the next sweep (this frame again, or the frame before)
+      const int64_t nrow = sub + 1 < m ? row : max(f - 1, F0) - A.f_lo;
+      for (int b0 = nb - 1; b0 >= 0; b0 -= kAdjRingG) {
+        const bool tail = b0 - kAdjRingG < 0;
+#pragma unroll
+        for (int k = 0; k < kAdjRingG; ++k) {
+          adj_slot<3>(phib, chib, gofs, re[k], rc[k], rj[k], cf);
+          adj_slot<2>(phib, chib, gofs, re[k], rc[k], rj[k], cf);
+          adj_slot<1>(phib, chib, gofs, re[k], rc[k], rj[k], cf);
+          adj_slot<0>(phib, chib, gofs, re[k], rc[k], rj[k], cf);
+          const int64_t o = (int64_t)(tail ? nb - 1 - k : b0 - kAdjRingG - k) * 4 * R;
+          const int64_t r2 = tail ? nrow : row;
+          re[k] = gld(A.steps + loff + o);
+          rc[k] = gld(A.cs + r2 * K + loff + o);
+          rj[k] = gld(A.jf + r2 * K + loff + o);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      ring_key = tau * m + sub + 1;
+      cg += step ? (double)cf : 0.0;
+    }
+    if (t == T - 1) {  // start: psi(T-1) and lambda(T-1) = g(T-1) mu(T-1)
+      const float2 gv = gld(A.g + (T - 1));
+      const float *mu = A.mu_rec + (A.mu_stride ? f * A.mu_stride : 0) + 1 + (int64_t)x * N;
+      for (int i = q; i < N; i += R) {
+        const float mm = gld(mu + i);
+        phi[i * G + g] = gld(A.ck + (ck0 + A.nck - 1) * N + i);
+        chi[i * G + g] = make_float2(__fmul_rn(gv.x, mm), __fmul_rn(gv.y, mm));
+      }
     }
   }
 #pragma unroll

@@ -150,6 +150,17 @@ __device__ __forceinline__ void gst(T *p, T v) {
   *(__attribute__((address_space(1))) W *)p = w;
 }
 
+// Value of quad lane j in every lane of the quad (DPP quad_perm; the
+// compiler folds it into the VALU instruction that uses the value).
+template <int j>
+__device__ __forceinline__ uint32_t quad_bcast(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, j * 0x55, 0xF, 0xF, false);
+}
+template <int j>
+__device__ __forceinline__ float quad_bcast(float x) {
+  return __uint_as_float(quad_bcast<j>(__float_as_uint(x)));
+}
+
 // Complex helpers with the reference's operation order.
 // src/propagate.c:871-872 / :892-893: (cr*re - ci*im, ci*re + cr*im)
 __device__ __forceinline__ float2 cmul_diag(float2 c, float2 u) {

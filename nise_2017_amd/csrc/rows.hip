@@ -52,16 +52,6 @@ __device__ __forceinline__ void rot_step(unsigned char *smem, uint32_t gofs, uin
   *pb_ = pb;
 }
 
-// Value of quad lane j in every lane of the quad.
-template <int j>
-__device__ __forceinline__ uint32_t quad_bcast(uint32_t x) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, j * 0x55, 0xF, 0xF, false);
-}
-template <int j>
-__device__ __forceinline__ float quad_bcast(float x) {
-  return __uint_as_float(quad_bcast<j>(__float_as_uint(x)));
-}
-
 // Layer j of a ring slot: the entry loaded by quad lane j.
 template <int j>
 __device__ __forceinline__ void rot_slot(unsigned char *smem, uint32_t gofs, uint2 e, float2 c) {
