@@ -260,6 +260,9 @@ def main():
     ap.add_argument("--multi-systems", type=int, default=96)
     ap.add_argument("--adjoint-every", default="16,32,64", help="checkpoint intervals K")
     ap.add_argument("--fd-params", type=int, default=3, help="offsets checked by finite differences")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first copy 1 GiB on the device (a kernel of known traffic for scaling "
+                         "FETCH_SIZE / WRITE_SIZE counter passes)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -275,6 +278,13 @@ def main():
     from nise_2017_amd import abi
 
     torch.cuda.set_device(0)
+    if args.copy_probe:  # 1 GiB read + 1 GiB written by one copy kernel
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     L, nfr, T, sr, ns = args.lattice, args.frames, args.tmax, args.samplerate, args.samples
     N = L * L
     need = (ns - 1) * sr + T
