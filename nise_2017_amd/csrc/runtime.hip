@@ -70,29 +70,68 @@ void build_schedule(int N, const std::vector<uint8_t> &mask, Schedule &s) {
     s.ab[fill[lay[k]]++] = (uint32_t)la[k] | ((uint32_t)lb[k] << 16);
 }
 
-// Layers of at most R rotations: each rotation of the lexicographic list
-// goes to the first layer after the last ones touching its sites that still
-// has room, so every site keeps its order.  Each layer is padded to exactly
-// R entries; the padding entry of row q is the dummy pair (N + q, N + q),
+// Layers of at most R rotations by list scheduling on the dependency graph
+// of the lexicographic sweep: a rotation follows the previous rotation on
+// each of its two sites, so every site keeps its order.  Layer by layer, the
+// ready rotations with the longest remaining chain (ties: lexicographic
+// order) fill the R rows; this keeps the layer count near
+// max(critical path, rotations / R).  Each layer is padded to exactly R
+// entries; the padding entry of row q is the dummy pair (N + q, N + q),
 // which the table kernel turns into an identity and the kernels apply to a
 // scratch site.  The layer count is padded to a multiple of kMinLayers.
 void build_schedule_rows(int N, const std::vector<uint8_t> &mask, int R, Schedule &s) {
   constexpr int kMinLayers = 8;
   s.N = N;
-  std::vector<int> last(N, -1);
-  std::vector<std::vector<uint32_t>> layers;
+  std::vector<uint32_t> rot;  // lexicographic order
   int64_t p = 0;
-  int K = 0;
   for (int a = 0; a < N; ++a)
-    for (int b = a + 1; b < N; ++b, ++p) {
-      if (!mask[p]) continue;
-      size_t l = (size_t)(1 + (last[a] > last[b] ? last[a] : last[b]));
-      while (l < layers.size() && (int)layers[l].size() >= R) ++l;
-      if (l >= layers.size()) layers.resize(l + 1);
-      layers[l].push_back((uint32_t)a | ((uint32_t)b << 16));
-      last[a] = last[b] = (int)l;
-      ++K;
+    for (int b = a + 1; b < N; ++b, ++p)
+      if (mask[p]) rot.push_back((uint32_t)a | ((uint32_t)b << 16));
+  const int K = (int)rot.size();
+  // successors: the next rotation on each site of rotation k
+  std::vector<int> succ(2 * (size_t)K, -1), npred(K, 0), height(K, 1), lastk(N, -1);
+  for (int k = 0; k < K; ++k) {
+    const int site[2] = {(int)(rot[k] & 0xffffu), (int)(rot[k] >> 16)};
+    for (int e = 0; e < 2; ++e) {
+      const int j = lastk[site[e]];
+      if (j >= 0) {
+        succ[2 * (size_t)j + ((int)(rot[j] & 0xffffu) == site[e] ? 0 : 1)] = k;
+        ++npred[k];
+      }
+      lastk[site[e]] = k;
     }
+  }
+  for (int k = K - 1; k >= 0; --k)
+    for (int e = 0; e < 2; ++e) {
+      const int j = succ[2 * (size_t)k + e];
+      if (j >= 0) height[k] = std::max(height[k], height[j] + 1);
+    }
+  auto later = [&](int x, int y) { return height[x] != height[y] ? height[x] < height[y] : x > y; };
+  std::vector<int> ready;
+  for (int k = 0; k < K; ++k)
+    if (!npred[k]) ready.push_back(k);
+  std::make_heap(ready.begin(), ready.end(), later);
+  std::vector<std::vector<uint32_t>> layers;
+  std::vector<int> fresh;
+  while (!ready.empty()) {
+    std::vector<uint32_t> L;
+    fresh.clear();
+    while ((int)L.size() < R && !ready.empty()) {
+      std::pop_heap(ready.begin(), ready.end(), later);
+      const int k = ready.back();
+      ready.pop_back();
+      L.push_back(rot[k]);
+      for (int e = 0; e < 2; ++e) {
+        const int j = succ[2 * (size_t)k + e];
+        if (j >= 0 && --npred[j] == 0) fresh.push_back(j);  // ready from the next layer on
+      }
+    }
+    layers.push_back(std::move(L));
+    for (int j : fresh) {
+      ready.push_back(j);
+      std::push_heap(ready.begin(), ready.end(), later);
+    }
+  }
   // at least kMinLayers layers, and a multiple of them (the kernel's
   // operand ring runs across sweeps)
   const size_t nl = std::max<size_t>(kMinLayers, (layers.size() + kMinLayers - 1) / kMinLayers * kMinLayers);

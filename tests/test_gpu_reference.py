@@ -5,10 +5,14 @@ reference's unmodified sources (propagate.c, NISE_subs.c, read_trajectory.c,
 ...; oracle/Makefile) and travels with the tree as a built library.  Its
 absorption loop reads the same GROBIN files with the reference's read_He /
 read_mue and propagates with the reference's propagate_vec_coupling_S.
-The GPU result must match it to 1e-6 relative L2 in the response (the
-propagation is the same FP32 arithmetic; only the order of the response sums
-differs) and to 1e-4 in the spectrum.  Skipped where the reference build is
-absent."""
+The propagation is the same FP32 arithmetic (bit-exact per step, pinned by
+the golden-vector tests); the response differs only in its summation: the
+reference adds every product mu_i * psi_i straight into one FP32 running sum
+per time step (calc_S1, src/absorption.c:205-212; 9,600 additions per step at
+cfg0), this path sums in FP64.  On cfg0 that FP32 running sum alone differs
+from an FP64 sum of the identical FP32 propagation by 3.4e-6 relative L2, so
+the response must match to 1e-5 and the spectrum to the 1e-4 spectral
+tolerance.  Skipped where the reference build is absent."""
 import os
 
 import numpy as np
@@ -87,4 +91,4 @@ def test_absorption_vs_reference_build(ref, tmp_path, case):
     s_ref = A_re @ re_r.astype(np.float64) + A_im @ im_r.astype(np.float64)
     e_sp = rel(s_gpu, s_ref)
     print(f"[reference] {case}: response rel L2 {e_td:.1e}, spectrum rel L2 {e_sp:.1e}", flush=True)
-    assert e_td < 1e-6 and e_sp < 1e-4
+    assert e_td < 1e-5 and e_sp < 1e-4
