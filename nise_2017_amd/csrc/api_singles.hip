@@ -18,6 +18,8 @@ using namespace nise;
 namespace nise {
 int dia_propagate_host(const t_non *non, const float *H, float *cr, float *ci, int sign, int nvec,
                        hipStream_t st);
+int dia_s_propagate_host(const t_non *non, const float *H, float *cr, float *ci, int sign, int nvec,
+                         hipStream_t st, int *elements);
 }
 
 // ---------------------------------------------------------------------------
@@ -697,10 +699,20 @@ extern "C" void propagate_vec_coupling_S(t_non *non, float *Hamiltonian_i, float
     fatal("propagate_vec_coupling_S");
 }
 
+// The sparse scheme's report (src/propagate.c:29-35), printed by the first
+// sample's propagation when display is 0.
+static void sparse_report(const t_non *non, int elements, int samples, int display) {
+  if (samples != non->begin || display != 0) return;
+  const int N = non->singles;
+  const float scale = non->deltat * kIcm2ifs * kTwoPi / non->ts;
+  printf("Sparce matrix efficiency: %f pct.\n", (1 - (1.0 * elements / (N * N))) * 100);
+  printf("Pressent tuncation %f.\n", non->thres / (scale * scale));
+  printf("Suggested truncation %f.\n", 0.001);
+}
+
 // src/propagate.c:20-39
 extern "C" void propagate_vector(t_non *non, float *Hamil_i_e, float *vecr, float *veci, int sign,
                                  int samples, int display) {
-  (void)samples, (void)display;
   if (non->propagation == 1) {
     if (coupling_S_host(non->singles, Hamil_i_e, vecr, veci, 1, non->ts, sign, non->deltat,
                         non->couplingcut))
@@ -712,6 +724,13 @@ extern "C" void propagate_vector(t_non *non, float *Hamil_i_e, float *vecr, floa
       fatal("propagate_vector");
     return;
   }
+  if (non->propagation == 0) {  // threshold in (0, 1]: the sparse scheme
+    int elements = 0;
+    if (dia_s_propagate_host(non, Hamil_i_e, vecr, veci, sign, 1, default_stream(), &elements))
+      fatal("propagate_vector");
+    sparse_report(non, elements, samples, display);
+    return;
+  }
   printf("propagate_vector: propagation scheme %d (threshold %g) is not provided by the GPU "
          "library\n", non->propagation, non->thres);
   exit(1);
@@ -720,7 +739,6 @@ extern "C" void propagate_vector(t_non *non, float *Hamil_i_e, float *vecr, floa
 // src/propagate.c:43-52: the N columns of an N x N block, one launch
 extern "C" void propagate_matrix(t_non *non, float *Hamil_i_e, float *vecr, float *veci, int sign,
                                  int samples, int display) {
-  (void)samples, (void)display;
   int N = non->singles;
   if (non->propagation == 1) {
     if (coupling_S_host(N, Hamil_i_e, vecr, veci, N, non->ts, sign, non->deltat, non->couplingcut))
@@ -730,6 +748,13 @@ extern "C" void propagate_matrix(t_non *non, float *Hamil_i_e, float *vecr, floa
   if (non->propagation == 0 && (non->thres == 0 || non->thres > 1)) {
     if (dia_propagate_host(non, Hamil_i_e, vecr, veci, sign, N, default_stream()))
       fatal("propagate_matrix");
+    return;
+  }
+  if (non->propagation == 0) {  // the sparse scheme, column j reporting with display * j
+    int elements = 0;
+    if (dia_s_propagate_host(non, Hamil_i_e, vecr, veci, sign, N, default_stream(), &elements))
+      fatal("propagate_matrix");
+    for (int j = 0; j < N; ++j) sparse_report(non, elements, samples, display * j);
     return;
   }
   printf("propagate_matrix: propagation scheme %d is not provided by the GPU library\n",

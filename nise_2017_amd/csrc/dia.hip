@@ -183,9 +183,32 @@ __global__ void dia_apply_kernel(const double *w, const double *V, int N, float 
   }
 }
 
+// psi <- U psi with a (thresholded) propagator in FP32, the kept elements
+// accumulated in the reference's order (src/propagate.c:238-247).  One
+// workgroup per vector.
+__global__ void dense_apply_kernel(const float2 *U, int N, float2 *psi, int nvec) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float2 *c = reinterpret_cast<float2 *>(smem);
+  const int v = blockIdx.x, t = threadIdx.x;
+  if (v >= nvec) return;
+  float2 *p = psi + (int64_t)v * N;
+  for (int b = t; b < N; b += blockDim.x) c[b] = p[b];
+  __syncthreads();
+  for (int a = t; a < N; a += blockDim.x) {
+    float yr = 0.f, yi = 0.f;
+    for (int b = 0; b < N; ++b) {
+      const float2 u = U[a * N + b];
+      if (u.x == 0.f && u.y == 0.f) continue;  // dropped element
+      yr = __fadd_rn(yr, __fsub_rn(__fmul_rn(u.x, c[b].x), __fmul_rn(u.y, c[b].y)));
+      yi = __fadd_rn(yi, __fadd_rn(__fmul_rn(u.x, c[b].y), __fmul_rn(u.y, c[b].x)));
+    }
+    p[a] = make_float2(yr, yi);
+  }
+}
+
 namespace {
 struct DiaCtx {
-  DevBuf H, A, w, V, st, psi;
+  DevBuf H, A, w, V, st, psi, U;
   std::vector<float2> sbuf;
 };
 DiaCtx &dctx() {
@@ -232,6 +255,50 @@ int dia_propagate_host(const t_non *non, const float *H, float *cr, float *ci, i
                                 hipMemcpyDeviceToHost, st));
   NISE_HIP_CHECK(hipStreamSynchronize(st));
   if (status < 0) return set_error("Jacobi eigensolver did not converge"), NISE_ERR_NUMERIC;
+  for (size_t i = 0; i < c.sbuf.size(); ++i) cr[i] = c.sbuf[i].x, ci[i] = c.sbuf[i].y;
+  return NISE_OK;
+}
+
+// The sparse scheme (src/propagate.c:185-258): U as for DIA, elements with
+// |U_ab|^2 <= thres dropped, psi <- U psi; *elements counts the kept ones.
+int dia_s_propagate_host(const t_non *non, const float *H, float *cr, float *ci, int sign, int nvec,
+                         hipStream_t st, int *elements) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  const int N = non->singles;
+  if (N < 1 || N > kJacMaxN) return set_error("DIA on the GPU supports 1..%d sites, got %d", kJacMaxN, N), NISE_ERR_ARG;
+  DiaCtx &c = dctx();
+  const int nn2 = N * (N + 1) / 2;
+  if ((rc = up(c.H, H, nn2 * 4, st))) return rc;
+  if ((rc = c.A.alloc((size_t)N * N * 8)) || (rc = c.V.alloc((size_t)N * N * 8)) ||
+      (rc = c.w.alloc(N * 8)) || (rc = c.st.alloc(4)) || (rc = c.U.alloc((size_t)N * N * 8)))
+    return rc;
+  hipLaunchKernelGGL(dense_from_triangle, dim3((N * N + 255) / 256), dim3(256), 0, st, c.H.as<float>(), N,
+                     c.A.as<double>());
+  if ((rc = launch_jacobi(c.A.as<double>(), N, 1, c.w.as<double>(), c.V.as<double>(), c.st.as<int>(), st)))
+    return rc;
+  volatile float fac = non->deltat * kIcm2ifs;  // src/propagate.c:197, no /m
+  fac = fac * kTwoPi;
+  fac = fac * (float)sign;
+  if ((rc = launch_dia_u(c.w.as<double>(), c.V.as<double>(), N, 1, (float)fac, non->thres, c.U.as<float2>(), st)))
+    return rc;
+  c.sbuf.resize((size_t)nvec * N);
+  for (size_t i = 0; i < c.sbuf.size(); ++i) c.sbuf[i] = make_float2(cr[i], ci[i]);
+  if ((rc = up(c.psi, c.sbuf.data(), c.sbuf.size() * sizeof(float2), st))) return rc;
+  hipLaunchKernelGGL(dense_apply_kernel, dim3(nvec), dim3(64), N * sizeof(float2), st, c.U.as<float2>(), N,
+                     c.psi.as<float2>(), nvec);
+  NISE_HIP_CHECK(hipGetLastError());
+  std::vector<float2> hU((size_t)N * N);
+  int status = 0;
+  NISE_HIP_CHECK(hipMemcpyAsync(&status, c.st.p, 4, hipMemcpyDeviceToHost, st));
+  NISE_HIP_CHECK(hipMemcpyAsync(hU.data(), c.U.p, hU.size() * sizeof(float2), hipMemcpyDeviceToHost, st));
+  NISE_HIP_CHECK(hipMemcpyAsync(c.sbuf.data(), c.psi.p, c.sbuf.size() * sizeof(float2),
+                                hipMemcpyDeviceToHost, st));
+  NISE_HIP_CHECK(hipStreamSynchronize(st));
+  if (status < 0) return set_error("Jacobi eigensolver did not converge"), NISE_ERR_NUMERIC;
+  int kept = 0;
+  for (const float2 &u : hU) kept += (u.x != 0.f || u.y != 0.f);
+  if (elements) *elements = kept;
   for (size_t i = 0; i < c.sbuf.size(); ++i) cr[i] = c.sbuf[i].x, ci[i] = c.sbuf[i].y;
   return NISE_OK;
 }

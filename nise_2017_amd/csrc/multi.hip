@@ -43,7 +43,9 @@ __global__ void dia_dense_kernel(FrameSource s, int N, int64_t f_lo, double *A) 
 
 // U[a][b] = sum_c V[a][c] V[b][c] (cos, -sin)(e_c f) (src/propagate.c:88-106),
 // e_c rounded to float and multiplied by f in float as the reference does.
-__global__ void dia_u_kernel(const double *w, const double *V, int N, float fac, float2 *U) {
+// With thres > 0 (the sparse scheme, src/propagate.c:185-258) an element is
+// kept only where its FP32 |U_ab|^2 exceeds thres (:242), else it is 0.
+__global__ void dia_u_kernel(const double *w, const double *V, int N, float fac, float thres, float2 *U) {
   const int64_t m = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N * N) return;
@@ -56,7 +58,20 @@ __global__ void dia_u_kernel(const double *w, const double *V, int N, float fac,
     ur += p * cos((double)ef);
     ui -= p * sin((double)ef);
   }
-  U[m * N * N + i] = make_float2((float)ur, (float)ui);
+  float2 u = make_float2((float)ur, (float)ui);
+  if (thres > 0.f && !(__fadd_rn(__fmul_rn(u.x, u.x), __fmul_rn(u.y, u.y)) > thres)) u = make_float2(0.f, 0.f);
+  U[m * N * N + i] = u;
+}
+
+int launch_dia_u(const double *w, const double *V, int N, int64_t nmat, float fac, float thres, float2 *U,
+                 hipStream_t st) {
+  for (int64_t d0 = 0; d0 < nmat; d0 += 65535) {
+    const int64_t nd = std::min<int64_t>(65535, nmat - d0);
+    hipLaunchKernelGGL(dia_u_kernel, dim3((N * N + 255) / 256, (unsigned)nd), dim3(256), 0, st, w + d0 * N,
+                       V + d0 * N * N, N, fac, thres, U + d0 * N * N);
+    NISE_HIP_CHECK(hipGetLastError());
+  }
+  return NISE_OK;
 }
 
 // One workgroup (256 threads): vectors v0 .. v0+vp-1 of one system.  Thread
@@ -150,6 +165,7 @@ int up(DevBuf &d, const void *s, size_t n) {
 
 struct SysState {
   int N = 0, T = 0, dia = 0, nvec = 0, nwg = 0;  // nwg: rows of partial responses
+  float thres = 0.f;  // sparse DIA threshold (0: full DIA)
   int64_t f_lo = 0, f_hi = 0;
   FrameSource src;
   DevBuf H, mu, C, shs, sha, ab, diag, cs, steps, U, partial;
@@ -184,10 +200,9 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
     s.N = N, s.T = T;
     if (non->propagation == 0) {
       if (N > 64) return set_error("multi: DIA systems need N <= 64 (system %d has %d)", k, N), NISE_ERR_ARG;
-      if (non->thres > 0 && non->thres <= 1)  // src/propagate.c:25-28: the sparse scheme
-        return set_error("multi: system %d: the sparse DIA scheme (Threshold in (0,1]) is not provided", k),
-               NISE_ERR_ARG;
       s.dia = 1;
+      // src/propagate.c:25-28: a threshold in (0, 1] selects the sparse scheme
+      s.thres = (non->thres > 0 && non->thres <= 1) ? non->thres : 0.f;
     } else if (non->propagation != 1) {
       return set_error("multi: system %d: propagation %d not provided", k, non->propagation), NISE_ERR_ARG;
     }
@@ -306,13 +321,9 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
         }
         if ((rc = launch_jacobi(A.as<double>(), N, (int)nc, w.as<double>(), V.as<double>(), stt.as<int>(), st)))
           return rc;
-        for (int64_t d0 = 0; d0 < nc; d0 += 65535) {
-          const int64_t nd = std::min<int64_t>(65535, nc - d0);
-          hipLaunchKernelGGL(dia_u_kernel, dim3((N * N + 255) / 256, (unsigned)nd), dim3(256), 0, st,
-                             w.as<double>() + d0 * N, V.as<double>() + d0 * N * N, N, fac,
-                             s.U.as<float2>() + (c0 + d0) * N * N);
-          NISE_HIP_CHECK(hipGetLastError());
-        }
+        if ((rc = launch_dia_u(w.as<double>(), V.as<double>(), N, nc, fac, s.thres, s.U.as<float2>() + c0 * N * N,
+                               st)))
+          return rc;
         std::vector<int> sts(nc);
         NISE_HIP_CHECK(hipMemcpyAsync(sts.data(), stt.p, nc * 4, hipMemcpyDeviceToHost, st));
         NISE_HIP_CHECK(hipStreamSynchronize(st));

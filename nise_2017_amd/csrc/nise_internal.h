@@ -304,6 +304,10 @@ struct DiaSys {
   int64_t mu_stride = 0;
   double2 *partial = nullptr;    // [workgroups][tmax]
 };
+// U = V exp(-i E f) V^T for nmat eigensystems (multi.hip); thres > 0 drops
+// the elements with FP32 |U_ab|^2 <= thres (the sparse scheme).
+int launch_dia_u(const double *w, const double *V, int N, int64_t nmat, float fac, float thres, float2 *U,
+                 hipStream_t st);
 size_t rows_smem(int N, int R);
 int rows_for(int N, int Kreal, int nlayers_asap);
 void rows_device_order(const Schedule &s, int R, std::vector<uint32_t> &ab);

@@ -393,6 +393,50 @@ void or_DIA_f64(int N, const float *H, double *cr, double *ci, int sign,
   free(A), free(V), free(w), free(yr), free(yi);
 }
 
+/* Sparse scheme, src/propagate.c:185-258: U = V exp(-iEf) V^T in the site
+   basis (:217-234), then psi_a <- sum over the kept elements U_ab psi_b,
+   an element kept when |U_ab|^2 > thres (:242); the reference tests the FP32
+   elements, so the test here is on the FP32-rounded element. */
+int or_DIA_S_f64(int N, const float *H, double *cr, double *ci, int sign,
+                 float deltat, float thres) {
+  float ff = deltat * OR_icm2ifs * OR_twoPi * sign; /* :197, no /m */
+  double f = ff;
+  double *A = malloc(N * N * sizeof(double));
+  double *V = malloc(N * N * sizeof(double));
+  double *w = malloc(N * sizeof(double));
+  double *ur = malloc(N * N * sizeof(double)), *ui = malloc(N * N * sizeof(double));
+  double *yr = malloc(N * sizeof(double)), *yi = malloc(N * sizeof(double));
+  for (int a = 0; a < N; a++)
+    for (int b = 0; b < N; b++) A[a * N + b] = H[or_sindex(a, b, N)];
+  or_jacobi_f64(N, A, w, V);
+  for (int a = 0; a < N; a++)
+    for (int b = 0; b < N; b++) {
+      double sr = 0, si = 0;
+      for (int c = 0; c < N; c++) {
+        double p = V[a * N + c] * V[b * N + c];
+        sr += p * cos(w[c] * f);
+        si -= p * sin(w[c] * f);
+      }
+      ur[a * N + b] = sr, ui[a * N + b] = si;
+    }
+  int kept = 0;
+  for (int a = 0; a < N; a++) {
+    double sr = 0, si = 0;
+    for (int b = 0; b < N; b++) {
+      float r = (float)ur[a * N + b], i = (float)ui[a * N + b];
+      if (r * r + i * i > thres) {
+        kept++;
+        sr += ur[a * N + b] * cr[b] - ui[a * N + b] * ci[b];
+        si += ur[a * N + b] * ci[b] + ui[a * N + b] * cr[b];
+      }
+    }
+    yr[a] = sr, yi[a] = si;
+  }
+  for (int a = 0; a < N; a++) cr[a] = yr[a], ci[a] = yi[a];
+  free(A), free(V), free(w), free(ur), free(ui), free(yr), free(yi);
+  return kept;
+}
+
 /* ---------------------------------------------------------------------- */
 /* mu_ef application, src/NISE_subs.c:485-509 and :552-575                 */
 /* ---------------------------------------------------------------------- */
@@ -520,6 +564,8 @@ int or_absorption(const t_non *non, const float *H_rec, long long H_stride,
   int N = non->singles, tmax = non->tmax;
   int nn2 = N * (N + 1) / 2;
   if (non->propagation != 1 && non->propagation != 0) return NISE_ERR_ARG;
+  /* src/propagate.c:25-28: a threshold in (0, 1] selects the sparse scheme */
+  const int sparse = non->propagation == 0 && non->thres > 0 && non->thres <= 1;
   if (re_S) memset(re_S, 0, tmax * sizeof(float)), memset(im_S, 0, tmax * sizeof(float));
   if (re_S64) memset(re_S64, 0, tmax * sizeof(double)), memset(im_S64, 0, tmax * sizeof(double));
   int nsamp = non->end - non->begin;
@@ -564,7 +610,8 @@ int or_absorption(const t_non *non, const float *H_rec, long long H_stride,
               or_coupling_S_f32(N, He, vecr, veci, non->ts, 1, non->deltat, non->couplingcut);
             } else {
               for (int i = 0; i < N; i++) dr[i] = vecr[i], di[i] = veci[i];
-              or_DIA_f64(N, He, dr, di, 1, non->deltat);
+              if (sparse) or_DIA_S_f64(N, He, dr, di, 1, non->deltat, non->thres);
+              else or_DIA_f64(N, He, dr, di, 1, non->deltat);
               for (int i = 0; i < N; i++) vecr[i] = (float)dr[i], veci[i] = (float)di[i];
             }
           }
@@ -575,6 +622,8 @@ int or_absorption(const t_non *non, const float *H_rec, long long H_stride,
             }
             if (non->propagation == 1)
               or_coupling_S_f64(N, He, dr, di, non->ts, 1, non->deltat, non->couplingcut);
+            else if (sparse)
+              or_DIA_S_f64(N, He, dr, di, 1, non->deltat, non->thres);
             else
               or_DIA_f64(N, He, dr, di, 1, non->deltat);
           }

@@ -21,6 +21,10 @@
  *     norm conservation), substep identity, FP64 expm limit.
  *   or_DIA_f64 / or_jacobi_f64 : eigen-decomposition residuals and FP64
  *     matrix exponential (numpy eigh / scipy expm) in tests.
+ *   or_DIA_S_f64 : equals or_DIA_f64 when no element is dropped (tiny
+ *     threshold); a dimer's analytic propagator (off-diagonal magnitude
+ *     |sin(J f)|) with the off-diagonal pair dropped or kept on either side
+ *     of the threshold; uncoupled sites (diagonal U) unchanged (tests).
  *   or_absorption : per-step composition of the pinned primitives; compared
  *     with the reference-primitive driver in oracle/_ref.
  *   or_1dfft : numpy.fft with the reference's re/im swap and scaling.
@@ -64,6 +68,10 @@ void or_coupling_S_doubles_f64(int N, const float *H, double *cr, double *ci,
 int or_jacobi_f64(int N, double *A, double *w, double *V);
 void or_DIA_f64(int N, const float *H, double *cr, double *ci, int sign,
                 float deltat);
+/* the sparse scheme: U as in or_DIA_f64, elements with |U_ab|^2 <= thres
+   (tested on the FP32-rounded element) dropped; returns the kept count */
+int or_DIA_S_f64(int N, const float *H, double *cr, double *ci, int sign,
+                 float deltat, float thres);
 
 void or_dipole_double(int N, const float *dipole, const float *cr,
                       const float *ci, float *fr, float *fi, float *over,

@@ -52,6 +52,8 @@ def oracle():
                                             C.c_float, C.c_float]
     L.or_jacobi_f64.argtypes = [C.c_int, _f64, _f64, _f64]
     L.or_DIA_f64.argtypes = [C.c_int, _f32, _f64, _f64, C.c_int, C.c_float]
+    L.or_DIA_S_f64.argtypes = [C.c_int, _f32, _f64, _f64, C.c_int, C.c_float, C.c_float]
+    L.or_DIA_S_f64.restype = C.c_int
     L.or_dipole_double.argtypes = [C.c_int, _f32, _f32, _f32, _f32, _f32, _f32, C.c_float]
     L.or_dipole_double_last.argtypes = L.or_dipole_double.argtypes
     L.or_polarweight.argtypes = [C.c_int, C.c_int]
