@@ -216,6 +216,19 @@ int nise_abs_plan_create(nise_abs_plan **plan, const t_non *non,
                          const float *mu_rec, int64_t mu_stride,
                          int64_t nframes, const float *coupling_rec, int where,
                          int kernel);
+/* As nise_abs_plan_create for HamiltonianType TransitionDipole or
+ * ExtendedDipole: H_rec holds the N site energies per frame, the couplings
+ * are built per frame on the device from positions (pos_rec: read_mue
+ * records, int + 3N per frame, two per frame for ExtendedDipole) and the
+ * transition dipoles mu_rec, with the minimum image in the box of box_rec
+ * (int + 3 floats per frame) when it is given.  Replaces the couplings that
+ * read_He computes on the fly, src/read_trajectory.c:181-303. */
+int nise_abs_plan_create_onfly(nise_abs_plan **plan, const t_non *non,
+                               const float *H_rec, int64_t H_stride,
+                               const float *mu_rec, int64_t mu_stride,
+                               int64_t nframes, const float *pos_rec,
+                               int64_t pos_stride, const float *box_rec,
+                               int where, int kernel);
 /* Conveyor shape: vp vector slots per workgroup, samples_per_wg samples
  * (3 polarisations each, 3*samples_per_wg <= vp), block 256 or 512
  * (multi-wave), 64 (one wave) or -2 / -4 (one wave, 2 or 4 slots per lane). */

@@ -114,6 +114,8 @@ def lib():
                                           C.c_int, C.c_int, _fp, _fp]
     L.nise_abs_plan_create.argtypes = [C.POINTER(C.c_void_p), P, _vp, C.c_int64, _vp, C.c_int64,
                                        C.c_int64, _vp, C.c_int, C.c_int]
+    L.nise_abs_plan_create_onfly.argtypes = [C.POINTER(C.c_void_p), P, _vp, C.c_int64, _vp, C.c_int64,
+                                             C.c_int64, _vp, C.c_int64, _vp, C.c_int, C.c_int]
     L.nise_abs_plan_execute.argtypes = [_vp, _vp, _vp, C.c_int]
     L.nise_abs_plan_set_shape.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.nise_abs_plan_set_rows.argtypes = [_vp, C.c_int]
@@ -166,12 +168,19 @@ class AbsPlan:
     """Batched absorption plan over a trajectory (host or device records)."""
 
     def __init__(self, non, H_rec, H_stride, mu_rec, mu_stride, nframes, coupling_rec=None,
-                 where=NISE_HOST, kernel=0):
+                 where=NISE_HOST, kernel=0, positions=None, box=None):
+        """positions / box: geometry records for HamiltonianType TransitionDipole
+        or ExtendedDipole (nise_abs_plan_create_onfly)."""
         L = lib()
-        self._keep = (non, H_rec, mu_rec, coupling_rec)
+        self._keep = (non, H_rec, mu_rec, coupling_rec, positions, box)
         h = C.c_void_p()
-        check(L.nise_abs_plan_create(C.byref(h), C.byref(non), ptr(H_rec), H_stride, ptr(mu_rec),
-                                     mu_stride, nframes, ptr(coupling_rec), where, kernel))
+        if positions is not None:
+            check(L.nise_abs_plan_create_onfly(C.byref(h), C.byref(non), ptr(H_rec), H_stride, ptr(mu_rec),
+                                               mu_stride, nframes, ptr(positions), positions.shape[1],
+                                               ptr(box), where, kernel))
+        else:
+            check(L.nise_abs_plan_create(C.byref(h), C.byref(non), ptr(H_rec), H_stride, ptr(mu_rec),
+                                         mu_stride, nframes, ptr(coupling_rec), where, kernel))
         self.h = h
         self.tmax = non.tmax1
         self.N = non.singles

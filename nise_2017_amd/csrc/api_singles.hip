@@ -33,7 +33,7 @@ struct nise_abs_plan {
   FrameSource src;
   const float *mu_dev = nullptr;
   int64_t mu_stride = 0;
-  DevBuf H_own, mu_own, C_own, shift_site, shift_ab;
+  DevBuf H_own, mu_own, C_own, shift_site, shift_ab, pos_own, box_own;
   Schedule sch;                // active schedule (one of the two below)
   Schedule sch_asap, sch_rows; // layered (slot conveyor) / row schedule
   int rows = 0;                // rows of the row conveyor, 0: slot conveyor
@@ -207,14 +207,27 @@ int run_plain(nise_abs_plan *P) {
 
 }  // namespace
 
-extern "C" int nise_abs_plan_create(nise_abs_plan **out, const t_non *non, const float *H_rec,
-                                    int64_t H_stride, const float *mu_rec, int64_t mu_stride,
-                                    int64_t nframes, const float *coupling_rec, int where,
-                                    int kernel) {
+// On-the-fly coupling scheme of a Hamiltonian type: 1 TransitionDipole,
+// 2 ExtendedDipole (src/read_trajectory.c:180,224), 0 none.
+int onfly_scheme(const t_non *non) {
+  if (std::strcmp(non->hamiltonian, "TransitionDipole") == 0) return 1;
+  if (std::strcmp(non->hamiltonian, "ExtendedDipole") == 0) return 2;
+  return 0;
+}
+
+// Plan set-up shared by nise_abs_plan_create and nise_abs_plan_create_onfly
+// (pos_rec: position records for on-the-fly couplings, else null).
+static int plan_create(nise_abs_plan **out, const t_non *non, const float *H_rec, int64_t H_stride,
+                       const float *mu_rec, int64_t mu_stride, int64_t nframes, const float *coupling_rec,
+                       const float *pos_rec, int64_t pos_stride, const float *box_rec, int where, int kernel) {
   *out = nullptr;
   int rc = ensure_device();
   if (rc) return rc;
   if (!non || !H_rec || !mu_rec) return set_error("null argument"), NISE_ERR_ARG;
+  const int onfly = onfly_scheme(non);
+  if (onfly && !pos_rec)
+    return set_error("%s couplings need position records (nise_abs_plan_create_onfly)", non->hamiltonian),
+           NISE_ERR_ARG;
   auto P = new nise_abs_plan();
   std::unique_ptr<nise_abs_plan> guard(P);
   P->non = *non;
@@ -236,7 +249,8 @@ extern "C" int nise_abs_plan_create(nise_abs_plan **out, const t_non *non, const
                      (long long)P->f_hi), NISE_ERR_RANGE;
   const bool ctype = is_coupling_type(non);
   if (ctype && !coupling_rec) return set_error("Coupling type needs coupling_rec"), NISE_ERR_ARG;
-  if (H_stride < 1 + (ctype ? N : (int64_t)N * (N + 1) / 2) || mu_stride < 1 + 3 * (int64_t)N)
+  if (H_stride < 1 + ((ctype || onfly) ? N : (int64_t)N * (N + 1) / 2) || mu_stride < 1 + 3 * (int64_t)N ||
+      (onfly && pos_stride < 1 + 3 * (int64_t)N))
     return set_error("record stride too small"), NISE_ERR_ARG;
   NISE_HIP_CHECK(hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking));
   for (auto &e : P->ev) NISE_HIP_CHECK(hipEventCreate(&e));
@@ -260,6 +274,22 @@ extern "C" int nise_abs_plan_create(nise_abs_plan **out, const t_non *non, const
         return rc;
       mud = P->mu_own.as<float>() - P->f_lo * mu_stride;
     }
+  }
+  if (onfly) {  // positions (two records per frame for ExtendedDipole) and boxes
+    const int64_t per = onfly == 2 ? 2 : 1;
+    const float *pd = pos_rec, *bd = box_rec;
+    if (where == NISE_HOST) {
+      if ((rc = upload(P->pos_own, pos_rec + per * P->f_lo * pos_stride,
+                       (size_t)(per * (P->f_hi - P->f_lo) * pos_stride) * 4, P->st)))
+        return rc;
+      pd = P->pos_own.as<float>() - per * P->f_lo * pos_stride;
+      if (box_rec) {
+        if ((rc = upload(P->box_own, box_rec + P->f_lo * 4, (size_t)(P->f_hi - P->f_lo) * 16, P->st))) return rc;
+        bd = P->box_own.as<float>() - P->f_lo * 4;
+      }
+    }
+    P->src.onfly = onfly, P->src.pos_rec = pd, P->src.pos_stride = pos_stride, P->src.box_rec = bd;
+    P->src.dip_rec = mud, P->src.dip_stride = mu_stride;
   }
   P->mu_dev = mud, P->mu_stride = mu_eff_stride;
   P->src.H_rec = Hd, P->src.H_stride = H_stride, P->src.coupling_type = ctype, P->src.C_rec = Cd;
@@ -306,6 +336,25 @@ extern "C" int nise_abs_plan_create(nise_abs_plan **out, const t_non *non, const
   NISE_HIP_CHECK(hipStreamSynchronize(P->st));
   *out = guard.release();
   return NISE_OK;
+}
+
+extern "C" int nise_abs_plan_create(nise_abs_plan **out, const t_non *non, const float *H_rec,
+                                    int64_t H_stride, const float *mu_rec, int64_t mu_stride,
+                                    int64_t nframes, const float *coupling_rec, int where,
+                                    int kernel) {
+  return plan_create(out, non, H_rec, H_stride, mu_rec, mu_stride, nframes, coupling_rec, nullptr, 0, nullptr,
+                     where, kernel);
+}
+
+extern "C" int nise_abs_plan_create_onfly(nise_abs_plan **out, const t_non *non, const float *H_rec,
+                                          int64_t H_stride, const float *mu_rec, int64_t mu_stride,
+                                          int64_t nframes, const float *pos_rec, int64_t pos_stride,
+                                          const float *box_rec, int where, int kernel) {
+  if (!non || !onfly_scheme(non))
+    return set_error("on-the-fly couplings need HamiltonianType TransitionDipole or ExtendedDipole"), NISE_ERR_ARG;
+  if (!pos_rec) return set_error("null position records"), NISE_ERR_ARG;
+  return plan_create(out, non, H_rec, H_stride, mu_rec, mu_stride, nframes, nullptr, pos_rec, pos_stride, box_rec,
+                     where, kernel);
 }
 
 extern "C" int nise_abs_plan_enable_adjoint(nise_abs_plan *P, int every);

@@ -211,8 +211,13 @@ extern "C" void absorption(t_non *non) {
   if (non->Npsites != 0) die(1, "absorption: the GPU path does not implement Projection\n");
   if (non->cluster != -1) die(1, "absorption: the GPU path does not implement Cluster\n");
   const bool ctype = is_coupling(non);
-  if (!ctype && strcmp(non->hamiltonian, "Full") != 0)
+  // on-the-fly couplings from positions (src/read_trajectory.c:181-303)
+  const int onfly = strcmp(non->hamiltonian, "TransitionDipole") == 0 ? 1
+                  : strcmp(non->hamiltonian, "ExtendedDipole") == 0 ? 2 : 0;
+  if (!ctype && !onfly && strcmp(non->hamiltonian, "Full") != 0)
     die(1, "absorption: HamiltonianType %s is not provided by the GPU path\n", non->hamiltonian);
+  if (onfly && non->propagation != 1)
+    die(1, "absorption: %s couplings run with propagation Coupling on the GPU path\n", non->hamiltonian);
   read_shift_file(non);
   // src/NISE_subs.c:197-221
   const int N_samples = (non->length - non->tmax1 - 1) / non->sample + 1;
@@ -223,7 +228,7 @@ extern "C" void absorption(t_non *non) {
     die(0, "Endpoint larger than number of samples was specified.\nEndpoint was %d but cannot be larger than %d.\n",
         non->end, N_samples);
   const int64_t D = non->doubles;
-  const int64_t hw = ctype ? 1 + N : 1 + (int64_t)N * (N + 1) / 2 + D * (D + 1) / 2;
+  const int64_t hw = (ctype || onfly) ? 1 + N : 1 + (int64_t)N * (N + 1) / 2 + D * (D + 1) / 2;
   const int64_t mw = 1 + 3 * (int64_t)N + 3 * (int64_t)N * D;
   const int64_t f_lo = (int64_t)non->begin * non->sample;
   const int64_t f_hi = (int64_t)(non->end - 1) * non->sample + non->tmax1;
@@ -250,13 +255,39 @@ extern "C" void absorption(t_non *non) {
     if (!read_records(non->couplingFName, 1 + (int64_t)N * (N + 1) / 2 + D * (D + 1) / 2, 0, 1, Cc.p))
       die(1, "Coupling trajectory file to short, could not fill buffer!!!\n");
   }
+  Pinned pos, box;
+  const bool pbc = strlen(non->pbcFName) > 0;
+  if (onfly) {  // positions: read_mue records, two per frame for ExtendedDipole; boxes: int + 3 floats
+    const int64_t per = onfly == 2 ? 2 : 1;
+    probe = fopen(non->positionFName, "rb");
+    if (!probe) die(1, "Atom position file for TDC on the fly not found!\n");
+    fclose(probe);
+    if (!pos.alloc((size_t)(per * nfr * mw))) die(1, "absorption: could not allocate pinned host memory\n");
+    if (!read_records(non->positionFName, mw, per * f_lo, per * nfr, pos.p))
+      die(1, "Dipole trajectory file to short, could not fill buffer!!!\n");
+    if (pbc) {
+      probe = fopen(non->pbcFName, "rb");
+      if (!probe) die(1, "Periodic Boundary Condition file not found!\n");
+      fclose(probe);
+      if (!box.alloc((size_t)(nfr * 4)) || !read_records(non->pbcFName, 4, f_lo, nfr, box.p))
+        die(1, "Periodic Boundary Condition file to short, could not fill buffer!!!\n");
+    }
+  }
   std::vector<float> re(non->tmax1), im(non->tmax1);
-  // the records start at frame f_lo: shift the base so frame indices stay absolute
+  // the records start at frame f_lo: the run counts frames from there
   t_non loc = *non;
   loc.begin = 0, loc.end = non->end - non->begin;
-  const int rc = nise_absorption_batched(&loc, H.p - 0, hw, mu.p, mw, nfr, ctype ? Cc.p : nullptr,
-                                         NISE_HOST, 0, re.data(), im.data());
-  (void)f_lo;
+  int rc;
+  if (onfly) {
+    nise_abs_plan *plan = nullptr;
+    rc = nise_abs_plan_create_onfly(&plan, &loc, H.p, hw, mu.p, mw, nfr, pos.p, mw, pbc ? box.p : nullptr,
+                                    NISE_HOST, 0);
+    if (!rc) rc = nise_abs_plan_execute(plan, re.data(), im.data(), 0);
+    if (plan) nise_abs_plan_destroy(plan);
+  } else {
+    rc = nise_absorption_batched(&loc, H.p, hw, mu.p, mw, nfr, ctype ? Cc.p : nullptr, NISE_HOST, 0, re.data(),
+                                 im.data());
+  }
   if (rc) die(1, "absorption: GPU run failed: %s\n", nise_last_error());
   const int samples = non->end;  // src/absorption.c:185,192: loop exit value
   FILE *log = fopen("NISE.log", "a");

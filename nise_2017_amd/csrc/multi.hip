@@ -194,6 +194,9 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
     const nise_system &y = systems[k];
     const t_non *non = y.non;
     if (!non || !y.H_rec || !y.mu_rec) return set_error("multi: system %d has null inputs", k), NISE_ERR_ARG;
+    if (strcmp(non->hamiltonian, "TransitionDipole") == 0 || strcmp(non->hamiltonian, "ExtendedDipole") == 0)
+      return set_error("multi: system %d: on-the-fly couplings (%s) need nise_abs_plan_create_onfly", k,
+                       non->hamiltonian), NISE_ERR_ARG;
     const int N = non->singles, T = non->tmax1;
     if (N < 1 || N > 65535 || T < 1 || non->sample < 1 || non->ts < 1)
       return set_error("multi: system %d has bad sizes", k), NISE_ERR_ARG;

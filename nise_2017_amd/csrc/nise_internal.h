@@ -89,6 +89,14 @@ struct FrameSource {
   const float *delta = nullptr;  // device, per-site static offsets added to
                                  // the shifted diagonal (gradient runs)
   float cscale = 1.f;            // scale of the kept couplings
+  // Couplings built per frame from geometry (src/read_trajectory.c:181-303):
+  // 1 TransitionDipole, 2 ExtendedDipole; H_rec then holds N diagonal values.
+  int onfly = 0;
+  const float *pos_rec = nullptr;  // position records (int + 3N), one per
+  int64_t pos_stride = 0;          // frame (two per frame for ExtendedDipole)
+  const float *dip_rec = nullptr;  // transition dipole records (int + 3N)
+  int64_t dip_stride = 0;
+  const float *box_rec = nullptr;  // periodic box records (int + 3), or none
 };
 
 // ---------------------------------------------------------------------------
@@ -96,10 +104,68 @@ struct FrameSource {
 // ---------------------------------------------------------------------------
 __device__ inline float raw_diag(const FrameSource &s, int N, int64_t f, int a) {
   const float *rec = s.H_rec + f * s.H_stride + 1;
-  return s.coupling_type ? rec[a] : rec[sindex(a, a, N)];
+  return (s.coupling_type || s.onfly) ? rec[a] : rec[sindex(a, a, N)];
+}
+
+// Minimum image along axis x when a box is given (pbc1, src/NISE_subs.c:641-648).
+__device__ inline float pbc_image(float r, int x, const float *box) {
+  if (box && box[0] > 0.f) {
+    if (r > box[x] / 2) r = __fsub_rn(r, box[x]);
+    if (r < -box[x] / 2) r = __fadd_rn(r, box[x]);
+  }
+  return r;
+}
+
+// |p - q| with the minimum image, in the reference's FP32 order.
+__device__ inline float pbc_dist(const float *p, const float *q, int i, int j, int N, const float *box) {
+  const float rx = pbc_image(__fsub_rn(p[i], q[j]), 0, box);
+  const float ry = pbc_image(__fsub_rn(p[N + i], q[N + j]), 1, box);
+  const float rz = pbc_image(__fsub_rn(p[2 * N + i], q[2 * N + j]), 2, box);
+  return __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(rx, rx), __fmul_rn(ry, ry)), __fmul_rn(rz, rz)));
+}
+
+// Coupling of sites i < j at frame f built from geometry, FP32 in the
+// reference's order.  TransitionDipole (src/read_trajectory.c:198-216):
+// J = A (m_i.m_j - 3 (m_i.R)(m_j.R)/R^2) / R^3.  ExtendedDipole (:244-297):
+// charges q = |m|/d on the two position records, four charge-pair
+// distances; the reference takes q_j from site i's dipole (:266), kept so
+// that results are identical.
+__device__ inline float onfly_coupling(const FrameSource &s, int N, int64_t f, int i, int j) {
+  const float A = 5034.11861687f;  // Debye^2/Angstrom^3 to cm^-1
+  const float *box = s.box_rec ? s.box_rec + f * 4 + 1 : nullptr;
+  const float *mu = s.dip_rec + f * s.dip_stride + 1;
+  const float m1x = mu[i], m1y = mu[N + i], m1z = mu[2 * N + i];
+  if (s.onfly == 1) {
+    const float *R = s.pos_rec + f * s.pos_stride + 1;
+    const float m2x = mu[j], m2y = mu[N + j], m2z = mu[2 * N + j];
+    const float rx = pbc_image(__fsub_rn(R[i], R[j]), 0, box);
+    const float ry = pbc_image(__fsub_rn(R[N + i], R[N + j]), 1, box);
+    const float rz = pbc_image(__fsub_rn(R[2 * N + i], R[2 * N + j]), 2, box);
+    const float dist = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(rx, rx), __fmul_rn(ry, ry)), __fmul_rn(rz, rz)));
+    const float idist = __fdiv_rn(1.f, dist);
+    const float idist3 = __fmul_rn(__fmul_rn(idist, idist), idist);
+    const float f1 = __fadd_rn(__fadd_rn(__fmul_rn(m1x, m2x), __fmul_rn(m1y, m2y)), __fmul_rn(m1z, m2z));
+    const float d1 = __fadd_rn(__fadd_rn(__fmul_rn(m1x, rx), __fmul_rn(m1y, ry)), __fmul_rn(m1z, rz));
+    const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(m2x, rx), __fmul_rn(m2y, ry)), __fmul_rn(m2z, rz));
+    const float f2 = __fmul_rn(__fmul_rn(__fmul_rn(__fmul_rn(-3.f, d1), d2), idist), idist);
+    return __fmul_rn(__fmul_rn(A, __fadd_rn(f1, f2)), idist3);
+  }
+  const float *R = s.pos_rec + 2 * f * s.pos_stride + 1;  // charges at R ...
+  const float *R2 = R + s.pos_stride;                     // ... and at R2
+  // q = sqrt(|m|^2) / d: the square root and the division in double, rounded
+  // once when stored to float (C's sqrt of a float argument is double)
+  const double m1 = __dsqrt_rn((double)__fadd_rn(__fadd_rn(__fmul_rn(m1x, m1x), __fmul_rn(m1y, m1y)), __fmul_rn(m1z, m1z)));
+  const float q1 = (float)__ddiv_rn(m1, (double)pbc_dist(R2, R, i, i, N, box));
+  const float q2 = (float)__ddiv_rn(m1, (double)pbc_dist(R2, R, j, j, N, box));
+  float J = __fdiv_rn(1.f, pbc_dist(R, R, i, j, N, box));
+  J = __fsub_rn(J, __fdiv_rn(1.f, pbc_dist(R, R2, i, j, N, box)));
+  J = __fsub_rn(J, __fdiv_rn(1.f, pbc_dist(R2, R, i, j, N, box)));
+  J = __fadd_rn(J, __fdiv_rn(1.f, pbc_dist(R2, R2, i, j, N, box)));
+  return __fmul_rn(__fmul_rn(__fmul_rn(A, q1), q2), J);
 }
 
 __device__ inline float raw_coupling(const FrameSource &s, int N, int64_t f, int a, int b) {
+  if (s.onfly) return onfly_coupling(s, N, f, a, b);
   const float *rec = s.coupling_type ? (s.C_rec + 1) : (s.H_rec + f * s.H_stride + 1);
   return rec[sindex(a, b, N)];
 }

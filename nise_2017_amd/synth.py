@@ -234,3 +234,50 @@ def ring_2dir(N=32, nframes=4200, seed=SEED0 + 2, dt=2.0, mean=1650.0, sd=12.0, 
     H = full_records(diag, coup)
     mu = dipole_records(unit_vectors(rng, N), nframes)
     return H, mu
+
+
+def onfly_system(N, nframes, seed, kind=1, box=None, spacing=5.0, mu_debye=0.5, arm=0.6, dt=2.0,
+                 mean=1650.0, sd=10.0, tau=100.0, jitter=0.05, wobble=0.02):
+    """Records for HamiltonianType TransitionDipole (kind 1) or
+    ExtendedDipole (kind 2): site energies (T, 1 + N) as AR(1), transition
+    dipoles (T, 1 + 3N) of magnitude mu_debye wobbling about random
+    directions, positions (Angstrom) on a jittered cubic grid of the given
+    spacing moving by small random steps.  ExtendedDipole has two position
+    records per frame (2T, 1 + 3N): the charges at centre -/+ arm/2 along
+    the dipole.  box: cubic edge for the minimum image (positions wrapped
+    into [0, box)), giving box records (T, 4), or None.
+    Returns (H, mu, pos, box_records)."""
+    rng = np.random.default_rng(seed)
+    n = int(np.ceil(N ** (1.0 / 3.0)))
+    grid = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 3)
+    centre = (grid[:N] * spacing + rng.normal(0, 0.3, (N, 3))).astype(np.float64)
+    axis = unit_vectors(rng, N)
+    H = np.empty((nframes, 1 + N), np.float32)
+    H[:, 0] = headers(nframes)
+    H[:, 1:] = ar1(rng, nframes, N, mean, sd, tau, dt)
+    mu = np.empty((nframes, 1 + 3 * N), np.float32)
+    mu[:, 0] = headers(nframes)
+    per = 1 if kind == 1 else 2
+    pos = np.empty((per * nframes, 1 + 3 * N), np.float32)
+    pos[:, 0] = headers(per * nframes)
+    for f in range(nframes):
+        centre += rng.normal(0, jitter, (N, 3))
+        axis += rng.normal(0, wobble, (N, 3))
+        axis /= np.linalg.norm(axis, axis=1)[:, None]
+        m = mu_debye * axis
+        mu[f, 1:] = np.concatenate([m[:, 0], m[:, 1], m[:, 2]])
+        c = np.mod(centre, box) if box else centre
+        if kind == 1:
+            pos[f, 1:] = np.concatenate([c[:, 0], c[:, 1], c[:, 2]])
+        else:
+            for k, sgn in enumerate((-0.5, 0.5)):
+                p = c + sgn * arm * axis
+                if box:
+                    p = np.mod(p, box)
+                pos[2 * f + k, 1:] = np.concatenate([p[:, 0], p[:, 1], p[:, 2]])
+    boxes = None
+    if box:
+        boxes = np.empty((nframes, 4), np.float32)
+        boxes[:, 0] = headers(nframes)
+        boxes[:, 1:] = box
+    return H, mu, pos, boxes

@@ -517,6 +517,90 @@ static int is_coupling(const t_non *non) {
   return strcmp(non->hamiltonian, "Coupling") == 0;
 }
 
+/* On-the-fly couplings (src/read_trajectory.c:181-303): geometry records set
+   by or_set_geometry; 1 TransitionDipole, 2 ExtendedDipole.  Evaluated in
+   IEEE arithmetic in source order, as the reference's ACCURATE_MATHS build
+   does (under -ffast-math GCC turns 1.0/sqrt(x) into an approximate
+   reciprocal square root plus a Newton step, whose low bits depend on the
+   CPU). */
+#pragma GCC push_options
+#pragma GCC optimize("no-fast-math")
+static const float *g_pos, *g_dip, *g_box;
+static long long g_pos_stride, g_dip_stride;
+
+void or_set_geometry(const float *pos_rec, long long pos_stride, const float *dip_rec,
+                     long long dip_stride, const float *box_rec) {
+  g_pos = pos_rec, g_pos_stride = pos_stride, g_dip = dip_rec, g_dip_stride = dip_stride;
+  g_box = box_rec;
+}
+
+static int onfly_kind(const t_non *non) {
+  if (strcmp(non->hamiltonian, "TransitionDipole") == 0) return 1;
+  if (strcmp(non->hamiltonian, "ExtendedDipole") == 0) return 2;
+  return 0;
+}
+
+/* minimum image along axis x (pbc1, src/NISE_subs.c:641-648) */
+static float image(float r, int x, const float *box) {
+  if (box && box[0] > 0.0f) {
+    if (r > box[x] / 2) r = r - box[x];
+    if (r < -box[x] / 2) r = r + box[x];
+  }
+  return r;
+}
+
+/* |p_i - q_j| with the minimum image, float as the reference */
+static float sep(const float *p, const float *q, int i, int j, int N, const float *box) {
+  float x = image(p[i] - q[j], 0, box);
+  float y = image(p[N + i] - q[N + j], 1, box);
+  float z = image(p[2 * N + i] - q[2 * N + j], 2, box);
+  return sqrt(x * x + y * y + z * z);
+}
+
+/* Couplings of frame pos into the packed triangle He (i < j). */
+static void onfly_couplings(const t_non *non, long long pos, float *He) {
+  const int N = non->singles, kind = onfly_kind(non);
+  const float A = 5034.11861687f;
+  const float *box = g_box ? g_box + pos * 4 + 1 : NULL;
+  const float *m = g_dip + pos * g_dip_stride + 1;
+  if (kind == 1) { /* :198-216 */
+    const float *R = g_pos + pos * g_pos_stride + 1;
+    for (int i = 0; i < N; i++)
+      for (int j = i + 1; j < N; j++) {
+        float rx = image(R[i] - R[j], 0, box);
+        float ry = image(R[N + i] - R[N + j], 1, box);
+        float rz = image(R[2 * N + i] - R[2 * N + j], 2, box);
+        float d = sqrt(rx * rx + ry * ry + rz * rz);
+        float id = 1.0 / d;
+        float id3 = id * id * id;
+        float dot = m[i] * m[j] + m[N + i] * m[N + j] + m[2 * N + i] * m[2 * N + j];
+        float a1 = m[i] * rx + m[N + i] * ry + m[2 * N + i] * rz;
+        float a2 = m[j] * rx + m[N + j] * ry + m[2 * N + j] * rz;
+        float orient = -3 * a1 * a2 * id * id;
+        He[or_sindex(i, j, N)] = A * (dot + orient) * id3;
+      }
+  } else { /* :244-297: charges on the two position records of the frame */
+    const float *R = g_pos + 2 * pos * g_pos_stride + 1, *R2 = R + g_pos_stride;
+    for (int i = 0; i < N; i++) {
+      float mi2 = m[i] * m[i] + m[N + i] * m[N + i] + m[2 * N + i] * m[2 * N + i];
+      float qi = sqrt(mi2) / sep(R2, R, i, i, N, box);
+      for (int j = i + 1; j < N; j++) {
+        float qj = sqrt(mi2) / sep(R2, R, j, j, N, box); /* site i's dipole, as :266 */
+        /* each 1/r is rounded to float before it enters the float sum */
+        float J = 1.0 / sep(R, R, i, j, N, box);
+        float id = 1.0 / sep(R, R2, i, j, N, box);
+        J = J - id;
+        id = 1.0 / sep(R2, R, i, j, N, box);
+        J = J - id;
+        id = 1.0 / sep(R2, R2, i, j, N, box);
+        J = J + id;
+        He[or_sindex(i, j, N)] = A * qi * qj * J;
+      }
+    }
+  }
+}
+#pragma GCC pop_options
+
 void or_read_H(const t_non *non, const float *H_rec, long long H_stride,
                const float *C_rec, long long pos, float *He, int apply_shift) {
   int N = non->singles;
@@ -530,6 +614,15 @@ void or_read_H(const t_non *non, const float *H_rec, long long H_stride,
         if (fabs(He[or_sindex(i, j, N)]) < non->couplingcut) He[or_sindex(i, j, N)] = 0;
     const float *d = H_rec + pos * H_stride + 1;
     for (int i = 0; i < N; i++) He[i * N + i - (i * (i + 1)) / 2] = d[i] - non->shifte;
+  } else if (onfly_kind(non)) {
+    /* site energies from the diagonal records (:135-154), couplings from
+       the geometry (:181-303), then the cut (:305-311) */
+    const float *d = H_rec + pos * H_stride + 1;
+    for (int i = 0; i < N; i++) He[i * N + i - (i * (i + 1)) / 2] = d[i] - non->shifte;
+    onfly_couplings(non, pos, He);
+    for (int i = 0; i < N; i++)
+      for (int j = i + 1; j < N; j++)
+        if (fabs(He[or_sindex(i, j, N)]) < non->couplingcut) He[or_sindex(i, j, N)] = 0;
   } else {
     const float *h = H_rec + pos * H_stride + 1;
     for (int i = 0; i < N * (N + 1) / 2; i++) He[i] = h[i];

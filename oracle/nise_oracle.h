@@ -14,7 +14,8 @@
  *
  * Pinning (what fixes each function independently of this file):
  *   or_coupling_S_f32 / or_coupling_S_doubles_f32 / or_dipole_double* /
- *   or_polar* / or_read_H* : bit-identical to the reference's own
+ *   or_polar* / or_read_H* (including the TransitionDipole /
+ *   ExtendedDipole couplings built from geometry) : bit-identical to the reference's own
  *     propagate.c / NISE_subs.c / polar.c / read_trajectory.c compiled from
  *     the reference sources (oracle/_ref), and to tests/golden fixtures.
  *   or_coupling_S_f64 : closed forms (uncoupled phases, dimer D.R.D,
@@ -83,6 +84,11 @@ float or_polarweight(int pol, int x);
 void or_polar(int *xp, int x);
 
 /* One Hamiltonian frame as read_Hamiltonian()/read_He() leave it. */
+/* geometry for TransitionDipole / ExtendedDipole frames read by or_read_H
+   and or_absorption: position records (two per frame for ExtendedDipole),
+   transition dipole records, optional box records (int + 3 floats) */
+void or_set_geometry(const float *pos_rec, long long pos_stride, const float *dip_rec,
+                     long long dip_stride, const float *box_rec);
 void or_read_H(const t_non *non, const float *H_rec, long long H_stride,
                const float *C_rec, long long pos, float *He, int apply_shift);
 

@@ -14,6 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "_build", "libnise_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libnise_ref.so")
+REF_IEEE_SO = os.path.join(HERE, "_ref", "libnise_ref_ieee.so")  # ACCURATE_MATHS flags
 
 _f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64 = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -33,6 +34,7 @@ def build(ref=None):
 
 _o = None
 _r = None
+_ri = None
 
 
 def oracle():
@@ -54,6 +56,7 @@ def oracle():
     L.or_DIA_f64.argtypes = [C.c_int, _f32, _f64, _f64, C.c_int, C.c_float]
     L.or_DIA_S_f64.argtypes = [C.c_int, _f32, _f64, _f64, C.c_int, C.c_float, C.c_float]
     L.or_DIA_S_f64.restype = C.c_int
+    L.or_set_geometry.argtypes = [_vp, C.c_longlong, _vp, C.c_longlong, _vp]
     L.or_dipole_double.argtypes = [C.c_int, _f32, _f32, _f32, _f32, _f32, _f32, C.c_float]
     L.or_dipole_double_last.argtypes = L.or_dipole_double.argtypes
     L.or_polarweight.argtypes = [C.c_int, C.c_int]
@@ -75,11 +78,20 @@ def have_ref():
     return os.path.exists(REF_SO)
 
 
-def ref():
-    global _r
-    if _r is not None:
-        return _r
-    L = C.CDLL(REF_SO)
+def ref(ieee=False):
+    """The reference build (default flags, -ffast-math), or with ieee=True
+    the same sources built with the reference's ACCURATE_MATHS flags."""
+    global _r, _ri
+    if ieee:
+        if _ri is None:
+            _ri = _bind_ref(C.CDLL(REF_IEEE_SO))
+        return _ri
+    if _r is None:
+        _r = _bind_ref(C.CDLL(REF_SO))
+    return _r
+
+
+def _bind_ref(L):
     L.ref_coupling_S.argtypes = [C.c_int, _f32, _f32, _f32, C.c_int, C.c_int, C.c_float, C.c_float]
     L.ref_coupling_S_doubles.argtypes = [C.c_int, _f32, _f32, _f32, C.c_int, _f32, C.c_float,
                                          C.c_float, C.c_float]
@@ -95,7 +107,10 @@ def ref():
     L.ref_absorption_files.argtypes = [C.c_int, cs, cs, cs, cs, cs, C.c_float, C.c_float, C.c_float,
                                        C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_int, _f32, _f32]
-    _r = L
+    L.ref_read_frame_onfly.argtypes = [C.c_int, cs, cs, cs, cs, cs, C.c_float, C.c_float, C.c_int, _f32]
+    L.ref_absorption_files_geo.argtypes = [C.c_int, cs, cs, cs, cs, cs, cs, cs, C.c_float, C.c_float,
+                                           C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           C.c_int, C.c_int, _f32, _f32]
     return L
 
 

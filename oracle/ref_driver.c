@@ -125,26 +125,71 @@ EXPORT int ref_read_frame(int N, const char *hfile, const char *dfile,
   return 0;
 }
 
+/* One frame of a TransitionDipole / ExtendedDipole Hamiltonian through the
+ * reference's read_Hamiltonian (read_He builds the couplings from the
+ * position, dipole and optional pbc files). */
+EXPORT int ref_read_frame_onfly(int N, const char *hfile, const char *dfile,
+                                const char *posfile, const char *pbcfile,
+                                const char *htype, float shifte, float cut,
+                                int pos, float *He) {
+  t_non non;
+  init_non(&non, N, 1, cut);
+  copy_name(non.energyFName, hfile);
+  copy_name(non.dipoleFName, dfile);
+  copy_name(non.positionFName, posfile);
+  copy_name(non.pbcFName, pbcfile ? pbcfile : "");
+  copy_name(non.hamiltonian, htype);
+  non.shifte = shifte;
+  non.length = 1 << 29;
+  FILE *H_traj, *mu_traj, *Cfile = NULL;
+  open_files(&non, &H_traj, &mu_traj, &Cfile);
+  read_Hamiltonian(&non, He, H_traj, pos);
+  fclose(H_traj), fclose(mu_traj);
+  return 0;
+}
+
+static int absorption_loop(t_non *non, float *re_S, float *im_S);
+
 /* Loop of src/absorption.c:92-146 over the reference's primitives.  calc_S1
- * (src/absorption.c:205-212) is the two-line running sum written inline. */
-EXPORT int ref_absorption_files(int N, const char *hfile, const char *dfile,
-                                const char *cfile, const char *shiftfile,
-                                const char *htype, float min1, float max1,
-                                float deltat, float cut, int ts, int tmax,
-                                int sample, int begin, int end, int length,
-                                float *re_S, float *im_S) {
+ * (src/absorption.c:205-212) is the two-line running sum written inline.
+ * posfile / pbcfile: geometry for the on-the-fly Hamiltonian types. */
+EXPORT int ref_absorption_files_geo(int N, const char *hfile, const char *dfile,
+                                    const char *cfile, const char *shiftfile,
+                                    const char *posfile, const char *pbcfile,
+                                    const char *htype, float min1, float max1,
+                                    float deltat, float cut, int ts, int tmax,
+                                    int sample, int begin, int end, int length,
+                                    float *re_S, float *im_S) {
   t_non non;
   init_non(&non, N, deltat, cut);
   copy_name(non.energyFName, hfile);
   copy_name(non.dipoleFName, dfile);
   copy_name(non.couplingFName, cfile ? cfile : "");
   copy_name(non.singleShiftFName, shiftfile ? shiftfile : "");
+  copy_name(non.positionFName, posfile ? posfile : "");
+  copy_name(non.pbcFName, pbcfile ? pbcfile : "");
   copy_name(non.hamiltonian, htype);
   non.min1 = min1, non.max1 = max1;
   non.ts = ts;
   non.tmax = non.tmax1 = tmax;
   non.sample = sample, non.begin = begin, non.end = end;
   non.length = length;
+  return absorption_loop(&non, re_S, im_S);
+}
+
+EXPORT int ref_absorption_files(int N, const char *hfile, const char *dfile,
+                                const char *cfile, const char *shiftfile,
+                                const char *htype, float min1, float max1,
+                                float deltat, float cut, int ts, int tmax,
+                                int sample, int begin, int end, int length,
+                                float *re_S, float *im_S) {
+  return ref_absorption_files_geo(N, hfile, dfile, cfile, shiftfile, "", "", htype, min1, max1, deltat, cut,
+                                  ts, tmax, sample, begin, end, length, re_S, im_S);
+}
+
+static int absorption_loop(t_non *non_, float *re_S, float *im_S) {
+  t_non non = *non_;
+  const int N = non.singles;
   non.shifte = (non.max1 + non.min1) / 2; /* src/absorption.c:57-59 */
   int nn2 = N * (N + 1) / 2;
   float *Hamil_i_e = calloc(nn2, sizeof(float));
