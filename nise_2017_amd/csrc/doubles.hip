@@ -100,86 +100,168 @@ int launch_doubles_tables(const FrameSource &src, int N, int K2, const uint32_t 
   return NISE_OK;
 }
 
+// Couplings staged in LDS per pass over the coupling list: (co, si, co2,
+// si2) rows and pair words of at most kDblCh couplings.
+constexpr int kDblCh = 512;
+__host__ __device__ inline int dbl_chunk(int K2) {
+  return K2 <= 64 ? 64 : K2 < kDblCh ? (K2 + 63) / 64 * 64 : kDblCh;
+}
+// Vectors per lane: with np <= 32 lanes per vector each lane rotates the
+// same pair of states in two vectors, so one index computation serves both.
+__host__ __device__ constexpr int dbl_vpl(int np) { return np <= 32 ? 1 : 1; }
+// LDS bytes of a doubles step: states [N2][vectors per wave], the D2 row
+// when d2_lds, then the coupling chunk.  The D2 row is staged when
+// everything fits.
+__host__ __device__ inline size_t doubles_smem(int N, int np, int K2, bool d2_lds) {
+  const size_t N2 = (size_t)N * (N + 1) / 2, vw = (size_t)(64 / np) * dbl_vpl(np);
+  return ((N2 * (vw + (d2_lds ? 1 : 0)) * 8 + 15) & ~(size_t)15) + (size_t)dbl_chunk(K2) * 20;
+}
+__host__ __device__ inline bool doubles_d2_lds(int N, int np, int K2) {
+  return doubles_smem(N, np, K2, true) <= 160 * 1024;
+}
+
 // One frame of m Trotter substeps for groups of vectors.  Group g (one
 // wave) holds vectors vec_begin[g] .. vec_begin[g+1]-1 of the state buffer
-// (at most 64/NP of them) and uses table row rows[g].
-template <int NP>
+// (at most 64/NP x VPL of them) and uses table row rows[g].  The states,
+// the frame's D2 row and its coupling rows are staged in LDS first (loads in
+// flight together), so the ordered rotation chain waits on LDS only.  Lane
+// group q = t / NP owns vectors q VPL .. q VPL + VPL - 1, lane c0 = t % NP
+// the rotation of site c.
+template <int NP, int VPL>
 __global__ void __launch_bounds__(64) doubles_step_kernel(DoublesArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int VW = 64 / NP;  // vectors per wave
-  const int N = A.N, N2 = N * (N + 1) / 2, K2 = A.K2;
+  constexpr int VW = 64 / NP * VPL;  // vectors per wave
+  const int N = A.N, N2 = N * (N + 1) / 2, K2 = A.K2, ch = dbl_chunk(K2);
+  const bool d2l = doubles_d2_lds(N, NP, K2);
   float2 *psi = reinterpret_cast<float2 *>(smem);  // [N2][VW]
+  float4 *csL = reinterpret_cast<float4 *>(smem + (((size_t)N2 * (VW + d2l) * 8 + 15) & ~(size_t)15));  // [ch]
+  uint32_t *abL = reinterpret_cast<uint32_t *>(csL + ch);                                              // [ch]
   const int g = blockIdx.x, t = threadIdx.x;
   const int v0 = A.vec_begin[g], nv = A.vec_begin[g + 1] - v0;
   const int64_t row = A.rows[g];
-  const float2 *D2 = A.D2 + row * N2;
   const float4 *cs2 = A.cs2 + row * K2;
-  // load: state[v0 + w][j] -> psi[j][w]
-  for (int i = t; i < N2 * VW; i += 64) {
-    const int w = i / N2, j = i % N2;
-    psi[j * VW + w] = (w < nv) ? A.state[(int64_t)(v0 + w) * N2 + j] : make_float2(0.f, 0.f);
+  const float2 *d2L = psi + N2 * VW, *d2g = A.D2 + row * N2;  // the D2 row in LDS (d2l) or device memory
+  // one diagonal half-step on the LDS states (LDS and global reads kept
+  // apart: a pointer that may be either compiles to flat loads)
+  auto diag = [&]() {
+    if (d2l) {
+#pragma unroll 4
+      for (int j = t; j < N2 * VW; j += 64) psi[j] = cmul_diag2(psi[j], d2L[j / VW]);
+    } else {
+      for (int j = t; j < N2 * VW; j += 64) psi[j] = cmul_diag2(psi[j], gld(d2g + j / VW));
+    }
+    asm volatile("" ::: "memory");
+  };
+  // Staging in one pass over the float2 words [states of the nv vectors |
+  // D2 row | coupling rows]: state[v0 + w][j] -> psi[j][w]; the pair words
+  // (at most kDblCh, 8 per lane) are loaded before it and stored after, so
+  // all of a thread's loads are in flight together.
+  const bool once = K2 <= ch;
+  for (int w = nv; w < VW; ++w)
+    for (int j = t; j < N2; j += 64) psi[j * VW + w] = make_float2(0.f, 0.f);
+  uint32_t abv[kDblCh / 64];
+  if (once) {
+#pragma unroll
+    for (int u = 0; u < kDblCh / 64; ++u)
+      if (t + 64 * u < K2) abv[u] = gld(A.ab2 + t + 64 * u);
+  }
+  {
+    const int nS = nv * N2, nD = d2l ? N2 : 0, nC = once ? 2 * K2 : 0;
+    const float2 *sg = A.state + (int64_t)v0 * N2, *cg = reinterpret_cast<const float2 *>(cs2);
+    const int cofs = (int)(reinterpret_cast<float2 *>(csL) - psi);
+    gather_lds<32>(psi, nS + nD + nC,
+                   [=](int e) { return e < nS ? sg + e : e < nS + nD ? d2g + (e - nS) : cg + (e - nS - nD); },
+                   [=](int e) {
+                     if (e < nS) {
+                       int w = 0;
+#pragma unroll
+                       for (int u = 1; u < VW; ++u) w += e >= u * N2;
+                       return (e - w * N2) * VW + w;
+                     }
+                     return e < nS + nD ? N2 * VW + (e - nS) : cofs + (e - nS - nD);
+                   });
+  }
+  if (once) {
+#pragma unroll
+    for (int u = 0; u < kDblCh / 64; ++u)
+      if (t + 64 * u < K2) abL[t + 64 * u] = abv[u];
   }
   __syncthreads();
-  const int w = t / NP, c0 = t % NP;
-  const bool on = w < nv;
+  const int q = t / NP, c0 = t % NP;
+  const int rs0 = (c0 * (2 * N - c0 - 1)) >> 1;  // row start of |c0 c0> in the packed basis
   for (int sub = 0; sub < A.m; ++sub) {
-    for (int j = t; j < N2 * VW; j += 64) {
-      const int jj = j / VW;
-      psi[j] = cmul_diag2(psi[j], D2[jj]);
-    }
-    asm volatile("" ::: "memory");
-    for (int k = 0; k < K2; ++k) {
-      const uint32_t p = A.ab2[k];
-      const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
-      const float4 cc = cs2[k];
-      for (int cb = 0; cb < N; cb += NP) {
-        const int c = cb + c0;
-        // round 1: every c except b (c == a uses the sqrt(2) angle)
-        if (on && c < N && c != b) {
-          const int i1 = sindex(c, a, N), i2 = sindex(c, b, N);
-          float2 p1 = psi[i1 * VW + w], p2 = psi[i2 * VW + w];
-          if (c == a) rot2(p1, p2, cc.z, cc.w);
-          else rot2(p1, p2, cc.x, cc.y);
-          psi[i1 * VW + w] = p1;
-          psi[i2 * VW + w] = p2;
+    diag();
+    for (int k0 = 0; k0 < K2; k0 += ch) {
+      const int nk = min(ch, K2 - k0);
+      if (!once) {
+        __syncthreads();
+        stage_lds<4>(csL, cs2 + k0, nk);
+        stage_lds<4>(abL, A.ab2 + k0, nk);
+        __syncthreads();
+      }
+      for (int k = 0; k < nk; ++k) {
+        // the coupling is wave-uniform: pair and angles in scalar registers
+        const uint32_t p = __builtin_amdgcn_readfirstlane(abL[k]);
+        const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
+        const int rs_a = (a * (2 * N - a - 1)) >> 1, rs_b = (b * (2 * N - b - 1)) >> 1;
+        const float4 cv = csL[k];
+        const float co = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cv.x)));
+        const float si = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cv.y)));
+        const float co2 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cv.z)));
+        const float si2 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cv.w)));
+        for (int cb = 0; cb < N; cb += NP) {
+          const int c = cb + c0;
+          // every c except b, ascending; c == a (sqrt(2) angle) also does the
+          // c == b rotation (|ab>, |bb>): it shares |ab>, and no rotation of
+          // a c in between touches |ab> or |bb>
+          if (c < N && c != b) {
+            const int rs_c = cb == 0 ? rs0 : (c * (2 * N - c - 1)) >> 1;
+            const int i1 = (c <= a ? a + rs_c : c + rs_a) * VW + q * VPL;
+            const int i2 = (c <= b ? b + rs_c : c + rs_b) * VW + q * VPL;
+            const int i3 = (b + rs_b) * VW + q * VPL;
+            const bool ca = c == a;
+            const float cl = ca ? co2 : co, sl = ca ? si2 : si;
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+              if (q * VPL + v < nv) {
+                float2 p1 = psi[i1 + v], p2 = psi[i2 + v], p3 = make_float2(0.f, 0.f);
+                if (ca) p3 = psi[i3 + v];  // read with the pair: one LDS round trip
+                rot2(p1, p2, cl, sl);
+                psi[i1 + v] = p1;
+                if (ca) {
+                  rot2(p2, p3, co2, si2);
+                  psi[i3 + v] = p3;
+                }
+                psi[i2 + v] = p2;
+              }
+            }
+          }
+          asm volatile("" ::: "memory");
         }
-        asm volatile("" ::: "memory");
       }
-      // round 2: c == b, after c == a (they share |ab>)
-      if (on && c0 == (b % NP)) {
-        const int i1 = sindex(a, b, N), i2 = sindex(b, b, N);
-        float2 p1 = psi[i1 * VW + w], p2 = psi[i2 * VW + w];
-        rot2(p1, p2, cc.z, cc.w);
-        psi[i1 * VW + w] = p1;
-        psi[i2 * VW + w] = p2;
-      }
-      asm volatile("" ::: "memory");
     }
-    for (int j = t; j < N2 * VW; j += 64) {
-      const int jj = j / VW;
-      psi[j] = cmul_diag2(psi[j], D2[jj]);
-    }
-    asm volatile("" ::: "memory");
+    if (sub + 1 < A.m) diag();
   }
   __syncthreads();
-  for (int i = t; i < N2 * VW; i += 64) {
-    const int ww = i / N2, j = i % N2;
-    if (ww < nv) A.state[(int64_t)(v0 + ww) * N2 + j] = psi[j * VW + ww];
-  }
+  // the last diagonal half-step, fused into the store
+  for (int ww = 0; ww < nv; ++ww)
+    for (int j = t; j < N2; j += 64)
+      gst(A.state + (int64_t)(v0 + ww) * N2 + j,
+          cmul_diag2(psi[j * VW + ww], d2l ? d2L[j] : gld(d2g + j)));
 }
 
 int doubles_np(int N) { return N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64; }
+int doubles_vw(int N) { return 64 / doubles_np(N) * dbl_vpl(doubles_np(N)); }
 
 int launch_doubles_step(const DoublesArgs &a, int ngroups, hipStream_t st) {
   if (ngroups == 0) return NISE_OK;
   const int np = doubles_np(a.N);
-  const int N2 = a.N * (a.N + 1) / 2;
-  const size_t smem = (size_t)N2 * (64 / np) * sizeof(float2);
+  const size_t smem = doubles_smem(a.N, np, a.K2, doubles_d2_lds(a.N, np, a.K2));
   if (smem > 160 * 1024) return set_error("doubles: N=%d too large for LDS", a.N), NISE_ERR_ARG;
-  const void *fn = np == 8 ? (const void *)doubles_step_kernel<8>
-                 : np == 16 ? (const void *)doubles_step_kernel<16>
-                 : np == 32 ? (const void *)doubles_step_kernel<32>
-                            : (const void *)doubles_step_kernel<64>;
+  const void *fn = np == 8 ? (const void *)doubles_step_kernel<8, dbl_vpl(8)>
+                 : np == 16 ? (const void *)doubles_step_kernel<16, dbl_vpl(16)>
+                 : np == 32 ? (const void *)doubles_step_kernel<32, dbl_vpl(32)>
+                            : (const void *)doubles_step_kernel<64, dbl_vpl(64)>;
   NISE_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
   DoublesArgs args = a;
   void *kargs[] = {&args};
@@ -228,7 +310,7 @@ int doubles_host(int N, const float *H, float *const *cr, float *const *ci, int 
   std::vector<uint32_t> ab;
   kept_pairs(N, H, cut, ab);
   const int K2 = (int)ab.size();
-  const int np = doubles_np(N), vw = 64 / np;
+  const int vw = doubles_vw(N);
   const int ng = (nvec + vw - 1) / vw;
   std::vector<int> vb(ng + 1);
   for (int g = 0; g <= ng; ++g) vb[g] = std::min(nvec, g * vw);

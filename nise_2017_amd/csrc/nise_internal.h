@@ -216,6 +216,38 @@ __device__ __forceinline__ void gst(T *p, T v) {
   *(__attribute__((address_space(1))) W *)p = w;
 }
 
+// dst[map(i)] = *src(i) for i < n, device memory to LDS, by the whole
+// workgroup with U loads in flight per thread before the first LDS store (a
+// plain copy loop waits for each load before it issues the next).
+struct IdentityMap {
+  __device__ int operator()(int i) const { return i; }
+};
+template <int U, class T, class Src, class Map = IdentityMap>
+__device__ __forceinline__ void gather_lds(T *dst, int n, Src src, Map map = Map()) {
+  const int nt = blockDim.x;
+  int i = threadIdx.x;
+  for (; i + (U - 1) * nt < n; i += U * nt) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = gld(src(i + u * nt));
+#pragma unroll
+    for (int u = 0; u < U; ++u) dst[map(i + u * nt)] = v[u];
+  }
+  if (i < n) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i + u * nt < n) v[u] = gld(src(i + u * nt));
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i + u * nt < n) dst[map(i + u * nt)] = v[u];
+  }
+}
+template <int U, class T, class Map = IdentityMap>
+__device__ __forceinline__ void stage_lds(T *dst, const T *src, int n, Map map = Map()) {
+  gather_lds<U>(dst, n, [=](int i) { return src + i; }, map);
+}
+
 // Value of quad lane j in every lane of the quad (DPP quad_perm; the
 // compiler folds it into the VALU instruction that uses the value).
 template <int j>
@@ -337,6 +369,7 @@ int launch_doubles_tables(const FrameSource &src, int N, int K2, const uint32_t 
                           float2 *D2, float4 *cs2, hipStream_t st);
 int launch_doubles_step(const DoublesArgs &a, int ngroups, hipStream_t st);
 int doubles_np(int N);
+int doubles_vw(int N);  // vectors per doubles group (one wave)
 
 // Row conveyor (rows.hip), one descriptor per system.  Vector v = 3 s + x
 // is polarisation x of sample begin + s, starting at frame (begin + s) *

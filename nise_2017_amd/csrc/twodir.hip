@@ -193,69 +193,205 @@ __global__ void dd_kernel(TwoDirArgs A) {
   A.dbl[((int64_t)s * (9 + 9 * A.T1) + vec) * N2 + j] = o;
 }
 
-// Element k of dipole_double_last(dip, f) in the reference's order
-// (src/NISE_subs.c:552-575): sum over i < k of dip_i f_ik, then over_k f_kk,
-// then dip_j f_kj for j > k.
-__device__ inline float2 ddl_elem(const float *dip, const float *over, const float2 *f, int N, int k) {
-  float r = 0.f, q = 0.f;
-  for (int i = 0; i < k; ++i) {
-    const float2 v = f[sindex(i, k, N)];
-    r = __fadd_rn(r, __fmul_rn(dip[i], v.x)), q = __fadd_rn(q, __fmul_rn(dip[i], v.y));
+// Element k of dipole_double_last(dip, f) (src/NISE_subs.c:552-575) for NIT
+// dipole sets at once, in the reference's order: sum over i < k of
+// dip_i f_ik, then over_k f_kk, then dip_j f_kj for j > k, i.e. one pass over
+// i = 0..N-1 with the weight over_k at i = k.  Each element f is read once
+// for all NIT sets.  f_ik with i <= k sits at k + rs(i), rs(i) = i(2N-i-1)/2,
+// and f_ki with i > k at rs(k) + i (src/NISE_subs.h:54-65).
+template <int NIT>
+__device__ __forceinline__ void ddl_multi(const float2 *f, int N, int k, const float *dip0, const float *dip1,
+                                 const float *dip2, const float ov[NIT], float2 out[NIT]) {
+  const float *dip[3] = {dip0, dip1, dip2};
+  float r[NIT], q[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) r[it] = 0.f, q[it] = 0.f;
+  const int rsk = k * ((N << 1) - k - 1) / 2;
+  int rsi = 0;
+  for (int i = 0; i < N; ++i) {
+    const float2 v = f[(i <= k) ? k + rsi : rsk + i];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      // dip[i] is wave-uniform: read it once into a scalar register (a
+      // convergent read, so it is not turned into a masked per-lane load)
+      const float dv = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(dip[it][i])));
+      const float w = (i == k) ? ov[it] : dv;
+      r[it] = __fadd_rn(r[it], __fmul_rn(w, v.x)), q[it] = __fadd_rn(q[it], __fmul_rn(w, v.y));
+    }
+    rsi += N - i - 1;
   }
-  const float2 d = f[sindex(k, k, N)];
-  r = __fadd_rn(r, __fmul_rn(over[k], d.x)), q = __fadd_rn(q, __fmul_rn(over[k], d.y));
-  for (int j = k + 1; j < N; ++j) {
-    const float2 v = f[sindex(k, j, N)];
-    r = __fadd_rn(r, __fmul_rn(dip[j], v.x)), q = __fadd_rn(q, __fmul_rn(dip[j], v.y));
-  }
-  return make_float2(r, q);
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) out[it] = make_float2(r[it], q[it]);
 }
 
-// EA terms at one t3 (src/calc_2DIR.c:448-488), one workgroup per
-// (s, x2, x3), one thread per t1; adds into E.
-__global__ void ea_kernel(TwoDirArgs A, int t3) {
+// dip_x(tl) and the overtone dipoles over_x(tl) for x = 0..2 into LDS
+// (src/calc_2DIR.c:450-454; sqrt(2) mu for a fixed anharmonicity,
+// src/NISE_subs.c:556-560, else the overtone records).
+__device__ inline void load_dip_over(const TwoDirArgs &A, int64_t tl, float *dipL, float *ovL) {
+  const int N = A.N;
+  for (int e = threadIdx.x; e < 3 * N; e += blockDim.x) {
+    const int x = e / N, i = e % N;
+    const float m = mu_at(A, tl, x)[i];
+    dipL[e] = m;
+    ovL[e] = (A.anh != 0.f) ? __fmul_rn(kSqrt2, m) : A.over_rec[tl * (1 + 3 * (int64_t)N) + 1 + (int64_t)x * N + i];
+  }
+}
+
+// leftrr_EA at one t3 for every (s, x1, x2, x3): dipole_double_last of
+// fr[x1][x2] with the x3 dipoles of frame tl (src/calc_2DIR.c:455-456).
+// One workgroup per (s, x1 x2); the doubles vector is staged in LDS when
+// kStage.  lrr[s][x1 x2][x3][N].
+template <bool kStage>
+__global__ void lrr_kernel(TwoDirArgs A, int t3, float2 *lrr) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = A.N, N2 = N * (N + 1) / 2;
-  const int s = blockIdx.x, x2 = blockIdx.y / 3, x3 = blockIdx.y % 3;
-  float *dip = reinterpret_cast<float *>(smem);   // [N]
-  float *over = dip + N;                          // [N]
-  float2 *lrr = reinterpret_cast<float2 *>(over + N + (N & 1));  // [3][N] leftrr_EA per x1
+  const int s = blockIdx.x, v = blockIdx.y;
+  float *dipL = reinterpret_cast<float *>(smem), *ovL = dipL + 3 * N;
+  float2 *fL = reinterpret_cast<float2 *>(ovL + 3 * N);
+  const int64_t tl = tj_of(A, s) + t3;
+  const float2 *fg = A.dbl + ((int64_t)s * (9 + 9 * A.T1) + v) * N2;
+  load_dip_over(A, tl, dipL, ovL);
+  if (kStage)
+    for (int e = threadIdx.x; e < N2; e += blockDim.x) fL[e] = fg[e];
+  __syncthreads();
+  const float2 *f = kStage ? fL : fg;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) {
+    const float ov[3] = {ovL[k], ovL[N + k], ovL[2 * N + k]};
+    float2 o[3];
+    ddl_multi<3>(f, N, k, dipL, dipL + N, dipL + 2 * N, ov, o);
+    float2 *out = lrr + ((int64_t)s * 9 + v) * 3 * N + k;
+    out[0] = o[0], out[N] = o[1], out[2 * N] = o[2];
+  }
+}
+
+// EA terms at one t3 (src/calc_2DIR.c:448-469).  One workgroup per
+// (s, x0 x2, chunk of C t1 values): the C doubles vectors ft[x0][x2][t1] are
+// contiguous and staged in LDS with coalesced loads (kStage), each element
+// is contracted once for the x3 of every polarisation item with this
+// (x0, x2) (NIT = 2 or 3 items), then one thread per (t1, item, part) forms
+// the two ordered dot products of that part and adds them into E.
+template <int NIT, bool kStage>
+__device__ __forceinline__ void ea_block(const TwoDirArgs &A, int t3, int C, int3 mi, const float2 *lrr,
+                         unsigned char *smem) {
+  const int N = A.N, N2 = N * (N + 1) / 2;
+  const int s = blockIdx.z, x0 = blockIdx.y / 3, x2 = blockIdx.y % 3;
+  const int t1_0 = blockIdx.x * C, Cc = min(C, A.T1 - t1_0);
+  float *dipL = reinterpret_cast<float *>(smem), *ovL = dipL + 3 * N;
+  float2 *RnL = reinterpret_cast<float2 *>(ovL + 3 * N);  // [3][N] M[x1](t3)
+  float2 *RL = RnL + 3 * N;                               // [C][N] L[x0][t1](t3)
+  float2 *lrL = RL + (int64_t)C * N;                      // [NIT][N] leftrr_EA of each item
+  float2 *LvL = lrL + NIT * N;                            // [C][NIT][N]
+  float2 *ftL = LvL + (int64_t)C * NIT * N;               // [C][N2]
   const int64_t tk = tj_of(A, s), tl = tk + t3;
-  const float *m4 = mu_at(A, tl, x3);
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
-    dip[i] = m4[i];
-    over[i] = (A.anh != 0.f) ? __fmul_rn(kSqrt2, m4[i])
-                             : A.over_rec[tl * (1 + 3 * (int64_t)N) + 1 + (int64_t)x3 * N + i];
-  }
-  __syncthreads();
-  const float2 *dbl = A.dbl + (int64_t)s * (9 + 9 * A.T1) * N2;
-  for (int e = threadIdx.x; e < 3 * N; e += blockDim.x) {
-    const int x1 = e / N, k = e % N;
-    lrr[e] = ddl_elem(dip, over, dbl + (int64_t)(x1 * 3 + x2) * N2, N, k);
-  }
-  __syncthreads();
-  for (int t1 = threadIdx.x; t1 < A.T1; t1 += blockDim.x) {
-    for (int m = 0; m < 21; ++m) {
-      if (c_polar[m][2] != x2 || c_polar[m][3] != x3) continue;
-      const int x0 = c_polar[m][0], x1 = c_polar[m][1];
-      const float2 *R = A.hist + (hrow_L(A, s, x0, t1) + t3) * N;  // conj -> rightrr[t1]
-      const float2 *Rn = A.hist + (hrow_M(A, s, x1) + t3) * N;     // conj -> rightnr
-      const float2 *ft = dbl + (int64_t)(9 + (x0 * 3 + x2) * A.T1 + t1) * N2;
-      const float2 *lr = lrr + x1 * N;
-      float rrI = 0.f, riI = 0.f, rrII = 0.f, riII = 0.f;
-      for (int i = 0; i < N; ++i) {
-        const float Rr = R[i].x, Ri = -R[i].y, Nr = Rn[i].x, Ni = -Rn[i].y;
-        const float2 Lv = ddl_elem(dip, over, ft, N, i);
-        rrI = __fadd_rn(rrI, __fadd_rn(__fmul_rn(lr[i].y, Rr), __fmul_rn(lr[i].x, Ri)));
-        riI = __fadd_rn(riI, __fsub_rn(__fmul_rn(lr[i].x, Rr), __fmul_rn(Ri, lr[i].y)));
-        rrII = __fadd_rn(rrII, __fadd_rn(__fmul_rn(Nr, Lv.y), __fmul_rn(Ni, Lv.x)));
-        riII = __fadd_rn(riII, __fsub_rn(__fmul_rn(Nr, Lv.x), __fmul_rn(Ni, Lv.y)));
+  const float2 *ftg = A.dbl + ((int64_t)s * (9 + 9 * A.T1) + 9 + (x0 * 3 + x2) * A.T1 + t1_0) * N2;
+  const int x3a = c_polar[mi.x][3], x3b = c_polar[mi.y][3], x3c = NIT > 2 ? c_polar[mi.z][3] : 0;
+  // dipoles first (into registers when 3N <= 2 blockDim, stored after the
+  // main pass), then one pass over the float2 words [M rows | t1 rows |
+  // items' leftrr rows | doubles vectors (kStage)]
+  const float *dg = mu_at(A, tl, 0);  // 3N contiguous: x N + i
+  const float *og = A.over_rec + tl * (1 + 3 * (int64_t)N) + 1;
+  const bool dreg = 3 * N <= 2 * (int)blockDim.x, ovr = A.anh == 0.f;
+  float dv[2] = {0.f, 0.f}, ovv[2] = {0.f, 0.f};
+  if (dreg) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = threadIdx.x + u * blockDim.x;
+      if (e < 3 * N) {
+        dv[u] = gld(dg + e);
+        if (ovr) ovv[u] = gld(og + e);
       }
-      float *E = A.E + (((int64_t)s * 21 + m) * 4) * A.T3 * A.T1 + (int64_t)t3 * A.T1 + t1;
-      const int64_t plane = (int64_t)A.T3 * A.T1;
-      E[0] += rrI, E[plane] += riI, E[2 * plane] += rrII, E[3 * plane] += riII;
     }
   }
+  const float2 *Mrow = A.hist + (hrow_M(A, s, 0) + t3) * N;  // M[x] rows are T3 N apart
+  const float2 *Lrow = A.hist + (hrow_L(A, s, x0, t1_0) + t3) * N;  // L[x0][t1] rows likewise
+  const int64_t rstride = (int64_t)A.T3 * N;
+  const float2 *lrr_s = lrr + (int64_t)s * 27 * N;
+  const int lo0 = ((c_polar[mi.x][1] * 3 + x2) * 3 + x3a) * N, lo1 = ((c_polar[mi.y][1] * 3 + x2) * 3 + x3b) * N;
+  const int lo2 = NIT > 2 ? ((c_polar[mi.z][1] * 3 + x2) * 3 + x3c) * N : 0;
+  const int nsm = (3 + Cc + NIT) * N, fofs = (int)(ftL - RnL);
+  gather_lds<16>(RnL, nsm + (kStage ? Cc * N2 : 0), [=](int e) {
+    if (e >= nsm) return ftg + (e - nsm);
+    const int r = e / N, i = e % N;
+    if (r < 3) return Mrow + r * rstride + i;
+    if (r < 3 + Cc) return Lrow + (r - 3) * rstride + i;
+    const int it = r - 3 - Cc;
+    return lrr_s + lo0 + (it >= 1) * (lo1 - lo0) + (it >= 2) * (lo2 - lo1) + i;
+  }, [=](int e) {
+    if (e >= nsm) return fofs + (e - nsm);  // doubles vectors, contiguous
+    return e < (3 + Cc) * N ? e : e + (C - Cc) * N;
+  });
+  if (dreg) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = threadIdx.x + u * blockDim.x;
+      if (e < 3 * N) {
+        dipL[e] = dv[u];
+        if (ovr) ovL[e] = ovv[u];
+      }
+    }
+  } else {
+    stage_lds<4>(dipL, dg, 3 * N);
+    if (ovr) stage_lds<4>(ovL, og, 3 * N);
+  }
+  __syncthreads();
+  const int x3v[3] = {x3a, x3b, x3c};
+  for (int u = threadIdx.x; u < Cc * N; u += blockDim.x) {
+    const int c = u / N, k = u % N;
+    float ov[NIT];
+    float2 o[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int e = x3v[it] * N + k;
+      ov[it] = (A.anh != 0.f) ? __fmul_rn(kSqrt2, dipL[e]) : ovL[e];
+    }
+    const float2 *f = kStage ? ftL + c * N2 : ftg + (int64_t)c * N2;
+    ddl_multi<NIT>(f, N, k, dipL + x3a * N, dipL + x3b * N, dipL + x3c * N, ov, o);
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) LvL[(c * NIT + it) * N + k] = o[it];
+  }
+  __syncthreads();
+  const int64_t plane = (int64_t)A.T3 * A.T1;
+  for (int j = threadIdx.x; j < 2 * NIT * Cc; j += blockDim.x) {
+    const int part = j & 1, it = (j >> 1) % NIT, c = (j >> 1) / NIT, t1 = t1_0 + c;
+    const int m = it == 0 ? mi.x : it == 1 ? mi.y : mi.z, x1 = c_polar[m][1];
+    float rr = 0.f, ri = 0.f;
+    if (part == 0) {  // rightrr = conj(L[x0][t1](t3)) against leftrr_EA (:465-466)
+      const float2 *R = RL + c * N, *lr = lrL + it * N;
+      for (int i = 0; i < N; ++i) {
+        const float2 l = lr[i], Rv = R[i];
+        const float Rr = Rv.x, Ri = -Rv.y;
+        rr = __fadd_rn(rr, __fadd_rn(__fmul_rn(l.y, Rr), __fmul_rn(l.x, Ri)));
+        ri = __fadd_rn(ri, __fsub_rn(__fmul_rn(l.x, Rr), __fmul_rn(Ri, l.y)));
+      }
+    } else {  // rightnr = conj(M[x1](t3)) against leftnr_EA[t1] (:457-459, :468-469)
+      const float2 *Rn = RnL + x1 * N, *Lv = LvL + (c * NIT + it) * N;
+      for (int i = 0; i < N; ++i) {
+        const float2 l = Lv[i], Rv = Rn[i];
+        const float Nr = Rv.x, Ni = -Rv.y;
+        rr = __fadd_rn(rr, __fadd_rn(__fmul_rn(Nr, l.y), __fmul_rn(Ni, l.x)));
+        ri = __fadd_rn(ri, __fsub_rn(__fmul_rn(Nr, l.x), __fmul_rn(Ni, l.y)));
+      }
+    }
+    float *E = A.E + (((int64_t)s * 21 + m) * 4 + 2 * part) * plane + (int64_t)t3 * A.T1 + t1;
+    E[0] += rr, E[plane] += ri;
+  }
+}
+
+template <bool kStage>
+__global__ void ea_kernel(TwoDirArgs A, int t3, int C, const float2 *lrr) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int x0 = blockIdx.y / 3, x2 = blockIdx.y % 3;
+  int3 mi = make_int3(0, 0, 0);
+  int n = 0;
+  for (int m = 0; m < 21; ++m)
+    if (c_polar[m][0] == x0 && c_polar[m][2] == x2) {
+      if (n == 0) mi.x = m;
+      else if (n == 1) mi.y = m;
+      else mi.z = m;
+      ++n;
+    }
+  if (n == 3) ea_block<3, kStage>(A, t3, C, mi, lrr, smem);
+  else ea_block<2, kStage>(A, t3, C, mi, lrr, smem);
 }
 
 // R[k][t3][t1] += sum_s sum_m E[s][m][q][t3][t1] * (w(p, m) lt(t3, t1)),
@@ -290,6 +426,28 @@ int upl(DevBuf &d, const void *s, size_t n) {
   return NISE_OK;
 }
 bool tech_is(const t_non *n, const char *t) { return strcmp(n->technique, t) == 0; }
+
+// EA launch shape: doubles vectors are staged in LDS when one fits in
+// 96 KB; C vectors per workgroup (at most 8), about 40 KB of them.
+struct EaShape {
+  bool stage;
+  int C, nchunk, block, lblock;
+  size_t smem, lsmem;
+};
+EaShape ea_shape(int N, int T1) {
+  EaShape e;
+  const int N2 = N * (N + 1) / 2;
+  e.stage = (size_t)N2 * 8 <= 96 * 1024;
+  e.C = e.stage ? std::max(1, std::min({8, T1, 40960 / (N2 * 8)})) : 1;
+  e.nchunk = (T1 + e.C - 1) / e.C;
+  e.block = std::min(256, std::max(64, (e.C * N + 63) / 64 * 64));
+  e.lblock = std::min(256, std::max(64, (N + 63) / 64 * 64));
+  // dip, over, M rows, items' leftrr rows; per t1: its L row, contractions
+  const size_t fixed = (size_t)6 * N * 4 + (size_t)6 * N * 8;
+  e.smem = fixed + (size_t)e.C * 4 * N * 8 + (e.stage ? (size_t)e.C * N2 * 8 : 0);
+  e.lsmem = (size_t)6 * N * 4 + (e.stage ? (size_t)N2 * 8 : 0);
+  return e;
+}
 }  // namespace
 
 extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H_stride,
@@ -411,6 +569,8 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
     }
   }
   if (!vp) return set_error("2DIR: N=%d does not fit the conveyor's LDS image", N), NISE_ERR_ARG;
+  if (do_ea && std::max(ea_shape(N, T1).smem, ea_shape(N, T1).lsmem) > 160 * 1024)
+    return set_error("2DIR: N=%d does not fit the EA kernel's LDS", N), NISE_ERR_ARG;
   if ((rc = upl(stp, steps.data(), steps.size() * 8))) return rc;
 
   // response weights: polarweight x lifetime factor (src/calc_2DIR.c:105-115)
@@ -433,10 +593,11 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
   const double per_sample = 8.0 * 3 * T1 * N + 8.0 * (3 * T1 + 3) * T3 * N +
                             (do_ea ? 8.0 * nvec * N2 : 0.0) + 4.0 * 84 * plane;
   const int chunk = (int)std::max(1.0, std::min<double>({(double)total, 1024.0, 8.0e9 / per_sample}));
-  const int vw = 64 / doubles_np(N);
+  const int vw = doubles_vw(N);
   const int gps = (nvec + vw - 1) / vw;  // doubles groups per sample
 
-  DevBuf L0, hist, dbl, E, slots, wgb, vb, rows;
+  DevBuf L0, hist, dbl, E, slots, wgb, vb, rows, lrrb;
+  const EaShape ea = ea_shape(N, T1);
   ConveyorArgs a;
   a.N = N, a.K = sch.K, a.nlayers = sch.nlayers(), a.m = non->ts, a.f_lo = f_lo;
   a.ab = ab.as<uint32_t>(), a.layer_off = lay.as<int>(), a.diag = diag.as<float2>(), a.cs = cs.as<float2>();
@@ -514,13 +675,27 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
           for (int g = 0; g < gps; ++g)
             rowh[((size_t)t3 * S + s) * gps + g] = (int64_t)(begin + c0 + s) * non->sample + T1 + t3 - f_lo;
       if ((rc = upl(vb, vbh.data(), vbh.size() * 4)) || (rc = upl(rows, rowh.data(), rowh.size() * 8))) return rc;
+      if ((rc = lrrb.alloc((size_t)S * 27 * N * 8))) return rc;
+      const void *fl = ea.stage ? (const void *)lrr_kernel<true> : (const void *)lrr_kernel<false>;
+      const void *fe = ea.stage ? (const void *)ea_kernel<true> : (const void *)ea_kernel<false>;
+      NISE_HIP_CHECK(hipFuncSetAttribute(fl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ea.lsmem));
+      NISE_HIP_CHECK(hipFuncSetAttribute(fe, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ea.smem));
       DoublesArgs d;
       d.N = N, d.K2 = (int)lex.size(), d.m = non->ts, d.ab2 = ab2.as<uint32_t>();
       d.D2 = D2.as<float2>(), d.cs2 = cs2.as<float4>(), d.state = dbl.as<float2>();
       d.vec_begin = vb.as<int>();
-      const size_t ea_smem = (size_t)(2 * N + (N & 1)) * 4 + (size_t)3 * N * 8;
       for (int t3 = 0; t3 < T3; ++t3) {
-        hipLaunchKernelGGL(ea_kernel, dim3(S, 9), dim3(64), ea_smem, st, A, t3);
+        if (ea.stage) {
+          hipLaunchKernelGGL(lrr_kernel<true>, dim3(S, 9), dim3(ea.lblock), ea.lsmem, st, A, t3, lrrb.as<float2>());
+          NISE_HIP_CHECK(hipGetLastError());
+          hipLaunchKernelGGL(ea_kernel<true>, dim3(ea.nchunk, 9, S), dim3(ea.block), ea.smem, st, A, t3, ea.C,
+                             (const float2 *)lrrb.as<float2>());
+        } else {
+          hipLaunchKernelGGL(lrr_kernel<false>, dim3(S, 9), dim3(ea.lblock), ea.lsmem, st, A, t3, lrrb.as<float2>());
+          NISE_HIP_CHECK(hipGetLastError());
+          hipLaunchKernelGGL(ea_kernel<false>, dim3(ea.nchunk, 9, S), dim3(ea.block), ea.smem, st, A, t3, ea.C,
+                             (const float2 *)lrrb.as<float2>());
+        }
         NISE_HIP_CHECK(hipGetLastError());
         if (t3 + 1 < T3) {  // the state after the last t3 is not used
           d.rows = rows.as<int64_t>() + (size_t)t3 * S * gps;

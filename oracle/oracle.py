@@ -152,11 +152,15 @@ def fft1d(non, re_S, im_S, samples):
     return fr[:k], sr[:k], si[:k]
 
 
-def twodir(non, H, mu, C_rec=None, A_rec=None, over_rec=None):
+def twodir(non, H, mu, C_rec=None, A_rec=None, over_rec=None, acc64=False):
     """Oracle 2DIR: the 12 response planes (12, tmax3, tmax1) float32 summed
     over samples begin..end-1 and the 21 polarisation items in the
     reference's work order (src/MPI_subs.c:74-95), un-normalised.  Sets
-    non.shifte = (min1 + max1) / 2 as src/calc_2DIR.c:79-81 does."""
+    non.shifte = (min1 + max1) / 2 as src/calc_2DIR.c:79-81 does.
+
+    acc64: each item's contribution is formed in FP32 as the reference does
+    and the items are summed in FP64 (float64 planes), which removes the
+    rounding of the reference's FP32 running sums over items."""
     H = np.ascontiguousarray(H, np.float32)
     mu = np.ascontiguousarray(mu, np.float32)
     Cr = None if C_rec is None else np.ascontiguousarray(C_rec, np.float32)
@@ -164,15 +168,20 @@ def twodir(non, H, mu, C_rec=None, A_rec=None, over_rec=None):
     Or = None if over_rec is None else np.ascontiguousarray(over_rec, np.float32)
     non.shifte = np.float32((np.float32(non.max1) + np.float32(non.min1)) / np.float32(2))
     R = np.zeros((12, non.tmax3, non.tmax1), np.float32)
+    R64 = np.zeros(R.shape, np.float64) if acc64 else None
     end = non.end
     if end == 0:
         end = (H.shape[0] - non.tmax1 - non.tmax2 - non.tmax3 - 1) // non.sample + 1
     for s in range(non.begin, end):
         for m in range(21):
+            if acc64:
+                R[...] = 0
             rc = oracle().or_2dir_item(C.byref(non), ptr(H), H.shape[1], ptr(mu), mu.shape[1],
                                        ptr(Cr), ptr(Ar), ptr(Or), s, m, R)
             assert rc == 0
-    return R
+            if acc64:
+                R64 += R
+    return R64 if acc64 else R
 
 
 def absorption_param_f64(non, H, mu, delta=None, cscale=1.0, C_rec=None, nthreads=1):

@@ -102,6 +102,37 @@ def test_larger_system_vs_oracle(orc):
     assert rel(R, ref) < TOL
 
 
+def test_partial_t1_chunk_vs_oracle(orc):
+    """N = 32, T1 = 13: the EA kernel stages 8 doubles vectors per workgroup,
+    so the second t1 chunk is partial."""
+    say("N=32 T1=13 partial EA chunk")
+    N, T1, T3 = 32, 13, 5
+    H, mu, anh, over = synth.twodir_system(N=N, nframes=60, seed=49)
+    non = mk(N, T1, T3, 8, 0, 3, H, lifetime=300.0)
+    ref = orc.twodir(non, H, mu)
+    non = mk(N, T1, T3, 8, 0, 3, H, lifetime=300.0)
+    R = abi.twodir_batched(non, H, mu)
+    assert rel(R, ref) < TOL
+
+
+def test_wide_system_unstaged_vs_oracle(orc):
+    """N = 160: a doubles vector (12880 states) exceeds the EA kernel's LDS
+    staging budget, so it reads the vectors from device memory.  The
+    responses grow with N and the imaginary planes are small differences of
+    them: the reference's FP32 running sums over items quantise those planes
+    at about 2^-15 (the FP32 and FP64 sums of the same per-item planes differ
+    by 7e-5 relative L2), so the oracle sums its per-item FP32 planes in FP64
+    here."""
+    say("N=160 unstaged EA")
+    N, T1, T3 = 160, 3, 3
+    H, mu, anh, over = synth.twodir_system(N=N, nframes=30, seed=48)
+    non = mk(N, T1, T3, 9, 0, 2, H)
+    ref = orc.twodir(non, H, mu, acc64=True)
+    non = mk(N, T1, T3, 9, 0, 2, H)
+    R = abi.twodir_batched(non, H, mu)
+    assert rel(R, ref) < TOL
+
+
 def test_device_records_match_host():
     import torch
     say("device records")
