@@ -334,6 +334,30 @@ typedef struct {
 int nise_absorption_multi(const nise_system *systems, int nsys, int where,
                           float *re_S, float *im_S, double *done_ms);
 
+/* Replaces the 2DFFT post-processing program (src/2DFFT.c:20-830, a
+ * separate executable in the reference, run on the NISE input file) in
+ * process.  Reads the response files calc_2DIR writes in the current
+ * directory (R{par,per,cro}{I,II}.dat; R*_<t>fs.dat for each waiting time
+ * listed in Waitingtime.dat; R*.IRraman.dat for the 2DIRraman techniques),
+ * applies the apodisation (homogen, inhomogen, window), the edge halving and
+ * the zero padding to fft x fft, transforms every response on the GPU
+ * (forward 2D DFT in double precision) and writes the same files as the
+ * program: Rw*.I.dat, Rw*.II.dat, 2D.*.dat and PP.*.dat.  The program's
+ * input keywords are the t_non fields fft, deltat, homogen, inhomogen,
+ * window, tmax1, tmax3, min1..max3 and technique; format is its Format
+ * keyword: 0 Dislin / Python, 1 Matlab (kI / kII files and waxis.dat only),
+ * 2 Gnuplot.  Returns NISE_OK, or NISE_ERR_ARG where the program prints a
+ * message and exits (negative or inverted frequency ranges, fft == 0). */
+int nise_2dfft(const t_non *non, int format);
+
+/* The forward 2D transform of nise_2dfft on its own (no counterpart in the
+ * reference): P grids of fft x fft complex doubles (interleaved re, im; row
+ * major; host memory), out[p][k0][k1] = sum in[p][j0][j1]
+ * exp(-2 pi i (j0 k0 + j1 k1) / fft), the sign and layout of FFTW_FORWARD.
+ * Only rows < nrows and columns < ncols of the input may be nonzero. */
+int nise_dft2d(const double *in, int P, int fft, int nrows, int ncols,
+               double *out);
+
 /* Last error message of the calling thread (static storage). */
 const char *nise_last_error(void);
 

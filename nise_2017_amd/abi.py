@@ -129,6 +129,8 @@ def lib():
     L.nise_abs_plan_enable_adjoint.argtypes = [_vp, C.c_int]
     L.nise_abs_plan_adjoint.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double)]
     L.nise_absorption_multi.argtypes = [_vp, C.c_int, C.c_int, _fp, _fp, _vp]
+    L.nise_2dfft.argtypes = [P, C.c_int]
+    L.nise_dft2d.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp]
     L.nise_2dir_batched.argtypes = [P, _vp, C.c_int64, _vp, C.c_int64, C.c_int64, _vp, _vp, _vp,
                                     C.c_int, _fp]
     for name in ("propagate_vec_coupling_S",):
