@@ -124,6 +124,12 @@ void propagate_vec_coupling_S(t_non *non, float *Hamiltonian_i, float *cr,
 void propagate_vec_coupling_S_doubles(t_non *non, float *Hamiltonian_i,
                                       float *cr, float *ci, int m, float *Anh);
 
+/* Replaces src/propagate.c:1051-1181 (prototype src/propagate.h:16), the
+ * two-exciton step of two-level sites used by calc_2DES: no anharmonicity,
+ * no rotations for c == a or c == b. */
+void propagate_vec_coupling_S_doubles_ES(t_non *non, float *Hamiltonian_i,
+                                         float *cr, float *ci, int m);
+
 /* Replaces src/propagate.c:60-125 (prototype src/propagate.h:7). */
 void propagate_vec_DIA(t_non *non, float *Hamiltonian_i, float *cr, float *ci,
                        int sign);
@@ -146,6 +152,15 @@ void propagate_double_2DIR_control(t_non *non, float *Hamil_i_e, float **ft1r,
                                    int currentSample, int molPol, int t3,
                                    float *Anh);
 
+/* Replaces src/propagate.c:1594-1709 (prototype src/propagate.h:26-28) for
+ * non->propagation == 1: as propagate_double_2DIR_control with the
+ * two-level doubles step. */
+void propagate_double_2DES_control(t_non *non, float *Hamil_i_e, float **ft1r,
+                                   float **ft1i, float *fr, float *fi,
+                                   float **rightrr, float **rightri,
+                                   float *rightnr, float *rightni,
+                                   int currentSample, int molPol, int t3);
+
 /* Replaces src/absorption.c:205-212 (prototype src/absorption.h:4). */
 void calc_S1(float *re_S_1, float *im_S_1, int t1, t_non *non, float *cr,
              float *ci, float *mu);
@@ -163,6 +178,12 @@ void do_1DFFT(t_non *non, char fname[], float *re_S_1, float *im_S_1,
  * computes every (sample, molPol) work item on its GPU and writes the six
  * R{par,per,cro}{I,II}.dat files; other ranks return at once. */
 void calc_2DIR(t_non *non, int parentRank, int parentSize, int subRank,
+               int subSize, MPI_Comm subComm, MPI_Comm rootComm);
+
+/* Replaces src/calc_2DES.c:22-586 (prototype src/calc_2DES.h:5), the 2DUVvis
+ * techniques (2DUVvis, GBUVvis, SEUVvis, EAUVvis, noEAUVvis): calc_2DIR for
+ * two-level sites, with the same files and messages. */
+void calc_2DES(t_non *non, int parentRank, int parentSize, int subRank,
                int subSize, MPI_Comm subComm, MPI_Comm rootComm);
 
 /* ------------------------------------------------------------------------ */
@@ -299,7 +320,9 @@ int nise_row_schedule(int N, const uint8_t *mask, int R, uint32_t *ab_out,
  * non->singles, tmax1, tmax3 (tmax2 must be 0), sample, begin, end (0: as
  * many samples as the trajectory holds), ts, deltat, couplingcut, min1 and
  * max1 (shifte = (min1+max1)/2), lifetime, anharmonicity, technique
- * ("2DIR", "GBIR", "SEIR", "EAIR", "noEAIR"), hamiltonian ("Full" or
+ * ("2DIR", "GBIR", "SEIR", "EAIR", "noEAIR"; or the two-level-site forms of
+ * calc_2DES: "2DUVvis", "GBUVvis", "SEUVvis", "EAUVvis", "noEAUVvis", which
+ * take no anharmonicity), hamiltonian ("Full" or
  * "Coupling"); propagation must be 1.  anh_rec (1 + N words per frame) and
  * over_rec (1 + 3N words per frame) are read when anharmonicity == 0 and
  * the technique has the EA pathway.  R12 (host, 12 x tmax3 x tmax1, t1

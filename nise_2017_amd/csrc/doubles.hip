@@ -60,7 +60,8 @@ __global__ void doubles_tables_kernel(FrameSource s, int N, int K2, const uint32
     const float eb = __fsub_rn(raw_diag(s, N, f, b), s.shifte);
     float h0 = __fadd_rn(ea, eb);
     if (a == b) {
-      const float d = (anh_fixed == 0.f) ? anh_rec[f * (1 + (int64_t)N) + 1 + a] : anh_fixed;
+      // no records and no fixed value: two-level sites, no anharmonicity
+      const float d = (anh_fixed == 0.f) ? (anh_rec ? anh_rec[f * (1 + (int64_t)N) + 1 + a] : 0.f) : anh_fixed;
       h0 = __fsub_rn(h0, d);
     }
     const float arg = __fmul_rn(__fmul_rn(0.5f, h0), fac);
@@ -214,7 +215,7 @@ __global__ void __launch_bounds__(64) doubles_step_kernel(DoublesArgs A) {
           // every c except b, ascending; c == a (sqrt(2) angle) also does the
           // c == b rotation (|ab>, |bb>): it shares |ab>, and no rotation of
           // a c in between touches |ab> or |bb>
-          if (c < N && c != b) {
+          if (c < N && c != b && !(A.es && c == a)) {
             const int rs_c = cb == 0 ? rs0 : (c * (2 * N - c - 1)) >> 1;
             const int i1 = (c <= a ? a + rs_c : c + rs_a) * VW + q * VPL;
             const int i2 = (c <= b ? b + rs_c : c + rs_b) * VW + q * VPL;
@@ -296,8 +297,10 @@ int up2(DevBuf &d, const void *s, size_t n) {
 
 // nvec doubles vectors (stride N(N+1)/2) through one frame H (packed,
 // diagonal already shifted), m substeps.
+// es: two-level sites (src/propagate.c:1051-1181): no anharmonicity, no
+// rotations for c == a or c == b; Anh and anharmonicity are not used.
 int doubles_host(int N, const float *H, float *const *cr, float *const *ci, int nvec, int m,
-                 const float *Anh, float anharmonicity, float deltat, float cut) {
+                 const float *Anh, float anharmonicity, float deltat, float cut, int es) {
   int rc = ensure_device();
   if (rc) return rc;
   if (N < 1 || m < 1 || nvec < 1) return set_error("doubles: bad arguments"), NISE_ERR_ARG;
@@ -323,7 +326,8 @@ int doubles_host(int N, const float *H, float *const *cr, float *const *ci, int 
     return rc;
   std::vector<int64_t> rows64(ng, 0);
   if ((rc = up2(c.rows, rows64.data(), rows64.size() * 8))) return rc;
-  if (anharmonicity == 0.f) {
+  if (es) anharmonicity = 0.f, Anh = nullptr;
+  if (anharmonicity == 0.f && Anh) {
     std::vector<float> arec(1 + N);
     arec[0] = 0.f;
     std::memcpy(arec.data() + 1, Anh, N * 4);
@@ -335,13 +339,14 @@ int doubles_host(int N, const float *H, float *const *cr, float *const *ci, int 
   src.H_rec = c.H.as<float>(), src.H_stride = 1 + N2, src.shifte = 0.f, src.cut = cut, src.apply_shift = 0;
   const float fac = trotter_factor(deltat, 1, m);
   if ((rc = launch_doubles_tables(src, N, K2, c.ab.as<uint32_t>(), fac, anharmonicity,
-                                  anharmonicity == 0.f ? c.anh.as<float>() : nullptr, 0, 1,
+                                  (anharmonicity == 0.f && Anh) ? c.anh.as<float>() : nullptr, 0, 1,
                                   c.D2.as<float2>(), c.cs2.as<float4>(), st)))
     return rc;
   DoublesArgs a;
   a.N = N, a.K2 = K2, a.m = m, a.ab2 = c.ab.as<uint32_t>();
   a.D2 = c.D2.as<float2>(), a.cs2 = c.cs2.as<float4>();
   a.state = c.state.as<float2>(), a.vec_begin = c.vb.as<int>(), a.rows = c.rows.as<int64_t>();
+  a.es = es;
   if ((rc = launch_doubles_step(a, ng, st))) return rc;
   NISE_HIP_CHECK(hipMemcpyAsync(c.sbuf.data(), c.state.p, c.sbuf.size() * sizeof(float2),
                                 hipMemcpyDeviceToHost, st));
@@ -366,32 +371,41 @@ extern "C" void propagate_vec_coupling_S_doubles(t_non *non, float *Hamiltonian_
                                                  float *ci, int m, float *Anh) {
   float *r[1] = {cr}, *i[1] = {ci};
   if (doubles_host(non->singles, Hamiltonian_i, r, i, 1, m, Anh, non->anharmonicity, non->deltat,
-                   non->couplingcut)) {
+                   non->couplingcut, 0)) {
     printf("propagate_vec_coupling_S_doubles: GPU propagation failed: %s\n", get_error());
     exit(1);
   }
 }
 
-// src/propagate.c:1476-1591, Coupling scheme (non->propagation == 1): the
-// doubles vectors fr and ft1r[0..tmax1) in one launch, the singles vectors
-// rightnr and rightrr[0..tmax1) with sign -1 in another.
-extern "C" void propagate_double_2DIR_control(t_non *non, float *Hamil_i_e, float **ft1r,
-                                              float **ft1i, float *fr, float *fi, float **rightrr,
-                                              float **rightri, float *rightnr, float *rightni,
-                                              int currentSample, int molPol, int t3, float *Anh) {
-  (void)currentSample, (void)molPol, (void)t3;
+// src/propagate.c:1051-1181 (two-level sites, calc_2DES)
+extern "C" void propagate_vec_coupling_S_doubles_ES(t_non *non, float *Hamiltonian_i, float *cr,
+                                                    float *ci, int m) {
+  float *r[1] = {cr}, *i[1] = {ci};
+  if (doubles_host(non->singles, Hamiltonian_i, r, i, 1, m, nullptr, 0.f, non->deltat,
+                   non->couplingcut, 1)) {
+    printf("propagate_vec_coupling_S_doubles_ES: GPU propagation failed: %s\n", get_error());
+    exit(1);
+  }
+}
+
+namespace {
+// The Coupling scheme of the doubles control routines: the doubles vectors
+// fr and ft1r[0..tmax1) in one launch, the singles vectors rightnr and
+// rightrr[0..tmax1) with sign -1 in another.
+void doubles_control(const char *who, t_non *non, float *Hamil_i_e, float **ft1r, float **ft1i, float *fr,
+                     float *fi, float **rightrr, float **rightri, float *rightnr, float *rightni, float *Anh,
+                     int es) {
   if (non->propagation != 1) {
-    printf("propagate_double_2DIR_control: propagation scheme %d is not provided by the GPU "
-           "library\n", non->propagation);
+    printf("%s: propagation scheme %d is not provided by the GPU library\n", who, non->propagation);
     exit(1);
   }
   const int T1 = non->tmax1, N = non->singles;
   std::vector<float *> r(T1 + 1), i(T1 + 1);
   r[0] = fr, i[0] = fi;
   for (int t = 0; t < T1; ++t) r[t + 1] = ft1r[t], i[t + 1] = ft1i[t];
-  if (doubles_host(N, Hamil_i_e, r.data(), i.data(), T1 + 1, non->ts, Anh, non->anharmonicity,
-                   non->deltat, non->couplingcut)) {
-    printf("propagate_double_2DIR_control: GPU propagation failed: %s\n", get_error());
+  if (doubles_host(N, Hamil_i_e, r.data(), i.data(), T1 + 1, non->ts, Anh, non->anharmonicity, non->deltat,
+                   non->couplingcut, es)) {
+    printf("%s: GPU propagation failed: %s\n", who, get_error());
     exit(1);
   }
   std::vector<float> sr((size_t)(T1 + 1) * N), si((size_t)(T1 + 1) * N);
@@ -403,7 +417,7 @@ extern "C" void propagate_double_2DIR_control(t_non *non, float *Hamil_i_e, floa
   }
   if (coupling_S_host(N, Hamil_i_e, sr.data(), si.data(), T1 + 1, non->ts, -1, non->deltat,
                       non->couplingcut)) {
-    printf("propagate_double_2DIR_control: GPU propagation failed: %s\n", get_error());
+    printf("%s: GPU propagation failed: %s\n", who, get_error());
     exit(1);
   }
   std::memcpy(rightnr, sr.data(), N * 4);
@@ -412,4 +426,26 @@ extern "C" void propagate_double_2DIR_control(t_non *non, float *Hamil_i_e, floa
     std::memcpy(rightrr[t], sr.data() + (size_t)(t + 1) * N, N * 4);
     std::memcpy(rightri[t], si.data() + (size_t)(t + 1) * N, N * 4);
   }
+}
+}  // namespace
+
+// src/propagate.c:1476-1591, Coupling scheme (non->propagation == 1)
+extern "C" void propagate_double_2DIR_control(t_non *non, float *Hamil_i_e, float **ft1r,
+                                              float **ft1i, float *fr, float *fi, float **rightrr,
+                                              float **rightri, float *rightnr, float *rightni,
+                                              int currentSample, int molPol, int t3, float *Anh) {
+  (void)currentSample, (void)molPol, (void)t3;
+  doubles_control("propagate_double_2DIR_control", non, Hamil_i_e, ft1r, ft1i, fr, fi, rightrr, rightri,
+                  rightnr, rightni, Anh, 0);
+}
+
+// src/propagate.c:1594-1709, Coupling scheme (non->propagation == 1): the
+// two-level doubles of calc_2DES
+extern "C" void propagate_double_2DES_control(t_non *non, float *Hamil_i_e, float **ft1r,
+                                              float **ft1i, float *fr, float *fi, float **rightrr,
+                                              float **rightri, float *rightnr, float *rightni,
+                                              int currentSample, int molPol, int t3) {
+  (void)currentSample, (void)molPol, (void)t3;
+  doubles_control("propagate_double_2DES_control", non, Hamil_i_e, ft1r, ft1i, fr, fi, rightrr, rightri,
+                  rightnr, rightni, nullptr, 1);
 }

@@ -363,6 +363,8 @@ struct DoublesArgs {
   float2 *state = nullptr;         // [vectors][N2]
   const int *vec_begin = nullptr;  // [groups + 1]
   const int64_t *rows = nullptr;   // [groups] table row of each group
+  int es = 0;                      // two-level sites (src/propagate.c:1051-1181):
+                                   // no rotation for c == a or c == b
 };
 int launch_doubles_tables(const FrameSource &src, int N, int K2, const uint32_t *ab2, float fac,
                           float anh_fixed, const float *anh_rec, int64_t f_lo, int64_t nfr,

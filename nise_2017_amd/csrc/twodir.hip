@@ -50,6 +50,7 @@ struct TwoDirArgs {
   int N, T1, T3, S;         // sites, t1 and t3 points, samples in this chunk
   int s0, sample;           // first sample (absolute), Samplerate
   int do_gb, do_se, do_ea;
+  int es;                   // two-level sites (calc_2DES): no |aa> states, no overtones
   float anh;                // fixed anharmonicity (0: per-frame records)
   const float *mu_rec;      // dipole records (absolute frame index)
   int64_t mu_stride;        // 0: static dipoles
@@ -182,10 +183,11 @@ __global__ void dd_kernel(TwoDirArgs A) {
   while (rem >= N - a) { rem -= N - a; ++a; }
   const int b = a + rem;
   float2 o;
-  if (a == b) {
-    const float ov = (A.anh != 0.f) ? __fmul_rn(kSqrt2, dip[a])
-                                    : A.over_rec[tk * (1 + 3 * (int64_t)N) + 1 + (int64_t)x2 * N + a];
-    o = make_float2(__fmul_rn(ov, c[a].x), __fmul_rn(ov, c[a].y));
+  if (a == b) {  // two-level sites hold no |aa> (src/NISE_subs.c:533-549)
+    const float ov = A.es ? 0.f
+                     : (A.anh != 0.f) ? __fmul_rn(kSqrt2, dip[a])
+                                      : A.over_rec[tk * (1 + 3 * (int64_t)N) + 1 + (int64_t)x2 * N + a];
+    o = A.es ? make_float2(0.f, 0.f) : make_float2(__fmul_rn(ov, c[a].x), __fmul_rn(ov, c[a].y));
   } else {
     o = make_float2(__fadd_rn(__fmul_rn(dip[a], c[b].x), __fmul_rn(dip[b], c[a].x)),
                     __fadd_rn(__fmul_rn(dip[a], c[b].y), __fmul_rn(dip[b], c[a].y)));
@@ -233,7 +235,10 @@ __device__ inline void load_dip_over(const TwoDirArgs &A, int64_t tl, float *dip
     const int x = e / N, i = e % N;
     const float m = mu_at(A, tl, x)[i];
     dipL[e] = m;
-    ovL[e] = (A.anh != 0.f) ? __fmul_rn(kSqrt2, m) : A.over_rec[tl * (1 + 3 * (int64_t)N) + 1 + (int64_t)x * N + i];
+    // two-level sites: no overtone term (src/NISE_subs.c:578-593); over_k
+    // = 0 adds 0 * f_kk = 0 (f_kk stays 0) to the same running sum
+    ovL[e] = A.es ? 0.f
+             : (A.anh != 0.f) ? __fmul_rn(kSqrt2, m) : A.over_rec[tl * (1 + 3 * (int64_t)N) + 1 + (int64_t)x * N + i];
   }
 }
 
@@ -290,7 +295,7 @@ __device__ __forceinline__ void ea_block(const TwoDirArgs &A, int t3, int C, int
   // items' leftrr rows | doubles vectors (kStage)]
   const float *dg = mu_at(A, tl, 0);  // 3N contiguous: x N + i
   const float *og = A.over_rec + tl * (1 + 3 * (int64_t)N) + 1;
-  const bool dreg = 3 * N <= 2 * (int)blockDim.x, ovr = A.anh == 0.f;
+  const bool dreg = 3 * N <= 2 * (int)blockDim.x, ovr = !A.es && A.anh == 0.f;
   float dv[2] = {0.f, 0.f}, ovv[2] = {0.f, 0.f};
   if (dreg) {
 #pragma unroll
@@ -342,7 +347,7 @@ __device__ __forceinline__ void ea_block(const TwoDirArgs &A, int t3, int C, int
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const int e = x3v[it] * N + k;
-      ov[it] = (A.anh != 0.f) ? __fmul_rn(kSqrt2, dipL[e]) : ovL[e];
+      ov[it] = A.es ? 0.f : (A.anh != 0.f) ? __fmul_rn(kSqrt2, dipL[e]) : ovL[e];
     }
     const float2 *f = kStage ? ftL + c * N2 : ftg + (int64_t)c * N2;
     ddl_multi<NIT>(f, N, k, dipL + x3a * N, dipL + x3b * N, dipL + x3c * N, ov, o);
@@ -464,15 +469,22 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
     return set_error("2DIR: only propagation 1 (Coupling) is provided"), NISE_ERR_ARG;
   if (non->tmax2 != 0) return set_error("2DIR: waiting time tmax2 > 0 is not provided"), NISE_ERR_ARG;
   if (non->cluster != -1) return set_error("2DIR: cluster selection is not provided"), NISE_ERR_ARG;
-  const int do_gb = tech_is(non, "GBIR") || tech_is(non, "2DIR") || tech_is(non, "noEAIR");
-  const int do_se = tech_is(non, "SEIR") || tech_is(non, "2DIR") || tech_is(non, "noEAIR");
-  const int do_ea = tech_is(non, "EAIR") || tech_is(non, "2DIR");
+  // calc_2DES (src/calc_2DES.c:22-586) is this computation for two-level
+  // sites: the UVvis techniques, two-exciton states without |aa>
+  const int es = tech_is(non, "GBUVvis") || tech_is(non, "SEUVvis") || tech_is(non, "EAUVvis") ||
+                 tech_is(non, "2DUVvis") || tech_is(non, "noEAUVvis");
+  const int do_gb = tech_is(non, "GBIR") || tech_is(non, "2DIR") || tech_is(non, "noEAIR") ||
+                    tech_is(non, "GBUVvis") || tech_is(non, "2DUVvis") || tech_is(non, "noEAUVvis");
+  const int do_se = tech_is(non, "SEIR") || tech_is(non, "2DIR") || tech_is(non, "noEAIR") ||
+                    tech_is(non, "SEUVvis") || tech_is(non, "2DUVvis") || tech_is(non, "noEAUVvis");
+  const int do_ea = tech_is(non, "EAIR") || tech_is(non, "2DIR") || tech_is(non, "EAUVvis") ||
+                    tech_is(non, "2DUVvis");
   if (!do_gb && !do_se && !do_ea) return set_error("2DIR: unknown technique %s", non->technique), NISE_ERR_ARG;
   const bool ctype = strcmp(non->hamiltonian, "Coupling") == 0;
   if (!ctype && strcmp(non->hamiltonian, "Full") != 0)
     return set_error("2DIR: HamiltonianType %s is not provided", non->hamiltonian), NISE_ERR_ARG;
   if (ctype && !coupling_rec) return set_error("2DIR: Coupling type needs coupling_rec"), NISE_ERR_ARG;
-  if (non->anharmonicity == 0.f && do_ea && (!anh_rec || !over_rec))
+  if (!es && non->anharmonicity == 0.f && do_ea && (!anh_rec || !over_rec))
     return set_error("2DIR: anharmonicity 0 needs anharmonicity and overtone records"), NISE_ERR_ARG;
   const int begin = non->begin;
   int end = non->end;
@@ -486,7 +498,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
   const int64_t nfr = f_hi - f_lo;
   hipStream_t st = default_stream();
   const int N2 = N * (N + 1) / 2;
-  const bool anh_file = do_ea && non->anharmonicity == 0.f;
+  const bool anh_file = do_ea && !es && non->anharmonicity == 0.f;
 
   // inputs on the device (host records are uploaded for [f_lo, f_hi) and
   // indexed by absolute frame through a shifted base pointer)
@@ -553,8 +565,8 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
         (rc = cs2.alloc((size_t)nfr * std::max((int)lex.size(), 1) * 16)))
       return rc;
     if ((rc = launch_doubles_tables(src, N, (int)lex.size(), ab2.as<uint32_t>(), fac,
-                                    non->anharmonicity, Ad, f_lo, nfr, D2.as<float2>(),
-                                    cs2.as<float4>(), st)))
+                                    es ? 0.f : non->anharmonicity, es ? nullptr : Ad, f_lo, nfr,
+                                    D2.as<float2>(), cs2.as<float4>(), st)))
       return rc;
   }
   // conveyor shape: the largest slot count whose LDS image fits
@@ -608,7 +620,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
     const int S = std::min(chunk, total - c0);
     TwoDirArgs A;
     A.N = N, A.T1 = T1, A.T3 = T3, A.S = S, A.s0 = begin + c0, A.sample = non->sample;
-    A.do_gb = do_gb, A.do_se = do_se, A.do_ea = do_ea, A.anh = non->anharmonicity;
+    A.do_gb = do_gb, A.do_se = do_se, A.do_ea = do_ea, A.es = es, A.anh = non->anharmonicity;
     A.mu_rec = mud, A.mu_stride = ctype ? 0 : mu_stride, A.over_rec = Od;
     if ((rc = L0.alloc((size_t)S * 3 * T1 * N * 8)) ||
         (rc = hist.alloc((size_t)S * (3 * T1 + 3) * T3 * N * 8)) ||
@@ -682,7 +694,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
       NISE_HIP_CHECK(hipFuncSetAttribute(fe, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ea.smem));
       DoublesArgs d;
       d.N = N, d.K2 = (int)lex.size(), d.m = non->ts, d.ab2 = ab2.as<uint32_t>();
-      d.D2 = D2.as<float2>(), d.cs2 = cs2.as<float4>(), d.state = dbl.as<float2>();
+      d.D2 = D2.as<float2>(), d.cs2 = cs2.as<float4>(), d.state = dbl.as<float2>(), d.es = es;
       d.vec_begin = vb.as<int>();
       for (int t3 = 0; t3 < T3; ++t3) {
         if (ea.stage) {
@@ -745,13 +757,17 @@ void print2D_file(const char *name, float *R, float *I, const t_non *non, int sa
 }
 }  // namespace
 
-// src/calc_2DIR.c:22-629.  Rank 0 runs every work item on its GPU; other
-// ranks have nothing to add to the sums and return.
-extern "C" void calc_2DIR(t_non *non, int parentRank, int parentSize, int subRank, int subSize,
-                          MPI_Comm subComm, MPI_Comm rootComm) {
-  (void)parentSize, (void)subRank, (void)subSize, (void)subComm, (void)rootComm;
+namespace {
+// The file flow of src/calc_2DIR.c:22-629 and src/calc_2DES.c:22-586 (the
+// same program for anharmonic oscillators and for two-level sites): rank 0
+// runs every work item on its GPU; other ranks have nothing to add to the
+// sums and return.
+void run_2d_files(t_non *non, int parentRank, const char *name, bool es) {
   if (parentRank != 0) return;
-  if (non->cluster != -1) die2d(1, "calc_2DIR: the GPU path does not implement Cluster\n");
+  if (non->cluster != -1) {
+    printf("%s: the GPU path does not implement Cluster\n", name);
+    die2d(1, "");
+  }
   const int N = non->singles;
   const int totalSampleCount = (non->length - non->tmax1 - non->tmax2 - non->tmax3 - 1) / non->sample + 1;
   if (totalSampleCount > 0) printf("Making %d samples!\n", totalSampleCount);
@@ -770,7 +786,10 @@ extern "C" void calc_2DIR(t_non *non, int parentRank, int parentSize, int subRan
   const int64_t mw = 1 + 3 * (int64_t)N + 3 * (int64_t)N * D;
   const int64_t f_lo = (int64_t)non->begin * non->sample;
   const int64_t f_hi = (int64_t)(non->end - 1) * non->sample + non->tmax1 + non->tmax2 + non->tmax3;
-  if (non->end <= non->begin) die2d(1, "calc_2DIR: no samples between Begin and End\n");
+  if (non->end <= non->begin) {
+    printf("%s: no samples between Begin and End\n", name);
+    die2d(1, "");
+  }
   const int64_t nfr = f_hi - f_lo;
   FILE *probe = fopen(non->energyFName, "rb");
   if (!probe) die2d(1, "Hamiltonian file not found!\n");
@@ -781,7 +800,8 @@ extern "C" void calc_2DIR(t_non *non, int parentRank, int parentSize, int subRan
     die2d(1, "");
   }
   fclose(probe);
-  const bool anh_file = non->anharmonicity == 0;  // src/calc_2DIR.c:132-146
+  // src/calc_2DIR.c:132-146; calc_2DES reads no anharmonicities
+  const bool anh_file = !es && non->anharmonicity == 0;
   if (anh_file) {
     if (!(probe = fopen(non->anharFName, "rb"))) {
       printf("Anharmonicity file %s not found!\n", non->anharFName);
@@ -795,8 +815,10 @@ extern "C" void calc_2DIR(t_non *non, int parentRank, int parentSize, int subRan
     fclose(probe);
   }
   Pinned H, mu, Cc, Ah, Oh;
-  if (!H.alloc((size_t)(nfr * hw)) || !mu.alloc((size_t)((ctype ? 1 : nfr) * mw)))
-    die2d(1, "calc_2DIR: could not allocate pinned host memory\n");
+  if (!H.alloc((size_t)(nfr * hw)) || !mu.alloc((size_t)((ctype ? 1 : nfr) * mw))) {
+    printf("%s: could not allocate pinned host memory\n", name);
+    die2d(1, "");
+  }
   if (!read_records(non->energyFName, hw, f_lo, nfr, H.p))
     die2d(1, "Hamiltonian trajectory file to short, could not fill buffer!!!\n");
   if (!read_records(non->dipoleFName, mw, ctype ? 0 : f_lo, ctype ? 1 : nfr, mu.p))
@@ -809,8 +831,10 @@ extern "C" void calc_2DIR(t_non *non, int parentRank, int parentSize, int subRan
       die2d(1, "Coupling trajectory file to short, could not fill buffer!!!\n");
   }
   if (anh_file && ea) {
-    if (!Ah.alloc((size_t)(nfr * (1 + N))) || !Oh.alloc((size_t)(nfr * (1 + 3 * N))))
-      die2d(1, "calc_2DIR: could not allocate pinned host memory\n");
+    if (!Ah.alloc((size_t)(nfr * (1 + N))) || !Oh.alloc((size_t)(nfr * (1 + 3 * N)))) {
+      printf("%s: could not allocate pinned host memory\n", name);
+      die2d(1, "");
+    }
     if (!read_records(non->anharFName, 1 + N, f_lo, nfr, Ah.p) ||
         !read_records(non->overdipFName, 1 + 3 * (int64_t)N, f_lo, nfr, Oh.p))
       die2d(1, "Anharmonicity or overtone dipole trajectory file to short, could not fill buffer!!!\n");
@@ -822,7 +846,7 @@ extern "C" void calc_2DIR(t_non *non, int parentRank, int parentSize, int subRan
   const int rc = nise_2dir_batched(&loc, H.p, hw, mu.p, mw, nfr, ctype ? Cc.p : nullptr,
                                    Ah.p, Oh.p, NISE_HOST, R.data());
   if (rc) {
-    printf("calc_2DIR: GPU run failed: %s\n", nise_last_error());
+    printf("%s: GPU run failed: %s\n", name, nise_last_error());
     die2d(1, "");
   }
   printf("Samples %d\n", sampleCount);
@@ -833,6 +857,21 @@ extern "C" void calc_2DIR(t_non *non, int parentRank, int parentSize, int subRan
       print2D_file(names[2 * p + k], R.data() + (4 * p + 2 * k) * pl, R.data() + (4 * p + 2 * k + 1) * pl, non,
                    sampleCount);
   printf("----------------------------------------\n");
-  printf(" 2DIR calculation succesfully completed\n");
+  printf(" %s calculation succesfully completed\n", es ? "2DES" : "2DIR");
   printf("----------------------------------------\n\n");
+}
+}  // namespace
+
+// src/calc_2DIR.c:22-629
+extern "C" void calc_2DIR(t_non *non, int parentRank, int parentSize, int subRank, int subSize,
+                          MPI_Comm subComm, MPI_Comm rootComm) {
+  (void)parentSize, (void)subRank, (void)subSize, (void)subComm, (void)rootComm;
+  run_2d_files(non, parentRank, "calc_2DIR", false);
+}
+
+// src/calc_2DES.c:22-586 (2DUVvis and its pathway techniques)
+extern "C" void calc_2DES(t_non *non, int parentRank, int parentSize, int subRank, int subSize,
+                          MPI_Comm subComm, MPI_Comm rootComm) {
+  (void)parentSize, (void)subRank, (void)subSize, (void)subComm, (void)rootComm;
+  run_2d_files(non, parentRank, "calc_2DES", true);
 }

@@ -257,6 +257,47 @@ void or_coupling_S_doubles_f32(int N, const float *H, float *cr, float *ci,
   free(re_U), free(im_U), free(ocr), free(oci);
 }
 
+/* Two-level (fermionic) doubles, src/propagate.c:1051-1181: H0_ab = e_a +
+   e_b with no anharmonicity; for each kept coupling (a, b) only the
+   rotations of (|ca>, |cb>) with c != a, b, all with the angle of J.       */
+void or_coupling_S_doubles_ES_f32(int N, const float *H, float *cr, float *ci,
+                                  int m, float deltat, float couplingcut) {
+  int N2 = N * (N + 1) / 2;
+  const float f = deltat * OR_icm2ifs * OR_twoPi / m; /* :1055 */
+  float *re_U = malloc(N2 * sizeof(float)), *im_U = malloc(N2 * sizeof(float));
+  float *ocr = malloc(N2 * sizeof(float)), *oci = malloc(N2 * sizeof(float));
+  for (int a = 0; a < N; a++) /* :1066-1074, :1091-1094 (float libm) */
+    for (int b = a; b < N; b++) {
+      float h0 = H[or_sindex(a, a, N)] + H[or_sindex(b, b, N)];
+      re_U[or_sindex(a, b, N)] = cosf(0.5f * h0 * f);
+      im_U[or_sindex(a, b, N)] = -sinf(0.5f * h0 * f);
+    }
+  for (int i = 0; i < m; i++) {
+    for (int a = 0; a < N2; a++) {
+      ocr[a] = cr[a] * re_U[a] - ci[a] * im_U[a];
+      oci[a] = cr[a] * im_U[a] + ci[a] * re_U[a];
+    }
+    for (int a = 0; a < N; a++) /* kept couplings in lexicographic order */
+      for (int b = a + 1; b < N; b++) {
+        float Hab = H[b + a * ((N << 1) - a - 1) / 2];
+        if (!(fabsf(Hab) > couplingcut)) continue;
+        float J = Hab * f;
+        float si = -sinf(J);
+        float co = sqrtf(1 - si * si);
+        for (int c = 0; c < N; c++) {
+          if (c == a || c == b) continue;
+          int i1 = or_sindex(c, a, N), i2 = or_sindex(c, b, N);
+          OR_ROT(float, co, si, i1, i2);
+        }
+      }
+    for (int a = 0; a < N2; a++) {
+      cr[a] = ocr[a] * re_U[a] - oci[a] * im_U[a];
+      ci[a] = ocr[a] * im_U[a] + oci[a] * re_U[a];
+    }
+  }
+  free(re_U), free(im_U), free(ocr), free(oci);
+}
+
 void or_coupling_S_doubles_f64(int N, const float *H, double *cr, double *ci,
                                int m, const float *Anh, float anharmonicity,
                                float deltat, float couplingcut) {
@@ -479,6 +520,34 @@ void or_dipole_double_last(int N, const float *dipole, const float *cr,
       fi[i] += dipole[j] * ci[index];
     }
   }
+}
+
+/* Two-level mu_ef, src/NISE_subs.c:533-549 and :578-593: no |aa> states. */
+void or_dipole_double_ES(int N, const float *dipole, const float *cr,
+                         const float *ci, float *fr, float *fi) {
+  int N2 = N * (N + 1) / 2;
+  for (int i = 0; i < N2; i++) fr[i] = 0, fi[i] = 0;
+  for (int i = 0; i < N; i++)
+    for (int j = i + 1; j < N; j++) {
+      int index = or_sindex(i, j, N);
+      fr[index] += dipole[i] * cr[j];
+      fi[index] += dipole[i] * ci[j];
+      fr[index] += dipole[j] * cr[i];
+      fi[index] += dipole[j] * ci[i];
+    }
+}
+
+void or_dipole_double_last_ES(int N, const float *dipole, const float *cr,
+                              const float *ci, float *fr, float *fi) {
+  for (int i = 0; i < N; i++) fr[i] = 0, fi[i] = 0;
+  for (int i = 0; i < N; i++)
+    for (int j = i + 1; j < N; j++) {
+      int index = or_sindex(i, j, N);
+      fr[j] += dipole[i] * cr[index];
+      fi[j] += dipole[i] * ci[index];
+      fr[i] += dipole[j] * cr[index];
+      fi[i] += dipole[j] * ci[index];
+    }
 }
 
 /* Isotropic weights and molecular-frame indices, src/polar.c:12-105. */
@@ -876,9 +945,16 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
   int nn2 = N * (N + 1) / 2;
   if (non->tmax2 != 0 || non->propagation != 1) return NISE_ERR_ARG;
   const char *tech = non->technique;
-  int do_gb = !strcmp(tech, "GBIR") || !strcmp(tech, "2DIR") || !strcmp(tech, "noEAIR");
-  int do_se = !strcmp(tech, "SEIR") || !strcmp(tech, "2DIR") || !strcmp(tech, "noEAIR");
-  int do_ea = !strcmp(tech, "EAIR") || !strcmp(tech, "2DIR");
+  /* calc_2DES (src/calc_2DES.c:22-586) is calc_2DIR for two-level sites:
+     its techniques end in UVvis and its doubles are the _ES forms */
+  int es = !strcmp(tech, "GBUVvis") || !strcmp(tech, "SEUVvis") || !strcmp(tech, "EAUVvis") ||
+           !strcmp(tech, "2DUVvis") || !strcmp(tech, "noEAUVvis");
+  int do_gb = !strcmp(tech, "GBIR") || !strcmp(tech, "2DIR") || !strcmp(tech, "noEAIR") ||
+              !strcmp(tech, "GBUVvis") || !strcmp(tech, "2DUVvis") || !strcmp(tech, "noEAUVvis");
+  int do_se = !strcmp(tech, "SEIR") || !strcmp(tech, "2DIR") || !strcmp(tech, "noEAIR") ||
+              !strcmp(tech, "SEUVvis") || !strcmp(tech, "2DUVvis") || !strcmp(tech, "noEAUVvis");
+  int do_ea = !strcmp(tech, "EAIR") || !strcmp(tech, "2DIR") || !strcmp(tech, "EAUVvis") ||
+              !strcmp(tech, "2DUVvis");
   size_t plane = (size_t)T3 * T1;
 #define RA(q, t3, t1) R12[(size_t)(q) * plane + (size_t)(t3) * T1 + (t1)]
   float *He = malloc(nn2 * sizeof(float));
@@ -946,7 +1022,12 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
   read_mu(non, mu_rec, mu_stride, tj, px[1], leftrr);
   for (int i = 0; i < N; i++) leftri[i] = 0;
   read_mu(non, mu_rec, mu_stride, tk, px[2], mut3r);
-  if (do_ea) { /* :331-348 */
+  if (do_ea && es) { /* src/calc_2DES.c:318-323 */
+    or_dipole_double_ES(N, mut3r, leftrr, leftri, fr, fi);
+    for (int t1 = 0; t1 < T1; t1++)
+      or_dipole_double_ES(N, mut3r, leftnr + (size_t)t1 * N, leftni + (size_t)t1 * N,
+                          ft1r + (size_t)t1 * nn2, ft1i + (size_t)t1 * nn2);
+  } else if (do_ea) { /* :331-348 */
     if (non->anharmonicity == 0) {
       const float *o = over_rec + (long long)tk * (1 + 3 * N) + 1 + px[2] * N;
       for (int i = 0; i < N; i++) over[i] = o[i];
@@ -955,6 +1036,8 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
     for (int t1 = 0; t1 < T1; t1++)
       or_dipole_double(N, mut3r, leftnr + (size_t)t1 * N, leftni + (size_t)t1 * N,
                        ft1r + (size_t)t1 * nn2, ft1i + (size_t)t1 * nn2, over, non->anharmonicity);
+  }
+  if (do_ea) {
     memcpy(rightrr, leftnr, (size_t)T1 * N * sizeof(float));
     memcpy(rightri, leftni, (size_t)T1 * N * sizeof(float));
     for (size_t i = 0; i < (size_t)T1 * N; i++) rightri[i] = -rightri[i];
@@ -1004,15 +1087,22 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
     for (int t3 = 0; t3 < T3; t3++) {
       int tl = tk + t3;
       read_mu(non, mu_rec, mu_stride, tl, px[3], mut4);
-      if (non->anharmonicity == 0) {
-        const float *o = over_rec + (long long)tl * (1 + 3 * N) + 1 + px[3] * N;
-        for (int i = 0; i < N; i++) over[i] = o[i];
+      if (es) { /* src/calc_2DES.c:428-432 */
+        or_dipole_double_last_ES(N, mut4, fr, fi, leftrr, leftri);
+        for (int t1 = 0; t1 < T1; t1++)
+          or_dipole_double_last_ES(N, mut4, ft1r + (size_t)t1 * nn2, ft1i + (size_t)t1 * nn2,
+                                   leftnr + (size_t)t1 * N, leftni + (size_t)t1 * N);
+      } else {
+        if (non->anharmonicity == 0) {
+          const float *o = over_rec + (long long)tl * (1 + 3 * N) + 1 + px[3] * N;
+          for (int i = 0; i < N; i++) over[i] = o[i];
+        }
+        or_dipole_double_last(N, mut4, fr, fi, leftrr, leftri, over, non->anharmonicity);
+        for (int t1 = 0; t1 < T1; t1++)
+          or_dipole_double_last(N, mut4, ft1r + (size_t)t1 * nn2, ft1i + (size_t)t1 * nn2,
+                                leftnr + (size_t)t1 * N, leftni + (size_t)t1 * N, over,
+                                non->anharmonicity);
       }
-      or_dipole_double_last(N, mut4, fr, fi, leftrr, leftri, over, non->anharmonicity);
-      for (int t1 = 0; t1 < T1; t1++)
-        or_dipole_double_last(N, mut4, ft1r + (size_t)t1 * nn2, ft1i + (size_t)t1 * nn2,
-                              leftnr + (size_t)t1 * N, leftni + (size_t)t1 * N, over,
-                              non->anharmonicity);
       for (int t1 = 0; t1 < T1; t1++) {
         float rrI = 0, riI = 0, rrII = 0, riII = 0;
         const float *Rr = rightrr + (size_t)t1 * N, *Ri = rightri + (size_t)t1 * N;
@@ -1032,14 +1122,21 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
         }
       }
       or_read_H(non, H_rec, H_stride, C_rec, tl, He, 0);
-      if (non->anharmonicity == 0) {
-        const float *a = A_rec + (long long)tl * (1 + N) + 1;
-        for (int i = 0; i < N; i++) Anh[i] = a[i];
+      if (es) { /* propagate_double_2DES_control, src/propagate.c:1664-1683 */
+        or_coupling_S_doubles_ES_f32(N, He, fr, fi, ts, dt, cut);
+        for (int t1 = 0; t1 < T1; t1++)
+          or_coupling_S_doubles_ES_f32(N, He, ft1r + (size_t)t1 * nn2, ft1i + (size_t)t1 * nn2, ts,
+                                       dt, cut);
+      } else {
+        if (non->anharmonicity == 0) {
+          const float *a = A_rec + (long long)tl * (1 + N) + 1;
+          for (int i = 0; i < N; i++) Anh[i] = a[i];
+        }
+        or_coupling_S_doubles_f32(N, He, fr, fi, ts, Anh, non->anharmonicity, dt, cut);
+        for (int t1 = 0; t1 < T1; t1++)
+          or_coupling_S_doubles_f32(N, He, ft1r + (size_t)t1 * nn2, ft1i + (size_t)t1 * nn2, ts,
+                                    Anh, non->anharmonicity, dt, cut);
       }
-      or_coupling_S_doubles_f32(N, He, fr, fi, ts, Anh, non->anharmonicity, dt, cut);
-      for (int t1 = 0; t1 < T1; t1++)
-        or_coupling_S_doubles_f32(N, He, ft1r + (size_t)t1 * nn2, ft1i + (size_t)t1 * nn2, ts,
-                                  Anh, non->anharmonicity, dt, cut);
       or_coupling_S_f32(N, He, rightnr, rightni, ts, -1, dt, cut);
       for (int t1 = 0; t1 < T1; t1++)
         or_coupling_S_f32(N, He, rightrr + (size_t)t1 * N, rightri + (size_t)t1 * N, ts, -1, dt, cut);

@@ -13,7 +13,7 @@
  * figure against which every FP32 result is reported.
  *
  * Pinning (what fixes each function independently of this file):
- *   or_coupling_S_f32 / or_coupling_S_doubles_f32 / or_dipole_double* /
+ *   or_coupling_S_f32 / or_coupling_S_doubles_f32 / _ES_f32 / or_dipole_double* /
  *   or_polar* / or_read_H* (including the TransitionDipole /
  *   ExtendedDipole couplings built from geometry) : bit-identical to the reference's own
  *     propagate.c / NISE_subs.c / polar.c / read_trajectory.c compiled from
@@ -62,6 +62,8 @@ void or_coupling_S_inv_f64(int N, const float *H, double *cr, double *ci,
 void or_coupling_S_doubles_f32(int N, const float *H, float *cr, float *ci,
                                int m, const float *Anh, float anharmonicity,
                                float deltat, float couplingcut);
+void or_coupling_S_doubles_ES_f32(int N, const float *H, float *cr, float *ci,
+                                  int m, float deltat, float couplingcut);
 void or_coupling_S_doubles_f64(int N, const float *H, double *cr, double *ci,
                                int m, const float *Anh, float anharmonicity,
                                float deltat, float couplingcut);
@@ -77,6 +79,10 @@ int or_DIA_S_f64(int N, const float *H, double *cr, double *ci, int sign,
 void or_dipole_double(int N, const float *dipole, const float *cr,
                       const float *ci, float *fr, float *fi, float *over,
                       float anharmonicity);
+void or_dipole_double_ES(int N, const float *dipole, const float *cr,
+                         const float *ci, float *fr, float *fi);
+void or_dipole_double_last_ES(int N, const float *dipole, const float *cr,
+                              const float *ci, float *fr, float *fi);
 void or_dipole_double_last(int N, const float *dipole, const float *cr,
                            const float *ci, float *fr, float *fi, float *over,
                            float anharmonicity);

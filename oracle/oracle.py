@@ -59,6 +59,9 @@ def oracle():
     L.or_set_geometry.argtypes = [_vp, C.c_longlong, _vp, C.c_longlong, _vp]
     L.or_dipole_double.argtypes = [C.c_int, _f32, _f32, _f32, _f32, _f32, _f32, C.c_float]
     L.or_dipole_double_last.argtypes = L.or_dipole_double.argtypes
+    L.or_coupling_S_doubles_ES_f32.argtypes = [C.c_int, _f32, _f32, _f32, C.c_int, C.c_float, C.c_float]
+    L.or_dipole_double_ES.argtypes = [C.c_int, _f32, _f32, _f32, _f32, _f32]
+    L.or_dipole_double_last_ES.argtypes = L.or_dipole_double_ES.argtypes
     L.or_polarweight.argtypes = [C.c_int, C.c_int]
     L.or_polarweight.restype = C.c_float
     L.or_polar.argtypes = [_i32, C.c_int]
@@ -97,6 +100,9 @@ def _bind_ref(L):
                                          C.c_float, C.c_float]
     L.ref_dipole_double.argtypes = [C.c_int, _f32, _f32, _f32, _f32, _f32, _f32, C.c_float]
     L.ref_dipole_double_last.argtypes = L.ref_dipole_double.argtypes
+    L.ref_coupling_S_doubles_ES.argtypes = [C.c_int, _f32, _f32, _f32, C.c_int, C.c_float, C.c_float]
+    L.ref_dipole_double_ES.argtypes = [C.c_int, _f32, _f32, _f32, _f32, _f32]
+    L.ref_dipole_double_last_ES.argtypes = L.ref_dipole_double_ES.argtypes
     L.ref_polarweight.argtypes = [C.c_int, C.c_int]
     L.ref_polarweight.restype = C.c_float
     L.ref_polar.argtypes = [_i32, C.c_int]

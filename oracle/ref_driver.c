@@ -74,6 +74,27 @@ EXPORT void ref_dipole_double_last(int N, float *dipole, float *cr, float *ci,
   dipole_double_last(&non, dipole, cr, ci, fr, fi, over);
 }
 
+EXPORT void ref_coupling_S_doubles_ES(int N, float *H, float *cr, float *ci,
+                                      int m, float deltat, float cut) {
+  t_non non;
+  init_non(&non, N, deltat, cut);
+  propagate_vec_coupling_S_doubles_ES(&non, H, cr, ci, m);
+}
+
+EXPORT void ref_dipole_double_ES(int N, float *dipole, float *cr, float *ci,
+                                 float *fr, float *fi) {
+  t_non non;
+  init_non(&non, N, 1, 0);
+  dipole_double_ES(&non, dipole, cr, ci, fr, fi);
+}
+
+EXPORT void ref_dipole_double_last_ES(int N, float *dipole, float *cr, float *ci,
+                                      float *fr, float *fi) {
+  t_non non;
+  init_non(&non, N, 1, 0);
+  dipole_double_last_ES(&non, dipole, cr, ci, fr, fi);
+}
+
 EXPORT float ref_polarweight(int pol, int x) { return polarweight(pol, x); }
 EXPORT void ref_polar(int *xp, int x) { polar(xp, x); }
 EXPORT int ref_sindex(int a, int b, int N) { return Sindex(a, b, N); }

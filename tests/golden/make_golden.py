@@ -146,6 +146,37 @@ def main():
         out["abs_coup_args"] = np.array([N, T, sr, ns, lo, hi], np.float64)
         out["abs_coup_H"], out["abs_coup_mu"], out["abs_coup_C"] = Hc, mur, Cr
         out["abs_coup_re"], out["abs_coup_im"] = re, im
+    # two-level doubles and mu_ef of calc_2DES, from a stream of their own
+    # (the fixtures above stay as they were)
+    rng2 = np.random.default_rng(20261203)
+    for i, (N, m, cut) in enumerate([(5, 1, 0.0), (12, 2, 8.0)]):
+        H = frame(rng2, N)
+        N2 = N * (N + 1) // 2
+        fr = rng2.normal(size=N2).astype(np.float32)
+        fi = rng2.normal(size=N2).astype(np.float32)
+        r, q = fr.copy(), fi.copy()
+        for _ in range(2):
+            R.ref_coupling_S_doubles_ES(N, H, r, q, m, 2.0, cut)
+        out[f"cE{i}_args"] = np.array([N, m, cut], np.float64)
+        out[f"cE{i}_H"] = H
+        out[f"cE{i}_in"], out[f"cE{i}_out"] = np.stack([fr, fi]), np.stack([r, q])
+    for i, N in enumerate([4, 9]):
+        N2 = N * (N + 1) // 2
+        dip = rng2.normal(size=N).astype(np.float32)
+        cr = rng2.normal(size=N).astype(np.float32)
+        ci = rng2.normal(size=N).astype(np.float32)
+        fr = np.zeros(N2, np.float32)
+        fi = np.zeros(N2, np.float32)
+        R.ref_dipole_double_ES(N, dip, cr, ci, fr, fi)
+        gr = rng2.normal(size=N2).astype(np.float32)
+        gi = rng2.normal(size=N2).astype(np.float32)
+        lr = np.zeros(N, np.float32)
+        li = np.zeros(N, np.float32)
+        R.ref_dipole_double_last_ES(N, dip, gr, gi, lr, li)
+        out[f"de{i}_in"] = np.concatenate([dip, cr, ci])
+        out[f"de{i}_out"] = np.concatenate([fr, fi])
+        out[f"dle{i}_in"] = np.concatenate([gr, gi])
+        out[f"dle{i}_out"] = np.concatenate([lr, li])
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {len(out)} arrays)")
 

@@ -152,7 +152,7 @@ def test_errors():
     L = abi.lib()
     R = np.zeros((12, 4, 4), np.float32)
     for kw, code in [(dict(tmax2=2), abi.NISE_ERR_ARG), (dict(propagation=0), abi.NISE_ERR_ARG),
-                     (dict(technique="2DUVvis"), abi.NISE_ERR_ARG),
+                     (dict(technique="2DIRraman"), abi.NISE_ERR_ARG),
                      (dict(end=30), abi.NISE_ERR_RANGE),
                      (dict(anharmonicity=0.0), abi.NISE_ERR_ARG)]:
         base = dict(singles=3, tmax1=4, tmax3=4, sample=2, begin=0, end=3, ts=1, propagation=1,

@@ -79,3 +79,20 @@ def test_closed_form_detects_frame_shift(orc):
     Hs = np.roll(H, -1, axis=0)
     mus = np.roll(mu, -1, axis=0)
     assert _rel(R, CF.two_level(non, Hs, mus)) > 1e-3
+
+
+@pytest.mark.parametrize("technique,paths", [("GBUVvis", ("GB",)), ("SEUVvis", ("SE",)),
+                                              ("EAUVvis", ("EA",)), ("noEAUVvis", ("GB", "SE")),
+                                              ("2DUVvis", ("GB", "SE", "EA"))])
+def test_two_level_sites_uncoupled(orc, technique, paths):
+    """calc_2DES (2DUVvis): uncoupled two-level sites, each pathway alone and
+    together; the overtone terms are absent, whatever the anharmonicity."""
+    N, T1, T3 = 3, 5, 6
+    H, mu, anh, over = synth.twodir_system(N=N, nframes=60, coupled=False, seed=21)
+    non = _non(N, T1, T3, 4, 1, 4, H, technique=technique, lifetime=300.0, anh=16.0)
+    R = orc.twodir(non, H, mu)
+    assert _rel(R, CF.response(non, H, mu, pathways=paths, es=True)) < 2e-5
+    if technique == "2DUVvis":
+        assert _rel(R, CF.two_level(non, H, mu, es=True)) < 2e-5
+        # the anharmonic-oscillator result differs: the overtone pathway matters
+        assert _rel(R, CF.two_level(non, H, mu)) > 1e-2

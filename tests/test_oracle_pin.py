@@ -86,6 +86,56 @@ def test_dipole_double_golden(orc, gold):
         assert np.array_equal(np.concatenate([lr, li]), gold[f"dl{i}_out"])
 
 
+def test_doubles_two_level_golden(orc, gold):
+    """propagate_vec_coupling_S_doubles_ES and dipole_double(_last)_ES
+    (calc_2DES) against the reference build's outputs."""
+    for i in range(2):
+        N, m, cut = gold[f"cE{i}_args"]
+        N, m = int(N), int(m)
+        r, q = gold[f"cE{i}_in"][0].copy(), gold[f"cE{i}_in"][1].copy()
+        for _ in range(2):
+            orc.oracle().or_coupling_S_doubles_ES_f32(N, gold[f"cE{i}_H"], r, q, m, 2.0, float(cut))
+        assert np.array_equal(r, gold[f"cE{i}_out"][0]) and np.array_equal(q, gold[f"cE{i}_out"][1])
+    for i, N in enumerate([4, 9]):
+        N2 = N * (N + 1) // 2
+        x = gold[f"de{i}_in"]
+        fr, fi = np.zeros(N2, np.float32), np.zeros(N2, np.float32)
+        orc.oracle().or_dipole_double_ES(N, x[:N], x[N:2 * N], x[2 * N:], fr, fi)
+        assert np.array_equal(np.concatenate([fr, fi]), gold[f"de{i}_out"])
+        g = gold[f"dle{i}_in"]
+        lr, li = np.zeros(N, np.float32), np.zeros(N, np.float32)
+        orc.oracle().or_dipole_double_last_ES(N, x[:N], g[:N2].copy(), g[N2:].copy(), lr, li)
+        assert np.array_equal(np.concatenate([lr, li]), gold[f"dle{i}_out"])
+
+
+def test_doubles_two_level_closed_form(orc):
+    """Two-level doubles: uncoupled sites give |ab> the phase of e_a + e_b
+    (no anharmonicity, |aa> untouched by couplings); a coupled dimer has
+    no c outside (a, b), so its single |01> state only takes the phase."""
+    N = 3
+    H = np.zeros(6, np.float32)
+    e = np.array([11.0, -7.0, 23.0], np.float32)
+    for a in range(N):
+        H[synth.sindex(a, a, N)] = e[a]
+    H[synth.sindex(0, 1, N)] = 0.0
+    f = np.float32(2.0) * np.float32(2.99792458e-5) * np.float32(2 * 3.14159265359)
+    fr = np.ones(6, np.float32)
+    fi = np.zeros(6, np.float32)
+    orc.oracle().or_coupling_S_doubles_ES_f32(N, H, fr, fi, 1, 2.0, 0.0)
+    for a in range(N):
+        for b in range(a, N):
+            ph = np.exp(-1j * float(f) * float(e[a] + e[b]))
+            k = synth.sindex(a, b, N)
+            assert abs(complex(fr[k], fi[k]) - ph) < 1e-6
+    # dimer with a coupling: the only two-exciton state |01> is not rotated
+    H2 = np.array([5.0, 30.0, -3.0], np.float32)  # |00>, |01> coupling 30, |11>
+    fr = np.array([0.0, 1.0, 0.0], np.float32)
+    fi = np.zeros(3, np.float32)
+    orc.oracle().or_coupling_S_doubles_ES_f32(2, H2, fr, fi, 1, 2.0, 0.0)
+    assert abs(complex(fr[1], fi[1]) - np.exp(-1j * float(f) * 2.0)) < 1e-6
+    assert fr[0] == 0 and fr[2] == 0 and fi[0] == 0 and fi[2] == 0
+
+
 def test_polar_golden(orc, gold):
     for x in range(21):
         t = np.zeros(4, np.int32)
@@ -164,6 +214,10 @@ def test_live_reference_random(orc):
         a, b, c, d = fr.copy(), fi.copy(), fr.copy(), fi.copy()
         orc.oracle().or_coupling_S_doubles_f32(N, H, a, b, m, Anh, anh, 2.0, cut)
         R.ref_coupling_S_doubles(N, H, c, d, m, Anh, anh, 2.0, cut)
+        assert np.array_equal(a, c) and np.array_equal(b, d)
+        a, b, c, d = fr.copy(), fi.copy(), fr.copy(), fi.copy()
+        orc.oracle().or_coupling_S_doubles_ES_f32(N, H, a, b, m, 2.0, cut)
+        R.ref_coupling_S_doubles_ES(N, H, c, d, m, 2.0, cut)
         assert np.array_equal(a, c) and np.array_equal(b, d)
 
 

@@ -28,6 +28,11 @@ perpendicular and cross lab polarisations:
 In the sum of all three pathways the cross-site terms cancel and each site
 contributes the two-level result -2 O e^{...} (1 - e^{i Delta t3}) of an
 anharmonic oscillator.  Signals are stored as rrX = Im S_X, riX = Re S_X.
+
+es=True gives the same sums for two-level sites (calc_2DES, 2DUVvis): no
+site holds two excitations, so the overtone terms are absent; the states
+|ab>, a != b, have the energy e_a + e_b and the same mu_ef as above, so the
+cross-site terms still cancel and each site contributes -2 O e^{...}.
 """
 import numpy as np
 
@@ -41,7 +46,7 @@ def orient(A, B, Cv, D):
                      2.5 * (ac * bd - ad * bc) / 30.0])
 
 
-def response(non, H, mu, anh=None, over=None, pathways=("GB", "SE", "EA")):
+def response(non, H, mu, anh=None, over=None, pathways=("GB", "SE", "EA"), es=False):
     """(12, tmax3, tmax1) float64 planes for uncoupled sites (Full records)."""
     N, T1, T3 = non.singles, non.tmax1, non.tmax3
     shift = np.float32((np.float32(non.max1) + np.float32(non.min1)) / np.float32(2))
@@ -50,7 +55,9 @@ def response(non, H, mu, anh=None, over=None, pathways=("GB", "SE", "EA")):
     diag = H[:, 1:][:, a_idx == b_idx]
     e = (diag - shift).astype(np.float32).astype(np.float64)  # (T, N)
     mu3 = mu[:, 1:].reshape(-1, 3, N).transpose(0, 2, 1).astype(np.float64)  # (T, N, 3)
-    if non.anharmonicity != 0:
+    if es:
+        ov3, dl = np.zeros_like(mu3), np.zeros((H.shape[0], N))
+    elif non.anharmonicity != 0:
         ov3 = np.float64(np.float32(np.sqrt(2.0))) * mu3
         dl = np.full((H.shape[0], N), float(np.float32(non.anharmonicity)))
     else:
@@ -95,7 +102,7 @@ def response(non, H, mu, anh=None, over=None, pathways=("GB", "SE", "EA")):
                                    orient(A[c], B[a], Cv[c], D[a])) * np.conj(z1[c]) * z3[a]
                             sII += (orient(A[c], B[c], Cv[a], D[a]) * z1[c] * z3[a] +
                                     orient(A[a], B[c], Cv[c], D[a]) * z1[a] * z3[a])
-                    if "EA" in pathways:
+                    if "EA" in pathways and not es:
                         o = orient(A[a], B[a], Cp[a], Dp[a])
                         sI += o * np.conj(z1[a]) * z3[a] * zD[a]
                         sII += o * z1[a] * z3[a] * zD[a]
@@ -108,7 +115,7 @@ def response(non, H, mu, anh=None, over=None, pathways=("GB", "SE", "EA")):
     return R
 
 
-def two_level(non, H, mu, anh=None, over=None):
+def two_level(non, H, mu, anh=None, over=None, es=False):
     """The sum of all pathways written per site, without cross terms."""
     N, T1, T3 = non.singles, non.tmax1, non.tmax3
     shift = np.float32((np.float32(non.max1) + np.float32(non.min1)) / np.float32(2))
@@ -116,7 +123,9 @@ def two_level(non, H, mu, anh=None, over=None):
     a_idx, b_idx = np.triu_indices(N)
     e = (H[:, 1:][:, a_idx == b_idx] - shift).astype(np.float32).astype(np.float64)
     mu3 = mu[:, 1:].reshape(-1, 3, N).transpose(0, 2, 1).astype(np.float64)
-    if non.anharmonicity != 0:
+    if es:
+        ov3, dl = np.zeros_like(mu3), np.zeros((H.shape[0], N))
+    elif non.anharmonicity != 0:
         ov3 = np.float64(np.float32(np.sqrt(2.0))) * mu3
         dl = np.full((H.shape[0], N), float(np.float32(non.anharmonicity)))
     else:
