@@ -22,24 +22,9 @@
 #include <vector>
 
 #include "nise_internal.h"
+#include "doubles_dev.h"
 
 namespace nise {
-
-__device__ __forceinline__ float2 cmul_diag2(float2 c, float2 u) {
-  // src/propagate.c:960-961: (cr*re - ci*im, cr*im + ci*re)
-  return make_float2(__fsub_rn(__fmul_rn(c.x, u.x), __fmul_rn(c.y, u.y)),
-                     __fadd_rn(__fmul_rn(c.x, u.y), __fmul_rn(c.y, u.x)));
-}
-
-__device__ __forceinline__ void rot2(float2 &p1, float2 &p2, float co, float si) {
-  // src/propagate.c:986-990 (same form for all five c ranges)
-  float cr1 = __fsub_rn(__fmul_rn(co, p1.x), __fmul_rn(si, p2.y));
-  float ci1 = __fadd_rn(__fmul_rn(co, p1.y), __fmul_rn(si, p2.x));
-  float cr2 = __fsub_rn(__fmul_rn(co, p2.x), __fmul_rn(si, p1.y));
-  float ci2 = __fadd_rn(__fmul_rn(co, p2.y), __fmul_rn(si, p1.x));
-  p1 = make_float2(cr1, ci1);
-  p2 = make_float2(cr2, ci2);
-}
 
 // Per-frame doubles tables: D2[row][N2] and (co, si, co2, si2)[row][K2]
 // for the kept couplings in lexicographic order (ab2).  anh_rec != null:
@@ -101,15 +86,6 @@ int launch_doubles_tables(const FrameSource &src, int N, int K2, const uint32_t 
   return NISE_OK;
 }
 
-// Couplings staged in LDS per pass over the coupling list: (co, si, co2,
-// si2) rows and pair words of at most kDblCh couplings.
-constexpr int kDblCh = 512;
-__host__ __device__ inline int dbl_chunk(int K2) {
-  return K2 <= 64 ? 64 : K2 < kDblCh ? (K2 + 63) / 64 * 64 : kDblCh;
-}
-// Vectors per lane: with np <= 32 lanes per vector each lane rotates the
-// same pair of states in two vectors, so one index computation serves both.
-__host__ __device__ constexpr int dbl_vpl(int np) { return np <= 32 ? 1 : 1; }
 // LDS bytes of a doubles step: states [N2][vectors per wave], the D2 row
 // when d2_lds, then the coupling chunk.  The D2 row is staged when
 // everything fits.
@@ -200,46 +176,7 @@ __global__ void __launch_bounds__(64) doubles_step_kernel(DoublesArgs A) {
         stage_lds<4>(abL, A.ab2 + k0, nk);
         __syncthreads();
       }
-      for (int k = 0; k < nk; ++k) {
-        // the coupling is wave-uniform: pair and angles in scalar registers
-        const uint32_t p = __builtin_amdgcn_readfirstlane(abL[k]);
-        const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
-        const int rs_a = (a * (2 * N - a - 1)) >> 1, rs_b = (b * (2 * N - b - 1)) >> 1;
-        const float4 cv = csL[k];
-        const float co = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cv.x)));
-        const float si = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cv.y)));
-        const float co2 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cv.z)));
-        const float si2 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cv.w)));
-        for (int cb = 0; cb < N; cb += NP) {
-          const int c = cb + c0;
-          // every c except b, ascending; c == a (sqrt(2) angle) also does the
-          // c == b rotation (|ab>, |bb>): it shares |ab>, and no rotation of
-          // a c in between touches |ab> or |bb>
-          if (c < N && c != b && !(A.es && c == a)) {
-            const int rs_c = cb == 0 ? rs0 : (c * (2 * N - c - 1)) >> 1;
-            const int i1 = (c <= a ? a + rs_c : c + rs_a) * VW + q * VPL;
-            const int i2 = (c <= b ? b + rs_c : c + rs_b) * VW + q * VPL;
-            const int i3 = (b + rs_b) * VW + q * VPL;
-            const bool ca = c == a;
-            const float cl = ca ? co2 : co, sl = ca ? si2 : si;
-#pragma unroll
-            for (int v = 0; v < VPL; ++v) {
-              if (q * VPL + v < nv) {
-                float2 p1 = psi[i1 + v], p2 = psi[i2 + v], p3 = make_float2(0.f, 0.f);
-                if (ca) p3 = psi[i3 + v];  // read with the pair: one LDS round trip
-                rot2(p1, p2, cl, sl);
-                psi[i1 + v] = p1;
-                if (ca) {
-                  rot2(p2, p3, co2, si2);
-                  psi[i3 + v] = p3;
-                }
-                psi[i2 + v] = p2;
-              }
-            }
-          }
-          asm volatile("" ::: "memory");
-        }
-      }
+      dbl_rotate<NP, VPL>(psi, N, nv, nk, abL, csL, A.es, q, c0, rs0);
     }
     if (sub + 1 < A.m) diag();
   }

@@ -216,6 +216,18 @@ __device__ __forceinline__ void gst(T *p, T v) {
   *(__attribute__((address_space(1))) W *)p = w;
 }
 
+// Load of wave-uniform, read-only data through the constant address space,
+// which the compiler turns into scalar loads (p must be the same in every
+// lane and the data must not be written while the kernel runs).
+template <class T>
+__device__ __forceinline__ T cld(const T *p) {
+  typedef typename GWord<sizeof(T)>::type W;
+  const W w = *(const __attribute__((address_space(4))) W *)p;
+  T v;
+  __builtin_memcpy(&v, &w, sizeof(T));
+  return v;
+}
+
 // dst[map(i)] = *src(i) for i < n, device memory to LDS, by the whole
 // workgroup with U loads in flight per thread before the first LDS store (a
 // plain copy loop waits for each load before it issues the next).

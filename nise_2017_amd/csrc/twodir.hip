@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "nise_internal.h"
+#include "doubles_dev.h"
 
 namespace nise {
 
@@ -200,8 +201,9 @@ __global__ void dd_kernel(TwoDirArgs A) {
 // dip_i f_ik, then over_k f_kk, then dip_j f_kj for j > k, i.e. one pass over
 // i = 0..N-1 with the weight over_k at i = k.  Each element f is read once
 // for all NIT sets.  f_ik with i <= k sits at k + rs(i), rs(i) = i(2N-i-1)/2,
-// and f_ki with i > k at rs(k) + i (src/NISE_subs.h:54-65).
-template <int NIT>
+// and f_ki with i > k at rs(k) + i (src/NISE_subs.h:54-65).  Element j of
+// f is read at f[j * FS].
+template <int NIT, int FS = 1>
 __device__ __forceinline__ void ddl_multi(const float2 *f, int N, int k, const float *dip0, const float *dip1,
                                  const float *dip2, const float ov[NIT], float2 out[NIT]) {
   const float *dip[3] = {dip0, dip1, dip2};
@@ -211,7 +213,7 @@ __device__ __forceinline__ void ddl_multi(const float2 *f, int N, int k, const f
   const int rsk = k * ((N << 1) - k - 1) / 2;
   int rsi = 0;
   for (int i = 0; i < N; ++i) {
-    const float2 v = f[(i <= k) ? k + rsi : rsk + i];
+    const float2 v = f[((i <= k) ? k + rsi : rsk + i) * FS];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       // dip[i] is wave-uniform: read it once into a scalar register (a
@@ -397,6 +399,279 @@ __global__ void ea_kernel(TwoDirArgs A, int t3, int C, const float2 *lrr) {
     }
   if (n == 3) ea_block<3, kStage>(A, t3, C, mi, lrr, smem);
   else ea_block<2, kStage>(A, t3, C, mi, lrr, smem);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent EA + doubles kernel: one wave per group of doubles vectors of
+// one sample, holding them in LDS for every t3.  At each
+// t3 it adds the EA terms its vectors own and then applies the doubles step
+// of frame tk + t3, so the two-exciton states never return to device memory.
+//   kind 1 (ft vectors of one (x0, x2), t1 = t1_0 + c): the part of the EA
+//     response against rightnr = conj(M[x1](t3)) (src/calc_2DIR.c:457-459,
+//     :468-469) for every item with that (x0, x2);
+//   kind 0 (fr[x1][x2] vectors): leftrr_EA for x3 = 0..2 (:455-456) and the
+//     part against rightrr[t1] = conj(L[x0][t1](t3)) (:465-466) for every
+//     item with that (x1, x2) and every t1.
+// The operations and their order are those of lrr_kernel, ea_kernel and
+// doubles_step_kernel, so the results are identical to that path.
+// ---------------------------------------------------------------------------
+struct DblGroup {
+  int s, kind, v0, nv;  // sample, 0 fr / 1 ft, first vector (per-sample index), count
+};
+
+struct PersistArgs {
+  TwoDirArgs A;
+  const DblGroup *groups;
+  int K2, m;
+  const uint32_t *ab2;   // kept couplings (lexicographic)
+  const float2 *D2;      // [frame - f_lo][N2]
+  const float4 *cs2;     // [frame - f_lo][K2]
+  int64_t f_lo;
+};
+
+// Workgroups of kPW waves: the waves hold groups of the same sample and
+// share the staged frame tables, dipoles and M rows.
+constexpr int kPW = 4;
+
+struct PersistLayout {
+  size_t ix, d2, rn, dip, ov, psi, lv, total;  // psi and lv: kPW per-wave blocks
+};
+// ix: per coupling k, site block cb and lane c0 the packed state indices
+// (i1 | i2 << 16) of the rotation of site c = cb NP + c0, or ~0u for none.
+__host__ __device__ inline PersistLayout persist_layout(int N, int VW, int K2) {
+  const size_t N2 = (size_t)N * (N + 1) / 2;
+  const int NP = 64 / VW, nb = (N + NP - 1) / NP;
+  PersistLayout L;
+  size_t o = 0;
+  L.ix = o, o += (size_t)K2 * nb * NP * 4, o = (o + 15) & ~(size_t)15;
+  L.d2 = o, o += N2 * 8;
+  L.rn = o, o += (size_t)3 * N * 8;
+  L.dip = o, o += (size_t)3 * N * 4;
+  L.ov = o, o += (size_t)3 * N * 4, o = (o + 15) & ~(size_t)15;
+  L.psi = o, o += kPW * N2 * VW * 8;
+  L.lv = o, o += (size_t)kPW * VW * 3 * N * 8;
+  L.total = o;
+  return L;
+}
+
+// sum_i (l, conj R) in the reference's order, R read from device memory
+// eight elements ahead (part I, :465-466)
+__device__ __forceinline__ void dot_part1(const float2 *lr, const float2 *R, int N, float &rr, float &ri) {
+  for (int i0 = 0; i0 < N; i0 += 8) {
+    float2 rv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) rv[u] = (i0 + u < N) ? gld(R + i0 + u) : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (i0 + u < N) {
+        const float2 l = lr[i0 + u];
+        const float Rr = rv[u].x, Ri = -rv[u].y;
+        rr = __fadd_rn(rr, __fadd_rn(__fmul_rn(l.y, Rr), __fmul_rn(l.x, Ri)));
+        ri = __fadd_rn(ri, __fsub_rn(__fmul_rn(l.x, Rr), __fmul_rn(Ri, l.y)));
+      }
+  }
+}
+
+// the slot-th item with (x1, x2) in molecular-polarisation order, or -1
+__device__ inline int item_x1x2(int x1, int x2, int slot) {
+  for (int m = 0; m < 21; ++m)
+    if (c_polar[m][1] == x1 && c_polar[m][2] == x2 && slot-- == 0) return m;
+  return -1;
+}
+
+// One wave's EA part for ft vectors (lane = threadIdx.x % 64).  Lanes read
+// what other lanes of the same wave wrote to LDS: a wave's LDS operations
+// run in order, so a compiler fence between the phases suffices.
+template <int NP, int NIT>
+__device__ __forceinline__ void persist_ft(const TwoDirArgs &A, int lane, int s, int t3, int v0, int nv,
+                                           int3 mi, const float2 *psi, const float2 *RnL, float2 *LvL,
+                                           const float *dipL, const float *ovL) {
+  constexpr int VW = 64 / NP;
+  const int N = A.N, T1 = A.T1, t1_0 = (v0 - 9) % T1;
+  const int x3a = c_polar[mi.x][3], x3b = c_polar[mi.y][3], x3c = NIT > 2 ? c_polar[mi.z][3] : 0;
+  const int x3v[3] = {x3a, x3b, x3c};
+  for (int u = lane; u < nv * N; u += 64) {
+    const int c = u / N, k = u % N;
+    float ov[NIT];
+    float2 o[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) ov[it] = ovL[x3v[it] * N + k];
+    ddl_multi<NIT, VW>(psi + c, N, k, dipL + x3a * N, dipL + x3b * N, dipL + x3c * N, ov, o);
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) LvL[(c * NIT + it) * N + k] = o[it];
+  }
+  asm volatile("" ::: "memory");
+  const int64_t plane = (int64_t)A.T3 * T1;
+  for (int j = lane; j < NIT * nv; j += 64) {
+    const int it = j % NIT, c = j / NIT, t1 = t1_0 + c;
+    const int m = it == 0 ? mi.x : it == 1 ? mi.y : mi.z, x1 = c_polar[m][1];
+    const float2 *Rn = RnL + x1 * N, *Lv = LvL + (c * NIT + it) * N;
+    float rr = 0.f, ri = 0.f;
+    for (int i = 0; i < N; ++i) {
+      const float2 l = Lv[i], Rv = Rn[i];
+      const float Nr = Rv.x, Ni = -Rv.y;
+      rr = __fadd_rn(rr, __fadd_rn(__fmul_rn(Nr, l.y), __fmul_rn(Ni, l.x)));
+      ri = __fadd_rn(ri, __fsub_rn(__fmul_rn(Nr, l.x), __fmul_rn(Ni, l.y)));
+    }
+    float *E = A.E + (((int64_t)s * 21 + m) * 4 + 2) * plane + (int64_t)t3 * T1 + t1;
+    E[0] += rr, E[plane] += ri;
+  }
+  asm volatile("" ::: "memory");
+}
+
+// One wave's EA part for fr vectors.
+template <int NP>
+__device__ __forceinline__ void persist_fr(const TwoDirArgs &A, int lane, int s, int t3, int v0, int nv,
+                                           const float2 *psi, float2 *LvL, const float *dipL, const float *ovL) {
+  constexpr int VW = 64 / NP;
+  const int N = A.N, T1 = A.T1;
+  for (int u = lane; u < nv * N; u += 64) {
+    const int c = u / N, k = u % N;
+    const float ov[3] = {ovL[k], ovL[N + k], ovL[2 * N + k]};
+    float2 o[3];
+    ddl_multi<3, VW>(psi + c, N, k, dipL, dipL + N, dipL + 2 * N, ov, o);
+    LvL[(c * 3 + 0) * N + k] = o[0], LvL[(c * 3 + 1) * N + k] = o[1], LvL[(c * 3 + 2) * N + k] = o[2];
+  }
+  asm volatile("" ::: "memory");
+  const int64_t plane = (int64_t)A.T3 * T1;
+  for (int j = lane; j < nv * 3 * T1; j += 64) {
+    const int t1 = j % T1, slot = (j / T1) % 3, c = j / (3 * T1), v = v0 + c;
+    const int m = item_x1x2(v / 3, v % 3, slot);
+    if (m < 0) continue;
+    const int x0 = c_polar[m][0], x3 = c_polar[m][3];
+    float rr = 0.f, ri = 0.f;
+    dot_part1(LvL + (c * 3 + x3) * N, A.hist + (hrow_L(A, s, x0, t1) + t3) * N, N, rr, ri);
+    float *E = A.E + (((int64_t)s * 21 + m) * 4 + 0) * plane + (int64_t)t3 * T1 + t1;
+    E[0] += rr, E[plane] += ri;
+  }
+  asm volatile("" ::: "memory");
+}
+
+// The rotations of one substep (as dbl_rotate) with the per-lane state
+// indices from the table ix, each lane's next entry read one rotation ahead,
+// and the pair words and angles read by scalar loads (they are wave-uniform
+// and read-only).
+template <int NP>
+__device__ __forceinline__ void dbl_rotate_tab(float2 *psi, int N, int nv, int K2, const uint32_t *ix,
+                                               const uint32_t *ab2, const float4 *cs, int q, int c0) {
+  constexpr int VW = 64 / NP;
+  const int nb = (N + NP - 1) / NP, total = K2 * nb;
+  if (total == 0) return;
+  uint32_t e = ix[c0];
+  uint32_t p = cld(ab2);
+  float4 cv = cld(cs);
+  int k = 0, cb = 0;
+  for (int j = 0; j < total; ++j) {
+    const uint32_t en = j + 1 < total ? ix[(j + 1) * NP + c0] : ~0u;
+    const int kn = min(cb + 1 == nb ? k + 1 : k, K2 - 1);
+    const uint32_t pn = cld(ab2 + kn);
+    const float4 cvn = cld(cs + kn);
+    const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
+    const int i3 = (b + ((b * (2 * N - b - 1)) >> 1)) * VW + q;
+    if (e != ~0u && q < nv) {
+      const int i1 = (int)(e & 0xffffu) * VW + q, i2 = (int)(e >> 16) * VW + q;
+      const bool ca = cb * NP + c0 == a;
+      const float cl = ca ? cv.z : cv.x, sl = ca ? cv.w : cv.y;
+      float2 p1 = psi[i1], p2 = psi[i2], p3 = make_float2(0.f, 0.f);
+      if (ca) p3 = psi[i3];
+      rot2(p1, p2, cl, sl);
+      psi[i1] = p1;
+      if (ca) {
+        rot2(p2, p3, cv.z, cv.w);
+        psi[i3] = p3;
+      }
+      psi[i2] = p2;
+    }
+    asm volatile("" ::: "memory");
+    e = en, p = pn, cv = cvn;
+    if (++cb == nb) cb = 0, ++k;
+  }
+}
+
+template <int NP>
+__global__ void __launch_bounds__(64 * kPW) dbl_ea_kernel(PersistArgs P) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int VW = 64 / NP;
+  const TwoDirArgs &A = P.A;
+  const int N = A.N, N2 = N * (N + 1) / 2, K2 = P.K2, T1 = A.T1, T3 = A.T3, t = threadIdx.x;
+  const int wave = t >> 6, lane = t & 63;
+  const PersistLayout Ly = persist_layout(N, VW, K2);
+  uint32_t *ixL = reinterpret_cast<uint32_t *>(smem + Ly.ix);
+  float2 *d2 = reinterpret_cast<float2 *>(smem + Ly.d2);
+  float2 *RnL = reinterpret_cast<float2 *>(smem + Ly.rn);
+  float *dipL = reinterpret_cast<float *>(smem + Ly.dip);
+  float *ovL = reinterpret_cast<float *>(smem + Ly.ov);
+  float2 *psi = reinterpret_cast<float2 *>(smem + Ly.psi) + (size_t)wave * N2 * VW;
+  float2 *LvL = reinterpret_cast<float2 *>(smem + Ly.lv) + (size_t)wave * VW * 3 * N;
+  const DblGroup gd = P.groups[(size_t)blockIdx.x * kPW + wave];
+  const int s = P.groups[(size_t)blockIdx.x * kPW].s;  // every wave of the workgroup: one sample
+  const int kind = gd.kind, v0 = gd.v0, nv = gd.nv;
+  const int nvec = 9 + 9 * T1;
+  // this wave's vectors at t3 = 0 (psi[j][w]); the pair words
+  for (int w = 0; w < VW; ++w) {
+    const float2 *src = A.dbl + ((int64_t)s * nvec + v0 + w) * N2;
+    for (int j = lane; j < N2; j += 64) psi[j * VW + w] = (w < nv) ? gld(src + j) : make_float2(0.f, 0.f);
+  }
+  // the rotation index table: every c except b, ascending; c == a also does
+  // the c == b rotation (|ab>, |bb>); two-level sites skip c == a
+  {
+    const int nb = (N + NP - 1) / NP;
+    for (int e = t; e < K2 * nb * NP; e += 64 * kPW) {
+      const int k = e / (nb * NP), c = e % (nb * NP);
+      const uint32_t p = P.ab2[k];
+      const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
+      uint32_t v = ~0u;
+      if (c < N && c != b && !(A.es && c == a)) {
+        const int rs_c = (c * (2 * N - c - 1)) >> 1;
+        const int rs_a = (a * (2 * N - a - 1)) >> 1, rs_b = (b * (2 * N - b - 1)) >> 1;
+        const int i1 = c <= a ? a + rs_c : c + rs_a, i2 = c <= b ? b + rs_c : c + rs_b;
+        v = (uint32_t)i1 | ((uint32_t)i2 << 16);
+      }
+      ixL[e] = v;
+    }
+  }
+  // ft groups: the items with this (x0, x2)
+  int3 mi = make_int3(0, 0, 0);
+  int nit = 0;
+  if (kind == 1 && nv > 0) {
+    const int idx = v0 - 9, x0 = idx / (3 * T1), x2 = (idx / T1) % 3;
+    for (int m = 0; m < 21; ++m)
+      if (c_polar[m][0] == x0 && c_polar[m][2] == x2) {
+        if (nit == 0) mi.x = m;
+        else if (nit == 1) mi.y = m;
+        else mi.z = m;
+        ++nit;
+      }
+  }
+  const int q = lane / NP, c0 = lane % NP;
+  const int64_t tk = tj_of(A, s);
+  for (int t3 = 0; t3 < T3; ++t3) {
+    const int64_t tl = tk + t3;
+    load_dip_over(A, tl, dipL, ovL);
+    for (int e = t; e < 3 * N; e += 64 * kPW) RnL[e] = A.hist[(hrow_M(A, s, e / N) + t3) * N + e % N];
+    __syncthreads();
+    if (nv > 0) {
+      if (kind == 1) {
+        if (nit == 3) persist_ft<NP, 3>(A, lane, s, t3, v0, nv, mi, psi, RnL, LvL, dipL, ovL);
+        else persist_ft<NP, 2>(A, lane, s, t3, v0, nv, mi, psi, RnL, LvL, dipL, ovL);
+      } else {
+        persist_fr<NP>(A, lane, s, t3, v0, nv, psi, LvL, dipL, ovL);
+      }
+    }
+    if (t3 + 1 < T3) {  // the doubles step of frame tk + t3 (the state after the last t3 is not used)
+      const int64_t row = tk + t3 - P.f_lo;
+      stage_lds<8>(d2, P.D2 + row * N2, N2);
+      __syncthreads();
+      for (int sub = 0; sub < P.m; ++sub) {
+        for (int j = lane; j < N2 * VW; j += 64) psi[j] = cmul_diag2(psi[j], d2[j / VW]);
+        asm volatile("" ::: "memory");
+        dbl_rotate_tab<NP>(psi, N, nv, K2, ixL, P.ab2, P.cs2 + row * K2, q, c0);
+        for (int j = lane; j < N2 * VW; j += 64) psi[j] = cmul_diag2(psi[j], d2[j / VW]);
+        asm volatile("" ::: "memory");
+      }
+    }
+    __syncthreads();  // the staged rows are overwritten at the next t3
+  }
 }
 
 // R[k][t3][t1] += sum_s sum_m E[s][m][q][t3][t1] * (w(p, m) lt(t3, t1)),
@@ -608,7 +883,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
   const int vw = doubles_vw(N);
   const int gps = (nvec + vw - 1) / vw;  // doubles groups per sample
 
-  DevBuf L0, hist, dbl, E, slots, wgb, vb, rows, lrrb;
+  DevBuf L0, hist, dbl, E, slots, wgb, vb, rows, lrrb, grp;
   const EaShape ea = ea_shape(N, T1);
   ConveyorArgs a;
   a.N = N, a.K = sch.K, a.nlayers = sch.nlayers(), a.m = non->ts, a.f_lo = f_lo;
@@ -674,7 +949,33 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
     NISE_HIP_CHECK(hipGetLastError());
     // (4) EA: doubles vectors at tk, then per t3 the responses and one
     //     doubles step with frame tk + t3
-    if (do_ea) {
+    const int K2 = (int)lex.size(), np = doubles_np(N), pvw = 64 / np;
+    const size_t psmem = persist_layout(N, pvw, K2).total;
+    if (do_ea && N2 <= 65535 && psmem <= 160 * 1024) {
+      // every doubles vector stays in the LDS of its workgroup for all t3
+      hipLaunchKernelGGL(dd_kernel, dim3((N2 + 127) / 128, S, nvec), dim3(128), 0, st, A);
+      NISE_HIP_CHECK(hipGetLastError());
+      std::vector<DblGroup> gh;
+      for (int s = 0; s < S; ++s) {
+        for (int v0 = 0; v0 < 9; v0 += pvw) gh.push_back(DblGroup{s, 0, v0, std::min(pvw, 9 - v0)});
+        for (int p = 0; p < 9; ++p)
+          for (int t1 = 0; t1 < T1; t1 += pvw) gh.push_back(DblGroup{s, 1, 9 + p * T1 + t1, std::min(pvw, T1 - t1)});
+        while (gh.size() % kPW) gh.push_back(DblGroup{s, 0, 0, 0});  // workgroups never mix samples
+      }
+      if ((rc = upl(grp, gh.data(), gh.size() * sizeof(DblGroup)))) return rc;
+      PersistArgs P;
+      P.A = A, P.groups = grp.as<DblGroup>(), P.K2 = K2, P.m = non->ts, P.ab2 = ab2.as<uint32_t>();
+      P.D2 = D2.as<float2>(), P.cs2 = cs2.as<float4>(), P.f_lo = f_lo;
+      const void *fp = np == 8 ? (const void *)dbl_ea_kernel<8>
+                     : np == 16 ? (const void *)dbl_ea_kernel<16>
+                     : np == 32 ? (const void *)dbl_ea_kernel<32>
+                                : (const void *)dbl_ea_kernel<64>;
+      NISE_HIP_CHECK(hipFuncSetAttribute(fp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)psmem));
+      void *kargs[] = {&P};
+      NISE_HIP_CHECK(hipLaunchKernel(fp, dim3((unsigned)(gh.size() / kPW)), dim3(64 * kPW), kargs, psmem, st));
+    } else if (do_ea) {
+      // per t3: leftrr, the EA responses from the doubles vectors in device
+      // memory, then one doubles step of every vector
       hipLaunchKernelGGL(dd_kernel, dim3((N2 + 127) / 128, S, nvec), dim3(128), 0, st, A);
       NISE_HIP_CHECK(hipGetLastError());
       std::vector<int> vbh((size_t)S * gps + 1);
