@@ -86,76 +86,115 @@ __global__ void seed0_kernel(TwoDirArgs A, float2 *L0) {
     L0[(((int64_t)s * 3 + x) * A.T1) * A.N + i] = make_float2(mu[i], 0.f);
 }
 
-// GB and SE terms from the recorded history, one workgroup per (s, t3),
-// one thread per t1.  Writes E = -GB - SE for all 21 items.
-__global__ void gbse_kernel(TwoDirArgs A) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int s = blockIdx.x, t3 = blockIdx.y, N = A.N;
-  float2 *sc = reinterpret_cast<float2 *>(smem);  // [3][3] t3n_GB, [3][3] t3r_SE, [3][3] t3n_SE
-  const int64_t tj = tj_of(A, s), tk = tj, tl = tk + t3;
-  const int t = threadIdx.x;
-  if (t < 27) {
-    const int kind = t / 9, a = (t / 3) % 3, b = t % 3;
-    float r = 0.f, q = 0.f;
-    if (kind == 0) {  // GB: t3n[px2=a][px3=b] = mu_px3(tl) . M[px2](t3)  (:265-268)
-      const float *m4 = mu_at(A, tl, b);
-      const float2 *v = A.hist + (hrow_M(A, s, a) + t3) * N;
-      for (int i = 0; i < N; ++i) r = __fadd_rn(r, __fmul_rn(m4[i], v[i].x)), q = __fadd_rn(q, __fmul_rn(m4[i], v[i].y));
-    } else if (kind == 1) {  // SE: t3r[px1=a][px3=b] = mu_px3(tl) . leftrr(t3)  (:378-381)
-      const float *m4 = mu_at(A, tl, b);
-      const float2 *v = A.hist + (hrow_M(A, s, a) + t3) * N;
-      for (int i = 0; i < N; ++i) r = __fadd_rn(r, __fmul_rn(m4[i], v[i].x)), q = __fadd_rn(q, __fmul_rn(m4[i], v[i].y));
-    } else {  // SE static: t3n[px1=a][px2=b] = leftrr(tk) . mu_px2(tk), im with -=  (:353-357)
-      const float *m3 = mu_at(A, tk, b);
-      const float2 *v = A.hist + (hrow_M(A, s, a) + 0) * N;
-      for (int i = 0; i < N; ++i) r = __fadd_rn(r, __fmul_rn(v[i].x, m3[i])), q = __fsub_rn(q, __fmul_rn(v[i].y, m3[i]));
-    }
-    sc[t] = make_float2(r, q);
-  }
-  __syncthreads();
-  for (int t1 = t; t1 < A.T1; t1 += blockDim.x) {
-    // per-t1 dots: GB t1n[x0][x1] (static, :244-250), SE t1n[x0][x3] (:384-390),
-    // SE static t1r[x0][x2] (:363-369)
-    float2 gb1[3][3], se1[3][3], se1r[3][3];
-    for (int x0 = 0; x0 < 3; ++x0) {
-      const float2 *l0 = A.L0 + (((int64_t)s * 3 + x0) * A.T1 + t1) * N;
-      const float2 *lt = A.hist + (hrow_L(A, s, x0, t1) + t3) * N;
-      for (int y = 0; y < 3; ++y) {
-        const float *mj = mu_at(A, tj, y), *ml = mu_at(A, tl, y), *mk = mu_at(A, tk, y);
-        float ar = 0.f, ai = 0.f, br = 0.f, bi = 0.f, cr = 0.f, ci = 0.f;
-        for (int i = 0; i < N; ++i) {
-          ar = __fadd_rn(ar, __fmul_rn(mj[i], l0[i].x)), ai = __fadd_rn(ai, __fmul_rn(mj[i], l0[i].y));
-          br = __fadd_rn(br, __fmul_rn(lt[i].x, ml[i])), bi = __fadd_rn(bi, __fmul_rn(lt[i].y, ml[i]));
-          cr = __fadd_rn(cr, __fmul_rn(l0[i].x, mk[i])), ci = __fsub_rn(ci, __fmul_rn(l0[i].y, mk[i]));
-        }
-        gb1[x0][y] = make_float2(ar, ai);
-        se1[x0][y] = make_float2(br, bi);
-        se1r[x0][y] = make_float2(cr, ci);
+// acc[a][b] = sum_i mu_b[i] v_a[i] (complex v, real mu) for three rows v_a
+// and three rows mu_b, each sum in ascending i with rounded products and
+// sums, as the reference's dot loops (the product's operand order does not
+// change a rounded product).
+__device__ __forceinline__ void dot9(const float2 *v0, const float2 *v1, const float2 *v2, const float *mu,
+                                     int mstride, int N, float2 acc[9]) {
+#pragma unroll
+  for (int j = 0; j < 9; ++j) acc[j] = make_float2(0.f, 0.f);
+#pragma unroll 2
+  for (int i = 0; i < N; ++i) {
+    const float2 v[3] = {gld(v0 + i), gld(v1 + i), gld(v2 + i)};
+    const float m[3] = {mu[i], mu[mstride + i], mu[2 * mstride + i]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        float2 &r = acc[a * 3 + b];
+        r.x = __fadd_rn(r.x, __fmul_rn(m[b], v[a].x));
+        r.y = __fadd_rn(r.y, __fmul_rn(m[b], v[a].y));
       }
+  }
+}
+
+// The GB and SE dots that do not depend on both t1 and t3, one workgroup
+// per sample:
+//   g1[s][x0*3+y][t1] = mu_y(tj) . L0[x0][t1]          GB t1n (src/calc_2DIR.c:244-250)
+//   g3[s][t3][a*3+b]  = mu_b(tk + t3) . M[a](t3)       GB t3n (:265-268), SE t3r (:378-381)
+// With tk = tj the reference's SE t1r[x0][x2] (:363-369) is conj(g1[x0][x2])
+// and its SE static t3n[x1][x2] (:353-357) is conj(g3[0][x1*3+x2]): the same
+// rounded products and sums with the imaginary sign flipped (x - p rounds to
+// -(-x + p)).
+__global__ void gbse_static_kernel(TwoDirArgs A, float2 *g1, float2 *g3) {
+  extern __shared__ __align__(16) float muj[];  // [3][N] mu(tj)
+  const int s = blockIdx.x, N = A.N, T1 = A.T1, T3 = A.T3;
+  const int64_t tj = tj_of(A, s);
+  for (int e = threadIdx.x; e < 3 * N; e += blockDim.x) muj[e] = mu_at(A, tj, e / N)[e % N];
+  __syncthreads();
+  for (int e = threadIdx.x; e < T1 + T3; e += blockDim.x) {
+    float2 acc[9];
+    if (e < T1) {
+      const float2 *l0 = A.L0 + ((int64_t)s * 3 * T1 + e) * N;
+      dot9(l0, l0 + (int64_t)T1 * N, l0 + (int64_t)2 * T1 * N, muj, N, N, acc);
+#pragma unroll
+      for (int j = 0; j < 9; ++j) g1[((int64_t)s * 9 + j) * T1 + e] = acc[j];
+    } else {
+      const int t3 = e - T1;
+      const float2 *m0 = A.hist + (hrow_M(A, s, 0) + t3) * N;
+      const int64_t rs = (int64_t)T3 * N;
+      dot9(m0, m0 + rs, m0 + 2 * rs, mu_at(A, tj + t3, 0), N, N, acc);
+#pragma unroll
+      for (int j = 0; j < 9; ++j) g3[((int64_t)s * T3 + t3) * 9 + j] = acc[j];
     }
+  }
+}
+
+// GB and SE terms, one workgroup per (s, t3), one thread per t1: the t3
+// dots SE t1n[x0][x3] = L[x0][t1](t3) . mu_x3(tk + t3) (src/calc_2DIR.c
+// :384-390) with the static dots of gbse_static_kernel.  Writes
+// E = -GB - SE for all 21 items.
+__global__ void gbse_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3) {
+  extern __shared__ __align__(16) float mul[];  // [3][N] mu(tk + t3)
+  constexpr int kPol[21][4] = {
+      {0, 0, 0, 0}, {1, 1, 1, 1}, {2, 2, 2, 2}, {0, 0, 1, 1}, {0, 0, 2, 2}, {1, 1, 0, 0},
+      {1, 1, 2, 2}, {2, 2, 0, 0}, {2, 2, 1, 1}, {0, 1, 0, 1}, {0, 2, 0, 2}, {1, 0, 1, 0},
+      {1, 2, 1, 2}, {2, 0, 2, 0}, {2, 1, 2, 1}, {0, 1, 1, 0}, {0, 2, 2, 0}, {1, 0, 0, 1},
+      {1, 2, 2, 1}, {2, 0, 0, 2}, {2, 1, 1, 2}};
+  const int s = blockIdx.x, t3 = blockIdx.y, N = A.N, T1 = A.T1;
+  const int64_t tl = tj_of(A, s) + t3, plane = (int64_t)A.T3 * T1;
+  for (int e = threadIdx.x; e < 3 * N; e += blockDim.x) mul[e] = mu_at(A, tl, e / N)[e % N];
+  __syncthreads();
+  float2 t3n[9], t3s[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    t3n[j] = cld(g3 + ((int64_t)s * A.T3 + t3) * 9 + j);
+    t3s[j] = cld(g3 + (int64_t)s * A.T3 * 9 + j);
+    t3s[j].y = -t3s[j].y;
+  }
+  for (int t1 = threadIdx.x; t1 < T1; t1 += blockDim.x) {
+    float2 se1[9], gb1[9];
+    const float2 *lt = A.hist + (hrow_L(A, s, 0, t1) + t3) * N;
+    const int64_t rs = (int64_t)T1 * A.T3 * N;
+    dot9(lt, lt + rs, lt + 2 * rs, mul, N, N, se1);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) gb1[j] = gld(g1 + ((int64_t)s * 9 + j) * T1 + t1);
+#pragma unroll
     for (int m = 0; m < 21; ++m) {
-      const int x0 = c_polar[m][0], x1 = c_polar[m][1], x2 = c_polar[m][2], x3 = c_polar[m][3];
+      const int x0 = kPol[m][0], x1 = kPol[m][1], x2 = kPol[m][2], x3 = kPol[m][3];
       float e[4] = {0.f, 0.f, 0.f, 0.f};
       if (A.do_gb) {  // src/calc_2DIR.c:272-275
-        const float t3nr = sc[x2 * 3 + x3].x, t3ni = sc[x2 * 3 + x3].y;
-        const float t1nr = gb1[x0][x1].x, t1ni = gb1[x0][x1].y;
+        const float t3nr = t3n[x2 * 3 + x3].x, t3ni = t3n[x2 * 3 + x3].y;
+        const float t1nr = gb1[x0 * 3 + x1].x, t1ni = gb1[x0 * 3 + x1].y;
         e[0] -= __fsub_rn(__fmul_rn(t3ni, t1nr), __fmul_rn(t3nr, t1ni));
         e[1] -= __fadd_rn(__fmul_rn(t3nr, t1nr), __fmul_rn(t3ni, t1ni));
         e[2] -= __fadd_rn(__fmul_rn(t3ni, t1nr), __fmul_rn(t3nr, t1ni));
         e[3] -= __fsub_rn(__fmul_rn(t3nr, t1nr), __fmul_rn(t3ni, t1ni));
       }
       if (A.do_se) {  // src/calc_2DIR.c:397-400
-        const float t3rr = sc[9 + x1 * 3 + x3].x, t3ri = sc[9 + x1 * 3 + x3].y;
-        const float t3nr = sc[18 + x1 * 3 + x2].x, t3ni = sc[18 + x1 * 3 + x2].y;
-        const float t1rr = se1r[x0][x2].x, t1ri = se1r[x0][x2].y;
-        const float t1nr = se1[x0][x3].x, t1ni = se1[x0][x3].y;
+        const float t3rr = t3n[x1 * 3 + x3].x, t3ri = t3n[x1 * 3 + x3].y;
+        const float t3nr = t3s[x1 * 3 + x2].x, t3ni = t3s[x1 * 3 + x2].y;
+        const float t1rr = gb1[x0 * 3 + x2].x, t1ri = -gb1[x0 * 3 + x2].y;
+        const float t1nr = se1[x0 * 3 + x3].x, t1ni = se1[x0 * 3 + x3].y;
         e[0] -= __fadd_rn(__fmul_rn(t3rr, t1ri), __fmul_rn(t3ri, t1rr));
         e[1] -= __fsub_rn(__fmul_rn(t3rr, t1rr), __fmul_rn(t3ri, t1ri));
         e[2] -= __fadd_rn(__fmul_rn(t3ni, t1nr), __fmul_rn(t3nr, t1ni));
         e[3] -= __fsub_rn(__fmul_rn(t3nr, t1nr), __fmul_rn(t3ni, t1ni));
       }
-      float *E = A.E + (((int64_t)s * 21 + m) * 4) * A.T3 * A.T1;
-      for (int q = 0; q < 4; ++q) E[((int64_t)q * A.T3 + t3) * A.T1 + t1] = e[q];
+      float *E = A.E + (((int64_t)s * 21 + m) * 4) * plane + (int64_t)t3 * T1 + t1;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) gst(E + q * plane, e[q]);
     }
   }
 }
@@ -674,23 +713,40 @@ __global__ void __launch_bounds__(64 * kPW) dbl_ea_kernel(PersistArgs P) {
   }
 }
 
-// R[k][t3][t1] += sum_s sum_m E[s][m][q][t3][t1] * (w(p, m) lt(t3, t1)),
-// k = 4p + q, in a fixed order (double accumulation).
-__global__ void reduce2d_kernel(const float *E, int S, int T3, int T1, const float *wpl,
-                                const float *lt, double *R) {
+// part[c][k][t3][t1] = sum over samples s of chunk c (kRedS samples) and
+// items m of E[s][m][q][t3][t1] * (w(p, m) lt(t3, t1)), k = 4p + q, in
+// ascending (s, m) order and double accumulation; reduce2d_final_kernel adds
+// the chunks in ascending order to R.
+constexpr int kRedS = 16;
+__global__ void reduce2d_part_kernel(const float *E, int S, int T3, int T1, const float *wpl, const float *lt,
+                                     double *part) {
   const int64_t plane = (int64_t)T3 * T1;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= 12 * plane) return;
-  const int k = (int)(idx / plane), p = k / 4, q = k % 4;
-  const int64_t e0 = idx % plane;
+  const int64_t e0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y, p = k / 4, q = k % 4, c = blockIdx.z;
+  if (e0 >= plane) return;
   const float l = lt[e0];
+  float pw[21];
+#pragma unroll
+  for (int m = 0; m < 21; ++m) pw[m] = __fmul_rn(cld(wpl + p * 21 + m), l);
   double acc = 0.0;
-  for (int s = 0; s < S; ++s)
-    for (int m = 0; m < 21; ++m) {
-      const float pw = __fmul_rn(wpl[p * 21 + m], l);
-      if (pw == 0.f) continue;
-      acc += (double)__fmul_rn(E[((((int64_t)s * 21 + m) * 4) + q) * plane + e0], pw);
-    }
+  const int s1 = min(S, (c + 1) * kRedS);
+  for (int s = c * kRedS; s < s1; ++s) {
+    const float *Es = E + ((int64_t)s * 84 + q) * plane + e0;
+    float ev[21];
+#pragma unroll
+    for (int m = 0; m < 21; ++m) ev[m] = gld(Es + (int64_t)m * 4 * plane);
+#pragma unroll
+    for (int m = 0; m < 21; ++m)
+      if (pw[m] != 0.f) acc += (double)__fmul_rn(ev[m], pw[m]);
+  }
+  part[((int64_t)c * 12 + k) * plane + e0] = acc;
+}
+
+__global__ void reduce2d_final_kernel(const double *part, int nc, int64_t n, double *R) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  double acc = 0.0;
+  for (int c = 0; c < nc; ++c) acc += part[(int64_t)c * n + idx];
   R[idx] += acc;
 }
 
@@ -883,7 +939,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
   const int vw = doubles_vw(N);
   const int gps = (nvec + vw - 1) / vw;  // doubles groups per sample
 
-  DevBuf L0, hist, dbl, E, slots, wgb, vb, rows, lrrb, grp;
+  DevBuf L0, hist, dbl, E, slots, wgb, vb, rows, lrrb, grp, g1b, g3b, part;
   const EaShape ea = ea_shape(N, T1);
   ConveyorArgs a;
   a.N = N, a.K = sch.K, a.nlayers = sch.nlayers(), a.m = non->ts, a.f_lo = f_lo;
@@ -899,7 +955,9 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
     A.mu_rec = mud, A.mu_stride = ctype ? 0 : mu_stride, A.over_rec = Od;
     if ((rc = L0.alloc((size_t)S * 3 * T1 * N * 8)) ||
         (rc = hist.alloc((size_t)S * (3 * T1 + 3) * T3 * N * 8)) ||
-        (rc = E.alloc((size_t)S * 84 * plane * 4)))
+        (rc = E.alloc((size_t)S * 84 * plane * 4)) || (rc = g1b.alloc((size_t)S * 9 * T1 * 8)) ||
+        (rc = g3b.alloc((size_t)S * 9 * T3 * 8)) ||
+        (rc = part.alloc((size_t)((S + kRedS - 1) / kRedS) * 12 * plane * 8)))
       return rc;
     if (do_ea && (rc = dbl.alloc((size_t)S * nvec * N2 * 8))) return rc;
     A.L0 = L0.as<float2>(), A.hist = hist.as<float2>(), A.dbl = dbl.as<float2>(), A.E = E.as<float>();
@@ -945,7 +1003,11 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
     a.state = L0.as<float2>(), a.writeback = 0, a.hist = hist.as<float2>();
     if ((rc = launch_conveyor(a, vp, block, st))) return rc;
     // (3) GB and SE terms from the history
-    hipLaunchKernelGGL(gbse_kernel, dim3(S, T3), dim3(64), 27 * sizeof(float2), st, A);
+    hipLaunchKernelGGL(gbse_static_kernel, dim3(S), dim3(256), 3 * N * sizeof(float), st, A, g1b.as<float2>(),
+                       g3b.as<float2>());
+    NISE_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(gbse_kernel, dim3(S, T3), dim3(64), 3 * N * sizeof(float), st, A,
+                       (const float2 *)g1b.as<float2>(), (const float2 *)g3b.as<float2>());
     NISE_HIP_CHECK(hipGetLastError());
     // (4) EA: doubles vectors at tk, then per t3 the responses and one
     //     doubles step with frame tk + t3
@@ -1017,8 +1079,12 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
       }
     }
     // (5) fixed-order reduction of this chunk into the 12 planes
-    hipLaunchKernelGGL(reduce2d_kernel, dim3((unsigned)((12 * plane + 127) / 128)), dim3(128), 0, st,
-                       E.as<float>(), S, T3, T1, wpl.as<float>(), ltb.as<float>(), Rd.as<double>());
+    const int nred = (S + kRedS - 1) / kRedS;
+    hipLaunchKernelGGL(reduce2d_part_kernel, dim3((unsigned)((plane + 255) / 256), 12, nred), dim3(256), 0, st,
+                       E.as<float>(), S, T3, T1, wpl.as<float>(), ltb.as<float>(), part.as<double>());
+    NISE_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(reduce2d_final_kernel, dim3((unsigned)((12 * plane + 255) / 256)), dim3(256), 0, st,
+                       (const double *)part.as<double>(), nred, 12 * plane, Rd.as<double>());
     NISE_HIP_CHECK(hipGetLastError());
     NISE_HIP_CHECK(hipStreamSynchronize(st));  // host slot vectors are rebuilt next chunk
   }
