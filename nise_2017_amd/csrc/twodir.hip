@@ -249,23 +249,27 @@ __device__ __forceinline__ void ddl_multi(const float2 *f, int N, int k, const f
   float r[NIT], q[NIT];
 #pragma unroll
   for (int it = 0; it < NIT; ++it) r[it] = 0.f, q[it] = 0.f;
-  const int rsk = k * ((N << 1) - k - 1) / 2;
-  int rsi = 0;
+  // state (i, k): index k + rs(i) for i <= k, rs(k) + i above, rs(i) =
+  // i (2N - i - 1) / 2; stepped from i = 0 (index k)
+  const char *fb = reinterpret_cast<const char *>(f);
+  int ob = k * FS * 8;
   for (int i = 0; i < N; ++i) {
-    const float2 v = f[((i <= k) ? k + rsi : rsk + i) * FS];
+    const float2 v = *reinterpret_cast<const float2 *>(fb + ob);
+    const bool d = i == k;
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       // dip[i] is wave-uniform: read it once into a scalar register (a
       // convergent read, so it is not turned into a masked per-lane load)
       const float dv = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(dip[it][i])));
-      const float w = (i == k) ? ov[it] : dv;
+      const float w = d ? ov[it] : dv;
       r[it] = __fadd_rn(r[it], __fmul_rn(w, v.x)), q[it] = __fadd_rn(q[it], __fmul_rn(w, v.y));
     }
-    rsi += N - i - 1;
+    ob += (i < k) ? (N - i - 1) * FS * 8 : FS * 8;
   }
 #pragma unroll
   for (int it = 0; it < NIT; ++it) out[it] = make_float2(r[it], q[it]);
 }
+
 
 // dip_x(tl) and the overtone dipoles over_x(tl) for x = 0..2 into LDS
 // (src/calc_2DIR.c:450-454; sqrt(2) mu for a fixed anharmonicity,
@@ -466,29 +470,30 @@ struct PersistArgs {
   const float2 *D2;      // [frame - f_lo][N2]
   const float4 *cs2;     // [frame - f_lo][K2]
   int64_t f_lo;
+  int npw;               // waves per workgroup
 };
-
-// Workgroups of kPW waves: the waves hold groups of the same sample and
-// share the staged frame tables, dipoles and M rows.
-constexpr int kPW = 4;
 
 struct PersistLayout {
-  size_t ix, d2, rn, dip, ov, psi, lv, total;  // psi and lv: kPW per-wave blocks
+  size_t ix, d2, rn, dip, ov, psi, lv, total;  // psi and lv: npw per-wave blocks
 };
-// ix: per coupling k, site block cb and lane c0 the packed state indices
-// (i1 | i2 << 16) of the rotation of site c = cb NP + c0, or ~0u for none.
-__host__ __device__ inline PersistLayout persist_layout(int N, int VW, int K2) {
+// Workgroups of npw waves: the waves hold groups of the same sample and
+// share the staged frame tables, dipoles and M rows.
+// ix: per coupling k, site block cb and lane c0 the packed byte offsets
+// (o1 | o2 << 16, o = state index x VW x 8) of the rotation of site c =
+// cb NP + c0 in a wave's vectors, or ~0u for none; one more row of ~0u
+// ends the table.
+__host__ __device__ inline PersistLayout persist_layout(int N, int VW, int K2, int npw) {
   const size_t N2 = (size_t)N * (N + 1) / 2;
   const int NP = 64 / VW, nb = (N + NP - 1) / NP;
   PersistLayout L;
   size_t o = 0;
-  L.ix = o, o += (size_t)K2 * nb * NP * 4, o = (o + 15) & ~(size_t)15;
+  L.ix = o, o += ((size_t)K2 * nb + 2) * NP * 4, o = (o + 15) & ~(size_t)15;
   L.d2 = o, o += N2 * 8;
   L.rn = o, o += (size_t)3 * N * 8;
   L.dip = o, o += (size_t)3 * N * 4;
   L.ov = o, o += (size_t)3 * N * 4, o = (o + 15) & ~(size_t)15;
-  L.psi = o, o += kPW * N2 * VW * 8;
-  L.lv = o, o += (size_t)kPW * VW * 3 * N * 8;
+  L.psi = o, o += (size_t)npw * N2 * VW * 8;
+  L.lv = o, o += (size_t)npw * VW * 3 * N * 8;
   L.total = o;
   return L;
 }
@@ -529,6 +534,11 @@ __device__ __forceinline__ void persist_ft(const TwoDirArgs &A, int lane, int s,
   const int N = A.N, T1 = A.T1, t1_0 = (v0 - 9) % T1;
   const int x3a = c_polar[mi.x][3], x3b = c_polar[mi.y][3], x3c = NIT > 2 ? c_polar[mi.z][3] : 0;
   const int x3v[3] = {x3a, x3b, x3c};
+  // the lane of (c, k) forms the k-th rounded term of each item's dot
+  // (leftnr_EA[t1] against rightnr = conj(M[x1](t3)), src/calc_2DIR.c
+  // :457-459, :468-469); one lane per (c, item) then adds the terms in
+  // ascending k, the reference's order
+  const int x1v[3] = {c_polar[mi.x][1], c_polar[mi.y][1], NIT > 2 ? c_polar[mi.z][1] : 0};
   for (int u = lane; u < nv * N; u += 64) {
     const int c = u / N, k = u % N;
     float ov[NIT];
@@ -537,20 +547,23 @@ __device__ __forceinline__ void persist_ft(const TwoDirArgs &A, int lane, int s,
     for (int it = 0; it < NIT; ++it) ov[it] = ovL[x3v[it] * N + k];
     ddl_multi<NIT, VW>(psi + c, N, k, dipL + x3a * N, dipL + x3b * N, dipL + x3c * N, ov, o);
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) LvL[(c * NIT + it) * N + k] = o[it];
+    for (int it = 0; it < NIT; ++it) {
+      const float2 l = o[it], Rv = RnL[x1v[it] * N + k];
+      const float Nr = Rv.x, Ni = -Rv.y;
+      LvL[(c * NIT + it) * N + k] = make_float2(__fadd_rn(__fmul_rn(Nr, l.y), __fmul_rn(Ni, l.x)),
+                                                __fsub_rn(__fmul_rn(Nr, l.x), __fmul_rn(Ni, l.y)));
+    }
   }
   asm volatile("" ::: "memory");
   const int64_t plane = (int64_t)A.T3 * T1;
   for (int j = lane; j < NIT * nv; j += 64) {
     const int it = j % NIT, c = j / NIT, t1 = t1_0 + c;
-    const int m = it == 0 ? mi.x : it == 1 ? mi.y : mi.z, x1 = c_polar[m][1];
-    const float2 *Rn = RnL + x1 * N, *Lv = LvL + (c * NIT + it) * N;
+    const int m = it == 0 ? mi.x : it == 1 ? mi.y : mi.z;
+    const float2 *Tm = LvL + (c * NIT + it) * N;
     float rr = 0.f, ri = 0.f;
-    for (int i = 0; i < N; ++i) {
-      const float2 l = Lv[i], Rv = Rn[i];
-      const float Nr = Rv.x, Ni = -Rv.y;
-      rr = __fadd_rn(rr, __fadd_rn(__fmul_rn(Nr, l.y), __fmul_rn(Ni, l.x)));
-      ri = __fadd_rn(ri, __fsub_rn(__fmul_rn(Nr, l.x), __fmul_rn(Ni, l.y)));
+    for (int k = 0; k < N; ++k) {
+      const float2 v = Tm[k];
+      rr = __fadd_rn(rr, v.x), ri = __fadd_rn(ri, v.y);
     }
     float *E = A.E + (((int64_t)s * 21 + m) * 4 + 2) * plane + (int64_t)t3 * T1 + t1;
     E[0] += rr, E[plane] += ri;
@@ -587,54 +600,85 @@ __device__ __forceinline__ void persist_fr(const TwoDirArgs &A, int lane, int s,
 }
 
 // The rotations of one substep (as dbl_rotate) with the per-lane state
-// indices from the table ix, each lane's next entry read one rotation ahead,
+// offsets from the table ix, each lane's next entry read one rotation ahead,
 // and the pair words and angles read by scalar loads (they are wave-uniform
-// and read-only).
+// and read-only).  The lane of c == a also rotates (|ab>, |bb>); the offset
+// of |bb> and the test c == a use the scalar pair word.
+template <int NP>
+__device__ __forceinline__ void rot_tab_one(char *base, int N, bool act, uint32_t e, uint32_t p, float4 cv, int cb,
+                                            int c0) {
+  constexpr int VW = 64 / NP;
+  const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
+  const int o3 = (b + ((b * (2 * N - b - 1)) >> 1)) * (VW * 8);
+  const bool ca = c0 == a - cb * NP;
+  if (e != ~0u && act) {
+    float2 *a1 = reinterpret_cast<float2 *>(base + (e & 0xffffu));
+    float2 *a2 = reinterpret_cast<float2 *>(base + (e >> 16));
+    float2 *a3 = reinterpret_cast<float2 *>(base + o3);
+    const float cl = ca ? cv.z : cv.x, sl = ca ? cv.w : cv.y;
+    float2 p1 = *a1, p2 = *a2, p3;
+    if (ca) p3 = *a3;
+    rot2(p1, p2, cl, sl);
+    *a1 = p1;
+    if (ca) {
+      rot2(p2, p3, cv.z, cv.w);
+      *a3 = p3;
+    }
+    *a2 = p2;
+  }
+}
 template <int NP>
 __device__ __forceinline__ void dbl_rotate_tab(float2 *psi, int N, int nv, int K2, const uint32_t *ix,
                                                const uint32_t *ab2, const float4 *cs, int q, int c0) {
-  constexpr int VW = 64 / NP;
   const int nb = (N + NP - 1) / NP, total = K2 * nb;
   if (total == 0) return;
-  uint32_t e = ix[c0];
-  uint32_t p = cld(ab2);
-  float4 cv = cld(cs);
+  char *base = reinterpret_cast<char *>(psi) + q * 8;
+  const bool act = q < nv;
   int k = 0, cb = 0;
-  for (int j = 0; j < total; ++j) {
-    const uint32_t en = j + 1 < total ? ix[(j + 1) * NP + c0] : ~0u;
-    const int kn = min(cb + 1 == nb ? k + 1 : k, K2 - 1);
-    const uint32_t pn = cld(ab2 + kn);
-    const float4 cvn = cld(cs + kn);
-    const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
-    const int i3 = (b + ((b * (2 * N - b - 1)) >> 1)) * VW + q;
-    if (e != ~0u && q < nv) {
-      const int i1 = (int)(e & 0xffffu) * VW + q, i2 = (int)(e >> 16) * VW + q;
-      const bool ca = cb * NP + c0 == a;
-      const float cl = ca ? cv.z : cv.x, sl = ca ? cv.w : cv.y;
-      float2 p1 = psi[i1], p2 = psi[i2], p3 = make_float2(0.f, 0.f);
-      if (ca) p3 = psi[i3];
-      rot2(p1, p2, cl, sl);
-      psi[i1] = p1;
-      if (ca) {
-        rot2(p2, p3, cv.z, cv.w);
-        psi[i3] = p3;
-      }
-      psi[i2] = p2;
-    }
-    asm volatile("" ::: "memory");
-    e = en, p = pn, cv = cvn;
-    if (++cb == nb) cb = 0, ++k;
+  uint32_t e0 = ix[c0];
+  uint32_t p0 = cld(ab2);
+  float4 v0 = cld(cs);
+  int j = 0;
+  for (; j + 1 < total; j += 2) {
+    const int cb1 = cb + 1 == nb ? 0 : cb + 1, k1 = cb + 1 == nb ? k + 1 : k;
+    const uint32_t e1 = ix[(j + 1) * NP + c0], p1 = cld(ab2 + k1);
+    const float4 v1 = cld(cs + k1);
+    rot_tab_one<NP>(base, N, act, e0, p0, v0, cb, c0);
+    const int cb2 = cb1 + 1 == nb ? 0 : cb1 + 1, k2 = min(cb1 + 1 == nb ? k1 + 1 : k1, K2 - 1);
+    e0 = ix[(j + 2) * NP + c0], p0 = cld(ab2 + k2), v0 = cld(cs + k2);
+    rot_tab_one<NP>(base, N, act, e1, p1, v1, cb1, c0);
+    cb = cb2, k = k2;
   }
+  if (j < total) rot_tab_one<NP>(base, N, act, e0, p0, v0, cb, c0);
+}
+
+// psi[j * VW + w] *= d2[j] for the states j < N2 of the VW vectors, two
+// entries per lane access when VW > 1 (src/propagate.c:960-961).
+template <int VW>
+__device__ __forceinline__ void dbl_diag(float2 *psi, const float2 *d2, int N2, int lane) {
+  if constexpr (VW == 1) {
+    for (int j = lane; j < N2; j += 64) psi[j] = cmul_diag2(psi[j], d2[j]);
+  } else {
+    float4 *p4 = reinterpret_cast<float4 *>(psi);
+    for (int j = lane; j < N2 * (VW / 2); j += 64) {
+      const float2 d = d2[j / (VW / 2)];
+      const float4 v = p4[j];
+      const float2 lo = cmul_diag2(make_float2(v.x, v.y), d), hi = cmul_diag2(make_float2(v.z, v.w), d);
+      p4[j] = make_float4(lo.x, lo.y, hi.x, hi.y);
+    }
+  }
+  asm volatile("" ::: "memory");
 }
 
 template <int NP>
-__global__ void __launch_bounds__(64 * kPW) dbl_ea_kernel(PersistArgs P) {
+__global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int VW = 64 / NP;
   const TwoDirArgs &A = P.A;
   const int N = A.N, N2 = N * (N + 1) / 2, K2 = P.K2, T1 = A.T1, T3 = A.T3, t = threadIdx.x;
   const int wave = t >> 6, lane = t & 63;
-  const PersistLayout Ly = persist_layout(N, VW, K2);
+  const int npw = P.npw, nt = blockDim.x;
+  const PersistLayout Ly = persist_layout(N, VW, K2, npw);
   uint32_t *ixL = reinterpret_cast<uint32_t *>(smem + Ly.ix);
   float2 *d2 = reinterpret_cast<float2 *>(smem + Ly.d2);
   float2 *RnL = reinterpret_cast<float2 *>(smem + Ly.rn);
@@ -642,8 +686,8 @@ __global__ void __launch_bounds__(64 * kPW) dbl_ea_kernel(PersistArgs P) {
   float *ovL = reinterpret_cast<float *>(smem + Ly.ov);
   float2 *psi = reinterpret_cast<float2 *>(smem + Ly.psi) + (size_t)wave * N2 * VW;
   float2 *LvL = reinterpret_cast<float2 *>(smem + Ly.lv) + (size_t)wave * VW * 3 * N;
-  const DblGroup gd = P.groups[(size_t)blockIdx.x * kPW + wave];
-  const int s = P.groups[(size_t)blockIdx.x * kPW].s;  // every wave of the workgroup: one sample
+  const DblGroup gd = P.groups[(size_t)blockIdx.x * npw + wave];
+  const int s = P.groups[(size_t)blockIdx.x * npw].s;  // every wave of the workgroup: one sample
   const int kind = gd.kind, v0 = gd.v0, nv = gd.nv;
   const int nvec = 9 + 9 * T1;
   // this wave's vectors at t3 = 0 (psi[j][w]); the pair words
@@ -655,16 +699,16 @@ __global__ void __launch_bounds__(64 * kPW) dbl_ea_kernel(PersistArgs P) {
   // the c == b rotation (|ab>, |bb>); two-level sites skip c == a
   {
     const int nb = (N + NP - 1) / NP;
-    for (int e = t; e < K2 * nb * NP; e += 64 * kPW) {
+    for (int e = t; e < (K2 * nb + 2) * NP; e += nt) {
       const int k = e / (nb * NP), c = e % (nb * NP);
-      const uint32_t p = P.ab2[k];
+      const uint32_t p = k < K2 ? P.ab2[k] : 0u;
       const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
       uint32_t v = ~0u;
-      if (c < N && c != b && !(A.es && c == a)) {
+      if (k < K2 && c < N && c != b && !(A.es && c == a)) {
         const int rs_c = (c * (2 * N - c - 1)) >> 1;
         const int rs_a = (a * (2 * N - a - 1)) >> 1, rs_b = (b * (2 * N - b - 1)) >> 1;
         const int i1 = c <= a ? a + rs_c : c + rs_a, i2 = c <= b ? b + rs_c : c + rs_b;
-        v = (uint32_t)i1 | ((uint32_t)i2 << 16);
+        v = (uint32_t)(i1 * VW * 8) | ((uint32_t)(i2 * VW * 8) << 16);
       }
       ixL[e] = v;
     }
@@ -687,7 +731,7 @@ __global__ void __launch_bounds__(64 * kPW) dbl_ea_kernel(PersistArgs P) {
   for (int t3 = 0; t3 < T3; ++t3) {
     const int64_t tl = tk + t3;
     load_dip_over(A, tl, dipL, ovL);
-    for (int e = t; e < 3 * N; e += 64 * kPW) RnL[e] = A.hist[(hrow_M(A, s, e / N) + t3) * N + e % N];
+    for (int e = t; e < 3 * N; e += nt) RnL[e] = A.hist[(hrow_M(A, s, e / N) + t3) * N + e % N];
     __syncthreads();
     if (nv > 0) {
       if (kind == 1) {
@@ -702,11 +746,9 @@ __global__ void __launch_bounds__(64 * kPW) dbl_ea_kernel(PersistArgs P) {
       stage_lds<8>(d2, P.D2 + row * N2, N2);
       __syncthreads();
       for (int sub = 0; sub < P.m; ++sub) {
-        for (int j = lane; j < N2 * VW; j += 64) psi[j] = cmul_diag2(psi[j], d2[j / VW]);
-        asm volatile("" ::: "memory");
+        dbl_diag<VW>(psi, d2, N2, lane);
         dbl_rotate_tab<NP>(psi, N, nv, K2, ixL, P.ab2, P.cs2 + row * K2, q, c0);
-        for (int j = lane; j < N2 * VW; j += 64) psi[j] = cmul_diag2(psi[j], d2[j / VW]);
-        asm volatile("" ::: "memory");
+        dbl_diag<VW>(psi, d2, N2, lane);
       }
     }
     __syncthreads();  // the staged rows are overwritten at the next t3
@@ -1012,8 +1054,21 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
     // (4) EA: doubles vectors at tk, then per t3 the responses and one
     //     doubles step with frame tk + t3
     const int K2 = (int)lex.size(), np = doubles_np(N), pvw = 64 / np;
-    const size_t psmem = persist_layout(N, pvw, K2).total;
-    if (do_ea && N2 <= 65535 && psmem <= 160 * 1024) {
+    // waves per workgroup: the most resident waves per CU (160 KB of LDS),
+    // counting the padding of each sample's groups to whole workgroups; at
+    // most 4 (larger workgroups wait longer at the per-t3 barriers: 7 waves
+    // of the N = 32 ring ran 15% slower than 4 at 2 more waves per CU)
+    const int gps_p = (9 + pvw - 1) / pvw + 9 * ((T1 + pvw - 1) / pvw);
+    int npw = 0;
+    double best = 0;
+    for (int w = 1; w <= 4; ++w) {
+      const size_t L = persist_layout(N, pvw, K2, w).total;
+      if (L > 160 * 1024) break;
+      const double score = (double)(160 * 1024 / L) * w * gps_p / (double)((gps_p + w - 1) / w * w);
+      if (score > best * 1.0001) best = score, npw = w;
+    }
+    const size_t psmem = npw ? persist_layout(N, pvw, K2, npw).total : 0;
+    if (do_ea && npw && (size_t)N2 * pvw * 8 <= 65536) {
       // every doubles vector stays in the LDS of its workgroup for all t3
       hipLaunchKernelGGL(dd_kernel, dim3((N2 + 127) / 128, S, nvec), dim3(128), 0, st, A);
       NISE_HIP_CHECK(hipGetLastError());
@@ -1022,19 +1077,19 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
         for (int v0 = 0; v0 < 9; v0 += pvw) gh.push_back(DblGroup{s, 0, v0, std::min(pvw, 9 - v0)});
         for (int p = 0; p < 9; ++p)
           for (int t1 = 0; t1 < T1; t1 += pvw) gh.push_back(DblGroup{s, 1, 9 + p * T1 + t1, std::min(pvw, T1 - t1)});
-        while (gh.size() % kPW) gh.push_back(DblGroup{s, 0, 0, 0});  // workgroups never mix samples
+        while (gh.size() % npw) gh.push_back(DblGroup{s, 0, 0, 0});  // workgroups never mix samples
       }
       if ((rc = upl(grp, gh.data(), gh.size() * sizeof(DblGroup)))) return rc;
       PersistArgs P;
       P.A = A, P.groups = grp.as<DblGroup>(), P.K2 = K2, P.m = non->ts, P.ab2 = ab2.as<uint32_t>();
-      P.D2 = D2.as<float2>(), P.cs2 = cs2.as<float4>(), P.f_lo = f_lo;
+      P.D2 = D2.as<float2>(), P.cs2 = cs2.as<float4>(), P.f_lo = f_lo, P.npw = npw;
       const void *fp = np == 8 ? (const void *)dbl_ea_kernel<8>
                      : np == 16 ? (const void *)dbl_ea_kernel<16>
                      : np == 32 ? (const void *)dbl_ea_kernel<32>
                                 : (const void *)dbl_ea_kernel<64>;
       NISE_HIP_CHECK(hipFuncSetAttribute(fp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)psmem));
       void *kargs[] = {&P};
-      NISE_HIP_CHECK(hipLaunchKernel(fp, dim3((unsigned)(gh.size() / kPW)), dim3(64 * kPW), kargs, psmem, st));
+      NISE_HIP_CHECK(hipLaunchKernel(fp, dim3((unsigned)(gh.size() / npw)), dim3(64 * npw), kargs, psmem, st));
     } else if (do_ea) {
       // per t3: leftrr, the EA responses from the doubles vectors in device
       // memory, then one doubles step of every vector
