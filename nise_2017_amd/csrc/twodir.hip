@@ -474,14 +474,16 @@ struct PersistArgs {
 };
 
 struct PersistLayout {
-  size_t ix, d2, rn, dip, ov, psi, lv, total;  // psi and lv: npw per-wave blocks
+  size_t ix, d2, rn, dip, ov, cs, psi, lv, total;  // psi and lv: npw per-wave blocks
 };
 // Workgroups of npw waves: the waves hold groups of the same sample and
 // share the staged frame tables, dipoles and M rows.
 // ix: per coupling k, site block cb and lane c0 the packed byte offsets
 // (o1 | o2 << 16, o = state index x VW x 8) of the rotation of site c =
-// cb NP + c0 in a wave's vectors, or ~0u for none; one more row of ~0u
-// ends the table.
+// cb NP + c0 in a wave's vectors; lanes with no rotation point both
+// offsets at a zero state after the N2 states of the wave (rotating zeros
+// leaves zeros), and two such rows end the table.  cs: the frame's angle
+// row.  psi: N2 + 1 states of VW vectors per wave.
 __host__ __device__ inline PersistLayout persist_layout(int N, int VW, int K2, int npw) {
   const size_t N2 = (size_t)N * (N + 1) / 2;
   const int NP = 64 / VW, nb = (N + NP - 1) / NP;
@@ -492,7 +494,8 @@ __host__ __device__ inline PersistLayout persist_layout(int N, int VW, int K2, i
   L.rn = o, o += (size_t)3 * N * 8;
   L.dip = o, o += (size_t)3 * N * 4;
   L.ov = o, o += (size_t)3 * N * 4, o = (o + 15) & ~(size_t)15;
-  L.psi = o, o += (size_t)npw * N2 * VW * 8;
+  L.cs = o, o += (size_t)K2 * 16;
+  L.psi = o, o += (size_t)npw * (N2 + 1) * VW * 8;
   L.lv = o, o += (size_t)npw * VW * 3 * N * 8;
   L.total = o;
   return L;
@@ -600,56 +603,56 @@ __device__ __forceinline__ void persist_fr(const TwoDirArgs &A, int lane, int s,
 }
 
 // The rotations of one substep (as dbl_rotate) with the per-lane state
-// offsets from the table ix, each lane's next entry read one rotation ahead,
-// and the pair words and angles read by scalar loads (they are wave-uniform
-// and read-only).  The lane of c == a also rotates (|ab>, |bb>); the offset
-// of |bb> and the test c == a use the scalar pair word.
+// offsets from the table ix, each lane's next entry read one rotation ahead.
+// The pair words are wave-uniform and read by scalar loads; the angles come
+// from the staged row csL by broadcast reads.  The lane of c == a also
+// rotates (|ab>, |bb>) (not for two-level sites); the offset of |bb> and the
+// test c == a use the scalar pair word.  Every lane rotates: lanes with no
+// rotation and vector slots past nv hold zeros.
 template <int NP>
-__device__ __forceinline__ void rot_tab_one(char *base, int N, bool act, uint32_t e, uint32_t p, float4 cv, int cb,
+__device__ __forceinline__ void rot_tab_one(char *base, int N, bool es, uint32_t e, uint32_t p, float4 cv, int cb,
                                             int c0) {
   constexpr int VW = 64 / NP;
   const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
   const int o3 = (b + ((b * (2 * N - b - 1)) >> 1)) * (VW * 8);
-  const bool ca = c0 == a - cb * NP;
-  if (e != ~0u && act) {
-    float2 *a1 = reinterpret_cast<float2 *>(base + (e & 0xffffu));
-    float2 *a2 = reinterpret_cast<float2 *>(base + (e >> 16));
-    float2 *a3 = reinterpret_cast<float2 *>(base + o3);
-    const float cl = ca ? cv.z : cv.x, sl = ca ? cv.w : cv.y;
-    float2 p1 = *a1, p2 = *a2, p3;
-    if (ca) p3 = *a3;
-    rot2(p1, p2, cl, sl);
-    *a1 = p1;
-    if (ca) {
-      rot2(p2, p3, cv.z, cv.w);
-      *a3 = p3;
-    }
-    *a2 = p2;
+  const bool ca = !es && c0 == a - cb * NP;
+  float2 *a1 = reinterpret_cast<float2 *>(base + (e & 0xffffu));
+  float2 *a2 = reinterpret_cast<float2 *>(base + (e >> 16));
+  float2 *a3 = reinterpret_cast<float2 *>(base + o3);
+  const float cl = ca ? cv.z : cv.x, sl = ca ? cv.w : cv.y;
+  float2 p1 = *a1, p2 = *a2, p3;
+  if (ca) p3 = *a3;
+  rot2(p1, p2, cl, sl);
+  *a1 = p1;
+  if (ca) {
+    rot2(p2, p3, cv.z, cv.w);
+    *a3 = p3;
   }
+  *a2 = p2;
 }
 template <int NP>
-__device__ __forceinline__ void dbl_rotate_tab(float2 *psi, int N, int nv, int K2, const uint32_t *ix,
-                                               const uint32_t *ab2, const float4 *cs, int q, int c0) {
+__device__ __forceinline__ void dbl_rotate_tab(float2 *psi, int N, bool es, int K2, const uint32_t *ix,
+                                               const uint32_t *ab2, const float4 *csL, int q, int c0) {
   const int nb = (N + NP - 1) / NP, total = K2 * nb;
   if (total == 0) return;
   char *base = reinterpret_cast<char *>(psi) + q * 8;
-  const bool act = q < nv;
   int k = 0, cb = 0;
   uint32_t e0 = ix[c0];
   uint32_t p0 = cld(ab2);
-  float4 v0 = cld(cs);
+  float4 v0 = csL[0];
   int j = 0;
   for (; j + 1 < total; j += 2) {
     const int cb1 = cb + 1 == nb ? 0 : cb + 1, k1 = cb + 1 == nb ? k + 1 : k;
     const uint32_t e1 = ix[(j + 1) * NP + c0], p1 = cld(ab2 + k1);
-    const float4 v1 = cld(cs + k1);
-    rot_tab_one<NP>(base, N, act, e0, p0, v0, cb, c0);
+    const float4 v1 = csL[k1];
+    rot_tab_one<NP>(base, N, es, e0, p0, v0, cb, c0);
     const int cb2 = cb1 + 1 == nb ? 0 : cb1 + 1, k2 = min(cb1 + 1 == nb ? k1 + 1 : k1, K2 - 1);
-    e0 = ix[(j + 2) * NP + c0], p0 = cld(ab2 + k2), v0 = cld(cs + k2);
-    rot_tab_one<NP>(base, N, act, e1, p1, v1, cb1, c0);
+    e0 = ix[(j + 2) * NP + c0], p0 = cld(ab2 + k2), v0 = csL[k2];
+    rot_tab_one<NP>(base, N, es, e1, p1, v1, cb1, c0);
     cb = cb2, k = k2;
   }
-  if (j < total) rot_tab_one<NP>(base, N, act, e0, p0, v0, cb, c0);
+  if (j < total) rot_tab_one<NP>(base, N, es, e0, p0, v0, cb, c0);
+  asm volatile("" ::: "memory");
 }
 
 // psi[j * VW + w] *= d2[j] for the states j < N2 of the VW vectors, two
@@ -684,7 +687,8 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
   float2 *RnL = reinterpret_cast<float2 *>(smem + Ly.rn);
   float *dipL = reinterpret_cast<float *>(smem + Ly.dip);
   float *ovL = reinterpret_cast<float *>(smem + Ly.ov);
-  float2 *psi = reinterpret_cast<float2 *>(smem + Ly.psi) + (size_t)wave * N2 * VW;
+  float4 *csL = reinterpret_cast<float4 *>(smem + Ly.cs);
+  float2 *psi = reinterpret_cast<float2 *>(smem + Ly.psi) + (size_t)wave * (N2 + 1) * VW;
   float2 *LvL = reinterpret_cast<float2 *>(smem + Ly.lv) + (size_t)wave * VW * 3 * N;
   const DblGroup gd = P.groups[(size_t)blockIdx.x * npw + wave];
   const int s = P.groups[(size_t)blockIdx.x * npw].s;  // every wave of the workgroup: one sample
@@ -695,6 +699,7 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
     const float2 *src = A.dbl + ((int64_t)s * nvec + v0 + w) * N2;
     for (int j = lane; j < N2; j += 64) psi[j * VW + w] = (w < nv) ? gld(src + j) : make_float2(0.f, 0.f);
   }
+  if (lane < VW) psi[N2 * VW + lane] = make_float2(0.f, 0.f);  // the zero state
   // the rotation index table: every c except b, ascending; c == a also does
   // the c == b rotation (|ab>, |bb>); two-level sites skip c == a
   {
@@ -703,7 +708,7 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
       const int k = e / (nb * NP), c = e % (nb * NP);
       const uint32_t p = k < K2 ? P.ab2[k] : 0u;
       const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
-      uint32_t v = ~0u;
+      uint32_t v = (uint32_t)(N2 * VW * 8) * 0x10001u;  // the zero state
       if (k < K2 && c < N && c != b && !(A.es && c == a)) {
         const int rs_c = (c * (2 * N - c - 1)) >> 1;
         const int rs_a = (a * (2 * N - a - 1)) >> 1, rs_b = (b * (2 * N - b - 1)) >> 1;
@@ -744,10 +749,11 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
     if (t3 + 1 < T3) {  // the doubles step of frame tk + t3 (the state after the last t3 is not used)
       const int64_t row = tk + t3 - P.f_lo;
       stage_lds<8>(d2, P.D2 + row * N2, N2);
+      for (int e = t; e < K2; e += nt) csL[e] = gld(P.cs2 + row * K2 + e);
       __syncthreads();
-      for (int sub = 0; sub < P.m; ++sub) {
+      for (int sub = 0; sub < (nv > 0 ? P.m : 0); ++sub) {
         dbl_diag<VW>(psi, d2, N2, lane);
-        dbl_rotate_tab<NP>(psi, N, nv, K2, ixL, P.ab2, P.cs2 + row * K2, q, c0);
+        dbl_rotate_tab<NP>(psi, N, A.es != 0, K2, ixL, P.ab2, csL, q, c0);
         dbl_diag<VW>(psi, d2, N2, lane);
       }
     }
@@ -1068,7 +1074,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
       if (score > best * 1.0001) best = score, npw = w;
     }
     const size_t psmem = npw ? persist_layout(N, pvw, K2, npw).total : 0;
-    if (do_ea && npw && (size_t)N2 * pvw * 8 <= 65536) {
+    if (do_ea && npw && (size_t)(N2 + 1) * pvw * 8 <= 65536) {
       // every doubles vector stays in the LDS of its workgroup for all t3
       hipLaunchKernelGGL(dd_kernel, dim3((N2 + 127) / 128, S, nvec), dim3(128), 0, st, A);
       NISE_HIP_CHECK(hipGetLastError());
