@@ -220,8 +220,14 @@ def twodir_leg(args, torch):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     items = ns * 21
+    # the reference's propagation work per item (src/calc_2DIR.c:184-506, t1 = t3 = T): singles vector-steps
+    # sum_t1 t1 (GB seeds) + T (GB t3) + (T + 1) T (SE) + (T + 1) T (EA right-hand vectors), and doubles
+    # vector-steps (T + 1) T; one site-step per amplitude and frame (10,400 x 32 + 4,160 x 528 at N = 32, T = 64)
+    singles = T * (T - 1) // 2 + T + 2 * (T + 1) * T
+    ref_site_steps = items * (singles * N + (T + 1) * T * (N * (N + 1) // 2))
     res = {"workload": f"N={N} ring, t1=t3={T}, {ns} samples x 21 items, anharmonicity 16",
-           "wall_ms": round(wall * 1e3, 3), "items_per_s": items / wall}
+           "wall_ms": round(wall * 1e3, 3), "items_per_s": items / wall,
+           "reference_site_steps_per_s": ref_site_steps / wall}
     log(f"2DIR leg: {wall * 1e3:.1f} ms for {items} work items")
     if not args.no_cpu:
         from oracle import oracle

@@ -115,6 +115,20 @@ def test_partial_t1_chunk_vs_oracle(orc):
     assert rel(R, ref) < TOL
 
 
+@pytest.mark.parametrize("tech", ["2DIR", "2DUVvis"])
+def test_ring48_one_vector_per_wave_vs_oracle(orc, tech):
+    """N = 48 ring: 48 kept couplings, so the two-exciton vectors (1176
+    states) stay in LDS with one vector per wave (64 lanes over the sites)."""
+    say(f"N=48 ring {tech}")
+    N, T1, T3 = 48, 5, 6
+    H, mu = synth.ring_2dir(N=N, nframes=40, seed=52)
+    non = mk(N, T1, T3, 6, 0, 3, H, technique=tech, lifetime=300.0)
+    ref = orc.twodir(non, H, mu, acc64=True)
+    non = mk(N, T1, T3, 6, 0, 3, H, technique=tech, lifetime=300.0)
+    R = abi.twodir_batched(non, H, mu)
+    assert rel(R, ref) < TOL
+
+
 def test_wide_system_unstaged_vs_oracle(orc):
     """N = 160: a doubles vector (12880 states) exceeds the EA kernel's LDS
     staging budget, so it reads the vectors from device memory.  The
