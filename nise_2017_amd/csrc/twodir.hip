@@ -270,6 +270,40 @@ __device__ __forceinline__ void ddl_multi(const float2 *f, int N, int k, const f
   for (int it = 0; it < NIT; ++it) out[it] = make_float2(r[it], q[it]);
 }
 
+// ddl_multi for N % 4 == 0 with 16-byte aligned dipole rows in LDS: one
+// broadcast 128-bit read per item covers four i, and the state address is
+// stepped from i = 0 (index k; + N - i - 1 while i < k, + 1 after).
+template <int NIT, int FS>
+__device__ __forceinline__ void ddl_multi4(const float2 *f, int N, int k, const float *dip0, const float *dip1,
+                                           const float *dip2, const float ov[NIT], float2 out[NIT]) {
+  const float *dip[3] = {dip0, dip1, dip2};
+  float r[NIT], q[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) r[it] = 0.f, q[it] = 0.f;
+  const char *fb = reinterpret_cast<const char *>(f);
+  int ob = k * FS * 8;
+  for (int i0 = 0; i0 < N; i0 += 4) {
+    float4 d4[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) d4[it] = *reinterpret_cast<const float4 *>(dip[it] + i0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + u;
+      const float2 v = *reinterpret_cast<const float2 *>(fb + ob);
+      const bool d = i == k;
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        const float dv = u == 0 ? d4[it].x : u == 1 ? d4[it].y : u == 2 ? d4[it].z : d4[it].w;
+        const float w = d ? ov[it] : dv;
+        r[it] = __fadd_rn(r[it], __fmul_rn(w, v.x)), q[it] = __fadd_rn(q[it], __fmul_rn(w, v.y));
+      }
+      ob += (i < k) ? (N - i - 1) * FS * 8 : FS * 8;
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) out[it] = make_float2(r[it], q[it]);
+}
+
 
 // dip_x(tl) and the overtone dipoles over_x(tl) for x = 0..2 into LDS
 // (src/calc_2DIR.c:450-454; sqrt(2) mu for a fixed anharmonicity,
@@ -548,7 +582,10 @@ __device__ __forceinline__ void persist_ft(const TwoDirArgs &A, int lane, int s,
     float2 o[NIT];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) ov[it] = ovL[x3v[it] * N + k];
-    ddl_multi<NIT, VW>(psi + c, N, k, dipL + x3a * N, dipL + x3b * N, dipL + x3c * N, ov, o);
+    if ((N & 3) == 0)
+      ddl_multi4<NIT, VW>(psi + c, N, k, dipL + x3a * N, dipL + x3b * N, dipL + x3c * N, ov, o);
+    else
+      ddl_multi<NIT, VW>(psi + c, N, k, dipL + x3a * N, dipL + x3b * N, dipL + x3c * N, ov, o);
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const float2 l = o[it], Rv = RnL[x1v[it] * N + k];
@@ -584,7 +621,10 @@ __device__ __forceinline__ void persist_fr(const TwoDirArgs &A, int lane, int s,
     const int c = u / N, k = u % N;
     const float ov[3] = {ovL[k], ovL[N + k], ovL[2 * N + k]};
     float2 o[3];
-    ddl_multi<3, VW>(psi + c, N, k, dipL, dipL + N, dipL + 2 * N, ov, o);
+    if ((N & 3) == 0)
+      ddl_multi4<3, VW>(psi + c, N, k, dipL, dipL + N, dipL + 2 * N, ov, o);
+    else
+      ddl_multi<3, VW>(psi + c, N, k, dipL, dipL + N, dipL + 2 * N, ov, o);
     LvL[(c * 3 + 0) * N + k] = o[0], LvL[(c * 3 + 1) * N + k] = o[1], LvL[(c * 3 + 2) * N + k] = o[2];
   }
   asm volatile("" ::: "memory");
