@@ -185,7 +185,10 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
   int rc = ensure_device();
   if (rc) return rc;
   if (nsys < 1 || !systems || !re_S || !im_S) return set_error("multi: bad arguments"), NISE_ERR_ARG;
-  hipStream_t st = default_stream();
+  // Trotter systems on the default stream, DIA systems (per-frame Jacobi,
+  // U and their conveyor) on a second one, so that the two overlap
+  hipStream_t st = default_stream(), sd = aux_stream();
+  if (!sd) return set_error("multi: no second stream"), NISE_ERR_HIP;
   std::vector<std::unique_ptr<SysState>> S(nsys);
   // 1. per-system inputs, structure and validation
   for (int k = 0; k < nsys; ++k) {
@@ -242,18 +245,30 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
         return rc;
       s.src.shift_site = s.shs.as<int>(), s.src.shift_ab = s.sha.as<float>();
     }
-    if (!s.dia) {
-      const int64_t npair = (int64_t)N * (N - 1) / 2;
-      s.mask.assign(npair, 0);
-      if (npair) {
-        DevBuf mk;
-        if ((rc = mk.alloc(npair)) || (rc = launch_union_mask(s.src, N, s.f_lo, nfr, mk.as<uint8_t>(), st)))
+  }
+  // the coupling masks of all Trotter systems in one device buffer, one
+  // copy back (no synchronisation per system)
+  {
+    std::vector<int64_t> moff(nsys + 1, 0);
+    for (int k = 0; k < nsys; ++k)
+      moff[k + 1] = moff[k] + (S[k]->dia ? 0 : (int64_t)S[k]->N * (S[k]->N - 1) / 2);
+    if (moff[nsys] > 0) {
+      DevBuf mk;
+      if ((rc = mk.alloc((size_t)moff[nsys]))) return rc;
+      for (int k = 0; k < nsys; ++k) {
+        SysState &s = *S[k];
+        if (moff[k + 1] > moff[k] &&
+            (rc = launch_union_mask(s.src, s.N, s.f_lo, s.f_hi - s.f_lo, mk.as<uint8_t>() + moff[k], st)))
           return rc;
-        NISE_HIP_CHECK(hipMemcpyAsync(s.mask.data(), mk.p, npair, hipMemcpyDeviceToHost, st));
-        NISE_HIP_CHECK(hipStreamSynchronize(st));
       }
-      build_schedule(N, s.mask, s.sch_asap);
+      std::vector<uint8_t> all((size_t)moff[nsys]);
+      NISE_HIP_CHECK(hipMemcpyAsync(all.data(), mk.p, all.size(), hipMemcpyDeviceToHost, st));
+      NISE_HIP_CHECK(hipStreamSynchronize(st));
+      for (int k = 0; k < nsys; ++k)
+        if (!S[k]->dia) S[k]->mask.assign(all.begin() + moff[k], all.begin() + moff[k + 1]);
     }
+    for (int k = 0; k < nsys; ++k)
+      if (!S[k]->dia) build_schedule(S[k]->N, S[k]->mask, S[k]->sch_asap);
   }
   // 2. one row count for the Trotter launch: the work-weighted median
   std::vector<std::pair<int, double>> rw;
@@ -271,7 +286,20 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
       if (acc >= 0.5 * tot) { R = e.first; break; }
     }
   }
-  // 3. tables
+  // 3. tables.  The DIA systems share FP64 work arrays sized for the
+  //    largest of them and one status word per frame, checked once after
+  //    every launch is queued.
+  std::vector<int64_t> soff(nsys + 1, 0);
+  int64_t wmax = 0;
+  for (int k = 0; k < nsys; ++k) {
+    const int64_t nfr = S[k]->f_hi - S[k]->f_lo;
+    soff[k + 1] = soff[k] + (S[k]->dia ? nfr : 0);
+    if (S[k]->dia) wmax = std::max(wmax, std::min<int64_t>(nfr, 4096) * S[k]->N * S[k]->N);
+  }
+  DevBuf A, V, w, stt;
+  if (wmax > 0 && ((rc = A.alloc((size_t)wmax * 8)) || (rc = V.alloc((size_t)wmax * 8)) ||
+                   (rc = w.alloc((size_t)wmax * 8)) || (rc = stt.alloc((size_t)soff[nsys] * 4))))
+    return rc;
   std::vector<RowsArgs> rows;
   std::vector<int> rows_of;
   std::vector<DiaSys> dias;
@@ -307,10 +335,6 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
       // U per frame, in chunks of frames for the FP64 work arrays
       if ((rc = s.U.alloc((size_t)nfr * N * N * 8))) return rc;
       const int64_t chunk = std::min<int64_t>(nfr, 4096);
-      DevBuf A, V, w, stt;
-      if ((rc = A.alloc((size_t)chunk * N * N * 8)) || (rc = V.alloc((size_t)chunk * N * N * 8)) ||
-          (rc = w.alloc((size_t)chunk * N * 8)) || (rc = stt.alloc((size_t)chunk * 4)))
-        return rc;
       volatile float fv = non->deltat * kIcm2ifs;  // src/propagate.c:69
       fv = fv * kTwoPi;
       const float fac = fv;
@@ -318,20 +342,16 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
         const int64_t nc = std::min(chunk, nfr - c0);
         for (int64_t d0 = 0; d0 < nc; d0 += 65535) {
           const int64_t nd = std::min<int64_t>(65535, nc - d0);
-          hipLaunchKernelGGL(dia_dense_kernel, dim3((N * N + 255) / 256, (unsigned)nd), dim3(256), 0, st, s.src, N,
+          hipLaunchKernelGGL(dia_dense_kernel, dim3((N * N + 255) / 256, (unsigned)nd), dim3(256), 0, sd, s.src, N,
                              s.f_lo + c0 + d0, A.as<double>() + d0 * N * N);
           NISE_HIP_CHECK(hipGetLastError());
         }
-        if ((rc = launch_jacobi(A.as<double>(), N, (int)nc, w.as<double>(), V.as<double>(), stt.as<int>(), st)))
+        if ((rc = launch_jacobi(A.as<double>(), N, (int)nc, w.as<double>(), V.as<double>(),
+                                stt.as<int>() + soff[k] + c0, sd)))
           return rc;
         if ((rc = launch_dia_u(w.as<double>(), V.as<double>(), N, nc, fac, s.thres, s.U.as<float2>() + c0 * N * N,
-                               st)))
+                               sd)))
           return rc;
-        std::vector<int> sts(nc);
-        NISE_HIP_CHECK(hipMemcpyAsync(sts.data(), stt.p, nc * 4, hipMemcpyDeviceToHost, st));
-        NISE_HIP_CHECK(hipStreamSynchronize(st));
-        for (int v : sts)
-          if (v < 0) return set_error("multi: Jacobi did not converge (system %d)", k), NISE_ERR_NUMERIC;
       }
       DiaSys d;
       d.N = N, d.Np = 1;
@@ -363,16 +383,25 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
       (rc = up(dsys, dias.data(), dias.size() * sizeof(DiaSys))) ||
       (rc = up(dwm, dmap.data(), dmap.size() * sizeof(int2))) || (rc = times.alloc(nt * 16 + 16)))
     return rc;
-  NISE_HIP_CHECK(hipStreamSynchronize(st));
   unsigned long long *tm = times.as<unsigned long long>();
   if (!rmap.empty() && (rc = launch_rows(rsys.as<RowsArgs>(), rwm.as<int2>(), (int)rmap.size(), R, rsmem, st, tm)))
     return rc;
   if (!dmap.empty()) {
     NISE_HIP_CHECK(hipFuncSetAttribute((const void *)dia_conveyor_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)dsmem));
-    hipLaunchKernelGGL(dia_conveyor_kernel, dim3((unsigned)dmap.size()), dim3(256), dsmem, st,
+    hipLaunchKernelGGL(dia_conveyor_kernel, dim3((unsigned)dmap.size()), dim3(256), dsmem, sd,
                        dsys.as<DiaSys>(), dwm.as<int2>(), tm + 2 * rmap.size());
     NISE_HIP_CHECK(hipGetLastError());
+  }
+  // the reductions and copies below wait for the DIA stream
+  std::vector<int> sts((size_t)soff[nsys]);
+  if (!sts.empty()) NISE_HIP_CHECK(hipMemcpyAsync(sts.data(), stt.p, sts.size() * 4, hipMemcpyDeviceToHost, sd));
+  {
+    hipEvent_t ev;
+    NISE_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipError_t e1 = hipEventRecord(ev, sd), e2 = e1 == hipSuccess ? hipStreamWaitEvent(st, ev, 0) : e1;
+    (void)hipEventDestroy(ev);
+    NISE_HIP_CHECK(e2);
   }
   // 5. per-system reductions and results
   std::vector<float> hr, hi;
@@ -392,6 +421,10 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
   NISE_HIP_CHECK(hipMemcpyAsync(ho.data(), out.p, ho.size() * 4, hipMemcpyDeviceToHost, st));
   if (nt) NISE_HIP_CHECK(hipMemcpyAsync(ht.data(), times.p, ht.size() * 8, hipMemcpyDeviceToHost, st));
   NISE_HIP_CHECK(hipStreamSynchronize(st));
+  NISE_HIP_CHECK(hipStreamSynchronize(sd));
+  for (int k = 0; k < nsys; ++k)
+    for (int64_t i = soff[k]; i < soff[k + 1]; ++i)
+      if (sts[i] < 0) return set_error("multi: Jacobi did not converge (system %d)", k), NISE_ERR_NUMERIC;
   off = 0;
   for (int k = 0; k < nsys; ++k) {
     std::memcpy(re_S + off, ho.data() + 2 * off, S[k]->T * 4);

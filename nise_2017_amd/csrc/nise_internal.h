@@ -472,6 +472,7 @@ void adjoint_steps(const Schedule &s, int vp, std::vector<int2> &steps);
 // Library-wide stream and device setup.
 int ensure_device();
 hipStream_t default_stream();
+hipStream_t aux_stream();
 
 // Host file helpers (driver.hip).
 // Read records [first, first+count) of a GROBIN file into dst; false if the

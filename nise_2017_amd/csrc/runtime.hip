@@ -181,6 +181,16 @@ hipStream_t default_stream() {
   return st;
 }
 
+// A second stream of the calling thread (default priority), for work that
+// is independent of the default stream's and may overlap it.
+hipStream_t aux_stream() {
+  static thread_local hipStream_t st = nullptr;
+  if (!st) {
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
+  }
+  return st;
+}
+
 }  // namespace nise
 
 extern "C" const char *nise_last_error(void) { return nise::get_error(); }
