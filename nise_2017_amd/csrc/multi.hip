@@ -173,6 +173,27 @@ struct SysState {
   int64_t mu_stride = 0;
   Schedule sch_asap, sch;
   std::vector<uint8_t> mask;
+  // everything but the device buffers, which the next call reuses
+  void reset() {
+    N = T = dia = nvec = nwg = 0;
+    thres = 0.f;
+    f_lo = f_hi = 0;
+    src = FrameSource();
+    mu_dev = nullptr;
+    mu_stride = 0;
+    sch_asap = Schedule();
+    sch = Schedule();
+    mask.clear();
+  }
+};
+
+// Device buffers of the calling thread's batches, kept between calls: a
+// batch of the same shape then allocates and frees nothing (hipMalloc and
+// hipFree of the cfg4 batch's buffers took about 55 ms of host time per
+// call).  Every call ends with both streams synchronised.
+struct MultiCache {
+  std::vector<std::unique_ptr<SysState>> sys;
+  DevBuf mk, A, V, w, stt, rsys, rwm, dsys, dwm, times, out;
 };
 }  // namespace
 
@@ -189,10 +210,16 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
   // U and their conveyor) on a second one, so that the two overlap
   hipStream_t st = default_stream(), sd = aux_stream();
   if (!sd) return set_error("multi: no second stream"), NISE_ERR_HIP;
-  std::vector<std::unique_ptr<SysState>> S(nsys);
+  static thread_local MultiCache cache;
+  // a call that returned early may have left work queued on the buffers
+  NISE_HIP_CHECK(hipStreamSynchronize(st));
+  NISE_HIP_CHECK(hipStreamSynchronize(sd));
+  std::vector<std::unique_ptr<SysState>> &S = cache.sys;
+  if ((int)S.size() < nsys) S.resize(nsys);
   // 1. per-system inputs, structure and validation
   for (int k = 0; k < nsys; ++k) {
-    S[k].reset(new SysState());
+    if (!S[k]) S[k].reset(new SysState());
+    S[k]->reset();
     SysState &s = *S[k];
     const nise_system &y = systems[k];
     const t_non *non = y.non;
@@ -253,7 +280,7 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
     for (int k = 0; k < nsys; ++k)
       moff[k + 1] = moff[k] + (S[k]->dia ? 0 : (int64_t)S[k]->N * (S[k]->N - 1) / 2);
     if (moff[nsys] > 0) {
-      DevBuf mk;
+      DevBuf &mk = cache.mk;
       if ((rc = mk.alloc((size_t)moff[nsys]))) return rc;
       for (int k = 0; k < nsys; ++k) {
         SysState &s = *S[k];
@@ -272,8 +299,8 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
   }
   // 2. one row count for the Trotter launch: the work-weighted median
   std::vector<std::pair<int, double>> rw;
-  for (auto &p : S)
-    if (!p->dia)
+  for (int k = 0; k < nsys; ++k)
+    if (const SysState *p = S[k].get(); !p->dia)
       rw.push_back({rows_for(p->N, p->sch_asap.Kreal, p->sch_asap.nlayers()),
                     (double)p->nvec * p->T * std::max(p->sch_asap.Kreal, 1)});
   int R = 16;
@@ -296,7 +323,7 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
     soff[k + 1] = soff[k] + (S[k]->dia ? nfr : 0);
     if (S[k]->dia) wmax = std::max(wmax, std::min<int64_t>(nfr, 4096) * S[k]->N * S[k]->N);
   }
-  DevBuf A, V, w, stt;
+  DevBuf &A = cache.A, &V = cache.V, &w = cache.w, &stt = cache.stt;
   if (wmax > 0 && ((rc = A.alloc((size_t)wmax * 8)) || (rc = V.alloc((size_t)wmax * 8)) ||
                    (rc = w.alloc((size_t)wmax * 8)) || (rc = stt.alloc((size_t)soff[nsys] * 4))))
     return rc;
@@ -376,7 +403,7 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
     for (int v = 0; v < dias[i].nvec; v += dias[i].vp) dmap.push_back(make_int2((int)i, v));
   }
   if (dsmem > 160 * 1024) return set_error("multi: DIA conveyor needs %zu B of LDS", dsmem), NISE_ERR_ARG;
-  DevBuf rsys, rwm, dsys, dwm, times;
+  DevBuf &rsys = cache.rsys, &rwm = cache.rwm, &dsys = cache.dsys, &dwm = cache.dwm, &times = cache.times;
   const size_t nt = rmap.size() + dmap.size();
   if ((rc = up(rsys, rows.data(), rows.size() * sizeof(RowsArgs))) ||
       (rc = up(rwm, rmap.data(), rmap.size() * sizeof(int2))) ||
@@ -406,9 +433,9 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
   // 5. per-system reductions and results
   std::vector<float> hr, hi;
   size_t off = 0;
-  DevBuf out;
+  DevBuf &out = cache.out;
   int64_t Tsum = 0;
-  for (auto &p : S) Tsum += p->T;
+  for (int k = 0; k < nsys; ++k) Tsum += S[k]->T;
   if ((rc = out.alloc((size_t)Tsum * 8))) return rc;
   for (int k = 0; k < nsys; ++k) {
     SysState &s = *S[k];

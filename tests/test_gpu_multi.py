@@ -68,3 +68,20 @@ def test_multi_matches_single_plans():
         r1, i1 = p.execute()
         p.close()
         assert rel(re + 1j * im, r1 + 1j * i1) < 1e-6
+
+
+def test_calls_of_changing_shape_reuse_buffers(orc):
+    """The batch keeps its device buffers between calls: a smaller batch
+    whose slots change kind (DIA <-> Trotter) and size, then the first batch
+    again, which must reproduce its first result bit for bit."""
+    a = [system(N, prop, seed, T=32, ns=5, sr=6) for N, prop, seed in [(40, 1, 31), (12, 0, 32), (90, 1, 33)]]
+    b = [system(N, prop, seed, T=24, ns=4, sr=5) for N, prop, seed in [(30, 0, 34), (60, 1, 35)]]
+    say("batch a, batch b, batch a")
+    out_a, _ = abi.absorption_multi(a)
+    out_b, _ = abi.absorption_multi(b)
+    for (non, H, mu, Cr), (re, im) in zip(b, out_b):
+        o_re, o_im = orc.absorption(non, H, mu, C_rec=Cr)
+        assert rel(re + 1j * im, o_re + 1j * o_im) < 1e-5
+    again, _ = abi.absorption_multi(a)
+    for (r0, i0), (r1, i1) in zip(out_a, again):
+        assert np.array_equal(r0, r1) and np.array_equal(i0, i1)
