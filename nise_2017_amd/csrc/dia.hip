@@ -31,33 +31,56 @@ __device__ __forceinline__ void rr_pair(int M, int r, int i, int &p, int &q) {
   if (p > q) { int t = p; p = q; q = t; }
 }
 
+// LDS row stride of the Jacobi matrices: odd (in doubles) when the padded
+// matrices fit, so that a wave walking down a column spreads over the banks
+// (a stride of N = 64 doubles put every lane on one bank).
+__host__ __device__ inline int jacobi_ld(int N) {
+  const int ld = N | 1;
+  return (size_t)2 * N * ld * sizeof(double) + 4096 <= 160 * 1024 ? ld : N;
+}
+
+// task = i N + k for 0 <= task < (M / 2) N, without an integer division.
+__device__ __forceinline__ void split_task(int task, int N, float invN, int &i, int &k) {
+  i = (int)((float)task * invN);
+  k = task - i * N;
+  if (k >= N) ++i, k -= N;
+  else if (k < 0) --i, k += N;
+}
+
 // A: row-major symmetric N x N (double), one matrix per workgroup.
 // Outputs: w[N] ascending eigenvalues, V[N][N] with V[i*N+j] = component i of
 // eigenvector j.  status[b] = sweeps used, or -1 without convergence.
+// In LDS both matrices have the row stride ld = jacobi_ld(N).
 __global__ void __launch_bounds__(kJacBlock) jacobi_kernel(const double *Ain, int N, double *w_out,
                                                            double *V_out, int *status) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const int ld = jacobi_ld(N);
   double *A = reinterpret_cast<double *>(smem);
-  double *V = A + N * N;
-  double *cs = V + N * N;                    // [M/2][2]
+  double *V = A + N * ld;
+  double *cs = V + N * ld;                   // [M/2][2]
   int *pq = reinterpret_cast<int *>(cs + N + 2);  // [M/2][2], even length
   double *red = reinterpret_cast<double *>(pq + ((N + 3) & ~1));  // [kJacBlock/64][2]
   const int t = threadIdx.x;
   const int64_t b = blockIdx.x;
   const int M = N + (N & 1);
-  for (int i = t; i < N * N; i += kJacBlock) {
-    A[i] = Ain[b * N * N + i];
-    V[i] = (i / N == i % N) ? 1.0 : 0.0;
+  const float invN = 1.0f / (float)N;
+  for (int e = t; e < N * N; e += kJacBlock) {
+    int r, c;
+    split_task(e, N, invN, r, c);
+    A[r * ld + c] = Ain[b * N * N + e];
+    V[r * ld + c] = (r == c) ? 1.0 : 0.0;
   }
   __syncthreads();
   int sweep = 0, converged = 0;
   for (; sweep < 40; ++sweep) {
     // convergence test: off-diagonal vs total Frobenius norm
     double off = 0, tot = 0;
-    for (int i = t; i < N * N; i += kJacBlock) {
-      double a = A[i] * A[i];
+    for (int e = t; e < N * N; e += kJacBlock) {
+      int r, c;
+      split_task(e, N, invN, r, c);
+      double a = A[r * ld + c] * A[r * ld + c];
       tot += a;
-      if (i / N != i % N) off += a;
+      if (r != c) off += a;
     }
     for (int o = 32; o > 0; o >>= 1) off += __shfl_xor(off, o), tot += __shfl_xor(tot, o);
     if ((t & 63) == 0) red[2 * (t >> 6)] = off, red[2 * (t >> 6) + 1] = tot;
@@ -72,9 +95,9 @@ __global__ void __launch_bounds__(kJacBlock) jacobi_kernel(const double *Ain, in
         rr_pair(M, r, i, p, q);
         double c = 1.0, s = 0.0;
         if (q < N) {
-          double apq = A[p * N + q];
+          double apq = A[p * ld + q];
           if (apq != 0.0) {
-            double tau = (A[q * N + q] - A[p * N + p]) / (2.0 * apq);
+            double tau = (A[q * ld + q] - A[p * ld + p]) / (2.0 * apq);
             double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
             c = 1.0 / sqrt(1.0 + tt * tt);
             s = tt * c;
@@ -88,27 +111,29 @@ __global__ void __launch_bounds__(kJacBlock) jacobi_kernel(const double *Ain, in
       __syncthreads();
       // A <- A J and V <- V J (columns p, q of each pair)
       for (int task = t; task < (M / 2) * N; task += kJacBlock) {
-        int i = task / N, k = task % N;
+        int i, k;
+        split_task(task, N, invN, i, k);
         int p = pq[2 * i], q = pq[2 * i + 1];
         if (p == q) continue;
         double c = cs[2 * i], s = cs[2 * i + 1];
-        double akp = A[k * N + p], akq = A[k * N + q];
-        A[k * N + p] = c * akp - s * akq;
-        A[k * N + q] = s * akp + c * akq;
-        double vkp = V[k * N + p], vkq = V[k * N + q];
-        V[k * N + p] = c * vkp - s * vkq;
-        V[k * N + q] = s * vkp + c * vkq;
+        double akp = A[k * ld + p], akq = A[k * ld + q];
+        A[k * ld + p] = c * akp - s * akq;
+        A[k * ld + q] = s * akp + c * akq;
+        double vkp = V[k * ld + p], vkq = V[k * ld + q];
+        V[k * ld + p] = c * vkp - s * vkq;
+        V[k * ld + q] = s * vkp + c * vkq;
       }
       __syncthreads();
       // A <- J^T A (rows p, q of each pair)
       for (int task = t; task < (M / 2) * N; task += kJacBlock) {
-        int i = task / N, k = task % N;
+        int i, k;
+        split_task(task, N, invN, i, k);
         int p = pq[2 * i], q = pq[2 * i + 1];
         if (p == q) continue;
         double c = cs[2 * i], s = cs[2 * i + 1];
-        double apk = A[p * N + k], aqk = A[q * N + k];
-        A[p * N + k] = c * apk - s * aqk;
-        A[q * N + k] = s * apk + c * aqk;
+        double apk = A[p * ld + k], aqk = A[q * ld + k];
+        A[p * ld + k] = c * apk - s * aqk;
+        A[q * ld + k] = s * apk + c * aqk;
       }
       __syncthreads();
     }
@@ -119,23 +144,24 @@ __global__ void __launch_bounds__(kJacBlock) jacobi_kernel(const double *Ain, in
     for (int i = 0; i < N; ++i) perm[i] = i;
     for (int i = 1; i < N; ++i) {  // insertion sort on the diagonal
       int k = perm[i];
-      double d = A[k * N + k];
+      double d = A[k * ld + k];
       int j = i - 1;
-      while (j >= 0 && A[perm[j] * N + perm[j]] > d) { perm[j + 1] = perm[j]; --j; }
+      while (j >= 0 && A[perm[j] * ld + perm[j]] > d) { perm[j + 1] = perm[j]; --j; }
       perm[j + 1] = k;
     }
     status[b] = converged ? sweep : -1;
   }
   __syncthreads();
-  for (int j = t; j < N; j += kJacBlock) w_out[b * N + j] = A[pq[j] * N + pq[j]];
-  for (int i = t; i < N * N; i += kJacBlock) {
-    int r = i / N, j = i % N;
-    V_out[b * N * N + i] = V[r * N + pq[j]];
+  for (int j = t; j < N; j += kJacBlock) w_out[b * N + j] = A[pq[j] * ld + pq[j]];
+  for (int e = t; e < N * N; e += kJacBlock) {
+    int r, j;
+    split_task(e, N, invN, r, j);
+    V_out[b * N * N + e] = V[r * ld + pq[j]];
   }
 }
 
 size_t jacobi_smem(int N) {
-  return (size_t)2 * N * N * sizeof(double) + (size_t)(N + 2) * sizeof(double) +
+  return (size_t)2 * N * jacobi_ld(N) * sizeof(double) + (size_t)(N + 2) * sizeof(double) +
          (size_t)((N + 3) & ~1) * sizeof(int) + (kJacBlock / 64) * 2 * sizeof(double) + 16;
 }
 
