@@ -273,49 +273,10 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
       s.src.shift_site = s.shs.as<int>(), s.src.shift_ab = s.sha.as<float>();
     }
   }
-  // the coupling masks of all Trotter systems in one device buffer, one
-  // copy back (no synchronisation per system)
-  {
-    std::vector<int64_t> moff(nsys + 1, 0);
-    for (int k = 0; k < nsys; ++k)
-      moff[k + 1] = moff[k] + (S[k]->dia ? 0 : (int64_t)S[k]->N * (S[k]->N - 1) / 2);
-    if (moff[nsys] > 0) {
-      DevBuf &mk = cache.mk;
-      if ((rc = mk.alloc((size_t)moff[nsys]))) return rc;
-      for (int k = 0; k < nsys; ++k) {
-        SysState &s = *S[k];
-        if (moff[k + 1] > moff[k] &&
-            (rc = launch_union_mask(s.src, s.N, s.f_lo, s.f_hi - s.f_lo, mk.as<uint8_t>() + moff[k], st)))
-          return rc;
-      }
-      std::vector<uint8_t> all((size_t)moff[nsys]);
-      NISE_HIP_CHECK(hipMemcpyAsync(all.data(), mk.p, all.size(), hipMemcpyDeviceToHost, st));
-      NISE_HIP_CHECK(hipStreamSynchronize(st));
-      for (int k = 0; k < nsys; ++k)
-        if (!S[k]->dia) S[k]->mask.assign(all.begin() + moff[k], all.begin() + moff[k + 1]);
-    }
-    for (int k = 0; k < nsys; ++k)
-      if (!S[k]->dia) build_schedule(S[k]->N, S[k]->mask, S[k]->sch_asap);
-  }
-  // 2. one row count for the Trotter launch: the work-weighted median
-  std::vector<std::pair<int, double>> rw;
-  for (int k = 0; k < nsys; ++k)
-    if (const SysState *p = S[k].get(); !p->dia)
-      rw.push_back({rows_for(p->N, p->sch_asap.Kreal, p->sch_asap.nlayers()),
-                    (double)p->nvec * p->T * std::max(p->sch_asap.Kreal, 1)});
-  int R = 16;
-  if (!rw.empty()) {
-    std::sort(rw.begin(), rw.end());
-    double tot = 0, acc = 0;
-    for (auto &e : rw) tot += e.second;
-    for (auto &e : rw) {
-      acc += e.second;
-      if (acc >= 0.5 * tot) { R = e.first; break; }
-    }
-  }
-  // 3. tables.  The DIA systems share FP64 work arrays sized for the
-  //    largest of them and one status word per frame, checked once after
-  //    every launch is queued.
+  // 2. the DIA systems' U per frame on the second stream, queued before
+  //    the host builds the Trotter schedules so that the two overlap.  The
+  //    DIA systems share FP64 work arrays sized for the largest of them and
+  //    one status word per frame, checked once after every launch is queued.
   std::vector<int64_t> soff(nsys + 1, 0);
   int64_t wmax = 0;
   for (int k = 0; k < nsys; ++k) {
@@ -336,29 +297,7 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
     const t_non *non = systems[k].non;
     const int N = s.N;
     const int64_t nfr = s.f_hi - s.f_lo;
-    if (!s.dia) {
-      build_schedule_rows(N, s.mask, R, s.sch);
-      std::vector<uint2> tab;
-      rows_step_table(s.sch, R, tab);
-      std::vector<uint32_t> abd;  // the frame tables follow the step table's order
-      rows_device_order(s.sch, R, abd);
-      if ((rc = up(s.ab, abd.data(), abd.size() * 4)) ||
-          (rc = up(s.steps, tab.data(), tab.size() * sizeof(uint2))) ||
-          (rc = s.diag.alloc((size_t)nfr * N * 8)) || (rc = s.cs.alloc((size_t)nfr * s.sch.K * 8)))
-        return rc;
-      if ((rc = launch_frame_tables(s.src, N, s.sch, s.ab.as<uint32_t>(), trotter_factor(non->deltat, 1, non->ts),
-                                    s.f_lo, nfr, s.diag.as<float2>(), s.cs.as<float2>(), st)))
-        return rc;
-      s.nwg = s.nvec;  // partial rows: one per vector
-      if ((rc = s.partial.alloc((size_t)s.nwg * s.T * sizeof(double2)))) return rc;
-      RowsArgs r;
-      r.N = N, r.tmax = s.T, r.m = non->ts, r.nlayers = s.sch.nlayers(), r.K = s.sch.K, r.f_lo = s.f_lo;
-      r.nvec = s.nvec, r.begin = non->begin, r.sample = non->sample;
-      r.steps = s.steps.as<uint2>(), r.cs = s.cs.as<float2>(), r.diag = s.diag.as<float2>();
-      r.mu_rec = s.mu_dev, r.mu_stride = s.mu_stride, r.partial = s.partial.as<double2>();
-      rows.push_back(r);
-      rows_of.push_back(k);
-    } else {
+    if (!s.dia) continue;
       // U per frame, in chunks of frames for the FP64 work arrays
       if ((rc = s.U.alloc((size_t)nfr * N * N * 8))) return rc;
       const int64_t chunk = std::min<int64_t>(nfr, 4096);
@@ -391,9 +330,77 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
       d.partial = s.partial.as<double2>();
       dias.push_back(d);
       dia_of.push_back(k);
+  }
+  // the coupling masks of all Trotter systems in one device buffer, one
+  // copy back (no synchronisation per system)
+  {
+    std::vector<int64_t> moff(nsys + 1, 0);
+    for (int k = 0; k < nsys; ++k)
+      moff[k + 1] = moff[k] + (S[k]->dia ? 0 : (int64_t)S[k]->N * (S[k]->N - 1) / 2);
+    if (moff[nsys] > 0) {
+      DevBuf &mk = cache.mk;
+      if ((rc = mk.alloc((size_t)moff[nsys]))) return rc;
+      for (int k = 0; k < nsys; ++k) {
+        SysState &s = *S[k];
+        if (moff[k + 1] > moff[k] &&
+            (rc = launch_union_mask(s.src, s.N, s.f_lo, s.f_hi - s.f_lo, mk.as<uint8_t>() + moff[k], st)))
+          return rc;
+      }
+      std::vector<uint8_t> all((size_t)moff[nsys]);
+      NISE_HIP_CHECK(hipMemcpyAsync(all.data(), mk.p, all.size(), hipMemcpyDeviceToHost, st));
+      NISE_HIP_CHECK(hipStreamSynchronize(st));
+      for (int k = 0; k < nsys; ++k)
+        if (!S[k]->dia) S[k]->mask.assign(all.begin() + moff[k], all.begin() + moff[k + 1]);
+    }
+    for (int k = 0; k < nsys; ++k)
+      if (!S[k]->dia) build_schedule(S[k]->N, S[k]->mask, S[k]->sch_asap);
+  }
+  // 3. one row count for the Trotter launch: the work-weighted median
+  std::vector<std::pair<int, double>> rw;
+  for (int k = 0; k < nsys; ++k)
+    if (const SysState *p = S[k].get(); !p->dia)
+      rw.push_back({rows_for(p->N, p->sch_asap.Kreal, p->sch_asap.nlayers()),
+                    (double)p->nvec * p->T * std::max(p->sch_asap.Kreal, 1)});
+  int R = 16;
+  if (!rw.empty()) {
+    std::sort(rw.begin(), rw.end());
+    double tot = 0, acc = 0;
+    for (auto &e : rw) tot += e.second;
+    for (auto &e : rw) {
+      acc += e.second;
+      if (acc >= 0.5 * tot) { R = e.first; break; }
     }
   }
-  // 4. the two launches (row conveyor for all Trotter systems, DIA conveyor
+  // 4. tables of the Trotter systems
+  for (int k = 0; k < nsys; ++k) {
+    SysState &s = *S[k];
+    const t_non *non = systems[k].non;
+    const int N = s.N;
+    const int64_t nfr = s.f_hi - s.f_lo;
+    if (s.dia) continue;
+      build_schedule_rows(N, s.mask, R, s.sch);
+      std::vector<uint2> tab;
+      rows_step_table(s.sch, R, tab);
+      std::vector<uint32_t> abd;  // the frame tables follow the step table's order
+      rows_device_order(s.sch, R, abd);
+      if ((rc = up(s.ab, abd.data(), abd.size() * 4)) ||
+          (rc = up(s.steps, tab.data(), tab.size() * sizeof(uint2))) ||
+          (rc = s.diag.alloc((size_t)nfr * N * 8)) || (rc = s.cs.alloc((size_t)nfr * s.sch.K * 8)))
+        return rc;
+      if ((rc = launch_frame_tables(s.src, N, s.sch, s.ab.as<uint32_t>(), trotter_factor(non->deltat, 1, non->ts),
+                                    s.f_lo, nfr, s.diag.as<float2>(), s.cs.as<float2>(), st)))
+        return rc;
+      s.nwg = s.nvec;  // partial rows: one per vector
+      if ((rc = s.partial.alloc((size_t)s.nwg * s.T * sizeof(double2)))) return rc;
+      RowsArgs r;
+      r.N = N, r.tmax = s.T, r.m = non->ts, r.nlayers = s.sch.nlayers(), r.K = s.sch.K, r.f_lo = s.f_lo;
+      r.nvec = s.nvec, r.begin = non->begin, r.sample = non->sample;
+      r.steps = s.steps.as<uint2>(), r.cs = s.cs.as<float2>(), r.diag = s.diag.as<float2>();
+      r.mu_rec = s.mu_dev, r.mu_stride = s.mu_stride, r.partial = s.partial.as<double2>();
+      rows.push_back(r);
+      rows_of.push_back(k);
+  }
+  // 5. the two launches (row conveyor for all Trotter systems, DIA conveyor
   //    for all DIA systems), timed on the device clock
   std::vector<int2> rmap, dmap;
   size_t rsmem = 0, dsmem = 0;
@@ -430,7 +437,7 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
     (void)hipEventDestroy(ev);
     NISE_HIP_CHECK(e2);
   }
-  // 5. per-system reductions and results
+  // 6. per-system reductions and results
   std::vector<float> hr, hi;
   size_t off = 0;
   DevBuf &out = cache.out;
