@@ -173,6 +173,9 @@ struct SysState {
   int64_t mu_stride = 0;
   Schedule sch_asap, sch;
   std::vector<uint8_t> mask;
+  // the row schedule of the previous call, reused while its inputs repeat
+  std::vector<uint8_t> rows_mask;
+  int rows_R = 0, rows_N = 0;
   // everything but the device buffers, which the next call reuses
   void reset() {
     N = T = dia = nvec = nwg = 0;
@@ -182,7 +185,6 @@ struct SysState {
     mu_dev = nullptr;
     mu_stride = 0;
     sch_asap = Schedule();
-    sch = Schedule();
     mask.clear();
   }
 };
@@ -378,7 +380,10 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
     const int N = s.N;
     const int64_t nfr = s.f_hi - s.f_lo;
     if (s.dia) continue;
-      build_schedule_rows(N, s.mask, R, s.sch);
+      if (s.rows_N != N || s.rows_R != R || s.rows_mask != s.mask) {
+        build_schedule_rows(N, s.mask, R, s.sch);
+        s.rows_mask = s.mask, s.rows_R = R, s.rows_N = N;
+      }
       std::vector<uint2> tab;
       rows_step_table(s.sch, R, tab);
       std::vector<uint32_t> abd;  // the frame tables follow the step table's order

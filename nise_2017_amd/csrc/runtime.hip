@@ -70,13 +70,78 @@ void build_schedule(int N, const std::vector<uint8_t> &mask, Schedule &s) {
     s.ab[fill[lay[k]]++] = (uint32_t)la[k] | ((uint32_t)lb[k] << 16);
 }
 
+// LDS bank cost of a layer's row order e[0..R) (padding rows ~0u stand for
+// the row's scratch site N + q): a row's G vectors occupy 2G consecutive
+// dwords, so in B banks its run falls in block slot mod (B / 2G); per group
+// of lanes the busiest block bounds the passes of an access.  The grouping
+// the hardware uses for 8-byte accesses is not pinned here, so the cost is
+// summed over 32-lane groups on 64 banks and 16-lane groups on 32 banks
+// (a grouping of all 64 lanes does not depend on the row order).
+struct BankModel {
+  int nblk, rpg;  // blocks, rows per lane group
+};
+static int group_cost(const uint32_t *e, int g0, int rpg, int nblk, int N, int R) {
+  int cost = 0;
+  for (int side = 0; side < 2; ++side) {
+    int cnt[64], mx = 0;
+    for (int k = 0; k < nblk; ++k) cnt[k] = 0;
+    for (int q = g0; q < std::min(R, g0 + rpg); ++q) {
+      const int slot = e[q] == ~0u ? N + q : side ? (int)(e[q] >> 16) : (int)(e[q] & 0xffffu);
+      mx = std::max(mx, ++cnt[slot & (nblk - 1)]);
+    }
+    cost += mx;
+  }
+  return cost;
+}
+
+// Rotations of one layer touch disjoint sites, so their order across the R
+// rows is free: passes of pairwise swaps between lane groups lower the
+// layer's bank cost (a swap changes only the two groups it touches).
+static void assign_rows(std::vector<uint32_t> &e, int R, int N) {
+  const int G = 64 / R;
+  BankModel bm[2];
+  int nm = 0;
+  for (const auto &m : {std::make_pair(64, 32), std::make_pair(32, 16)}) {
+    const int span = 2 * G, rpg = std::max(1, m.second / G);
+    if (span <= m.first && rpg < R) bm[nm++] = BankModel{m.first / span, rpg};
+  }
+  if (!nm) return;
+  for (int pass = 0; pass < 4; ++pass) {
+    bool improved = false;
+    for (int i = 0; i < R; ++i)
+      for (int j = i + 1; j < R; ++j) {
+        if (e[i] == ~0u && e[j] == ~0u) continue;
+        int before = 0, after = 0;
+        for (int k = 0; k < nm; ++k) {
+          const int gi = i / bm[k].rpg * bm[k].rpg, gj = j / bm[k].rpg * bm[k].rpg;
+          if (gi != gj)
+            before += group_cost(e.data(), gi, bm[k].rpg, bm[k].nblk, N, R) +
+                      group_cost(e.data(), gj, bm[k].rpg, bm[k].nblk, N, R);
+        }
+        if (!before) continue;
+        std::swap(e[i], e[j]);
+        for (int k = 0; k < nm; ++k) {
+          const int gi = i / bm[k].rpg * bm[k].rpg, gj = j / bm[k].rpg * bm[k].rpg;
+          if (gi != gj)
+            after += group_cost(e.data(), gi, bm[k].rpg, bm[k].nblk, N, R) +
+                     group_cost(e.data(), gj, bm[k].rpg, bm[k].nblk, N, R);
+        }
+        if (after < before) improved = true;
+        else std::swap(e[i], e[j]);
+      }
+    if (!improved) break;
+  }
+}
+
 // Layers of at most R rotations by list scheduling on the dependency graph
 // of the lexicographic sweep: a rotation follows the previous rotation on
 // each of its two sites, so every site keeps its order.  Layer by layer, the
 // ready rotations with the longest remaining chain (ties: lexicographic
 // order) fill the R rows; this keeps the layer count near
-// max(critical path, rotations / R).  Each layer is padded to exactly R
-// entries; the padding entry of row q is the dummy pair (N + q, N + q),
+// max(critical path, rotations / R).  The rows of a layer are then ordered
+// to spread its LDS accesses over the banks (assign_rows).  Each layer is
+// padded to exactly R entries; the padding entry of row q is the dummy pair
+// (N + q, N + q),
 // which the table kernel turns into an identity and the kernels apply to a
 // scratch site.  The layer count is padded to a multiple of kMinLayers.
 void build_schedule_rows(int N, const std::vector<uint8_t> &mask, int R, Schedule &s) {
@@ -140,9 +205,12 @@ void build_schedule_rows(int N, const std::vector<uint8_t> &mask, int R, Schedul
   s.K = (int)layers.size() * R;
   s.ab.clear();
   s.layer_off.assign(1, 0);
+  std::vector<uint32_t> e(R);
   for (auto &L : layers) {
+    for (int q = 0; q < R; ++q) e[q] = q < (int)L.size() ? L[q] : ~0u;
+    if (R >= 4 && R <= 64) assign_rows(e, R, N);
     for (int q = 0; q < R; ++q)
-      s.ab.push_back(q < (int)L.size() ? L[q] : (uint32_t)(N + q) | ((uint32_t)(N + q) << 16));
+      s.ab.push_back(e[q] != ~0u ? e[q] : (uint32_t)(N + q) | ((uint32_t)(N + q) << 16));
     s.layer_off.push_back((int)s.ab.size());
   }
 }
