@@ -874,6 +874,17 @@ EaShape ea_shape(int N, int T1) {
 }
 }  // namespace
 
+namespace {
+// Device buffers of the calling thread's 2D runs, kept between calls: a run
+// of the same shape allocates and frees nothing (the N = 32 ring run holds
+// about 3 GB of history, two-exciton vectors and item planes).  Every call
+// ends with its stream synchronised.
+struct TwoDirBufs {
+  DevBuf Hb, mub, Cb, Ab, Ob, mk, ab, lay, diag, cs, ab2, D2, cs2, stp, wpl, ltb, Rd;
+  DevBuf L0, hist, dbl, E, part, slots, wgb, slots2, wgb2, vb, rows, lrrb, grp, g1b, g3b;
+};
+}  // namespace
+
 extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H_stride,
                                  const float *mu_rec, int64_t mu_stride, int64_t nframes,
                                  const float *coupling_rec, const float *anh_rec,
@@ -918,10 +929,12 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
   hipStream_t st = default_stream();
   const int N2 = N * (N + 1) / 2;
   const bool anh_file = do_ea && !es && non->anharmonicity == 0.f;
+  static thread_local TwoDirBufs B;
+  NISE_HIP_CHECK(hipStreamSynchronize(st));  // a call that returned early may have left work queued
 
   // inputs on the device (host records are uploaded for [f_lo, f_hi) and
   // indexed by absolute frame through a shifted base pointer)
-  DevBuf Hb, mub, Cb, Ab, Ob;
+  DevBuf &Hb = B.Hb, &mub = B.mub, &Cb = B.Cb, &Ab = B.Ab, &Ob = B.Ob;
   const float *Hd = H_rec, *mud = mu_rec, *Cd = coupling_rec, *Ad = anh_rec, *Od = over_rec;
   if (where == NISE_HOST) {
     if ((rc = upl(Hb, H_rec + f_lo * H_stride, (size_t)(nfr * H_stride * 4)))) return rc;
@@ -953,7 +966,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
   const int64_t npair = (int64_t)N * (N - 1) / 2;
   std::vector<uint8_t> mask(npair);
   if (npair) {
-    DevBuf mk;
+    DevBuf &mk = B.mk;
     if ((rc = mk.alloc(npair))) return rc;
     if ((rc = launch_union_mask(src, N, f_lo, nfr, mk.as<uint8_t>(), st))) return rc;
     NISE_HIP_CHECK(hipMemcpyAsync(mask.data(), mk.p, npair, hipMemcpyDeviceToHost, st));
@@ -969,7 +982,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
         if (mask[p]) lex.push_back((uint32_t)a | ((uint32_t)b << 16));
   }
   const float fac = trotter_factor(non->deltat, 1, non->ts);
-  DevBuf ab, lay, diag, cs, ab2, D2, cs2, stp;
+  DevBuf &ab = B.ab, &lay = B.lay, &diag = B.diag, &cs = B.cs, &ab2 = B.ab2, &D2 = B.D2, &cs2 = B.cs2, &stp = B.stp;
   if ((rc = upl(ab, sch.ab.data(), sch.ab.size() * 4)) ||
       (rc = upl(lay, sch.layer_off.data(), sch.layer_off.size() * 4)))
     return rc;
@@ -1012,7 +1025,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
     for (int t1 = 0; t1 < T1; ++t1)
       lth[(size_t)t3 * T1 + t1] =
           (non->lifetime > 0) ? (float)exp(-(double)(t1 + t3) * non->deltat / (2 * non->lifetime)) : 1.0f;
-  DevBuf wpl, ltb, Rd;
+  DevBuf &wpl = B.wpl, &ltb = B.ltb, &Rd = B.Rd;
   const int64_t plane = (int64_t)T3 * T1;
   if ((rc = upl(wpl, wh.data(), wh.size() * 4)) || (rc = upl(ltb, lth.data(), lth.size() * 4))) return rc;
   if ((rc = Rd.alloc((size_t)12 * plane * 8))) return rc;
@@ -1027,7 +1040,11 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
   const int vw = doubles_vw(N);
   const int gps = (nvec + vw - 1) / vw;  // doubles groups per sample
 
-  DevBuf L0, hist, dbl, E, slots, wgb, vb, rows, lrrb, grp, g1b, g3b, part;
+  // the seed and history conveyors have their own slot buffers: a host copy
+  // into a buffer that a queued kernel still reads would not wait for it
+  DevBuf &L0 = B.L0, &hist = B.hist, &dbl = B.dbl, &E = B.E, &slots = B.slots, &wgb = B.wgb, &vb = B.vb,
+         &slots2 = B.slots2, &wgb2 = B.wgb2,
+         &rows = B.rows, &lrrb = B.lrrb, &grp = B.grp, &g1b = B.g1b, &g3b = B.g3b, &part = B.part;
   const EaShape ea = ea_shape(N, T1);
   ConveyorArgs a;
   a.N = N, a.K = sch.K, a.nlayers = sch.nlayers(), a.m = non->ts, a.f_lo = f_lo;
@@ -1085,9 +1102,9 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
       for (int b = 0; b < 3 * T1 + 3; b += vp) wb.push_back(base + b);
     }
     wb.push_back((int)sl.size());
-    if ((rc = upl(slots, sl.data(), sl.size() * sizeof(SlotDesc))) || (rc = upl(wgb, wb.data(), wb.size() * 4)))
+    if ((rc = upl(slots2, sl.data(), sl.size() * sizeof(SlotDesc))) || (rc = upl(wgb2, wb.data(), wb.size() * 4)))
       return rc;
-    a.slots = slots.as<SlotDesc>(), a.wg_slot_begin = wgb.as<int>(), a.nwg = (int)wb.size() - 1;
+    a.slots = slots2.as<SlotDesc>(), a.wg_slot_begin = wgb2.as<int>(), a.nwg = (int)wb.size() - 1;
     a.state = L0.as<float2>(), a.writeback = 0, a.hist = hist.as<float2>();
     if ((rc = launch_conveyor(a, vp, block, st))) return rc;
     // (3) GB and SE terms from the history
