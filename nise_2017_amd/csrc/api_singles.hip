@@ -67,9 +67,11 @@ namespace {
 // Allocates at least 16 bytes so that empty inputs (no couplings) still get
 // a valid device pointer; copies only when there is data.
 int upload(DevBuf &dst, const void *src, size_t bytes, hipStream_t st) {
+  // work queued on the plan's stream (an execute without a result fetch
+  // returns early) may still read dst: the copy below does not wait for it
+  if (st) NISE_HIP_CHECK(hipStreamSynchronize(st));
   int rc = dst.alloc(bytes < 16 ? 16 : bytes);
   if (rc) return rc;
-  (void)st;
   // synchronous: the source may be pageable memory owned by the caller
   if (bytes && src) NISE_HIP_CHECK(hipMemcpy(dst.p, src, bytes, hipMemcpyHostToDevice));
   return NISE_OK;
