@@ -201,11 +201,10 @@ __global__ void gbse_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3) {
 
 // fr and ft = dipole_double(mu_x2(tk), ...) (src/NISE_subs.c:485-509):
 // |aa> gets over_a c_a, |ab> gets (0 + mu_a c_b) + mu_b c_a.
-__global__ void dd_kernel(TwoDirArgs A) {
-  const int N = A.N, N2 = N * (N + 1) / 2;
-  const int s = blockIdx.y, vec = blockIdx.z;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N2) return;
+// State j of the two-exciton vector vec of sample s: dipole_double of
+// mu_x2(tk) on M[x1](tk) (vec < 9, x1 x2 = vec) or on L0[x0][t1].
+__device__ __forceinline__ float2 dd_value(const TwoDirArgs &A, int s, int vec, int j) {
+  const int N = A.N;
   const int64_t tk = tj_of(A, s);
   int x2;
   const float2 *c;
@@ -232,7 +231,15 @@ __global__ void dd_kernel(TwoDirArgs A) {
     o = make_float2(__fadd_rn(__fmul_rn(dip[a], c[b].x), __fmul_rn(dip[b], c[a].x)),
                     __fadd_rn(__fmul_rn(dip[a], c[b].y), __fmul_rn(dip[b], c[a].y)));
   }
-  A.dbl[((int64_t)s * (9 + 9 * A.T1) + vec) * N2 + j] = o;
+  return o;
+}
+
+__global__ void dd_kernel(TwoDirArgs A) {
+  const int N2 = A.N * (A.N + 1) / 2;
+  const int s = blockIdx.y, vec = blockIdx.z;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N2) return;
+  A.dbl[((int64_t)s * (9 + 9 * A.T1) + vec) * N2 + j] = dd_value(A, s, vec, j);
 }
 
 // Element k of dipole_double_last(dip, f) (src/NISE_subs.c:552-575) for NIT
@@ -733,12 +740,10 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
   const DblGroup gd = P.groups[(size_t)blockIdx.x * npw + wave];
   const int s = P.groups[(size_t)blockIdx.x * npw].s;  // every wave of the workgroup: one sample
   const int kind = gd.kind, v0 = gd.v0, nv = gd.nv;
-  const int nvec = 9 + 9 * T1;
-  // this wave's vectors at t3 = 0 (psi[j][w]); the pair words
-  for (int w = 0; w < VW; ++w) {
-    const float2 *src = A.dbl + ((int64_t)s * nvec + v0 + w) * N2;
-    for (int j = lane; j < N2; j += 64) psi[j * VW + w] = (w < nv) ? gld(src + j) : make_float2(0.f, 0.f);
-  }
+  // this wave's vectors at t3 = 0 (psi[j][w]), formed here from the seeds
+  // and the history (dd_value)
+  for (int w = 0; w < VW; ++w)
+    for (int j = lane; j < N2; j += 64) psi[j * VW + w] = (w < nv) ? dd_value(A, s, v0 + w, j) : make_float2(0.f, 0.f);
   if (lane < VW) psi[N2 * VW + lane] = make_float2(0.f, 0.f);  // the zero state
   // the rotation index table: every c except b, ascending; c == a also does
   // the c == b rotation (|ab>, |bb>); two-level sites skip c == a
@@ -1064,7 +1069,6 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
         (rc = g3b.alloc((size_t)S * 9 * T3 * 8)) ||
         (rc = part.alloc((size_t)((S + kRedS - 1) / kRedS) * 12 * plane * 8)))
       return rc;
-    if (do_ea && (rc = dbl.alloc((size_t)S * nvec * N2 * 8))) return rc;
     A.L0 = L0.as<float2>(), A.hist = hist.as<float2>(), A.dbl = dbl.as<float2>(), A.E = E.as<float>();
 
     // (1) GB seeds L0[x][t1]: mu_x(tj - t1) propagated through frames
@@ -1133,8 +1137,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
     const size_t psmem = npw ? persist_layout(N, pvw, K2, npw).total : 0;
     if (do_ea && npw && (size_t)(N2 + 1) * pvw * 8 <= 65536) {
       // every doubles vector stays in the LDS of its workgroup for all t3
-      hipLaunchKernelGGL(dd_kernel, dim3((N2 + 127) / 128, S, nvec), dim3(128), 0, st, A);
-      NISE_HIP_CHECK(hipGetLastError());
+      // (and is formed there: no doubles vectors in device memory)
       std::vector<DblGroup> gh;
       for (int s = 0; s < S; ++s) {
         for (int v0 = 0; v0 < 9; v0 += pvw) gh.push_back(DblGroup{s, 0, v0, std::min(pvw, 9 - v0)});
@@ -1156,6 +1159,8 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
     } else if (do_ea) {
       // per t3: leftrr, the EA responses from the doubles vectors in device
       // memory, then one doubles step of every vector
+      if ((rc = dbl.alloc((size_t)S * nvec * N2 * 8))) return rc;
+      A.dbl = dbl.as<float2>();
       hipLaunchKernelGGL(dd_kernel, dim3((N2 + 127) / 128, S, nvec), dim3(128), 0, st, A);
       NISE_HIP_CHECK(hipGetLastError());
       std::vector<int> vbh((size_t)S * gps + 1);
