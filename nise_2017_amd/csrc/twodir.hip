@@ -199,6 +199,88 @@ __global__ void gbse_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3) {
   }
 }
 
+// gbse_kernel with the history rows staged in LDS: per chunk of 64 t1 and
+// per x0 the workgroup loads the 64 rows L[x0][t1](t3) with consecutive
+// lanes on consecutive sites (row stride N + 1), then each lane forms its
+// three dots of that x0 (each sum still in ascending site order).
+__global__ void gbse_lds_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int kPol[21][4] = {
+      {0, 0, 0, 0}, {1, 1, 1, 1}, {2, 2, 2, 2}, {0, 0, 1, 1}, {0, 0, 2, 2}, {1, 1, 0, 0},
+      {1, 1, 2, 2}, {2, 2, 0, 0}, {2, 2, 1, 1}, {0, 1, 0, 1}, {0, 2, 0, 2}, {1, 0, 1, 0},
+      {1, 2, 1, 2}, {2, 0, 2, 0}, {2, 1, 2, 1}, {0, 1, 1, 0}, {0, 2, 2, 0}, {1, 0, 0, 1},
+      {1, 2, 2, 1}, {2, 0, 0, 2}, {2, 1, 1, 2}};
+  const int s = blockIdx.x, t3 = blockIdx.y, N = A.N, T1 = A.T1, t = threadIdx.x, ld = N + 1;
+  float *mul = reinterpret_cast<float *>(smem);                                    // [3][N] mu(tk + t3)
+  float2 *rowL = reinterpret_cast<float2 *>(smem + ((3 * N * 4 + 15) & ~15));    // [64][N + 1]
+  const int64_t tl = tj_of(A, s) + t3, plane = (int64_t)A.T3 * T1;
+  for (int e = t; e < 3 * N; e += 64) mul[e] = mu_at(A, tl, e / N)[e % N];
+  float2 t3n[9], t3s[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    t3n[j] = cld(g3 + ((int64_t)s * A.T3 + t3) * 9 + j);
+    t3s[j] = cld(g3 + (int64_t)s * A.T3 * 9 + j);
+    t3s[j].y = -t3s[j].y;
+  }
+  for (int c0 = 0; c0 < T1; c0 += 64) {  // uniform: every lane reaches the barriers
+    const int t1 = c0 + t, nr = min(64, T1 - c0);
+    float2 se1[9];
+#pragma unroll
+    for (int x0 = 0; x0 < 3; ++x0) {
+      __syncthreads();  // the previous rows are read
+      for (int e = t; e < nr * N; e += 64) {
+        const int r = e / N, i = e - r * N;
+        rowL[r * ld + i] = gld(A.hist + (hrow_L(A, s, x0, c0 + r) + t3) * N + i);
+      }
+      __syncthreads();
+      float2 acc[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+      if (t1 < T1) {
+        const float2 *v = rowL + t * ld;
+        for (int i = 0; i < N; ++i) {
+          const float2 vv = v[i];
+#pragma unroll
+          for (int y = 0; y < 3; ++y) {
+            acc[y].x = __fadd_rn(acc[y].x, __fmul_rn(mul[y * N + i], vv.x));
+            acc[y].y = __fadd_rn(acc[y].y, __fmul_rn(mul[y * N + i], vv.y));
+          }
+        }
+      }
+#pragma unroll
+      for (int y = 0; y < 3; ++y) se1[x0 * 3 + y] = acc[y];
+    }
+    if (t1 >= T1) continue;
+    float2 gb1[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) gb1[j] = gld(g1 + ((int64_t)s * 9 + j) * T1 + t1);
+#pragma unroll
+    for (int m = 0; m < 21; ++m) {
+      const int x0 = kPol[m][0], x1 = kPol[m][1], x2 = kPol[m][2], x3 = kPol[m][3];
+      float e[4] = {0.f, 0.f, 0.f, 0.f};
+      if (A.do_gb) {  // src/calc_2DIR.c:272-275
+        const float t3nr = t3n[x2 * 3 + x3].x, t3ni = t3n[x2 * 3 + x3].y;
+        const float t1nr = gb1[x0 * 3 + x1].x, t1ni = gb1[x0 * 3 + x1].y;
+        e[0] -= __fsub_rn(__fmul_rn(t3ni, t1nr), __fmul_rn(t3nr, t1ni));
+        e[1] -= __fadd_rn(__fmul_rn(t3nr, t1nr), __fmul_rn(t3ni, t1ni));
+        e[2] -= __fadd_rn(__fmul_rn(t3ni, t1nr), __fmul_rn(t3nr, t1ni));
+        e[3] -= __fsub_rn(__fmul_rn(t3nr, t1nr), __fmul_rn(t3ni, t1ni));
+      }
+      if (A.do_se) {  // src/calc_2DIR.c:397-400
+        const float t3rr = t3n[x1 * 3 + x3].x, t3ri = t3n[x1 * 3 + x3].y;
+        const float t3nr = t3s[x1 * 3 + x2].x, t3ni = t3s[x1 * 3 + x2].y;
+        const float t1rr = gb1[x0 * 3 + x2].x, t1ri = -gb1[x0 * 3 + x2].y;
+        const float t1nr = se1[x0 * 3 + x3].x, t1ni = se1[x0 * 3 + x3].y;
+        e[0] -= __fadd_rn(__fmul_rn(t3rr, t1ri), __fmul_rn(t3ri, t1rr));
+        e[1] -= __fsub_rn(__fmul_rn(t3rr, t1rr), __fmul_rn(t3ri, t1ri));
+        e[2] -= __fadd_rn(__fmul_rn(t3ni, t1nr), __fmul_rn(t3nr, t1ni));
+        e[3] -= __fsub_rn(__fmul_rn(t3nr, t1nr), __fmul_rn(t3ni, t1ni));
+      }
+      float *E = A.E + (((int64_t)s * 21 + m) * 4) * plane + (int64_t)t3 * T1 + t1;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) gst(E + q * plane, e[q]);
+    }
+  }
+}
+
 // fr and ft = dipole_double(mu_x2(tk), ...) (src/NISE_subs.c:485-509):
 // |aa> gets over_a c_a, |ab> gets (0 + mu_a c_b) + mu_b c_a.
 // State j of the two-exciton vector vec of sample s: dipole_double of
@@ -1115,8 +1197,16 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
     hipLaunchKernelGGL(gbse_static_kernel, dim3(S), dim3(256), 3 * N * sizeof(float), st, A, g1b.as<float2>(),
                        g3b.as<float2>());
     NISE_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(gbse_kernel, dim3(S, T3), dim3(64), 3 * N * sizeof(float), st, A,
-                       (const float2 *)g1b.as<float2>(), (const float2 *)g3b.as<float2>());
+    const size_t gsm = ((size_t)3 * N * 4 + 15) / 16 * 16 + (size_t)64 * (N + 1) * 8;
+    if (gsm <= 64 * 1024) {
+      NISE_HIP_CHECK(hipFuncSetAttribute((const void *)gbse_lds_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)gsm));
+      hipLaunchKernelGGL(gbse_lds_kernel, dim3(S, T3), dim3(64), gsm, st, A, (const float2 *)g1b.as<float2>(),
+                         (const float2 *)g3b.as<float2>());
+    } else {
+      hipLaunchKernelGGL(gbse_kernel, dim3(S, T3), dim3(64), 3 * N * sizeof(float), st, A,
+                         (const float2 *)g1b.as<float2>(), (const float2 *)g3b.as<float2>());
+    }
     NISE_HIP_CHECK(hipGetLastError());
     // (4) EA: doubles vectors at tk, then per t3 the responses and one
     //     doubles step with frame tk + t3

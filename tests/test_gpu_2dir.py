@@ -115,6 +115,19 @@ def test_partial_t1_chunk_vs_oracle(orc):
     assert rel(R, ref) < TOL
 
 
+def test_long_t1_two_chunks_vs_oracle(orc):
+    """T1 = 70: the GB/SE kernel stages the history rows of 64 t1 at a
+    time, so the second chunk holds 6 rows."""
+    say("N=4 T1=70 two t1 chunks")
+    N, T1, T3 = 4, 70, 3
+    H, mu, anh, over = synth.twodir_system(N=N, nframes=90, seed=54)
+    non = mk(N, T1, T3, 5, 0, 3, H, lifetime=300.0)
+    ref = orc.twodir(non, H, mu)
+    non = mk(N, T1, T3, 5, 0, 3, H, lifetime=300.0)
+    R = abi.twodir_batched(non, H, mu)
+    assert rel(R, ref) < TOL
+
+
 @pytest.mark.parametrize("tech", ["2DIR", "2DUVvis"])
 def test_ring48_one_vector_per_wave_vs_oracle(orc, tech):
     """N = 48 ring: 48 kept couplings, so the two-exciton vectors (1176
