@@ -156,25 +156,34 @@ def multi_leg(args, torch):
     cm^-1, 50 fs), Coupling type, dt 1 fs, 1503 frames; tmax 512, 100
     samples every 10 frames, 3 polarisations; N <= 64 on the exact
     exponential (DIA) path, the rest on the Trotter path.  One call runs
-    the per-frame tables, the two launches and the reductions."""
+    the per-frame tables, the two launches and the reductions.
+
+    Checked against the oracle: every system of the same batch run on its
+    first 2 samples (a second call), and the timed 100-sample results of the
+    smallest Trotter systems."""
     import numpy as np
 
     from nise_2017_amd import abi, synth
-    rng = np.random.default_rng(20261202 + 4)
-    Ns = np.round(np.exp(rng.uniform(np.log(8), np.log(1024), args.multi_systems))).astype(int)
     T, sr, ns = 512, 10, 100
-    nfr = (ns - 1) * sr + T + 1 + 2
-    log(f"multi leg: {len(Ns)} systems, sum N = {int(Ns.sum())}, generating")
-    systems = []
-    for k, N in enumerate(Ns):
-        H, Cr, mu = synth.geometric_system(int(N), nfr, seed=20261202 + 4000 + k)
-        mean = float(H[:, 1:].mean())
-        non = abi.make_non(singles=int(N), tmax1=T, sample=sr, begin=0, end=ns, ts=1,
-                           propagation=0 if N <= 64 else 1, deltat=1.0, min1=mean - 600.0,
-                           max1=mean + 600.0, hamiltonian="Coupling", couplingcut=0.5)
-        non.shifte = np.float32((np.float32(non.max1) + np.float32(non.min1)) / np.float32(2))
-        systems.append((non, torch.from_numpy(H).cuda(), torch.from_numpy(mu).cuda(),
-                        torch.from_numpy(Cr).cuda()))
+
+    def systems_for(samples, dev):
+        specs = synth.uneven_batch(nsys=args.multi_systems, samples=samples, tmax=T, samplerate=sr)
+        out = []
+        for s in specs:
+            non = abi.make_non(singles=s["N"], tmax1=T, sample=sr, begin=0, end=samples, ts=1,
+                               propagation=s["propagation"], deltat=1.0, min1=s["min1"], max1=s["max1"],
+                               hamiltonian="Coupling", couplingcut=0.5)
+            non.shifte = np.float32((np.float32(non.max1) + np.float32(non.min1)) / np.float32(2))
+            if dev:
+                out.append((non, torch.from_numpy(s["H"]).cuda(), torch.from_numpy(s["mu"]).cuda(),
+                            torch.from_numpy(s["C"]).cuda()))
+            else:
+                out.append((non, s["H"], s["mu"], s["C"]))
+        return out
+
+    log(f"multi leg: {args.multi_systems} systems, generating")
+    systems = systems_for(ns, True)
+    Ns = np.array([s[0].singles for s in systems])
     torch.cuda.synchronize()
     abi.absorption_multi(systems)  # warm
     torch.cuda.synchronize()
@@ -194,6 +203,41 @@ def multi_leg(args, torch):
                    "times are device-clock times of each system's last workgroup"}
     log(f"multi leg: {res['site_steps_per_s']:.3e} site-steps/s, p50 {res['completion_ms_p50']:.2f} ms, "
         f"p99 {res['completion_ms_p99']:.2f} ms")
+    if not args.no_cpu:
+        from concurrent.futures import ThreadPoolExecutor
+
+        from oracle import oracle
+        threads = max(1, min(16, os.cpu_count() or 1))
+
+        def rel(a, b):
+            return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+        def check(host_systems, results, idx):
+            errs = {}
+
+            def run(k):
+                non, H, mu, Cr = host_systems[k]
+                o_re, o_im = oracle.absorption(non, H, mu, C_rec=Cr)
+                re, im = results[k]
+                errs[k] = rel(re + 1j * im, o_re + 1j * o_im)
+            cost = lambda k: host_systems[k][0].singles ** (3 if host_systems[k][0].propagation == 0 else 2)  # noqa
+            with ThreadPoolExecutor(threads) as ex:
+                list(ex.map(run, sorted(idx, key=lambda k: -cost(k))))
+            return errs
+        t_a = time.perf_counter()
+        short_dev = systems_for(2, True)
+        short_out, _ = abi.absorption_multi(short_dev)
+        del short_dev
+        e_all = check(systems_for(2, False), short_out, range(len(Ns)))
+        small = [k for k in np.argsort(Ns) if Ns[k] > 64][:4]
+        e_full = check(systems_for(ns, False), out, small)
+        res["parity"] = {
+            "all_systems_2_samples_max_rel_l2": max(e_all.values()),
+            "timed_run_smallest_trotter": {f"N={int(Ns[k])}": e_full[k] for k in small},
+            "oracle": "FP32 oracle per system (DIA: FP64 eigensystem, FP32 vectors)",
+            "seconds": round(time.perf_counter() - t_a, 1)}
+        log(f"multi leg parity: all systems (2 samples) max rel L2 {max(e_all.values()):.2e}; "
+            f"timed run, smallest Trotter systems max {max(e_full.values()):.2e}")
     return res
 
 

@@ -105,9 +105,11 @@ typedef int MPI_Comm;
 /* ------------------------------------------------------------------------ */
 
 /* Replaces src/propagate.c:20-39 (prototype src/propagate.h:4).  Dispatch on
- * non->propagation: 1 -> Trotter coupling scheme, 0 with thres==0 or >1 ->
- * exact exponential (DIA).  Other schemes (RK4, sparse DIA_S) are outside
- * this library and report an error. */
+ * non->propagation: 1 -> Trotter coupling scheme; 0 -> exact exponential
+ * (DIA) when thres == 0 or thres > 1, else the sparse scheme (DIA_S: U with
+ * elements |U_ab|^2 <= thres dropped, with the reference's sparsity report).
+ * Only RK4 (propagation 3) is not built: it prints a message and exits, as
+ * the reference does for an unknown scheme. */
 void propagate_vector(t_non *non, float *Hamil_i_e, float *vecr, float *veci,
                       int sign, int samples, int display);
 

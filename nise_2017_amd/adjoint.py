@@ -96,15 +96,25 @@ def _autograd():
 
         @staticmethod
         def forward(ctx, delta, cscale, model):
-            re, im = model.forward(delta.detach().cpu().numpy(), float(cscale))
+            c = float(cscale)
+            re, im = model.forward(delta.detach().cpu().numpy(), c)
             ctx.model = model
-            return torch.from_numpy(np.stack([re, im])).to(delta.dtype)
+            # gradients go back on the device and in the dtype of each input;
+            # a plain number as the scale takes no gradient
+            ctx.d_meta = (delta.device, delta.dtype)
+            ctx.c_meta = (cscale.device, cscale.dtype) if torch.is_tensor(cscale) else None
+            return torch.from_numpy(np.stack([re, im])).to(device=delta.device, dtype=delta.dtype)
 
         @staticmethod
         def backward(ctx, gS):
-            g = gS.detach().cpu().numpy()
+            g = gS.detach().to("cpu", torch.float64).numpy()
             gd, gc = ctx.model.backward(g[0], g[1])
-            return torch.from_numpy(gd), torch.tensor(gc, dtype=torch.float64), None
+            dev, dt = ctx.d_meta
+            gdt = torch.from_numpy(np.asarray(gd, np.float64)).to(device=dev, dtype=dt)
+            gct = None
+            if ctx.c_meta is not None:
+                gct = torch.tensor(float(gc), dtype=torch.float64).to(device=ctx.c_meta[0], dtype=ctx.c_meta[1])
+            return gdt, gct, None
 
     return AbsorptionResponse
 

@@ -414,6 +414,12 @@ extern "C" int nise_abs_plan_autotune(nise_abs_plan *P, float *ms, int *codes, i
     if (rows_smem(P->N, R) <= 160 * 1024) cands.push_back({R, 0, 0, 0});
   const Cand slot[] = {{0, 8, 2, 256}, {0, 16, 5, 256}, {0, 8, 2, 64}};
   for (const Cand &c : slot) cands.push_back(c);
+  // the plan's configuration on entry, restored if a candidate fails to run
+  const Cand orig{P->rows, P->vp, P->spw, P->block};
+  auto restore = [&]() {
+    (void)(orig.rows ? nise_abs_plan_set_rows(P, orig.rows)
+                     : nise_abs_plan_set_shape(P, orig.vp, orig.spw, orig.block));
+  };
   float best = 1e30f;
   int bi = -1, k = 0;
   for (size_t i = 0; i < cands.size(); ++i) {
@@ -422,7 +428,10 @@ extern "C" int nise_abs_plan_autotune(nise_abs_plan *P, float *ms, int *codes, i
     if (rc) continue;  // shape does not fit this system
     float t = 1e30f;
     for (int r = 0; r < 2; ++r) {
-      if ((rc = nise_abs_plan_execute(P, nullptr, nullptr, 1))) return -rc;
+      if ((rc = nise_abs_plan_execute(P, nullptr, nullptr, 1))) {
+        restore();
+        return -rc;
+      }
       t = std::min(t, P->last_ms[1]);
     }
     if (k < n) {
@@ -432,7 +441,10 @@ extern "C" int nise_abs_plan_autotune(nise_abs_plan *P, float *ms, int *codes, i
     ++k;
     if (t < best) best = t, bi = (int)i;
   }
-  if (bi < 0) return set_error("autotune: no configuration ran"), -NISE_ERR_ARG;
+  if (bi < 0) {
+    restore();
+    return set_error("autotune: no configuration ran"), -NISE_ERR_ARG;
+  }
   const Cand &c = cands[bi];
   int rc = c.rows ? nise_abs_plan_set_rows(P, c.rows) : nise_abs_plan_set_shape(P, c.vp, c.spw, c.block);
   return rc ? -rc : k;

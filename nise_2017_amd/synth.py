@@ -281,3 +281,30 @@ def onfly_system(N, nframes, seed, kind=1, box=None, spacing=5.0, mu_debye=0.5, 
         boxes[:, 0] = headers(nframes)
         boxes[:, 1:] = box
     return H, mu, pos, boxes
+
+
+def uneven_batch(nsys=96, seed=SEED0 + 4, samples=100, tmax=512, samplerate=10, sys_seed=SEED0 + 4000,
+                 full_samples=100):
+    """The uneven-size batch: nsys systems with N = round(exp U[ln 8, ln
+    1024]), each a geometric_system (sites uniform in a cube at one per
+    (5 A)^3, couplings 8 cm^-1 at 5 A for kappa = 1 dropped below 0.5
+    cm^-1, AR(1) energies 15000 / 100 cm^-1 / 50 fs, dt 1 fs); system k
+    draws from seed sys_seed + k.  The trajectories are generated for
+    full_samples start points every samplerate frames over tmax steps, and
+    the window (min1, max1) is centred on their mean; the records returned
+    are cut to what `samples` start points need, so a short run is the
+    prefix of the full one.  N <= 64 runs on the exact exponential
+    (propagation 0), larger N on the Trotter scheme.
+    Returns a list of dicts: N, propagation, H (T, 1 + N), C (coupling
+    record), mu (dipole record), min1, max1."""
+    rng = np.random.default_rng(seed)
+    Ns = np.round(np.exp(rng.uniform(np.log(8), np.log(1024), nsys))).astype(int)
+    nfr = (full_samples - 1) * samplerate + tmax + 3
+    need = (samples - 1) * samplerate + tmax + 3
+    out = []
+    for k, N in enumerate(Ns):
+        H, Cr, mu = geometric_system(int(N), nfr, seed=sys_seed + k)
+        mean = float(H[:, 1:].mean())
+        out.append(dict(N=int(N), propagation=0 if N <= 64 else 1, H=np.ascontiguousarray(H[:need]), C=Cr,
+                        mu=mu, min1=mean - 600.0, max1=mean + 600.0))
+    return out

@@ -10,11 +10,12 @@
  * visibility and linked with --gc-sections, so functions that would need
  * LAPACK (ssyev_/sgeev_) are dropped; nothing here calls them.
  *
- * The two loop drivers (ref_absorption_files, ref_2dir_item_files) restate
- * the loop structure of src/absorption.c:92-146 and src/calc_2DIR.c:184-506
- * (those files need FFTW/MPI headers this image lacks), but every file read,
- * propagation, dipole and polarisation operation inside them is the
- * reference's own compiled function.
+ * The loop driver ref_absorption_files restates the loop structure of
+ * src/absorption.c:92-146 (that translation unit calls do_1DFFT, which needs
+ * FFTW, absent from this image), but every file read, propagation and
+ * response operation inside it is the reference's own compiled function.
+ * The 2D drivers calc_2DIR / calc_2DES are built whole, with MPI, by
+ * ref2d_main.c (oracle/Makefile, target ref2d).
  */
 #include <stddef.h>
 #include <stdio.h>

@@ -173,3 +173,27 @@ def test_autograd_harness(orc):
     model.close()
     assert np.allclose(d.grad.numpy(), gd, rtol=1e-6, atol=0)
     assert abs(c.grad.item() - gc) <= 1e-6 * abs(gc)
+
+
+def test_autograd_harness_cuda_float32(orc):
+    """Parameters on the GPU in float32, and a plain-number scale: the
+    gradient comes back on the parameters' device and dtype; the number
+    takes none."""
+    import torch
+    say("torch.autograd harness, cuda float32 parameters")
+    non, H, mu, Cr = make_case(N=6, T=32, seed=67)
+    freq, A_re, A_im = adjoint.spectrum_matrix(non, non.end)
+    target = torch.from_numpy(A_re @ orc.absorption_param_f64(non, H, mu, np.full(6, 1.5), 1.1)[0])
+    model = adjoint.AdjointAbsorption(non, H, mu, every=8)
+    d = torch.zeros(6, dtype=torch.float32, device="cuda", requires_grad=True)
+    S = adjoint.absorption_response(d, 1.0, model)
+    assert S.device.type == "cuda" and S.dtype == torch.float32
+    spec = torch.from_numpy(A_re).cuda() @ S[0].double() + torch.from_numpy(A_im).cuda() @ S[1].double()
+    loss = ((spec - target.cuda()) ** 2).sum()
+    loss.backward()
+    assert d.grad.device.type == "cuda" and d.grad.dtype == torch.float32
+    _, g_re, g_im = loss_grad((S[0].detach().double().cpu().numpy(), S[1].detach().double().cpu().numpy()),
+                              A_re, A_im, target.numpy())
+    gd, _ = model.backward(g_re, g_im)
+    model.close()
+    assert np.allclose(d.grad.cpu().numpy(), gd.astype(np.float32), rtol=1e-5, atol=1e-12)
