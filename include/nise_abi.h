@@ -229,6 +229,22 @@ int nise_absorption_batched(const t_non *non, const float *H_rec,
                             const float *coupling_rec, int where,
                             int kernel, float *re_S, float *im_S);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): what nise_absorption_batched computes for device
+ * records (where = NISE_DEVICE, kernel 0) with propagation 1, bit for bit,
+ * using only a library-owned 32 MiB device array per GPU (held until the
+ * process ends) for its mask, schedule, per-frame tables and partial sums.
+ * Runs needing more return NISE_ERR_RANGE with a message.  No hipMalloc /
+ * hipFree, no stream or event is created (null stream, synchronous copies;
+ * re_S / im_S are host arrays).  Calls on one GPU share the array: they are
+ * serialised inside this process, and overlapping use from elsewhere races. */
+int nise_absorption_batched_noalloc(const t_non *non, const float *H_rec,
+                                    int64_t H_stride, const float *mu_rec,
+                                    int64_t mu_stride, int64_t nframes,
+                                    const float *coupling_rec, float *re_S,
+                                    float *im_S);
+
 /* Absorption plan: the state of nise_absorption_batched kept on the device
  * so that a run can be repeated (timed, replayed as a hipGraph, re-shaped)
  * and differentiated.  Arguments as nise_absorption_batched; host records

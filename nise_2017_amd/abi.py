@@ -112,6 +112,7 @@ def lib():
     L.nise_version.restype = C.c_char_p
     L.nise_absorption_batched.argtypes = [P, _vp, C.c_int64, _vp, C.c_int64, C.c_int64, _vp,
                                           C.c_int, C.c_int, _fp, _fp]
+    L.nise_absorption_batched_noalloc.argtypes = [P, _vp, C.c_int64, _vp, C.c_int64, C.c_int64, _vp, _fp, _fp]
     L.nise_abs_plan_create.argtypes = [C.POINTER(C.c_void_p), P, _vp, C.c_int64, _vp, C.c_int64,
                                        C.c_int64, _vp, C.c_int, C.c_int]
     L.nise_abs_plan_create_onfly.argtypes = [C.POINTER(C.c_void_p), P, _vp, C.c_int64, _vp, C.c_int64,
