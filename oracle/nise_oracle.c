@@ -933,9 +933,232 @@ int or_1dfft(const t_non *non, const float *re_S, const float *im_S,
 }
 
 /* ---------------------------------------------------------------------- */
-/* One 2DIR work item, src/calc_2DIR.c:184-506 (Coupling propagation,      */
-/* tmax2 == 0).  Adds into R12 = 12 arrays [tmax3][tmax1] in the order of  */
-/* src/calc_2DIR.c:540-542.                                                */
+/* Truncated one-exciton propagator, src/propagate.c:1241-1302:            */
+/* U = V exp(-i E f/m) V^T (f = dt icm2ifs 2 pi / m, sign +1) in the site  */
+/* basis, elements listed a-major, b-minor where FP32 |U_ab|^2 > thres:    */
+/* Ur/Ui = U_ab, R = a, C = b.  The eigensystem is the FP64 Jacobi one     */
+/* (the reference's is LAPACK ssyev_ in FP32); each element is rounded to  */
+/* FP32 once, so U is exactly symmetric.  Returns the element count.      */
+/* ---------------------------------------------------------------------- */
+int or_time_evolution_mat(int N, const float *H, float deltat, int m, float thres,
+                          float *Ur, float *Ui, int *R, int *C) {
+  float ff = deltat * OR_icm2ifs * OR_twoPi / m; /* :1253 */
+  double f = ff;
+  double *A = malloc(N * N * sizeof(double));
+  double *V = malloc(N * N * sizeof(double));
+  double *w = malloc(N * sizeof(double));
+  for (int a = 0; a < N; a++)
+    for (int b = 0; b < N; b++) A[a * N + b] = H[or_sindex(a, b, N)];
+  or_jacobi_f64(N, A, w, V);
+  int elements = 0;
+  for (int a = 0; a < N; a++)
+    for (int b = 0; b < N; b++) {
+      double sr = 0, si = 0;
+      for (int c = 0; c < N; c++) {
+        double p = V[a * N + c] * V[b * N + c];
+        sr += p * cos(w[c] * f);
+        si -= p * sin(w[c] * f);
+      }
+      float r = (float)sr, i = (float)si;
+      if (r * r + i * i > thres) { /* :1291 */
+        Ur[elements] = r, Ui[elements] = i;
+        R[elements] = a, C[elements] = b;
+        elements++;
+      }
+    }
+  free(A), free(V), free(w);
+  return elements;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Sparse two-exciton step, src/propagate.c:1304-1398 (three-level sites)  */
+/* and :1400-1473 (two-level sites), in the reference's FP32 order: per    */
+/* substep the anharmonic half phase on |aa> (a copy for two-level sites), */
+/* the sum over element pairs b <= a of U_a U_b g f[Sindex(t,v)] into      */
+/* v[Sindex(s,u)] with (s,t) = (R[a],C[a]), (u,v) = (R[b],C[b]), and the   */
+/* second half phase.  g: sqrt2 when exactly one of s!=u, t!=v holds, else */
+/* 1 (two-level sites: 0 there, else 1).  The statements keep the          */
+/* reference's shapes: under -ffast-math the compiler's reassociation of   */
+/* each update depends on them, and the results are bit-identical only so. */
+/* ---------------------------------------------------------------------- */
+static void sparce3(int N, const float *Ur, const float *Ui, const int *R, const int *C,
+                    float *fr, float *fi, int elements, int m, const float *Anh,
+                    float anharmonicity, float deltat) {
+  int a, b, indexA, indexB;
+  int Nf, s, t, u, v;
+  float g;
+  float *vr, *vi;
+  float Uar, Ubr, Uai, Ubi;
+  float *si, *co;
+  float f, fm;
+  int i;
+  f = deltat * OR_icm2ifs * OR_twoPi;
+  fm = f * 0.5 / m;
+  Nf = N * (N + 1) / 2;
+  vr = (float *)calloc(Nf, sizeof(float));
+  vi = (float *)calloc(Nf, sizeof(float));
+  co = (float *)calloc(N, sizeof(float));
+  si = (float *)calloc(N, sizeof(float));
+  if (anharmonicity != 0) {
+    for (i = 0; i < N; i++) co[i] = cos(fm * anharmonicity), si[i] = sin(fm * anharmonicity);
+  } else {
+    for (i = 0; i < N; i++) co[i] = cos(fm * Anh[i]), si[i] = sin(fm * Anh[i]);
+  }
+  for (i = 0; i < m; i++) {
+    for (a = 0; a < N; a++)
+      for (b = 0; b <= a; b++) {
+        indexA = or_sindex(a, b, N);
+        vr[indexA] = fr[indexA];
+        vi[indexA] = fi[indexA];
+      }
+    for (a = 0; a < N; a++) {
+      indexA = or_sindex(a, a, N);
+      fr[indexA] = co[a] * vr[indexA] - si[a] * vi[indexA];
+      fi[indexA] = co[a] * vi[indexA] + si[a] * vr[indexA];
+    }
+    for (a = 0; a < N; a++)
+      for (b = 0; b <= a; b++) {
+        indexA = or_sindex(a, b, N);
+        vr[indexA] = 0;
+        vi[indexA] = 0;
+      }
+    for (a = 0; a < elements; a++) {
+      s = R[a], t = C[a], Uar = Ur[a], Uai = Ui[a];
+      for (b = 0; b <= a; b++) {
+        u = R[b], v = C[b], Ubr = Ur[b], Ubi = Ui[b];
+        g = 1;
+        indexA = or_sindex(s, u, N), indexB = or_sindex(t, v, N);
+        if (s != u) g = OR_sqrt2;
+        if (v != t) g = OR_sqrt2;
+        if (s != u && v != t) g = 1;
+        vr[indexA] += (Uar * Ubr - Uai * Ubi) * fr[indexB] * g;
+        vr[indexA] -= (Uai * Ubr + Uar * Ubi) * fi[indexB] * g;
+        vi[indexA] += (Uar * Ubr - Uai * Ubi) * fi[indexB] * g;
+        vi[indexA] += (Uai * Ubr + Uar * Ubi) * fr[indexB] * g;
+      }
+    }
+    for (a = 0; a < N; a++)
+      for (b = 0; b <= a; b++) {
+        indexA = or_sindex(a, b, N);
+        fr[indexA] = vr[indexA];
+        fi[indexA] = vi[indexA];
+      }
+    for (a = 0; a < N; a++) {
+      indexA = or_sindex(a, a, N);
+      fr[indexA] = co[a] * vr[indexA] - si[a] * vi[indexA];
+      fi[indexA] = co[a] * vi[indexA] + si[a] * vr[indexA];
+    }
+  }
+  free(si), free(co), free(vr), free(vi);
+}
+
+static void sparce2(int N, const float *Ur, const float *Ui, const int *R, const int *C,
+                    float *fr, float *fi, int elements, int m) {
+  int a, b, indexA, indexB;
+  int Nf, s, t, u, v;
+  float g;
+  float *vr, *vi;
+  float Uar, Ubr, Uai, Ubi;
+  int i;
+  Nf = N * (N + 1) / 2;
+  vr = (float *)calloc(Nf, sizeof(float));
+  vi = (float *)calloc(Nf, sizeof(float));
+  for (i = 0; i < m; i++) {
+    for (a = 0; a < N; a++)
+      for (b = 0; b <= a; b++) {
+        indexA = or_sindex(a, b, N);
+        vr[indexA] = fr[indexA];
+        vi[indexA] = fi[indexA];
+      }
+    for (a = 0; a < N; a++) {
+      indexA = or_sindex(a, a, N);
+      fr[indexA] = vr[indexA];
+      fi[indexA] = vi[indexA];
+    }
+    for (a = 0; a < N; a++)
+      for (b = 0; b <= a; b++) {
+        indexA = or_sindex(a, b, N);
+        vr[indexA] = 0;
+        vi[indexA] = 0;
+      }
+    for (a = 0; a < elements; a++) {
+      s = R[a], t = C[a], Uar = Ur[a], Uai = Ui[a];
+      for (b = 0; b <= a; b++) {
+        u = R[b], v = C[b], Ubr = Ur[b], Ubi = Ui[b];
+        g = 1;
+        indexA = or_sindex(s, u, N), indexB = or_sindex(t, v, N);
+        if (s != u) g = 0;
+        if (v != t) g = 0;
+        if (s != u && v != t) g = 1;
+        vr[indexA] += (Uar * Ubr - Uai * Ubi) * fr[indexB] * g;
+        vr[indexA] -= (Uai * Ubr + Uar * Ubi) * fi[indexB] * g;
+        vi[indexA] += (Uar * Ubr - Uai * Ubi) * fi[indexB] * g;
+        vi[indexA] += (Uai * Ubr + Uar * Ubi) * fr[indexB] * g;
+      }
+    }
+    for (a = 0; a < N; a++)
+      for (b = 0; b <= a; b++) {
+        indexA = or_sindex(a, b, N);
+        fr[indexA] = vr[indexA];
+        fi[indexA] = vi[indexA];
+      }
+    for (a = 0; a < N; a++) {
+      indexA = or_sindex(a, a, N);
+      fr[indexA] = vr[indexA];
+      fi[indexA] = vi[indexA];
+    }
+  }
+  free(vr), free(vi);
+}
+
+void or_double_sparce_f32(int N, const float *Ur, const float *Ui, const int *R,
+                          const int *C, float *fr, float *fi, int elements, int m,
+                          const float *Anh, float anharmonicity, float deltat,
+                          int es) {
+  if (es)
+    sparce2(N, Ur, Ui, R, C, fr, fi, elements, m);
+  else
+    sparce3(N, Ur, Ui, R, C, fr, fi, elements, m, Anh, anharmonicity, deltat);
+}
+
+/* One singles step of propagate_vector (src/propagate.c:20-39): the        */
+/* Trotter scheme for propagation 1; for propagation 0 the exact            */
+/* exponential when thres is 0 or above 1, else the sparse scheme (DIA_S,   */
+/* always when force_sparse, as src/propagate.c:1519-1525 calls it); the    */
+/* exponential schemes in FP64, the vector rounded to FP32 after the step.  */
+static void or_singles_step(const t_non *non, const float *He, float *cr, float *ci,
+                            int sign, int force_sparse) {
+  int N = non->singles;
+  if (non->propagation == 1) {
+    or_coupling_S_f32(N, He, cr, ci, non->ts, sign, non->deltat, non->couplingcut);
+    return;
+  }
+  double *dr = malloc(N * sizeof(double)), *di = malloc(N * sizeof(double));
+  for (int i = 0; i < N; i++) dr[i] = cr[i], di[i] = ci[i];
+  if (force_sparse || (non->thres > 0 && non->thres <= 1))
+    or_DIA_S_f64(N, He, dr, di, sign, non->deltat, non->thres);
+  else
+    or_DIA_f64(N, He, dr, di, sign, non->deltat);
+  for (int i = 0; i < N; i++) cr[i] = (float)dr[i], ci[i] = (float)di[i];
+  free(dr), free(di);
+}
+
+/* One waiting-time step, src/propagate.c:130-180: the exact exponential    */
+/* (no Trotter split, sign +1) for whatever propagation scheme is chosen.   */
+static void or_t2_step(int N, const float *He, float *cr, float *ci, float deltat) {
+  double *dr = malloc(N * sizeof(double)), *di = malloc(N * sizeof(double));
+  for (int i = 0; i < N; i++) dr[i] = cr[i], di[i] = ci[i];
+  or_DIA_f64(N, He, dr, di, 1, deltat);
+  for (int i = 0; i < N; i++) cr[i] = (float)dr[i], ci[i] = (float)di[i];
+  free(dr), free(di);
+}
+
+/* ---------------------------------------------------------------------- */
+/* One 2DIR work item, src/calc_2DIR.c:184-506 (propagation 1, the Trotter */
+/* scheme, or 0, the exact exponential / sparse scheme of                   */
+/* src/propagate.c:1476-1533; waiting time tmax2 >= 0 propagated as        */
+/* src/calc_2DIR.c:303-326 does).  Adds into R12 = 12 arrays [tmax3][tmax1] */
+/* in the order of src/calc_2DIR.c:540-542.                                */
 /* ---------------------------------------------------------------------- */
 int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
                  const float *mu_rec, long long mu_stride, const float *C_rec,
@@ -943,7 +1166,7 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
                  int molPol, float *R12) {
   int N = non->singles, T1 = non->tmax1, T3 = non->tmax3;
   int nn2 = N * (N + 1) / 2;
-  if (non->tmax2 != 0 || non->propagation != 1) return NISE_ERR_ARG;
+  if (non->tmax2 < 0 || (non->propagation != 1 && non->propagation != 0)) return NISE_ERR_ARG;
   const char *tech = non->technique;
   /* calc_2DES (src/calc_2DES.c:22-586) is calc_2DIR for two-level sites:
      its techniques end in UVvis and its doubles are the _ES forms */
@@ -970,6 +1193,8 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
   float *t1nr = calloc(T1, sizeof(float)), *t1ni = calloc(T1, sizeof(float));
   float *t1rr = calloc(T1, sizeof(float)), *t1ri = calloc(T1, sizeof(float));
   float *lt = malloc(plane * sizeof(float));
+  float *Urs = malloc((size_t)N * N * sizeof(float)), *Uis = malloc((size_t)N * N * sizeof(float));
+  int *Rs = malloc((size_t)N * N * sizeof(int)), *Cs = malloc((size_t)N * N * sizeof(int));
   for (int t1 = 0; t1 < T1; t1++)
     for (int t3 = 0; t3 < T3; t3++)
       lt[t3 * T1 + t1] = (non->lifetime > 0)
@@ -988,7 +1213,7 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
     read_mu(non, mu_rec, mu_stride, tj - t1, px[0], leftnr + (size_t)t1 * N);
     for (int tm = 0; tm < t1; tm++) {
       or_read_H(non, H_rec, H_stride, C_rec, tj - t1 + tm, He, 0);
-      or_coupling_S_f32(N, He, leftnr + (size_t)t1 * N, leftni + (size_t)t1 * N, ts, 1, dt, cut);
+      or_singles_step(non, He, leftnr + (size_t)t1 * N, leftni + (size_t)t1 * N, 1, 0);
     }
   }
   for (int t1 = 0; t1 < T1; t1++) { /* :244-250 */
@@ -1016,11 +1241,17 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
       }
     }
     or_read_H(non, H_rec, H_stride, C_rec, tl, He, 0);
-    or_coupling_S_f32(N, He, mut3r, mut3i, ts, 1, dt, cut);
+    or_singles_step(non, He, mut3r, mut3i, 1, 0);
   }
-  /* SE (:303-443); tmax2 == 0 so no t2 propagation */
+  /* SE (:303-443), after the waiting time (:305-326) */
   read_mu(non, mu_rec, mu_stride, tj, px[1], leftrr);
   for (int i = 0; i < N; i++) leftri[i] = 0;
+  for (int t2 = 0; t2 < non->tmax2; t2++) {
+    or_read_H(non, H_rec, H_stride, C_rec, tj + t2, He, 0);
+    or_t2_step(N, He, leftrr, leftri, dt);
+    for (int t1 = 0; t1 < T1; t1++)
+      or_t2_step(N, He, leftnr + (size_t)t1 * N, leftni + (size_t)t1 * N, dt);
+  }
   read_mu(non, mu_rec, mu_stride, tk, px[2], mut3r);
   if (do_ea && es) { /* src/calc_2DES.c:318-323 */
     or_dipole_double_ES(N, mut3r, leftrr, leftri, fr, fi);
@@ -1079,9 +1310,9 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
       }
     }
     or_read_H(non, H_rec, H_stride, C_rec, tl, He, 0);
-    or_coupling_S_f32(N, He, leftrr, leftri, ts, 1, dt, cut);
+    or_singles_step(non, He, leftrr, leftri, 1, 0);
     for (int t1 = 0; t1 < T1; t1++)
-      or_coupling_S_f32(N, He, leftnr + (size_t)t1 * N, leftni + (size_t)t1 * N, ts, 1, dt, cut);
+      or_singles_step(non, He, leftnr + (size_t)t1 * N, leftni + (size_t)t1 * N, 1, 0);
   }
   if (do_ea) { /* :445-506 */
     for (int t3 = 0; t3 < T3; t3++) {
@@ -1122,6 +1353,23 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
         }
       }
       or_read_H(non, H_rec, H_stride, C_rec, tl, He, 0);
+      if (non->propagation == 0) {
+        /* src/propagate.c:1486-1525 (and :1606-1645 for two-level sites):
+           the caller passes the row and column arrays swapped */
+        if (non->anharmonicity == 0 && !es) {
+          const float *a = A_rec + (long long)tl * (1 + N) + 1;
+          for (int i = 0; i < N; i++) Anh[i] = a[i];
+        }
+        int el = or_time_evolution_mat(N, He, dt, ts, non->thres, Urs, Uis, Cs, Rs);
+        or_double_sparce_f32(N, Urs, Uis, Rs, Cs, fr, fi, el, ts, Anh, non->anharmonicity, dt, es);
+        for (int t1 = 0; t1 < T1; t1++)
+          or_double_sparce_f32(N, Urs, Uis, Rs, Cs, ft1r + (size_t)t1 * nn2, ft1i + (size_t)t1 * nn2,
+                               el, ts, Anh, non->anharmonicity, dt, es);
+        or_singles_step(non, He, rightnr, rightni, -1, 1);
+        for (int t1 = 0; t1 < T1; t1++)
+          or_singles_step(non, He, rightrr + (size_t)t1 * N, rightri + (size_t)t1 * N, -1, 1);
+        continue;
+      }
       if (es) { /* propagate_double_2DES_control, src/propagate.c:1664-1683 */
         or_coupling_S_doubles_ES_f32(N, He, fr, fi, ts, dt, cut);
         for (int t1 = 0; t1 < T1; t1++)
@@ -1147,5 +1395,6 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
   free(rightrr), free(rightri), free(rightnr), free(rightni), free(mut2), free(mut3r);
   free(mut3i), free(mut4), free(fr), free(fi), free(ft1r), free(ft1i);
   free(t1nr), free(t1ni), free(t1rr), free(t1ri), free(lt);
+  free(Urs), free(Uis), free(Rs), free(Cs);
   return 0;
 }

@@ -33,11 +33,20 @@
  *     parameters; at delta = 0, cscale = 1 it equals or_absorption's FP64
  *     result exactly (tests), and an energy offset on an uncoupled site is a
  *     pure phase e^{-i f delta t} (closed form in tests).
+ *   or_time_evolution_mat / or_double_sparce_f32 : the sparse step is
+ *     bit-identical to the reference's propagate_double_sparce(_ES) given
+ *     the same element list; the element list agrees with the reference's
+ *     time_evolution_mat built against LAPACK (oracle/_ref, MKL) to the
+ *     eigensolver's rounding, with the same kept set (tests/golden).
+ *   or_DIA_f64 / or_DIA_S_f64 / the t2 step : also checked against the
+ *     reference's propagate_vec_DIA / _DIA_S / propagate_t2_DIA built
+ *     against LAPACK, to the eigensolver's rounding.
  *   or_2dir_item : composition of pinned primitives, checked against the
  *     closed form of uncoupled anharmonic sites (each pathway alone, the
- *     two-level sum, per-frame anharmonicities and overtone dipoles).  The
- *     order and signs of the 12 accumulators follow src/calc_2DIR.c line by
- *     line; calc_2DIR.c itself needs MPI and is not built here.
+ *     two-level sum, per-frame anharmonicities and overtone dipoles), and
+ *     against the R*.dat files of the reference's own calc_2DIR / calc_2DES
+ *     (oracle/_ref/nise_ref2d, MPI singleton) for coupled systems, the
+ *     waiting time and the sparse scheme (tests/golden/reference_2d.npz).
  */
 #ifndef NISE_ORACLE_H
 #define NISE_ORACLE_H
@@ -112,6 +121,17 @@ int or_absorption_param_f64(const t_non *non, const float *H_rec,
                             long long mu_stride, const float *C_rec,
                             const double *delta, double cscale,
                             double *re_S, double *im_S, int nthreads);
+
+/* src/propagate.c:1241-1302: kept elements of the one-exciton propagator
+   (Ur, Ui, R = row, C = column, a-major order); returns the count */
+int or_time_evolution_mat(int N, const float *H, float deltat, int m, float thres,
+                          float *Ur, float *Ui, int *R, int *C);
+/* src/propagate.c:1304-1473: the sparse two-exciton step (es = 1: two-level
+   sites) */
+void or_double_sparce_f32(int N, const float *Ur, const float *Ui, const int *R,
+                          const int *C, float *fr, float *fi, int elements, int m,
+                          const float *Anh, float anharmonicity, float deltat,
+                          int es);
 
 int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
                  const float *mu_rec, long long mu_stride, const float *C_rec,

@@ -71,6 +71,9 @@ def oracle():
     L.or_1dfft.argtypes = [P, _f32, _f32, C.c_int, _f64, _f64, _f64, C.POINTER(C.c_int)]
     L.or_absorption_param_f64.argtypes = [P, _vp, C.c_longlong, _vp, C.c_longlong, _vp, _vp,
                                           C.c_double, _f64, _f64, C.c_int]
+    L.or_time_evolution_mat.argtypes = [C.c_int, _f32, C.c_float, C.c_int, C.c_float, _f32, _f32, _i32, _i32]
+    L.or_double_sparce_f32.argtypes = [C.c_int, _f32, _f32, _i32, _i32, _f32, _f32, C.c_int, C.c_int, _vp,
+                                       C.c_float, C.c_float, C.c_int]
     L.or_2dir_item.argtypes = [P, _vp, C.c_longlong, _vp, C.c_longlong, _vp, _vp, _vp, C.c_int,
                                C.c_int, _f32]
     _o = L

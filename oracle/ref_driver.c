@@ -241,3 +241,55 @@ static int absorption_loop(t_non *non_, float *re_S, float *im_S) {
   free(Hamil_i_e), free(vecr), free(veci), free(mu_eg), free(mu_xyz);
   return 0;
 }
+
+#ifdef REF_WITH_LAPACK
+/* Exact-exponential primitives that call LAPACK's ssyev_ through
+ * diagonalizeLPD (src/propagate.c:1185-1221); built only into
+ * _ref/libnise_ref_lapack.so, which links MKL (oracle/Makefile). */
+EXPORT void ref_DIA(int N, float *H, float *cr, float *ci, int sign, float deltat) {
+  t_non non;
+  init_non(&non, N, deltat, 0);
+  propagate_vec_DIA(&non, H, cr, ci, sign);
+}
+
+EXPORT int ref_DIA_S(int N, float *H, float *cr, float *ci, int sign, float deltat, float thres) {
+  t_non non;
+  init_non(&non, N, deltat, 0);
+  non.thres = thres;
+  return propagate_vec_DIA_S(&non, H, cr, ci, sign);
+}
+
+/* propagate_t2_DIA on one vector and T1 more (vr: T1 x N rows). */
+EXPORT void ref_t2_DIA(int N, float *H, float *cr, float *ci, float *vr, float *vi, int T1,
+                       float deltat) {
+  t_non non;
+  init_non(&non, N, deltat, 0);
+  non.tmax1 = T1;
+  float **pr = malloc(T1 * sizeof(float *)), **pi = malloc(T1 * sizeof(float *));
+  for (int t = 0; t < T1; t++) pr[t] = vr + (size_t)t * N, pi[t] = vi + (size_t)t * N;
+  propagate_t2_DIA(&non, H, cr, ci, pr, pi, 1);
+  free(pr), free(pi);
+}
+
+EXPORT int ref_time_evolution_mat(int N, float *H, float deltat, int m, float thres, float *Ur,
+                                  float *Ui, int *R, int *C) {
+  t_non non;
+  init_non(&non, N, deltat, 0);
+  non.thres = thres;
+  return time_evolution_mat(&non, H, Ur, Ui, R, C, m);
+}
+#endif
+
+/* propagate_double_sparce(_ES) (src/propagate.c:1304-1473) with a given
+ * element list: no LAPACK involved. */
+EXPORT void ref_double_sparce(int N, float *Ur, float *Ui, int *R, int *C, float *fr, float *fi,
+                              int elements, int m, float *Anh, float anharmonicity, float deltat,
+                              int es) {
+  t_non non;
+  init_non(&non, N, deltat, 0);
+  non.anharmonicity = anharmonicity;
+  if (es)
+    propagate_double_sparce_ES(&non, Ur, Ui, R, C, fr, fi, elements, m);
+  else
+    propagate_double_sparce(&non, Ur, Ui, R, C, fr, fi, elements, m, Anh);
+}
