@@ -288,6 +288,181 @@ def twodir_leg(args, torch):
     return res
 
 
+def host_cpu():
+    """(nproc, CPU model name) of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def cpu_threads():
+    """Worker threads for the CPU baselines: every host core, up to the 16
+    hardware threads a one-GPU box allots to its job."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baseline_leg(args, N, T, sr, ns, lo, hi, non, diag, row, mus, H, mu, nfr):
+    """The reference's own compiled absorption loop (src/absorption.c:92-146
+    over the reference's read_He, read_mue and propagate_vec_coupling_S,
+    built from its sources into oracle/_ref) on a bounded sample of the
+    same workload, the samples split into contiguous BeginPoint/EndPoint
+    ranges, one per thread.  Two modes: frames read with fseek/fread per
+    step from files in the page cache (the reference's I/O), and the same
+    loop over the trajectory held in memory (stdio streams over RAM
+    buffers).  The oracle port is timed beside it.  Returns the
+    cpu_baseline object and the GPU's relative L2 against the oracle on the
+    same samples."""
+    import tempfile
+    import threading
+
+    import numpy as np
+
+    from nise_2017_amd import abi
+    from oracle import oracle
+    threads = cpu_threads()
+    nproc, model = host_cpu()
+    f_need = lambda k: (k - 1) * sr + T  # noqa: E731
+
+    def make(k):
+        n = abi.make_non(singles=N, tmax1=T, sample=sr, begin=0, end=k, ts=1, propagation=1,
+                         deltat=2.0, min1=lo, max1=hi)
+        n.shifte = non.shifte
+        return n
+
+    def ranges(k):
+        return [(k * i // threads, k * (i + 1) // threads) for i in range(threads)]
+
+    def run_threads(fn, k):
+        parts = [(np.zeros(T, np.float32), np.zeros(T, np.float32)) for _ in range(threads)]
+        ths = [threading.Thread(target=fn, args=(b, e, parts[i][0], parts[i][1]))
+               for i, (b, e) in enumerate(ranges(k)) if e > b]
+        t_a = time.perf_counter()
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        dt = time.perf_counter() - t_a
+        re = np.sum([p[0] for p in parts], axis=0)
+        im = np.sum([p[1] for p in parts], axis=0)
+        return dt, re, im
+
+    out = {"unit": "site-steps/s", "cores": threads, "nproc": nproc, "cpu_model": model}
+    ref_ok = oracle.have_ref()
+    # per-sample cost on one thread, to size the sample
+    Hh, muh = host_records(diag, row, mus, f_need(1))
+    if ref_ok:
+        R = oracle.ref()
+        re1, im1 = np.zeros(T, np.float32), np.zeros(T, np.float32)
+        t_a = time.perf_counter()
+        R.ref_absorption_mem(N, Hh.ctypes.data, Hh.nbytes, muh.ctypes.data, muh.nbytes, lo, hi, 2.0, 0.0, 1, T,
+                             sr, 0, 1, Hh.shape[0], re1, im1)
+        per_sample = time.perf_counter() - t_a
+    else:
+        t_a = time.perf_counter()
+        oracle.absorption(make(1), Hh, muh, nthreads=1)
+        per_sample = time.perf_counter() - t_a
+    # about cpu_seconds of work on every thread, at most 1200 samples (a
+    # 1.6 GB trajectory file at N = 256)
+    k = int(max(threads, min(ns, 1200, round(args.cpu_seconds * threads / per_sample))))
+    Hh, muh = host_records(diag, row, mus, f_need(k))
+    site_steps = k * 3 * T * N
+    sample_txt = (f"{k} of {ns} samples x 3 pols x {T} steps at N={N}, contiguous sample ranges on "
+                  f"{threads} threads")
+    if ref_ok:
+        shm = "/dev/shm" if os.path.isdir("/dev/shm") else None
+        with tempfile.TemporaryDirectory(dir=shm) as d:
+            hf, df = os.path.join(d, "H.bin"), os.path.join(d, "mu.bin")
+            Hh.tofile(hf)
+            muh.tofile(df)
+
+            def files(b, e, r, i):
+                R.ref_absorption_files(N, hf.encode(), df.encode(), b"", b"", b"Full", lo, hi, 2.0, 0.0, 1, T,
+                                       sr, b, e, Hh.shape[0], r, i)
+            run_threads(files, threads)  # page cache warm
+            dt_f, rf, imf = run_threads(files, k)
+
+            def mem(b, e, r, i):
+                R.ref_absorption_mem(N, Hh.ctypes.data, Hh.nbytes, muh.ctypes.data, muh.nbytes, lo, hi, 2.0, 0.0,
+                                     1, T, sr, b, e, Hh.shape[0], r, i)
+            dt_m, rm, imm = run_threads(mem, k)
+        out.update({"value": site_steps / dt_f, "kind": "reference",
+                    "sample": sample_txt + f", the reference's loop compiled from its sources, frames read "
+                                          f"per step from files in the page cache, {dt_f:.1f} s",
+                    "modes": {"file_read": {"value": site_steps / dt_f, "seconds": round(dt_f, 2)},
+                              "preloaded": {"value": site_steps / dt_m, "seconds": round(dt_m, 2)}},
+                    "single_thread_value": 3 * T * N / per_sample})
+        log(f"cpu baseline (reference loop, {threads} threads): files {site_steps / dt_f:.3e}, "
+            f"memory {site_steps / dt_m:.3e} site-steps/s")
+    # the oracle port on the same samples (and the GPU's parity against it)
+    t_a = time.perf_counter()
+    o_re, o_im = oracle.absorption(make(k), Hh, muh, nthreads=threads)
+    dt_o = time.perf_counter() - t_a
+    port = {"value": site_steps / dt_o, "seconds": round(dt_o, 2), "kind": "port", "cores": threads}
+    if ref_ok:
+        out["port"] = port
+        out["reference_vs_port_rel_l2"] = float(np.linalg.norm((rf - o_re) + 1j * (imf - o_im)) /
+                                                np.linalg.norm(o_re + 1j * o_im))
+    else:
+        out.update({"value": port["value"], "kind": "port", "sample": sample_txt + ", trajectory preloaded"})
+    pc = abi.AbsPlan(make(k), H, H.shape[1], mu, mu.shape[1], nfr, where=abi.NISE_DEVICE)
+    g_re, g_im = pc.execute()
+    pc.close()
+    err = float(np.linalg.norm((g_re - o_re) + 1j * (g_im - o_im)) / np.linalg.norm(o_re + 1j * o_im))
+    return out, err
+
+
+def chain16_leg(args, torch):
+    """The latency-bound case: N=16 chain, Full type, AR(1) energies (1650,
+    sd 10, 100 fs) and nearest-neighbour couplings N(-5, 2) + AR(1) sd 1,
+    4200 frames at dt 2 fs, tmax 128, 200 samples every 20 frames, 3
+    polarisations, Trotter 1: the whole run as one captured graph, and the
+    plain per-sample kernel on the same run."""
+    import numpy as np
+
+    from nise_2017_amd import abi, synth
+    N, T, sr, ns = 16, 128, 20, 200
+    H, mu = synth.chain_absorption(N=N, nframes=4200, seed=20261202)
+    lo, hi = synth.shifted_center(H, N)
+    non = abi.make_non(singles=N, tmax1=T, sample=sr, begin=0, end=ns, ts=1, propagation=1,
+                       deltat=2.0, min1=lo, max1=hi, fft=2048, lifetime=1300.0)
+    non.shifte = np.float32((np.float32(hi) + np.float32(lo)) / np.float32(2))
+    Hd, mud = torch.from_numpy(H).cuda(), torch.from_numpy(mu).cuda()
+    plan = abi.AbsPlan(non, Hd, H.shape[1], mud, mu.shape[1], H.shape[0], where=abi.NISE_DEVICE)
+    for _ in range(3):
+        plan.execute(flags=2, fetch=False)
+    torch.cuda.synchronize()
+    reps = 50
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.execute(flags=2, fetch=False)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    plan.execute(flags=1, fetch=False)
+    t = plan.timing()
+    info = plan.info()
+    plan.close()
+    pp = abi.AbsPlan(non, Hd, H.shape[1], mud, mu.shape[1], H.shape[0], where=abi.NISE_DEVICE, kernel=1)
+    pp.execute(flags=1, fetch=False)
+    pms = pp.timing()["conveyor_ms"]
+    pp.close()
+    site_steps = ns * 3 * T * N
+    res = {"workload": "absorption N=16 chain, Full FP32 frames, 4200 frames, tmax 128, 200 samples every 20 "
+                       "frames, 3 pols, Trotter 1",
+           "ms_per_run": round(ms, 4), "site_steps_per_s": site_steps / (ms * 1e-3),
+           "stages_ms": {k: round(v, 4) for k, v in t.items() if k != "adjoint_ms"},
+           "rows": info["rows"], "plain_kernel_ms": round(pms, 3),
+           "batched_over_plain": round(pms / ms, 2)}
+    log(f"N=16 chain: {ms:.3f} ms per run ({res['site_steps_per_s']:.3e} site-steps/s), plain {pms:.2f} ms")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -307,6 +482,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the N=128 adjoint leg")
     ap.add_argument("--no-multi", action="store_true", help="skip the uneven-size batch leg")
     ap.add_argument("--no-2dir", action="store_true", help="skip the 2DIR leg")
+    ap.add_argument("--no-cfg0", action="store_true", help="skip the N=16 chain leg")
     ap.add_argument("--multi-systems", type=int, default=96)
     ap.add_argument("--adjoint-every", default="16,32,64", help="checkpoint intervals K")
     ap.add_argument("--fd-params", type=int, default=3, help="offsets checked by finite differences")
@@ -400,6 +576,9 @@ def main():
     if os.path.exists(tf):
         with open(tf) as fh:
             roofline["traffic"] = json.load(fh).get("bytes_per_launch")
+        roofline["traffic_source"] = (f"profiles/{os.path.basename(tf)}: rocprofv3 FETCH_SIZE and WRITE_SIZE "
+                                      "passes over this workload, scaled by a copy kernel (not measured in "
+                                      "this run)")
 
     extra = {}
     if not args.no_plain and args.plain_samples > 0:
@@ -427,57 +606,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle
-        threads = max(1, min(16, os.cpu_count() or 1))
-        f_need = lambda k: (k - 1) * sr + T  # noqa: E731
-        Hh, muh = host_records(diag, row, mus, f_need(1))
-        non1 = abi.make_non(singles=N, tmax1=T, sample=sr, begin=0, end=1, ts=1, propagation=1,
-                            deltat=2.0, min1=lo, max1=hi)
-        non1.shifte = non.shifte
-        t_a = time.perf_counter()
-        oracle.absorption(non1, Hh, muh, nthreads=1)
-        per_sample = time.perf_counter() - t_a
-        k = int(max(threads, min(ns, round(args.cpu_seconds * threads / per_sample))))
-        Hh, muh = host_records(diag, row, mus, f_need(k))
-        nonk = abi.make_non(singles=N, tmax1=T, sample=sr, begin=0, end=k, ts=1, propagation=1,
-                            deltat=2.0, min1=lo, max1=hi)
-        nonk.shifte = non.shifte
-        t_a = time.perf_counter()
-        o_re, o_im = oracle.absorption(nonk, Hh, muh, nthreads=threads)
-        dt = time.perf_counter() - t_a
-        cpu = {"value": k * 3 * T * N / dt, "unit": "site-steps/s", "cores": threads,
-               "kind": "port",
-               "sample": f"{k} of {ns} samples x 3 pols x {T} steps at N={N}, trajectory "
-                         f"preloaded, OpenMP over samples, {dt:.1f} s",
-               "single_thread_value": 3 * T * N / per_sample}
-        log(f"cpu baseline: {cpu['value']:.3e} site-steps/s on {threads} threads")
-        if oracle.have_ref():
-            # the reference's own compiled absorption loop (propagate.c, read_He
-            # per step from files in the page cache), one thread, a few samples
-            import tempfile
-            kr = 2
-            Hr, mr = host_records(diag, row, mus, f_need(kr))
-            with tempfile.TemporaryDirectory() as d:
-                hf, df = os.path.join(d, "H.bin"), os.path.join(d, "mu.bin")
-                Hr.tofile(hf)
-                mr.tofile(df)
-                re_r = np.zeros(T, np.float32)
-                im_r = np.zeros(T, np.float32)
-                t_a = time.perf_counter()
-                oracle.ref().ref_absorption_files(N, hf.encode(), df.encode(), b"", b"", b"Full", lo, hi,
-                                                  2.0, 0.0, 1, T, sr, 0, kr, Hr.shape[0], re_r, im_r)
-                dtr = time.perf_counter() - t_a
-            cpu["reference_build"] = {"value": kr * 3 * T * N / dtr, "cores": 1, "kind": "reference",
-                                      "sample": f"{kr} samples, reference sources compiled here, "
-                                                f"frames read per step from files, {dtr:.1f} s"}
-            log(f"reference build: {cpu['reference_build']['value']:.3e} site-steps/s on 1 thread")
-        # spectral check of the GPU run against the oracle on the same samples
-        pc = abi.AbsPlan(nonk, H, H.shape[1], mu, mu.shape[1], nfr, where=abi.NISE_DEVICE)
-        g_re, g_im = pc.execute()
-        pc.close()
-        err = float(np.linalg.norm((g_re - o_re) + 1j * (g_im - o_im)) /
-                    np.linalg.norm(o_re + 1j * o_im))
-        extra["parity_rel_l2_vs_fp32_oracle"] = err
+        cpu, extra["parity_rel_l2_vs_fp32_oracle"] = cpu_baseline_leg(args, N, T, sr, ns, lo, hi, non, diag, row,
+                                                                      mus, H, mu, nfr)
+    if not args.no_cfg0:
+        extra["chain16"] = chain16_leg(args, torch)
 
     if not args.no_adjoint:
         extra["adjoint"] = adjoint_leg(args, torch)

@@ -40,7 +40,7 @@ struct nise_abs_plan {
   DevBuf ab, layer_off, diag, cs, slots, wg_begin, partial, out, steps, row_steps, row_sys, row_wmap;
   size_t row_smem = 0;
   int nsteps = 0;
-  DevBuf ws_cs, ws_ab, resp_plain;
+  DevBuf ws_cs, resp_plain;
   int vp = 16, spw = 5, nwg = 0, block = 256;
   int npart = 0;  // rows of partial responses (workgroups, or vectors for the row conveyor)
   // gradient runs: parameters and the adjoint's buffers (every == 0: off)
@@ -194,7 +194,10 @@ int run_plain(nise_abs_plan *P) {
   a.N = P->N, a.m = P->m, a.tmax = P->tmax;
   a.fac = trotter_factor(P->non.deltat, 1, P->m);
   a.src = P->src, a.mu_rec = P->mu_dev, a.mu_stride = P->mu_stride;
-  a.resp = resp, a.ws_cs = P->ws_cs.as<float2>(), a.ws_ab = P->ws_ab.as<uint32_t>();
+  a.resp = resp, a.ws_cs = P->ws_cs.as<float2>();
+  a.ab = P->ab.as<uint32_t>(), a.layer_off = P->layer_off.as<int>();
+  a.K = P->sch_asap.K, a.nlayers = P->sch_asap.nlayers();
+  a.cs_lds = plain_smem(P->N, a.K, true) <= 160 * 1024;
   for (int s = P->begin; s < P->end; ++s) {  // one launch per sample
     a.ti = (int64_t)s * P->sample;
     int rc = launch_plain_sample(a, P->st);
@@ -307,23 +310,26 @@ static int plan_create(nise_abs_plan **out, const t_non *non, const float *H_rec
   }
   if ((rc = P->out.alloc(2 * P->tmax * sizeof(float)))) return rc;
 
-  if (kernel == 1) {
-    int64_t npair = (int64_t)N * (N - 1) / 2 + 1;
-    if ((rc = P->ws_cs.alloc(npair * sizeof(float2)))) return rc;
-    if ((rc = P->ws_ab.alloc(npair * sizeof(uint32_t)))) return rc;
+  // structure: union of kept couplings over the frames in use
+  int64_t npair = (int64_t)N * (N - 1) / 2;
+  std::vector<uint8_t> mask(npair);
+  if (npair) {
+    DevBuf mk;
+    if ((rc = mk.alloc(npair))) return rc;
+    if ((rc = launch_union_mask(P->src, N, P->f_lo, P->f_hi - P->f_lo, mk.as<uint8_t>(), P->st))) return rc;
+    NISE_HIP_CHECK(hipMemcpyAsync(mask.data(), mk.p, npair, hipMemcpyDeviceToHost, P->st));
+    NISE_HIP_CHECK(hipStreamSynchronize(P->st));
+  }
+  build_schedule(N, mask, P->sch_asap);
+  if (kernel == 1) {  // the plain per-sample kernel sweeps the layered schedule
+    if ((rc = upload(P->ab, P->sch_asap.ab.data(), P->sch_asap.ab.size() * 4, P->st))) return rc;
+    if ((rc = upload(P->layer_off, P->sch_asap.layer_off.data(), P->sch_asap.layer_off.size() * 4, P->st)))
+      return rc;
+    if (plain_smem(N, P->sch_asap.K, true) > 160 * 1024 &&
+        (rc = P->ws_cs.alloc((size_t)std::max(P->sch_asap.K, 1) * sizeof(float2))))
+      return rc;
     if ((rc = P->resp_plain.alloc(P->tmax * sizeof(double2)))) return rc;
   } else {
-    // structure: union of kept couplings over the frames in use
-    int64_t npair = (int64_t)N * (N - 1) / 2;
-    std::vector<uint8_t> mask(npair);
-    if (npair) {
-      DevBuf mk;
-      if ((rc = mk.alloc(npair))) return rc;
-      if ((rc = launch_union_mask(P->src, N, P->f_lo, P->f_hi - P->f_lo, mk.as<uint8_t>(), P->st))) return rc;
-      NISE_HIP_CHECK(hipMemcpyAsync(mask.data(), mk.p, npair, hipMemcpyDeviceToHost, P->st));
-      NISE_HIP_CHECK(hipStreamSynchronize(P->st));
-    }
-    build_schedule(N, mask, P->sch_asap);
     int64_t nfr = P->f_hi - P->f_lo;
     if ((rc = P->diag.alloc((size_t)nfr * N * sizeof(float2)))) return rc;
     if ((rc = P->cs.alloc((size_t)nfr * std::max(P->sch_asap.K, 1) * sizeof(float2)))) return rc;

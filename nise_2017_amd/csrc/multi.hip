@@ -74,6 +74,33 @@ int launch_dia_u(const double *w, const double *V, int N, int64_t nmat, float fa
   return NISE_OK;
 }
 
+// Per-frame propagators U (FP32 [nfr][N][N]) of frames f_first .. for the
+// exact exponential with factor fac (thres > 0: the sparse scheme's dropped
+// elements zeroed), through FP64 work arrays of at most 4096 frames; one
+// Jacobi convergence word per frame in status (negative: not converged).
+int launch_dia_tables(const FrameSource &src, int N, int64_t f_first, int64_t nfr, float fac, float thres,
+                      float2 *U, DevBuf &A, DevBuf &V, DevBuf &w, int *status, hipStream_t st) {
+  if (nfr <= 0) return NISE_OK;
+  const int64_t chunk = std::min<int64_t>(nfr, 4096);
+  int rc;
+  if ((rc = A.alloc((size_t)chunk * N * N * 8)) || (rc = V.alloc((size_t)chunk * N * N * 8)) ||
+      (rc = w.alloc((size_t)chunk * N * 8)))
+    return rc;
+  for (int64_t c0 = 0; c0 < nfr; c0 += chunk) {
+    const int64_t nc = std::min(chunk, nfr - c0);
+    for (int64_t d0 = 0; d0 < nc; d0 += 65535) {
+      const int64_t nd = std::min<int64_t>(65535, nc - d0);
+      hipLaunchKernelGGL(dia_dense_kernel, dim3((N * N + 255) / 256, (unsigned)nd), dim3(256), 0, st, src, N,
+                         f_first + c0 + d0, A.as<double>() + d0 * N * N);
+      NISE_HIP_CHECK(hipGetLastError());
+    }
+    if ((rc = launch_jacobi(A.as<double>(), N, (int)nc, w.as<double>(), V.as<double>(), status + c0, st)))
+      return rc;
+    if ((rc = launch_dia_u(w.as<double>(), V.as<double>(), N, nc, fac, thres, U + c0 * N * N, st))) return rc;
+  }
+  return NISE_OK;
+}
+
 // One workgroup (256 threads): vectors v0 .. v0+vp-1 of one system.  Thread
 // (site a, group sg) handles sites a of vectors sg, sg + 256/Np, ...
 __global__ void __launch_bounds__(256) dia_conveyor_kernel(const DiaSys *sys, const int2 *wmap,

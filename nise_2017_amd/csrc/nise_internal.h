@@ -361,9 +361,13 @@ struct PlainArgs {
   int64_t mu_stride = 0;
   int64_t ti = 0;            // start frame of the sample
   double2 *resp = nullptr;   // [tmax] accumulated (3 polarisations)
-  float2 *ws_cs = nullptr;   // [N(N-1)/2] workspace
-  uint32_t *ws_ab = nullptr; // [N(N-1)/2] workspace
+  const uint32_t *ab = nullptr;     // layered schedule of the union structure
+  const int *layer_off = nullptr;
+  int K = 0, nlayers = 0;
+  int cs_lds = 1;            // rotation angles in LDS (else in ws_cs)
+  float2 *ws_cs = nullptr;   // [K] workspace when they do not fit
 };
+size_t plain_smem(int N, int K, bool cs_lds);
 int launch_plain_sample(const PlainArgs &a, hipStream_t st);
 
 // Doubles propagation (doubles.hip).
@@ -421,6 +425,57 @@ struct DiaSys {
 // the elements with FP32 |U_ab|^2 <= thres (the sparse scheme).
 int launch_dia_u(const double *w, const double *V, int N, int64_t nmat, float fac, float thres, float2 *U,
                  hipStream_t st);
+// Singles vectors through per-frame exact-exponential propagators
+// (twodir_dia.hip).  The slots of one workgroup share fs, nresp and nprop.
+struct DiaSlot {
+  int fs;        // first frame (absolute)
+  int nresp;     // history steps: psi before each of frames fs .. fs+nresp-1
+  int nprop;     // propagation steps
+  int x;         // dipole component of the initial state when in_row < 0
+  int in_row;    // initial state row of DiaSlotArgs::in, or -1: mu_x(fs)
+  int out_row;   // final state row of DiaSlotArgs::out, or -1
+  int hist_row;  // first history row of DiaSlotArgs::hist, or -1
+};
+struct DiaSlotArgs {
+  int N = 0;
+  int64_t f_lo = 0;                // absolute frame of table row 0
+  const float2 *U = nullptr;       // [rows][N][N]
+  const float *mu_rec = nullptr;
+  int64_t mu_stride = 0;           // 0: static dipoles
+  const DiaSlot *slots = nullptr;
+  const int *wg_begin = nullptr;   // [nwg + 1]
+  int nwg = 0;
+  const float2 *in = nullptr;
+  float2 *out = nullptr;
+  float2 *hist = nullptr;
+};
+int launch_dia_slots(const DiaSlotArgs &a, int max_ns, hipStream_t st);
+int dia_slots_max(int N);
+
+// The sparse two-exciton step (twodir_dia.hip): per-frame element lists of
+// the one-exciton propagator and the step on groups of vectors.
+int launch_sparse_lists(const float2 *U, int N, int64_t nrows, float thres, float4 *el, int *colptr, int *colidx,
+                        int *cnt, hipStream_t st);
+struct SparseDblArgs {
+  int N = 0, m = 1, es = 0;
+  float anh = 0.f;                 // fixed anharmonicity (0: per-frame records)
+  const float *anh_rec = nullptr;  // (1 + N) words per frame, absolute frame index
+  float fm = 0.f;                  // the half-phase factor f * 0.5 / m
+  int64_t f_lo = 0;                // absolute frame of list row 0
+  const float4 *el = nullptr;      // [rows][N*N]
+  const int *colptr = nullptr;     // [rows][N + 1]
+  const int *colidx = nullptr;     // [rows][N*N]
+  const int *cnt = nullptr;        // [rows]
+  float2 *state = nullptr;         // [vectors][N2]
+  const int *vec_begin = nullptr;  // [groups + 1]
+  const int64_t *rows = nullptr;   // [groups] list row of each group
+};
+int launch_sparse_doubles(const SparseDblArgs &a, int ngroups, hipStream_t st);
+
+// Per-frame exact-exponential propagators of frames f_first .. f_first+nfr
+// (multi.hip): dense H, FP64 Jacobi, U = V exp(-i E fac) V^T in FP32.
+int launch_dia_tables(const FrameSource &src, int N, int64_t f_first, int64_t nfr, float fac, float thres,
+                      float2 *U, DevBuf &A, DevBuf &V, DevBuf &w, int *status, hipStream_t st);
 size_t rows_smem(int N, int R);
 int rows_for(int N, int Kreal, int nlayers_asap);
 void rows_device_order(const Schedule &s, int R, std::vector<uint32_t> &ab);

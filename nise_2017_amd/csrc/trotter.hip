@@ -969,26 +969,25 @@ int launch_reduce_partials(const double2 *partial, int nwg, int tmax, float *re,
 }
 
 // ---------------------------------------------------------------------------
-// Plain per-sample kernel: one workgroup, the three polarisation vectors of
-// one sample, frames re-read from HBM at every step.  Mirrors the
-// reference's per-call structure (src/absorption.c:113-145 with
-// src/propagate.c:821-899 inside): per step, compact the kept couplings of
-// the frame (prefix sum over the triangle), evaluate the rotation angles,
-// then sweep the rotations in lexicographic order, one lane per
-// polarisation.
+// Plain per-sample kernel: the straightforward GPU port of the reference's
+// per-sample loop (src/absorption.c:113-145 with src/propagate.c:821-899
+// inside), used as the baseline the batched kernels are measured against.
+// One launch per start sample, one workgroup holding its three polarisation
+// vectors in LDS.  Per step the frame is read from HBM (half-step phases of
+// the diagonal, angles of the kept couplings of the union structure), then
+// the whole workgroup sweeps the layered schedule, one barrier per layer
+// (rotations of a layer touch disjoint sites, so every site sees the
+// reference's operations in the reference's order).
 // ---------------------------------------------------------------------------
 constexpr int kPlainBlock = 256;
 
 __global__ void __launch_bounds__(kPlainBlock) plain_sample_kernel(PlainArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int N = A.N;
-  const int npair = N * (N - 1) / 2;
+  const int N = A.N, K = A.K;
   float2 *psi = reinterpret_cast<float2 *>(smem);        // [3][N]
   float2 *D = psi + 3 * N;                               // [N]
-  int *cnt = reinterpret_cast<int *>(D + N);             // [kPlainBlock + 1]
-  float2 *cs = A.ws_cs;                                  // [npair] global
-  uint32_t *ab = A.ws_ab;                                // [npair] global
-  float2 *red = reinterpret_cast<float2 *>(cnt + kPlainBlock + 1);  // [3][waves]
+  float2 *red = D + N;                                   // [3][waves]
+  float2 *cs = A.cs_lds ? red + 3 * (kPlainBlock / 64) : A.ws_cs;  // [K]
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   constexpr int kWaves = kPlainBlock / 64;
 
@@ -999,7 +998,7 @@ __global__ void __launch_bounds__(kPlainBlock) plain_sample_kernel(PlainArgs A) 
   __syncthreads();
   for (int t1 = 0; t1 < A.tmax; ++t1) {
     const int64_t f = A.ti + t1;
-    // response for the three polarisations
+    // response for the three polarisations (src/absorption.c:205-212)
     for (int x = 0; x < 3; ++x) {
       const float *mu = A.mu_rec + (A.mu_stride ? f * A.mu_stride : 0) + 1 + x * N;
       float rr = 0.f, ri = 0.f;
@@ -1019,81 +1018,59 @@ __global__ void __launch_bounds__(kPlainBlock) plain_sample_kernel(PlainArgs A) 
       A.resp[t1].y += si;
     }
     if (t1 == A.tmax - 1) break;  // the last propagation is never observed
-    // diagonal phases
+    // the frame: half-step phases (src/propagate.c:863-866) and rotation
+    // angles (:881-886; an absent coupling is the identity rotation)
     for (int i = t; i < N; i += kPlainBlock) {
       float h = site_energy(A.src, N, f, i);
       double arg = 0.5 * (double)h * (double)A.fac;
       D[i] = make_float2((float)cos(arg), (float)(-sin(arg)));
     }
-    // compaction of kept couplings in lexicographic order: each thread owns
-    // a contiguous chunk of the pair list
-    const int chunk = (npair + kPlainBlock - 1) / kPlainBlock;
-    const int p0 = t * chunk, p1 = min(npair, p0 + chunk);
-    int a = 0, rem = p0;
-    while (a < N - 1 && rem >= N - 1 - a) { rem -= N - 1 - a; ++a; }
-    int b = a + 1 + rem;
-    int mine = 0;
-    {
-      int aa = a, bb = b;
-      for (int p = p0; p < p1; ++p) {
-        mine += kept_coupling(A.src, N, f, aa, bb) != 0.f;
-        if (++bb == N) { ++aa; bb = aa + 1; }
+    for (int k = t; k < K; k += kPlainBlock) {
+      const uint32_t p = A.ab[k];
+      const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
+      const float J = kept_coupling(A.src, N, f, a, b);
+      float co = 1.f, si = 0.f;
+      if (J != 0.f) {
+        const float Jf = __fmul_rn(J, A.fac);
+        si = (float)(-sin((double)Jf));
+        co = (float)sqrt((double)__fsub_rn(1.0f, __fmul_rn(si, si)));
       }
-    }
-    cnt[t] = mine;
-    __syncthreads();
-    if (t == 0) {
-      int s = 0;
-      for (int i = 0; i < kPlainBlock; ++i) { int c = cnt[i]; cnt[i] = s; s += c; }
-      cnt[kPlainBlock] = s;
+      cs[k] = make_float2(co, si);
     }
     __syncthreads();
-    {
-      int k = cnt[t];
-      int aa = a, bb = b;
-      for (int p = p0; p < p1; ++p) {
-        float J = kept_coupling(A.src, N, f, aa, bb);
-        if (J != 0.f) {
-          float Jf = __fmul_rn(J, A.fac);
-          float si = (float)(-sin((double)Jf));
-          float co = (float)sqrt((double)__fsub_rn(1.0f, __fmul_rn(si, si)));
-          cs[k] = make_float2(co, si);
-          ab[k] = (uint32_t)aa | ((uint32_t)bb << 16);
-          ++k;
-        }
-        if (++bb == N) { ++aa; bb = aa + 1; }
-      }
-    }
-    __syncthreads();
-    const int kmax = cnt[kPlainBlock];
     for (int sub = 0; sub < A.m; ++sub) {
       for (int i = t; i < 3 * N; i += kPlainBlock) psi[i] = cmul_diag(psi[i], D[i % N]);
       __syncthreads();
-      if (t < 3) {
-        float2 *p = psi + t * N;
-        for (int k = 0; k < kmax; ++k) {
-          uint32_t pk = ab[k];
-          int ka = (int)(pk & 0xffffu), kb = (int)(pk >> 16);
-          float2 pa = p[ka], pb = p[kb];
-          rotate_pair(pa, pb, cs[k].x, cs[k].y);
-          p[ka] = pa;
-          p[kb] = pb;
+      for (int l = 0; l < A.nlayers; ++l) {
+        const int k0 = A.layer_off[l], n = 3 * (A.layer_off[l + 1] - k0);
+        for (int e = t; e < n; e += kPlainBlock) {
+          const int k = k0 + e / 3, x = e % 3;
+          const uint32_t pk = A.ab[k];
+          float2 *pv = psi + x * N;
+          const int ka = (int)(pk & 0xffffu), kb = (int)(pk >> 16);
+          float2 pa = pv[ka], pb = pv[kb];
+          const float2 c = cs[k];
+          rotate_pair(pa, pb, c.x, c.y);
+          pv[ka] = pa;
+          pv[kb] = pb;
         }
+        __syncthreads();
       }
-      __syncthreads();
       for (int i = t; i < 3 * N; i += kPlainBlock) psi[i] = cmul_diag(psi[i], D[i % N]);
       __syncthreads();
     }
   }
 }
 
+size_t plain_smem(int N, int K, bool cs_lds) {
+  return (size_t)4 * N * sizeof(float2) + 3 * (kPlainBlock / 64) * sizeof(float2) +
+         (cs_lds ? (size_t)K * sizeof(float2) : 0);
+}
+
 int launch_plain_sample(const PlainArgs &a, hipStream_t st) {
-  int N = a.N, npair = N * (N - 1) / 2;
-  (void)npair;
-  size_t smem = (size_t)4 * N * sizeof(float2) + (kPlainBlock + 1) * sizeof(int) +
-                3 * (kPlainBlock / 64) * sizeof(float2);
+  const size_t smem = plain_smem(a.N, a.K, a.cs_lds);
   if (smem > 160 * 1024) {
-    set_error("plain kernel: N=%d needs %zu B of LDS", N, smem);
+    set_error("plain kernel: N=%d needs %zu B of LDS", a.N, smem);
     return NISE_ERR_ARG;
   }
   NISE_HIP_CHECK(hipFuncSetAttribute((const void *)plain_sample_kernel,

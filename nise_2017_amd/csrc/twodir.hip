@@ -8,11 +8,17 @@
 //   M[x]       mu_x(tk) propagated through t3                    (3)
 //   fr[x1][x2] dipole_double(mu_x2(tk), M[x1](tk))               (9, doubles)
 //   ft[x0][x2][t1] dipole_double(mu_x2(tk), L0[x0][t1])          (9 x T1)
-// with tj = sample*Samplerate + T1 and tk = tj (no waiting time).  The EA
-// "right" vectors are not propagated: the sign -1 step of conj(psi) is, bit
-// for bit, conj of the sign +1 step of psi (negation commutes with every
-// rounded product and sum of the step), so rightrr[t1](t3) =
-// conj(L[x0][t1](t3)) and rightnr(t3) = conj(M[x1](t3)).
+// with tj = sample*Samplerate + T1 and tk = tj + T2.  Without a waiting
+// time (T2 = 0) the SE and EA pathways start from the same vectors (M[x] is
+// leftrr, L0 is leftnr).  With T2 > 0 the SE/EA start vectors are mu_x(tj)
+// and L0[x][t1] carried through the waiting time by the exact exponential
+// (src/calc_2DIR.c:303-326): those are the rows the history records as "M"
+// and "L" (started from Linit), and the GB's own M[x] (mu_x(tk) through t3)
+// gets three more history rows.  The EA "right" vectors are not
+// propagated: the sign -1 step of conj(psi) is, bit for bit, conj of the
+// sign +1 step of psi (negation commutes with every rounded product and sum
+// of the step), so rightrr[t1](t3) = conj(L[x0][t1](t3)) and rightnr(t3) =
+// conj(M[x1](t3)).
 //
 // Stages: seeds (conveyor, write back), history (conveyor recording L and M
 // at every t3), GB/SE responses from the history, doubles setup, then per
@@ -49,6 +55,8 @@ static float polarweight_h(int pol, int x) {
 
 struct TwoDirArgs {
   int N, T1, T3, S;         // sites, t1 and t3 points, samples in this chunk
+  int T2;                   // waiting time steps: tk = tj + T2
+  int nrow;                 // history rows per sample: 3 T1 + 3, plus 3 GB rows when T2 > 0
   int s0, sample;           // first sample (absolute), Samplerate
   int do_gb, do_se, do_ea;
   int es;                   // two-level sites (calc_2DES): no |aa> states, no overtones
@@ -57,7 +65,9 @@ struct TwoDirArgs {
   int64_t mu_stride;        // 0: static dipoles
   const float *over_rec;    // overtone dipole records (1 + 3N) when anh == 0
   const float2 *L0;         // [S][3][T1][N]
-  const float2 *hist;       // [S][3T1+3][T3][N]: rows of L then M
+  const float2 *Linit;      // [S][3T1+3][N]: L[x][t1] and M[x] at tk (T2 > 0), else L0
+  int linit_stride;         // rows per sample of Linit (3 T1 + 3, or 3 T1 for L0)
+  const float2 *hist;       // [S][nrow][T3][N]: rows of L, then M, then the GB's M (T2 > 0)
   float2 *dbl;              // [S][9 + 9 T1][N2]
   float *E;                 // [S][21][4][T3][T1]
 };
@@ -69,13 +79,17 @@ __device__ inline const float *mu_at(const TwoDirArgs &A, int64_t f, int x) {
 __device__ inline int64_t tj_of(const TwoDirArgs &A, int s) {
   return (int64_t)(A.s0 + s) * A.sample + A.T1;
 }
+__device__ inline int64_t tk_of(const TwoDirArgs &A, int s) { return tj_of(A, s) + A.T2; }
 
-// history row of L[x][t1] / M[x]
+// history row of L[x][t1] / M[x] (the SE and EA vectors) / the GB's M[x]
 __device__ inline int64_t hrow_L(const TwoDirArgs &A, int s, int x, int t1) {
-  return (((int64_t)s * (3 * A.T1 + 3)) + x * A.T1 + t1) * A.T3;
+  return (((int64_t)s * A.nrow) + x * A.T1 + t1) * A.T3;
 }
 __device__ inline int64_t hrow_M(const TwoDirArgs &A, int s, int x) {
-  return (((int64_t)s * (3 * A.T1 + 3)) + 3 * A.T1 + x) * A.T3;
+  return (((int64_t)s * A.nrow) + 3 * A.T1 + x) * A.T3;
+}
+__device__ inline int64_t hrow_G(const TwoDirArgs &A, int s, int x) {
+  return (((int64_t)s * A.nrow) + 3 * A.T1 + (A.T2 > 0 ? 3 : 0) + x) * A.T3;
 }
 
 // L0[s][x][0] = mu_x(tj) (no propagation for t1 = 0)
@@ -112,31 +126,44 @@ __device__ __forceinline__ void dot9(const float2 *v0, const float2 *v1, const f
 // The GB and SE dots that do not depend on both t1 and t3, one workgroup
 // per sample:
 //   g1[s][x0*3+y][t1] = mu_y(tj) . L0[x0][t1]          GB t1n (src/calc_2DIR.c:244-250)
-//   g3[s][t3][a*3+b]  = mu_b(tk + t3) . M[a](t3)       GB t3n (:265-268), SE t3r (:378-381)
-// With tk = tj the reference's SE t1r[x0][x2] (:363-369) is conj(g1[x0][x2])
-// and its SE static t3n[x1][x2] (:353-357) is conj(g3[0][x1*3+x2]): the same
-// rounded products and sums with the imaginary sign flipped (x - p rounds to
-// -(-x + p)).
-__global__ void gbse_static_kernel(TwoDirArgs A, float2 *g1, float2 *g3) {
-  extern __shared__ __align__(16) float muj[];  // [3][N] mu(tj)
+//   g3[s][t3][a*3+b]  = mu_b(tk + t3) . Mgb[a](t3)     GB t3n (:265-268)
+// and for the SE
+//   g3P[s][t3][a*3+b] = mu_b(tk + t3) . M[a](t3)       SE t3r (:378-381)
+//   g1s[s][x0*3+y][t1] = mu_y(tk) . L[x0][t1](0)       conj: SE t1r (:363-369)
+// The reference's SE t1r[x0][x2] is conj(g1s[x0][x2]) and its SE static
+// t3n[x1][x2] (:353-357) is conj(g3P[0][x1*3+x2]): the same rounded products
+// and sums with the imaginary sign flipped (x - p rounds to -(-x + p)).
+// Without a waiting time g3P is g3 and g1s is g1 (the SE vectors are the GB
+// ones), and only g1 and g3 are formed.
+__global__ void gbse_static_kernel(TwoDirArgs A, float2 *g1, float2 *g3, float2 *g1s, float2 *g3P) {
+  extern __shared__ __align__(16) float muj[];  // [3][N] mu(tj), [3][N] mu(tk)
   const int s = blockIdx.x, N = A.N, T1 = A.T1, T3 = A.T3;
-  const int64_t tj = tj_of(A, s);
-  for (int e = threadIdx.x; e < 3 * N; e += blockDim.x) muj[e] = mu_at(A, tj, e / N)[e % N];
+  const int64_t tj = tj_of(A, s), tk = tk_of(A, s);
+  float *muk = muj + 3 * N;
+  for (int e = threadIdx.x; e < 3 * N; e += blockDim.x) {
+    muj[e] = mu_at(A, tj, e / N)[e % N];
+    muk[e] = mu_at(A, tk, e / N)[e % N];
+  }
   __syncthreads();
-  for (int e = threadIdx.x; e < T1 + T3; e += blockDim.x) {
+  const int ne = A.T2 > 0 ? 2 * (T1 + T3) : T1 + T3;
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
     float2 acc[9];
-    if (e < T1) {
-      const float2 *l0 = A.L0 + ((int64_t)s * 3 * T1 + e) * N;
-      dot9(l0, l0 + (int64_t)T1 * N, l0 + (int64_t)2 * T1 * N, muj, N, N, acc);
+    const bool se = e >= T1 + T3;  // the SE's own dots (T2 > 0)
+    const int r = se ? e - (T1 + T3) : e;
+    if (r < T1) {
+      const float2 *l0 = se ? A.Linit + ((int64_t)s * A.linit_stride + r) * N : A.L0 + ((int64_t)s * 3 * T1 + r) * N;
+      dot9(l0, l0 + (int64_t)T1 * N, l0 + (int64_t)2 * T1 * N, se ? muk : muj, N, N, acc);
+      float2 *o = se ? g1s : g1;
 #pragma unroll
-      for (int j = 0; j < 9; ++j) g1[((int64_t)s * 9 + j) * T1 + e] = acc[j];
+      for (int j = 0; j < 9; ++j) o[((int64_t)s * 9 + j) * T1 + r] = acc[j];
     } else {
-      const int t3 = e - T1;
-      const float2 *m0 = A.hist + (hrow_M(A, s, 0) + t3) * N;
+      const int t3 = r - T1;
+      const float2 *m0 = A.hist + ((se ? hrow_M(A, s, 0) : hrow_G(A, s, 0)) + t3) * N;
       const int64_t rs = (int64_t)T3 * N;
-      dot9(m0, m0 + rs, m0 + 2 * rs, mu_at(A, tj + t3, 0), N, N, acc);
+      dot9(m0, m0 + rs, m0 + 2 * rs, mu_at(A, tk + t3, 0), N, N, acc);
+      float2 *o = se ? g3P : g3;
 #pragma unroll
-      for (int j = 0; j < 9; ++j) g3[((int64_t)s * T3 + t3) * 9 + j] = acc[j];
+      for (int j = 0; j < 9; ++j) o[((int64_t)s * T3 + t3) * 9 + j] = acc[j];
     }
   }
 }
@@ -145,7 +172,8 @@ __global__ void gbse_static_kernel(TwoDirArgs A, float2 *g1, float2 *g3) {
 // dots SE t1n[x0][x3] = L[x0][t1](t3) . mu_x3(tk + t3) (src/calc_2DIR.c
 // :384-390) with the static dots of gbse_static_kernel.  Writes
 // E = -GB - SE for all 21 items.
-__global__ void gbse_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3) {
+__global__ void gbse_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3, const float2 *g1s,
+                            const float2 *g3P) {
   extern __shared__ __align__(16) float mul[];  // [3][N] mu(tk + t3)
   constexpr int kPol[21][4] = {
       {0, 0, 0, 0}, {1, 1, 1, 1}, {2, 2, 2, 2}, {0, 0, 1, 1}, {0, 0, 2, 2}, {1, 1, 0, 0},
@@ -153,23 +181,27 @@ __global__ void gbse_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3) {
       {1, 2, 1, 2}, {2, 0, 2, 0}, {2, 1, 2, 1}, {0, 1, 1, 0}, {0, 2, 2, 0}, {1, 0, 0, 1},
       {1, 2, 2, 1}, {2, 0, 0, 2}, {2, 1, 1, 2}};
   const int s = blockIdx.x, t3 = blockIdx.y, N = A.N, T1 = A.T1;
-  const int64_t tl = tj_of(A, s) + t3, plane = (int64_t)A.T3 * T1;
+  const int64_t tl = tk_of(A, s) + t3, plane = (int64_t)A.T3 * T1;
   for (int e = threadIdx.x; e < 3 * N; e += blockDim.x) mul[e] = mu_at(A, tl, e / N)[e % N];
   __syncthreads();
-  float2 t3n[9], t3s[9];
+  float2 t3n[9], t3p[9], t3s[9];
 #pragma unroll
   for (int j = 0; j < 9; ++j) {
     t3n[j] = cld(g3 + ((int64_t)s * A.T3 + t3) * 9 + j);
-    t3s[j] = cld(g3 + (int64_t)s * A.T3 * 9 + j);
+    t3p[j] = cld(g3P + ((int64_t)s * A.T3 + t3) * 9 + j);
+    t3s[j] = cld(g3P + (int64_t)s * A.T3 * 9 + j);
     t3s[j].y = -t3s[j].y;
   }
   for (int t1 = threadIdx.x; t1 < T1; t1 += blockDim.x) {
-    float2 se1[9], gb1[9];
+    float2 se1[9], gb1[9], gs1[9];
     const float2 *lt = A.hist + (hrow_L(A, s, 0, t1) + t3) * N;
     const int64_t rs = (int64_t)T1 * A.T3 * N;
     dot9(lt, lt + rs, lt + 2 * rs, mul, N, N, se1);
 #pragma unroll
-    for (int j = 0; j < 9; ++j) gb1[j] = gld(g1 + ((int64_t)s * 9 + j) * T1 + t1);
+    for (int j = 0; j < 9; ++j) {
+      gb1[j] = gld(g1 + ((int64_t)s * 9 + j) * T1 + t1);
+      gs1[j] = gld(g1s + ((int64_t)s * 9 + j) * T1 + t1);
+    }
 #pragma unroll
     for (int m = 0; m < 21; ++m) {
       const int x0 = kPol[m][0], x1 = kPol[m][1], x2 = kPol[m][2], x3 = kPol[m][3];
@@ -183,9 +215,9 @@ __global__ void gbse_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3) {
         e[3] -= __fsub_rn(__fmul_rn(t3nr, t1nr), __fmul_rn(t3ni, t1ni));
       }
       if (A.do_se) {  // src/calc_2DIR.c:397-400
-        const float t3rr = t3n[x1 * 3 + x3].x, t3ri = t3n[x1 * 3 + x3].y;
+        const float t3rr = t3p[x1 * 3 + x3].x, t3ri = t3p[x1 * 3 + x3].y;
         const float t3nr = t3s[x1 * 3 + x2].x, t3ni = t3s[x1 * 3 + x2].y;
-        const float t1rr = gb1[x0 * 3 + x2].x, t1ri = -gb1[x0 * 3 + x2].y;
+        const float t1rr = gs1[x0 * 3 + x2].x, t1ri = -gs1[x0 * 3 + x2].y;
         const float t1nr = se1[x0 * 3 + x3].x, t1ni = se1[x0 * 3 + x3].y;
         e[0] -= __fadd_rn(__fmul_rn(t3rr, t1ri), __fmul_rn(t3ri, t1rr));
         e[1] -= __fsub_rn(__fmul_rn(t3rr, t1rr), __fmul_rn(t3ri, t1ri));
@@ -203,7 +235,8 @@ __global__ void gbse_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3) {
 // per x0 the workgroup loads the 64 rows L[x0][t1](t3) with consecutive
 // lanes on consecutive sites (row stride N + 1), then each lane forms its
 // three dots of that x0 (each sum still in ascending site order).
-__global__ void gbse_lds_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3) {
+__global__ void gbse_lds_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3, const float2 *g1s,
+                                const float2 *g3P) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int kPol[21][4] = {
       {0, 0, 0, 0}, {1, 1, 1, 1}, {2, 2, 2, 2}, {0, 0, 1, 1}, {0, 0, 2, 2}, {1, 1, 0, 0},
@@ -213,13 +246,14 @@ __global__ void gbse_lds_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3
   const int s = blockIdx.x, t3 = blockIdx.y, N = A.N, T1 = A.T1, t = threadIdx.x, ld = N + 1;
   float *mul = reinterpret_cast<float *>(smem);                                    // [3][N] mu(tk + t3)
   float2 *rowL = reinterpret_cast<float2 *>(smem + ((3 * N * 4 + 15) & ~15));    // [64][N + 1]
-  const int64_t tl = tj_of(A, s) + t3, plane = (int64_t)A.T3 * T1;
+  const int64_t tl = tk_of(A, s) + t3, plane = (int64_t)A.T3 * T1;
   for (int e = t; e < 3 * N; e += 64) mul[e] = mu_at(A, tl, e / N)[e % N];
-  float2 t3n[9], t3s[9];
+  float2 t3n[9], t3p[9], t3s[9];
 #pragma unroll
   for (int j = 0; j < 9; ++j) {
     t3n[j] = cld(g3 + ((int64_t)s * A.T3 + t3) * 9 + j);
-    t3s[j] = cld(g3 + (int64_t)s * A.T3 * 9 + j);
+    t3p[j] = cld(g3P + ((int64_t)s * A.T3 + t3) * 9 + j);
+    t3s[j] = cld(g3P + (int64_t)s * A.T3 * 9 + j);
     t3s[j].y = -t3s[j].y;
   }
   for (int c0 = 0; c0 < T1; c0 += 64) {  // uniform: every lane reaches the barriers
@@ -249,9 +283,12 @@ __global__ void gbse_lds_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3
       for (int y = 0; y < 3; ++y) se1[x0 * 3 + y] = acc[y];
     }
     if (t1 >= T1) continue;
-    float2 gb1[9];
+    float2 gb1[9], gs1[9];
 #pragma unroll
-    for (int j = 0; j < 9; ++j) gb1[j] = gld(g1 + ((int64_t)s * 9 + j) * T1 + t1);
+    for (int j = 0; j < 9; ++j) {
+      gb1[j] = gld(g1 + ((int64_t)s * 9 + j) * T1 + t1);
+      gs1[j] = gld(g1s + ((int64_t)s * 9 + j) * T1 + t1);
+    }
 #pragma unroll
     for (int m = 0; m < 21; ++m) {
       const int x0 = kPol[m][0], x1 = kPol[m][1], x2 = kPol[m][2], x3 = kPol[m][3];
@@ -265,9 +302,9 @@ __global__ void gbse_lds_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3
         e[3] -= __fsub_rn(__fmul_rn(t3nr, t1nr), __fmul_rn(t3ni, t1ni));
       }
       if (A.do_se) {  // src/calc_2DIR.c:397-400
-        const float t3rr = t3n[x1 * 3 + x3].x, t3ri = t3n[x1 * 3 + x3].y;
+        const float t3rr = t3p[x1 * 3 + x3].x, t3ri = t3p[x1 * 3 + x3].y;
         const float t3nr = t3s[x1 * 3 + x2].x, t3ni = t3s[x1 * 3 + x2].y;
-        const float t1rr = gb1[x0 * 3 + x2].x, t1ri = -gb1[x0 * 3 + x2].y;
+        const float t1rr = gs1[x0 * 3 + x2].x, t1ri = -gs1[x0 * 3 + x2].y;
         const float t1nr = se1[x0 * 3 + x3].x, t1ni = se1[x0 * 3 + x3].y;
         e[0] -= __fadd_rn(__fmul_rn(t3rr, t1ri), __fmul_rn(t3ri, t1rr));
         e[1] -= __fsub_rn(__fmul_rn(t3rr, t1rr), __fmul_rn(t3ri, t1ri));
@@ -284,10 +321,10 @@ __global__ void gbse_lds_kernel(TwoDirArgs A, const float2 *g1, const float2 *g3
 // fr and ft = dipole_double(mu_x2(tk), ...) (src/NISE_subs.c:485-509):
 // |aa> gets over_a c_a, |ab> gets (0 + mu_a c_b) + mu_b c_a.
 // State j of the two-exciton vector vec of sample s: dipole_double of
-// mu_x2(tk) on M[x1](tk) (vec < 9, x1 x2 = vec) or on L0[x0][t1].
+// mu_x2(tk) on M[x1](tk) (vec < 9, x1 x2 = vec) or on L[x0][t1] at tk.
 __device__ __forceinline__ float2 dd_value(const TwoDirArgs &A, int s, int vec, int j) {
   const int N = A.N;
-  const int64_t tk = tj_of(A, s);
+  const int64_t tk = tk_of(A, s);
   int x2;
   const float2 *c;
   if (vec < 9) {
@@ -297,7 +334,7 @@ __device__ __forceinline__ float2 dd_value(const TwoDirArgs &A, int s, int vec, 
   } else {
     const int idx = vec - 9, x0 = idx / (3 * A.T1), t1 = idx % A.T1;
     x2 = (idx / A.T1) % 3;
-    c = A.L0 + (((int64_t)s * 3 + x0) * A.T1 + t1) * N;
+    c = A.Linit + ((int64_t)s * A.linit_stride + x0 * A.T1 + t1) * N;
   }
   const float *dip = mu_at(A, tk, x2);
   int a = 0, rem = j;
@@ -421,7 +458,7 @@ __global__ void lrr_kernel(TwoDirArgs A, int t3, float2 *lrr) {
   const int s = blockIdx.x, v = blockIdx.y;
   float *dipL = reinterpret_cast<float *>(smem), *ovL = dipL + 3 * N;
   float2 *fL = reinterpret_cast<float2 *>(ovL + 3 * N);
-  const int64_t tl = tj_of(A, s) + t3;
+  const int64_t tl = tk_of(A, s) + t3;
   const float2 *fg = A.dbl + ((int64_t)s * (9 + 9 * A.T1) + v) * N2;
   load_dip_over(A, tl, dipL, ovL);
   if (kStage)
@@ -455,7 +492,7 @@ __device__ __forceinline__ void ea_block(const TwoDirArgs &A, int t3, int C, int
   float2 *lrL = RL + (int64_t)C * N;                      // [NIT][N] leftrr_EA of each item
   float2 *LvL = lrL + NIT * N;                            // [C][NIT][N]
   float2 *ftL = LvL + (int64_t)C * NIT * N;               // [C][N2]
-  const int64_t tk = tj_of(A, s), tl = tk + t3;
+  const int64_t tk = tk_of(A, s), tl = tk + t3;
   const float2 *ftg = A.dbl + ((int64_t)s * (9 + 9 * A.T1) + 9 + (x0 * 3 + x2) * A.T1 + t1_0) * N2;
   const int x3a = c_polar[mi.x][3], x3b = c_polar[mi.y][3], x3c = NIT > 2 ? c_polar[mi.z][3] : 0;
   // dipoles first (into registers when 3N <= 2 blockDim, stored after the
@@ -859,7 +896,7 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
       }
   }
   const int q = lane / NP, c0 = lane % NP;
-  const int64_t tk = tj_of(A, s);
+  const int64_t tk = tk_of(A, s);
   for (int t3 = 0; t3 < T3; ++t3) {
     const int64_t tl = tk + t3;
     load_dip_over(A, tl, dipL, ovL);
@@ -969,7 +1006,21 @@ namespace {
 struct TwoDirBufs {
   DevBuf Hb, mub, Cb, Ab, Ob, mk, ab, lay, diag, cs, ab2, D2, cs2, stp, wpl, ltb, Rd;
   DevBuf L0, hist, dbl, E, part, slots, wgb, slots2, wgb2, vb, rows, lrrb, grp, g1b, g3b;
+  // waiting time and the exact-exponential / sparse schemes
+  DevBuf Li, g1s, g3P, Ul, Ut2, Ud, jA, jV, jw, jst, el, cptr, cidx, ecnt, dsl, dwb, dsl2, dwb2, dsl3, dwb3;
 };
+
+// Groups of DIA slots for launch_dia_slots: consecutive slots with the same
+// first frame and step counts, at most max_ns per workgroup.
+void dia_groups(const std::vector<DiaSlot> &sl, int max_ns, std::vector<int> &wb) {
+  wb.clear();
+  for (int i = 0; i < (int)sl.size(); ++i) {
+    const bool same = !wb.empty() && i - wb.back() < max_ns && sl[i].fs == sl[wb.back()].fs &&
+                      sl[i].nresp == sl[wb.back()].nresp && sl[i].nprop == sl[wb.back()].nprop;
+    if (!same) wb.push_back(i);
+  }
+  wb.push_back((int)sl.size());
+}
 }  // namespace
 
 extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H_stride,
@@ -982,9 +1033,15 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
   if (N < 1 || T1 < 1 || T3 < 1 || non->sample < 1 || non->ts < 1 || !R12)
     return set_error("2DIR: bad sizes"), NISE_ERR_ARG;
   if (N > 65535) return set_error("2DIR: N=%d exceeds 65535 sites", N), NISE_ERR_ARG;
-  if (non->propagation != 1)
-    return set_error("2DIR: only propagation 1 (Coupling) is provided"), NISE_ERR_ARG;
-  if (non->tmax2 != 0) return set_error("2DIR: waiting time tmax2 > 0 is not provided"), NISE_ERR_ARG;
+  const int prop = non->propagation, T2 = non->tmax2;
+  if (prop != 1 && prop != 0)
+    return set_error("2DIR: propagation %d is not provided (1 Coupling, 0 Sparse)", prop), NISE_ERR_ARG;
+  if (T2 < 0) return set_error("2DIR: tmax2 < 0"), NISE_ERR_ARG;
+  // the exact exponential diagonalises N x N frames (FP64 Jacobi in LDS);
+  // the sparse two-exciton step holds a frame's element lists in LDS
+  if ((T2 > 0 || prop == 0) && N > 100)
+    return set_error("2DIR: waiting time and propagation 0 support N <= 100, got %d", N), NISE_ERR_ARG;
+  if (prop == 0 && N > 64) return set_error("2DIR: propagation 0 supports N <= 64, got %d", N), NISE_ERR_ARG;
   if (non->cluster != -1) return set_error("2DIR: cluster selection is not provided"), NISE_ERR_ARG;
   // calc_2DES (src/calc_2DES.c:22-586) is this computation for two-level
   // sites: the UVvis techniques, two-exciton states without |aa>
@@ -1008,7 +1065,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
   if (end == 0) end = (int)((nframes - T1 - non->tmax2 - T3 - 1) / non->sample + 1);  // src/MPI_subs.c:51,62
   if (end <= begin) return set_error("2DIR: no samples (begin %d, end %d)", begin, end), NISE_ERR_RANGE;
   const int64_t f_lo = (int64_t)begin * non->sample;
-  const int64_t f_hi = (int64_t)(end - 1) * non->sample + T1 + T3;
+  const int64_t f_hi = (int64_t)(end - 1) * non->sample + T1 + T2 + T3;
   if (f_hi > nframes)
     return set_error("2DIR: trajectory has %lld frames, the samples need %lld", (long long)nframes,
                      (long long)f_hi), NISE_ERR_RANGE;
@@ -1073,12 +1130,86 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
   if ((rc = upl(ab, sch.ab.data(), sch.ab.size() * 4)) ||
       (rc = upl(lay, sch.layer_off.data(), sch.layer_off.size() * 4)))
     return rc;
-  if ((rc = diag.alloc((size_t)nfr * N * 8)) || (rc = cs.alloc((size_t)nfr * std::max(sch.K, 1) * 8)))
-    return rc;
-  if ((rc = launch_frame_tables(src, N, sch, ab.as<uint32_t>(), fac, f_lo, nfr, diag.as<float2>(),
-                                cs.as<float2>(), st)))
-    return rc;
-  if (do_ea) {
+  if (prop == 1) {
+    if ((rc = diag.alloc((size_t)nfr * N * 8)) || (rc = cs.alloc((size_t)nfr * std::max(sch.K, 1) * 8)))
+      return rc;
+    if ((rc = launch_frame_tables(src, N, sch, ab.as<uint32_t>(), fac, f_lo, nfr, diag.as<float2>(),
+                                  cs.as<float2>(), st)))
+      return rc;
+  }
+  // per-frame exact exponentials: the singles of propagation 0
+  // (propagate_vector: DIA when thres is 0 or above 1, else the sparse
+  // DIA_S, src/propagate.c:25-33; the EA's right vectors take DIA_S,
+  // :1519-1525, which keeps the same elements for thres <= 1), the waiting
+  // time (propagate_t2_DIA, no Trotter split, nothing dropped) and the
+  // one-exciton operator of the sparse two-exciton step (time_evolution_mat,
+  // f / Trotter, elements with |U|^2 <= thres dropped, src/propagate.c:1241-1302)
+  volatile float fexp_v = non->deltat * kIcm2ifs;  // src/propagate.c:69
+  fexp_v = fexp_v * kTwoPi;
+  const float fexp = fexp_v;
+  const float thres = non->thres;
+  const float thres_l = (thres > 0.f && thres <= 1.f) ? thres : 0.f;
+  const float fdbl = fexp / (float)non->ts;  // src/propagate.c:1253
+  const float fm_dbl = (float)((double)fexp * 0.5 / non->ts);  // src/propagate.c:1319
+  const float2 *Uleft = nullptr, *Utwo = nullptr, *Udbl = nullptr;
+  int64_t t2_lo = 0;
+  {
+    DevBuf &jst = B.jst;
+    if ((rc = jst.alloc((size_t)3 * nfr * 4))) return rc;
+    std::vector<int64_t> jn;  // frames per table, for the convergence check
+    if (prop == 0) {
+      if ((rc = B.Ul.alloc((size_t)nfr * N * N * 8))) return rc;
+      if ((rc = launch_dia_tables(src, N, f_lo, nfr, fexp, thres_l, B.Ul.as<float2>(), B.jA, B.jV, B.jw,
+                                  B.jst.as<int>(), st)))
+        return rc;
+      Uleft = B.Ul.as<float2>();
+      jn.push_back(nfr);
+      if (do_ea) {
+        if (non->ts == 1 && thres_l == thres) {
+          Udbl = Uleft;
+        } else {
+          if ((rc = B.Ud.alloc((size_t)nfr * N * N * 8))) return rc;
+          if ((rc = launch_dia_tables(src, N, f_lo, nfr, fdbl, thres, B.Ud.as<float2>(), B.jA, B.jV, B.jw,
+                                      B.jst.as<int>() + nfr, st)))
+            return rc;
+          Udbl = B.Ud.as<float2>();
+          jn.push_back(nfr);
+        }
+        const int64_t NN = (int64_t)N * N;
+        if ((rc = B.el.alloc((size_t)nfr * NN * 16)) || (rc = B.cidx.alloc((size_t)nfr * NN * 4)) ||
+            (rc = B.cptr.alloc((size_t)nfr * (N + 1) * 4)) || (rc = B.ecnt.alloc((size_t)nfr * 4)))
+          return rc;
+        if ((rc = launch_sparse_lists(Udbl, N, nfr, thres, B.el.as<float4>(), B.cptr.as<int>(), B.cidx.as<int>(),
+                                      B.ecnt.as<int>(), st)))
+          return rc;
+      }
+    }
+    if (T2 > 0) {  // frames tj .. tj + T2 - 1 of every sample
+      t2_lo = f_lo + T1;
+      const int64_t n2 = f_hi - T3 - t2_lo;
+      if (prop == 0 && thres_l == 0.f) {
+        Utwo = Uleft + (t2_lo - f_lo) * N * N;
+      } else {
+        if ((rc = B.Ut2.alloc((size_t)n2 * N * N * 8))) return rc;
+        if ((rc = launch_dia_tables(src, N, t2_lo, n2, fexp, 0.f, B.Ut2.as<float2>(), B.jA, B.jV, B.jw,
+                                    B.jst.as<int>() + 2 * nfr, st)))
+          return rc;
+        Utwo = B.Ut2.as<float2>();
+        jn.push_back(-n2);  // third slot of the status words
+      }
+    }
+    if (!jn.empty()) {  // every frame's eigensystem converged
+      std::vector<int> sts((size_t)3 * nfr, 0);
+      NISE_HIP_CHECK(hipMemcpyAsync(sts.data(), jst.p, sts.size() * 4, hipMemcpyDeviceToHost, st));
+      NISE_HIP_CHECK(hipStreamSynchronize(st));
+      for (size_t k = 0; k < jn.size(); ++k) {
+        const int64_t base = jn[k] < 0 ? 2 * nfr : (int64_t)k * nfr, n = jn[k] < 0 ? -jn[k] : jn[k];
+        for (int64_t i = 0; i < n; ++i)
+          if (sts[base + i] < 0) return set_error("2DIR: Jacobi did not converge for frame %lld", (long long)i), NISE_ERR_NUMERIC;
+      }
+    }
+  }
+  if (do_ea && prop == 1) {
     if ((rc = upl(ab2, lex.data(), lex.size() * 4))) return rc;
     if ((rc = D2.alloc((size_t)nfr * N2 * 8)) ||
         (rc = cs2.alloc((size_t)nfr * std::max((int)lex.size(), 1) * 16)))
@@ -1121,7 +1252,8 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
   // samples per chunk, sized to a device memory budget
   const int total = end - begin;
   const int nvec = 9 + 9 * T1;  // doubles vectors per sample
-  const double per_sample = 8.0 * 3 * T1 * N + 8.0 * (3 * T1 + 3) * T3 * N +
+  const int nrow = 3 * T1 + 3 + (T2 > 0 ? 3 : 0);  // history rows per sample
+  const double per_sample = 8.0 * 3 * T1 * N + 8.0 * nrow * T3 * N + (T2 > 0 ? 8.0 * (3 * T1 + 3) * N : 0.0) +
                             (do_ea ? 8.0 * nvec * N2 : 0.0) + 4.0 * 84 * plane;
   const int chunk = (int)std::max(1.0, std::min<double>({(double)total, 1024.0, 8.0e9 / per_sample}));
   const int vw = doubles_vw(N);
@@ -1143,15 +1275,26 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
     const int S = std::min(chunk, total - c0);
     TwoDirArgs A;
     A.N = N, A.T1 = T1, A.T3 = T3, A.S = S, A.s0 = begin + c0, A.sample = non->sample;
+    A.T2 = T2, A.nrow = nrow;
     A.do_gb = do_gb, A.do_se = do_se, A.do_ea = do_ea, A.es = es, A.anh = non->anharmonicity;
     A.mu_rec = mud, A.mu_stride = ctype ? 0 : mu_stride, A.over_rec = Od;
     if ((rc = L0.alloc((size_t)S * 3 * T1 * N * 8)) ||
-        (rc = hist.alloc((size_t)S * (3 * T1 + 3) * T3 * N * 8)) ||
+        (rc = hist.alloc((size_t)S * nrow * T3 * N * 8)) ||
         (rc = E.alloc((size_t)S * 84 * plane * 4)) || (rc = g1b.alloc((size_t)S * 9 * T1 * 8)) ||
         (rc = g3b.alloc((size_t)S * 9 * T3 * 8)) ||
         (rc = part.alloc((size_t)((S + kRedS - 1) / kRedS) * 12 * plane * 8)))
       return rc;
+    if (T2 > 0 && ((rc = B.Li.alloc((size_t)S * (3 * T1 + 3) * N * 8)) || (rc = B.g1s.alloc((size_t)S * 9 * T1 * 8)) ||
+                   (rc = B.g3P.alloc((size_t)S * 9 * T3 * 8))))
+      return rc;
     A.L0 = L0.as<float2>(), A.hist = hist.as<float2>(), A.dbl = dbl.as<float2>(), A.E = E.as<float>();
+    A.Linit = T2 > 0 ? B.Li.as<float2>() : L0.as<float2>();
+    A.linit_stride = T2 > 0 ? 3 * T1 + 3 : 3 * T1;
+    const float2 *g1s = T2 > 0 ? B.g1s.as<float2>() : g1b.as<float2>();
+    const float2 *g3P = T2 > 0 ? B.g3P.as<float2>() : g3b.as<float2>();
+    DiaSlotArgs ds;
+    ds.N = N, ds.f_lo = f_lo, ds.U = Uleft, ds.mu_rec = mud, ds.mu_stride = ctype ? 0 : mu_stride;
+    const int dmax = dia_slots_max(N);
 
     // (1) GB seeds L0[x][t1]: mu_x(tj - t1) propagated through frames
     //     tj-t1 .. tj-1 (src/calc_2DIR.c:215-239); t1 = 0 is mu_x(tj)
@@ -1159,53 +1302,122 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
     NISE_HIP_CHECK(hipGetLastError());
     std::vector<SlotDesc> sl;
     std::vector<int> wb;
-    for (int s = 0; s < S; ++s) {
-      const int tj = (begin + c0 + s) * non->sample + T1;
-      for (int x = 0; x < 3; ++x) {
-        const int first = (int)sl.size();
+    if (prop == 0) {  // the exact exponential / sparse scheme, three polarisations per (s, t1)
+      std::vector<DiaSlot> dl;
+      for (int s = 0; s < S; ++s) {
+        const int tj = (begin + c0 + s) * non->sample + T1;
         for (int t1 = 1; t1 < T1; ++t1)
-          sl.push_back(SlotDesc{tj - t1, 0, t1, x, 0, (s * 3 + x) * T1 + t1, 0, -1});
-        for (int b = first; b < (int)sl.size(); b += vp) wb.push_back(b);
+          for (int x = 0; x < 3; ++x) dl.push_back(DiaSlot{tj - t1, 0, t1, x, -1, (s * 3 + x) * T1 + t1, -1});
       }
+      dia_groups(dl, dmax, wb);
+      if (!dl.empty()) {
+        if ((rc = upl(B.dsl, dl.data(), dl.size() * sizeof(DiaSlot))) || (rc = upl(B.dwb, wb.data(), wb.size() * 4)))
+          return rc;
+        ds.slots = B.dsl.as<DiaSlot>(), ds.wg_begin = B.dwb.as<int>(), ds.nwg = (int)wb.size() - 1;
+        ds.in = nullptr, ds.out = L0.as<float2>(), ds.hist = nullptr;
+        if ((rc = launch_dia_slots(ds, dmax, st))) return rc;
+      }
+      wb.clear();
+    } else {
+      for (int s = 0; s < S; ++s) {
+        const int tj = (begin + c0 + s) * non->sample + T1;
+        for (int x = 0; x < 3; ++x) {
+          const int first = (int)sl.size();
+          for (int t1 = 1; t1 < T1; ++t1)
+            sl.push_back(SlotDesc{tj - t1, 0, t1, x, 0, (s * 3 + x) * T1 + t1, 0, -1});
+          for (int b = first; b < (int)sl.size(); b += vp) wb.push_back(b);
+        }
+      }
+      wb.push_back((int)sl.size());
     }
-    wb.push_back((int)sl.size());
-    if (!sl.empty()) {
+    if (prop == 1 && !sl.empty()) {
       if ((rc = upl(slots, sl.data(), sl.size() * sizeof(SlotDesc))) || (rc = upl(wgb, wb.data(), wb.size() * 4)))
         return rc;
       a.slots = slots.as<SlotDesc>(), a.wg_slot_begin = wgb.as<int>(), a.nwg = (int)wb.size() - 1;
       a.state = L0.as<float2>(), a.writeback = 1, a.hist = nullptr;
       if ((rc = launch_conveyor(a, vp, block, st))) return rc;
     }
-    // (2) history: L[x][t1] (from L0) and M[x] (from mu_x(tk)) at t3 = 0..T3-1
+    // (1b) waiting time: L0[x][t1] and mu_x(tj) through frames tj .. tj+T2-1
+    //      by the exact exponential (src/calc_2DIR.c:303-326) into Linit
+    if (T2 > 0) {
+      std::vector<DiaSlot> dl;
+      for (int s = 0; s < S; ++s) {
+        const int tj = (begin + c0 + s) * non->sample + T1, base = s * (3 * T1 + 3);
+        for (int x = 0; x < 3; ++x)
+          for (int t1 = 0; t1 < T1; ++t1)
+            dl.push_back(DiaSlot{tj, 0, T2, x, (s * 3 + x) * T1 + t1, base + x * T1 + t1, -1});
+        for (int x = 0; x < 3; ++x) dl.push_back(DiaSlot{tj, 0, T2, x, -1, base + 3 * T1 + x, -1});
+      }
+      dia_groups(dl, dmax, wb);
+      if ((rc = upl(B.dsl2, dl.data(), dl.size() * sizeof(DiaSlot))) || (rc = upl(B.dwb2, wb.data(), wb.size() * 4)))
+        return rc;
+      DiaSlotArgs d2 = ds;
+      d2.U = Utwo, d2.f_lo = t2_lo;
+      d2.slots = B.dsl2.as<DiaSlot>(), d2.wg_begin = B.dwb2.as<int>(), d2.nwg = (int)wb.size() - 1;
+      d2.in = L0.as<float2>(), d2.out = B.Li.as<float2>(), d2.hist = nullptr;
+      if ((rc = launch_dia_slots(d2, dmax, st))) return rc;
+    }
+    // (2) history at t3 = 0..T3-1 from tk: L[x][t1] (from Linit), M[x] (from
+    //     mu_x(tk), or from Linit after a waiting time) and, after a waiting
+    //     time, the GB's M[x] from mu_x(tk)
     sl.clear(), wb.clear();
+    std::vector<DiaSlot> hl;
     for (int s = 0; s < S; ++s) {
-      const int tk = (begin + c0 + s) * non->sample + T1;
-      const int base = s * (3 * T1 + 3);
+      const int tk = (begin + c0 + s) * non->sample + T1 + T2;
+      const int base = s * nrow, lb = s * A.linit_stride;
       for (int x = 0; x < 3; ++x)
-        for (int t1 = 0; t1 < T1; ++t1)
-          sl.push_back(SlotDesc{tk, T3, T3 - 1, x, 1, (s * 3 + x) * T1 + t1, 0, (base + x * T1 + t1) * T3});
-      for (int x = 0; x < 3; ++x) sl.push_back(SlotDesc{tk, T3, T3 - 1, x, 0, -1, 0, (base + 3 * T1 + x) * T3});
-      for (int b = 0; b < 3 * T1 + 3; b += vp) wb.push_back(base + b);
+        for (int t1 = 0; t1 < T1; ++t1) {
+          const int lrow = lb + x * T1 + t1, h = (base + x * T1 + t1) * T3;
+          sl.push_back(SlotDesc{tk, T3, T3 - 1, x, 1, lrow, 0, h});
+          hl.push_back(DiaSlot{tk, T3, T3 - 1, x, lrow, -1, h});
+        }
+      for (int x = 0; x < 3; ++x) {
+        const int h = (base + 3 * T1 + x) * T3;
+        if (T2 > 0) {
+          sl.push_back(SlotDesc{tk, T3, T3 - 1, x, 1, lb + 3 * T1 + x, 0, h});
+          hl.push_back(DiaSlot{tk, T3, T3 - 1, x, lb + 3 * T1 + x, -1, h});
+        } else {
+          sl.push_back(SlotDesc{tk, T3, T3 - 1, x, 0, -1, 0, h});
+          hl.push_back(DiaSlot{tk, T3, T3 - 1, x, -1, -1, h});
+        }
+      }
+      if (T2 > 0)
+        for (int x = 0; x < 3; ++x) {
+          const int h = (base + 3 * T1 + 3 + x) * T3;
+          sl.push_back(SlotDesc{tk, T3, T3 - 1, x, 0, -1, 0, h});
+          hl.push_back(DiaSlot{tk, T3, T3 - 1, x, -1, -1, h});
+        }
+      for (int b = 0; b < nrow; b += vp) wb.push_back(base + b);
     }
     wb.push_back((int)sl.size());
-    if ((rc = upl(slots2, sl.data(), sl.size() * sizeof(SlotDesc))) || (rc = upl(wgb2, wb.data(), wb.size() * 4)))
-      return rc;
-    a.slots = slots2.as<SlotDesc>(), a.wg_slot_begin = wgb2.as<int>(), a.nwg = (int)wb.size() - 1;
-    a.state = L0.as<float2>(), a.writeback = 0, a.hist = hist.as<float2>();
-    if ((rc = launch_conveyor(a, vp, block, st))) return rc;
+    if (prop == 1) {
+      if ((rc = upl(slots2, sl.data(), sl.size() * sizeof(SlotDesc))) || (rc = upl(wgb2, wb.data(), wb.size() * 4)))
+        return rc;
+      a.slots = slots2.as<SlotDesc>(), a.wg_slot_begin = wgb2.as<int>(), a.nwg = (int)wb.size() - 1;
+      a.state = const_cast<float2 *>(A.Linit), a.writeback = 0, a.hist = hist.as<float2>();
+      if ((rc = launch_conveyor(a, vp, block, st))) return rc;
+    } else {
+      dia_groups(hl, dmax, wb);
+      if ((rc = upl(B.dsl3, hl.data(), hl.size() * sizeof(DiaSlot))) || (rc = upl(B.dwb3, wb.data(), wb.size() * 4)))
+        return rc;
+      DiaSlotArgs d3 = ds;
+      d3.slots = B.dsl3.as<DiaSlot>(), d3.wg_begin = B.dwb3.as<int>(), d3.nwg = (int)wb.size() - 1;
+      d3.in = A.Linit, d3.out = nullptr, d3.hist = hist.as<float2>();
+      if ((rc = launch_dia_slots(d3, dmax, st))) return rc;
+    }
     // (3) GB and SE terms from the history
-    hipLaunchKernelGGL(gbse_static_kernel, dim3(S), dim3(256), 3 * N * sizeof(float), st, A, g1b.as<float2>(),
-                       g3b.as<float2>());
+    hipLaunchKernelGGL(gbse_static_kernel, dim3(S), dim3(256), 6 * N * sizeof(float), st, A, g1b.as<float2>(),
+                       g3b.as<float2>(), const_cast<float2 *>(g1s), const_cast<float2 *>(g3P));
     NISE_HIP_CHECK(hipGetLastError());
     const size_t gsm = ((size_t)3 * N * 4 + 15) / 16 * 16 + (size_t)64 * (N + 1) * 8;
     if (gsm <= 64 * 1024) {
       NISE_HIP_CHECK(hipFuncSetAttribute((const void *)gbse_lds_kernel,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)gsm));
       hipLaunchKernelGGL(gbse_lds_kernel, dim3(S, T3), dim3(64), gsm, st, A, (const float2 *)g1b.as<float2>(),
-                         (const float2 *)g3b.as<float2>());
+                         (const float2 *)g3b.as<float2>(), g1s, g3P);
     } else {
       hipLaunchKernelGGL(gbse_kernel, dim3(S, T3), dim3(64), 3 * N * sizeof(float), st, A,
-                         (const float2 *)g1b.as<float2>(), (const float2 *)g3b.as<float2>());
+                         (const float2 *)g1b.as<float2>(), (const float2 *)g3b.as<float2>(), g1s, g3P);
     }
     NISE_HIP_CHECK(hipGetLastError());
     // (4) EA: doubles vectors at tk, then per t3 the responses and one
@@ -1225,7 +1437,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
       if (score > best * 1.0001) best = score, npw = w;
     }
     const size_t psmem = npw ? persist_layout(N, pvw, K2, npw).total : 0;
-    if (do_ea && npw && (size_t)(N2 + 1) * pvw * 8 <= 65536) {
+    if (do_ea && prop == 1 && npw && (size_t)(N2 + 1) * pvw * 8 <= 65536) {
       // every doubles vector stays in the LDS of its workgroup for all t3
       // (and is formed there: no doubles vectors in device memory)
       std::vector<DblGroup> gh;
@@ -1261,7 +1473,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
       for (int t3 = 0; t3 < T3; ++t3)
         for (int s = 0; s < S; ++s)
           for (int g = 0; g < gps; ++g)
-            rowh[((size_t)t3 * S + s) * gps + g] = (int64_t)(begin + c0 + s) * non->sample + T1 + t3 - f_lo;
+            rowh[((size_t)t3 * S + s) * gps + g] = (int64_t)(begin + c0 + s) * non->sample + T1 + T2 + t3 - f_lo;
       if ((rc = upl(vb, vbh.data(), vbh.size() * 4)) || (rc = upl(rows, rowh.data(), rowh.size() * 8))) return rc;
       if ((rc = lrrb.alloc((size_t)S * 27 * N * 8))) return rc;
       const void *fl = ea.stage ? (const void *)lrr_kernel<true> : (const void *)lrr_kernel<false>;
@@ -1286,8 +1498,17 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
         }
         NISE_HIP_CHECK(hipGetLastError());
         if (t3 + 1 < T3) {  // the state after the last t3 is not used
-          d.rows = rows.as<int64_t>() + (size_t)t3 * S * gps;
-          if ((rc = launch_doubles_step(d, S * gps, st))) return rc;
+          if (prop == 1) {
+            d.rows = rows.as<int64_t>() + (size_t)t3 * S * gps;
+            if ((rc = launch_doubles_step(d, S * gps, st))) return rc;
+          } else {  // src/propagate.c:1486-1509 (and :1606-1629 for two-level sites)
+            SparseDblArgs sp;
+            sp.N = N, sp.m = non->ts, sp.es = es, sp.anh = non->anharmonicity, sp.anh_rec = es ? nullptr : Ad;
+            sp.fm = fm_dbl, sp.f_lo = f_lo, sp.el = B.el.as<float4>(), sp.colptr = B.cptr.as<int>();
+            sp.colidx = B.cidx.as<int>(), sp.cnt = B.ecnt.as<int>(), sp.state = dbl.as<float2>();
+            sp.vec_begin = vb.as<int>(), sp.rows = rows.as<int64_t>() + (size_t)t3 * S * gps;
+            if ((rc = launch_sparse_doubles(sp, S * gps, st))) return rc;
+          }
         }
       }
     }

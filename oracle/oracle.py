@@ -117,6 +117,9 @@ def _bind_ref(L):
                                        C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_int, _f32, _f32]
     L.ref_read_frame_onfly.argtypes = [C.c_int, cs, cs, cs, cs, cs, C.c_float, C.c_float, C.c_int, _f32]
+    L.ref_absorption_mem.argtypes = [C.c_int, _vp, C.c_longlong, _vp, C.c_longlong, C.c_float, C.c_float,
+                                     C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, _f32, _f32]
     L.ref_absorption_files_geo.argtypes = [C.c_int, cs, cs, cs, cs, cs, cs, cs, C.c_float, C.c_float,
                                            C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int,
                                            C.c_int, C.c_int, _f32, _f32]

@@ -209,7 +209,35 @@ EXPORT int ref_absorption_files(int N, const char *hfile, const char *dfile,
                                   ts, tmax, sample, begin, end, length, re_S, im_S);
 }
 
-static int absorption_loop(t_non *non_, float *re_S, float *im_S) {
+static int absorption_loop_streams(t_non *non_, FILE *H_traj, FILE *mu_traj, float *re_S, float *im_S);
+
+static int absorption_loop(t_non *non, float *re_S, float *im_S) {
+  FILE *H_traj, *mu_traj, *Cfile = NULL;
+  open_files(non, &H_traj, &mu_traj, &Cfile);
+  return absorption_loop_streams(non, H_traj, mu_traj, re_S, im_S);
+}
+
+/* The same loop over a trajectory held in memory: the reference's read_He
+ * and read_mue read it through stdio streams over the buffers (fmemopen),
+ * so no file system or page cache is involved.  Full-type records only. */
+EXPORT int ref_absorption_mem(int N, const void *H_buf, long long H_bytes, const void *mu_buf,
+                              long long mu_bytes, float min1, float max1, float deltat, float cut,
+                              int ts, int tmax, int sample, int begin, int end, int length,
+                              float *re_S, float *im_S) {
+  t_non non;
+  init_non(&non, N, deltat, cut);
+  non.min1 = min1, non.max1 = max1;
+  non.ts = ts;
+  non.tmax = non.tmax1 = tmax;
+  non.sample = sample, non.begin = begin, non.end = end;
+  non.length = length;
+  FILE *H_traj = fmemopen((void *)H_buf, (size_t)H_bytes, "rb");
+  FILE *mu_traj = fmemopen((void *)mu_buf, (size_t)mu_bytes, "rb");
+  if (!H_traj || !mu_traj) return 1;
+  return absorption_loop_streams(&non, H_traj, mu_traj, re_S, im_S);
+}
+
+static int absorption_loop_streams(t_non *non_, FILE *H_traj, FILE *mu_traj, float *re_S, float *im_S) {
   t_non non = *non_;
   const int N = non.singles;
   non.shifte = (non.max1 + non.min1) / 2; /* src/absorption.c:57-59 */
@@ -217,8 +245,7 @@ static int absorption_loop(t_non *non_, float *re_S, float *im_S) {
   float *Hamil_i_e = calloc(nn2, sizeof(float));
   float *vecr = calloc(N, sizeof(float)), *veci = calloc(N, sizeof(float));
   float *mu_eg = calloc(N, sizeof(float)), *mu_xyz = calloc(3 * N, sizeof(float));
-  FILE *H_traj, *mu_traj, *Cfile = NULL, *C_traj = NULL;
-  open_files(&non, &H_traj, &mu_traj, &Cfile);
+  FILE *C_traj = NULL;
   read_coupling(&non, C_traj, mu_traj, Hamil_i_e, mu_xyz);
   for (int samples = non.begin; samples < non.end; samples++) {
     int ti = samples * non.sample;

@@ -178,7 +178,8 @@ def test_errors():
     H, mu, anh, over = synth.twodir_system(N=3, nframes=40, seed=47)
     L = abi.lib()
     R = np.zeros((12, 4, 4), np.float32)
-    for kw, code in [(dict(tmax2=2), abi.NISE_ERR_ARG), (dict(propagation=0), abi.NISE_ERR_ARG),
+    for kw, code in [(dict(tmax2=-1), abi.NISE_ERR_ARG), (dict(propagation=3), abi.NISE_ERR_ARG),
+                     (dict(tmax2=40), abi.NISE_ERR_RANGE),
                      (dict(technique="2DIRraman"), abi.NISE_ERR_ARG),
                      (dict(end=30), abi.NISE_ERR_RANGE),
                      (dict(anharmonicity=0.0), abi.NISE_ERR_ARG)]:
