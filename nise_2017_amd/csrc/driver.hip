@@ -218,6 +218,14 @@ extern "C" void absorption(t_non *non) {
     die(1, "absorption: HamiltonianType %s is not provided by the GPU path\n", non->hamiltonian);
   if (onfly && non->propagation != 1)
     die(1, "absorption: %s couplings run with propagation Coupling on the GPU path\n", non->hamiltonian);
+  // the files are opened first (src/absorption.c:76, open_files
+  // src/read_trajectory.c:15-27), then the samples counted
+  FILE *probe = fopen(non->energyFName, "rb");
+  if (!probe) die(1, "Hamiltonian file not found!\n");
+  fclose(probe);
+  probe = fopen(non->dipoleFName, "rb");
+  if (!probe) die(1, "Dipole file %s not found!\n", non->dipoleFName);
+  fclose(probe);
   read_shift_file(non);
   // src/NISE_subs.c:197-221
   const int N_samples = (non->length - non->tmax1 - 1) / non->sample + 1;
@@ -236,12 +244,6 @@ extern "C" void absorption(t_non *non) {
   Pinned H, mu, Cc;
   if (!H.alloc((size_t)(nfr * hw)) || !mu.alloc((size_t)((ctype ? 1 : nfr) * mw)))
     die(1, "absorption: could not allocate pinned host memory\n");
-  FILE *probe = fopen(non->energyFName, "rb");
-  if (!probe) die(1, "Hamiltonian file not found!\n");
-  fclose(probe);
-  probe = fopen(non->dipoleFName, "rb");
-  if (!probe) die(1, "Dipole file %s not found!\n", non->dipoleFName);
-  fclose(probe);
   if (!read_records(non->energyFName, hw, f_lo, nfr, H.p))
     die(1, "Hamiltonian trajectory file to short, could not fill buffer!!!\n");
   if (!read_records(non->dipoleFName, mw, ctype ? 0 : f_lo, ctype ? 1 : nfr, mu.p))

@@ -320,3 +320,32 @@ EXPORT void ref_double_sparce(int N, float *Ur, float *Ui, int *R, int *C, float
   else
     propagate_double_sparce(&non, Ur, Ui, R, C, fr, fi, elements, m, Anh);
 }
+
+/* The order of src/absorption.c:76-92 before its sample loop, over the
+ * reference's own functions: open_files (missing files), control (its
+ * diagnostics; the caller ignores the result, :80), determine_samples
+ * (sample count and EndPoint checks, src/NISE_subs.c:196-221),
+ * read_coupling, then the loop of absorption_loop.  Fatal conditions print
+ * the reference's messages and exit with its codes; the tests run this in a
+ * child process.  determine_samples appends to NISE.log in the working
+ * directory. */
+EXPORT int ref_absorption_checked(int N, const char *hfile, const char *dfile, float min1, float max1,
+                                  float deltat, int tmax, int sample, int begin, int end, int length,
+                                  float *re_S, float *im_S) {
+  t_non non;
+  init_non(&non, N, deltat, 0);
+  copy_name(non.energyFName, hfile);
+  copy_name(non.dipoleFName, dfile);
+  copy_name(non.couplingFName, "");
+  copy_name(non.singleShiftFName, "");
+  non.min1 = min1, non.max1 = max1;
+  non.shifte = (max1 + min1) / 2;
+  non.tmax = non.tmax1 = tmax;
+  non.sample = sample, non.begin = begin, non.end = end;
+  non.length = length;
+  FILE *H_traj, *mu_traj, *Cfile = NULL;
+  open_files(&non, &H_traj, &mu_traj, &Cfile);
+  control(&non);
+  determine_samples(&non);
+  return absorption_loop_streams(&non, H_traj, mu_traj, re_S, im_S);
+}
