@@ -216,6 +216,12 @@ __device__ __forceinline__ void gst(T *p, T v) {
   *(__attribute__((address_space(1))) W *)p = w;
 }
 
+// Workgroup barrier for LDS data: this wave's LDS traffic complete, then
+// s_barrier.  Global loads already issued stay outstanding (__syncthreads
+// would wait for them too, which undoes a prefetch); use it only where no
+// global store of this workgroup must be visible to another wave.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // Load of wave-uniform, read-only data through the constant address space,
 // which the compiler turns into scalar loads (p must be the same in every
 // lane and the data must not be written while the kernel runs).

@@ -134,12 +134,6 @@ __device__ __forceinline__ int64_t hist_row(const SlotDesc &sd, int64_t rel, boo
   return -1;
 }
 
-__device__ __forceinline__ void lds_barrier() {
-  // LDS traffic of this wave complete, then the workgroup barrier; global
-  // loads already issued remain outstanding.
-  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
-
 struct ConvSmem {
   static __host__ __device__ size_t bytes(int N, int vp, int block, int tmax_acc, int nsteps) {
     return (size_t)N * vp * 8        // psi

@@ -47,6 +47,22 @@ def test_uneven_systems_vs_oracle(orc):
     assert np.all(done > 0)
 
 
+@pytest.mark.parametrize("sr", [3, 45])
+def test_dia_conveyor_shapes(orc, sr):
+    """DIA systems whose vectors fill one to four workgroups, with odd N and
+    N of one, two and three sites (vector groups of 1 to 64 lanes), windows
+    that overlap (sample 3 < tmax 30) and windows with gaps between them
+    (sample 45)."""
+    specs = [(1, 0, 41), (2, 0, 42), (3, 0, 43), (5, 0, 44), (17, 0, 45), (47, 0, 46), (64, 0, 47)]
+    systems = [system(N, prop, seed, T=30, ns=40, sr=sr) for N, prop, seed in specs]
+    out, _ = abi.absorption_multi(systems)
+    for (non, H, mu, Cr), (re, im) in zip(systems, out):
+        o_re, o_im = orc.absorption(non, H, mu, C_rec=Cr)
+        e = rel(re + 1j * im, o_re + 1j * o_im)
+        say(f"sample {sr} N={non.singles}: rel L2 {e:.1e}")
+        assert e < 1e-5
+
+
 def test_single_dia_system_through_batched(orc):
     non, H, mu, Cr = system(20, 0, 11, T=32, ns=9, sr=5)
     import ctypes as C
