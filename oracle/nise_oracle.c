@@ -1121,6 +1121,29 @@ void or_double_sparce_f32(int N, const float *Ur, const float *Ui, const int *R,
     sparce3(N, Ur, Ui, R, C, fr, fi, elements, m, Anh, anharmonicity, deltat);
 }
 
+/* src/NISE_subs.c:512-530: the ground-state double excitation, |ii> =    */
+/* over_i (over = sqrt(2) dipole when the anharmonicity is fixed, written  */
+/* back to over as the reference does), every other state 0.               */
+void or_dipole_double_ground(int N, const float *dipole, float *fr, float *fi, float *over,
+                             float anharmonicity) {
+  int N2 = N * (N + 1) / 2;
+  for (int i = 0; i < N2; i++) fr[i] = 0, fi[i] = 0;
+  if (anharmonicity != 0)
+    for (int i = 0; i < N; i++) over[i] = OR_sqrt2 * dipole[i];
+  for (int i = 0; i < N; i++) fr[or_sindex(i, i, N)] = over[i];
+}
+
+/* src/polar.c:107-116: the Raman tensor component (xx, xy, xz, yy, yz, zz) */
+/* of the polarisation pair (p2, p3), as its return value.                 */
+int or_raman_pol(int p2, int p3) {
+  if (p2 == 0 && p3 == 0) return 0;
+  if ((p2 == 0 && p3 == 1) || (p2 == 1 && p3 == 0)) return 1;
+  if ((p2 == 0 && p3 == 2) || (p2 == 2 && p3 == 0)) return 2;
+  if (p2 == 1 && p3 == 1) return 3;
+  if ((p2 == 1 && p3 == 2) || (p2 == 2 && p3 == 1)) return 4;
+  return 5;
+}
+
 /* One singles step of propagate_vector (src/propagate.c:20-39): the        */
 /* Trotter scheme for propagation 1; for propagation 0 the exact            */
 /* exponential when thres is 0 or above 1, else the sparse scheme (DIA_S,   */
@@ -1396,5 +1419,171 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
   free(mut3i), free(mut4), free(fr), free(fi), free(ft1r), free(ft1i);
   free(t1nr), free(t1ni), free(t1rr), free(t1ri), free(lt);
   free(Urs), free(Uis), free(Rs), free(Cs);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* One 2DIR-Raman work item, src/calc_2DIRraman.c:176-441 with             */
+/* propagate_1 (:591-633): the Trotter scheme, a fixed anharmonicity (the  */
+/* reference exits for a per-frame one, :222-226, :330-334).  Two changes, */
+/* both where the reference's result is not defined: the Raman component   */
+/* is find_raman_pol's return value (the reference discards it, :319, and  */
+/* reads its uninitialised alphapol), and lifetime 0 means no lifetime     */
+/* decay (the reference divides by 2 lifetime, :106-107).  Everything else */
+/* is the reference's, the technique variants' block conditions included   */
+/* (2DIRramanI computes diagram 1 without propagating fr and rightrr in t3; */
+/* 2DIRraman2 propagates them, :620-629).  Adds into R12 = 12 arrays        */
+/* [tmax3][tmax1] in the order of :466-468 (rrIpar, riIpar, rrIIpar, ...).  */
+/* alpha_rec: per frame 1 + 6N words, components xx xy xz yy yz zz         */
+/* (read_alpha, src/read_trajectory.c:380-399).                            */
+/* ---------------------------------------------------------------------- */
+static int tech_in(const char *t, const char *const *set, int n) {
+  for (int i = 0; i < n; i++)
+    if (!strcmp(t, set[i])) return 1;
+  return 0;
+}
+
+int or_2dirraman_item(const t_non *non, const float *H_rec, long long H_stride,
+                      const float *mu_rec, long long mu_stride, const float *C_rec,
+                      const float *alpha_rec, int sample, int molPol, float *R12) {
+  int N = non->singles, T1 = non->tmax1, T3 = non->tmax3;
+  int nn2 = N * (N + 1) / 2;
+  if (non->propagation != 1 || non->anharmonicity == 0) return NISE_ERR_ARG;
+  const char *tech = non->technique;
+  static const char *const s_k1[] = {"2DIRraman", "2DIRraman1", "2DIRramanI"};
+  static const char *const s_k2[] = {"2DIRraman", "2DIRraman2", "2DIRraman3", "2DIRramanII"};
+  static const char *const s_d2[] = {"2DIRraman", "2DIRraman2", "2DIRramanII"};
+  static const char *const s_d3[] = {"2DIRraman", "2DIRraman3", "2DIRramanII"};
+  static const char *const s_p1[] = {"2DIRraman", "2DIRraman1", "2DIRraman2"};
+  int k1 = tech_in(tech, s_k1, 3), k2 = tech_in(tech, s_k2, 4), d2 = tech_in(tech, s_d2, 3);
+  int d3 = tech_in(tech, s_d3, 3), p1 = tech_in(tech, s_p1, 3);
+  if (!k1 && !k2) return NISE_ERR_ARG;
+  size_t plane = (size_t)T3 * T1;
+#define RA(q, t3, t1) R12[(size_t)(q) * plane + (size_t)(t3) * T1 + (t1)]
+  float *He = malloc(nn2 * sizeof(float));
+  float *Anh = calloc(N, sizeof(float)), *over = calloc(N, sizeof(float));
+  float *leftrr = calloc(N, sizeof(float)), *leftri = calloc(N, sizeof(float));
+  float *leftnr = calloc((size_t)T1 * N, sizeof(float)), *leftni = calloc((size_t)T1 * N, sizeof(float));
+  float *rightrr = calloc((size_t)T1 * N, sizeof(float)), *rightri = calloc((size_t)T1 * N, sizeof(float));
+  float *rightnr = calloc(N, sizeof(float)), *rightni = calloc(N, sizeof(float));
+  float *l3r = calloc(N, sizeof(float)), *l3i = calloc(N, sizeof(float));
+  float *mut3r = calloc(N, sizeof(float)), *alpha4 = calloc(N, sizeof(float));
+  float *fr = calloc(nn2, sizeof(float)), *fi = calloc(nn2, sizeof(float));
+  float *ft1r = calloc((size_t)T1 * nn2, sizeof(float)), *ft1i = calloc((size_t)T1 * nn2, sizeof(float));
+  float *t1nr = calloc((size_t)T1 * N, sizeof(float)), *t1ni = calloc((size_t)T1 * N, sizeof(float));
+  float *lt = malloc(plane * sizeof(float));
+  for (int t1 = 0; t1 < T1; t1++)
+    for (int t3 = 0; t3 < T3; t3++)
+      lt[t3 * T1 + t1] = (non->lifetime > 0)
+          ? (float)exp(-(double)(t1 + t3) * non->deltat / (2 * non->lifetime)) : 1.0f;
+  float wpol[3];
+  for (int p = 0; p < 3; p++) wpol[p] = or_polarweight(p, molPol);
+  int tj = sample * non->sample + T1;
+  int px[4];
+  or_polar(px, molPol);
+  int apol = or_raman_pol(px[2], px[3]);
+  float dt = non->deltat, cut = non->couplingcut, anh = non->anharmonicity;
+  int ts = non->ts;
+
+  /* t1: kI singles and kII doubles from mu_x0(tj - t1) (:197-246) */
+  for (int t1 = 0; t1 < T1; t1++) {
+    float *lr = leftnr + (size_t)t1 * N, *li = leftni + (size_t)t1 * N;
+    float *gr = ft1r + (size_t)t1 * nn2, *gi = ft1i + (size_t)t1 * nn2;
+    read_mu(non, mu_rec, mu_stride, tj - t1, px[0], lr);
+    if (k2) or_dipole_double_ground(N, lr, gr, gi, over, anh);
+    for (int tm = 0; tm < t1; tm++) {
+      or_read_H(non, H_rec, H_stride, C_rec, tj - t1 + tm, He, 0);
+      if (k1) or_coupling_S_f32(N, He, lr, li, ts, 1, dt, cut);
+      if (k2) or_coupling_S_doubles_f32(N, He, gr, gi, ts, Anh, anh, dt, cut);
+    }
+  }
+  read_mu(non, mu_rec, mu_stride, tj, px[1], mut3r); /* :249 */
+  if (k1) { /* :255-264 */
+    or_dipole_double_ground(N, mut3r, fr, fi, over, anh);
+    memcpy(rightrr, leftnr, (size_t)T1 * N * sizeof(float));
+    memcpy(rightri, leftni, (size_t)T1 * N * sizeof(float));
+    for (size_t i = 0; i < (size_t)T1 * N; i++) rightri[i] = -rightri[i];
+  }
+  if (d2) { /* :267-274 */
+    memcpy(rightnr, mut3r, N * sizeof(float));
+    for (int i = 0; i < N; i++) rightni[i] = 0;
+  }
+  if (d3) /* :280-287 */
+    for (int t1 = 0; t1 < T1; t1++)
+      or_dipole_double_last(N, mut3r, ft1r + (size_t)t1 * nn2, ft1i + (size_t)t1 * nn2,
+                            t1nr + (size_t)t1 * N, t1ni + (size_t)t1 * N, over, anh);
+  const float one = 1, zero = 0; /* the ground-state right side of diagram 3, :389 */
+  for (int t3 = 0; t3 < T3; t3++) {
+    int tl = tj + t3;
+    const float *a = alpha_rec + (long long)tl * (1 + 6 * N) + 1 + (long long)apol * N;
+    for (int i = 0; i < N; i++) alpha4[i] = a[i];
+    if (k1) { /* diagram 1, :306-330 */
+      or_dipole_double_last(N, alpha4, fr, fi, leftrr, leftri, over, anh);
+      for (int t1 = 0; t1 < T1; t1++) {
+        float rrI = 0, riI = 0;
+        const float *Rr = rightrr + (size_t)t1 * N, *Ri = rightri + (size_t)t1 * N;
+        for (int i = 0; i < N; i++) {
+          rrI += leftri[i] * Rr[i] + leftrr[i] * Ri[i];
+          riI += leftrr[i] * Rr[i] - Ri[i] * leftri[i];
+        }
+        for (int p = 0; p < 3; p++) {
+          float pw = wpol[p] * lt[t3 * T1 + t1];
+          RA(4 * p + 0, t3, t1) += rrI * pw;
+          RA(4 * p + 1, t3, t1) += riI * pw;
+        }
+      }
+    }
+    if (d2) /* diagram 2, :333-356 */
+      for (int t1 = 0; t1 < T1; t1++) {
+        float *Lr = leftnr + (size_t)t1 * N, *Li = leftni + (size_t)t1 * N;
+        or_dipole_double_last(N, alpha4, ft1r + (size_t)t1 * nn2, ft1i + (size_t)t1 * nn2, Lr, Li,
+                              over, anh);
+        float rrII = 0, riII = 0;
+        for (int i = 0; i < N; i++) {
+          rrII += rightnr[i] * Li[i] + rightni[i] * Lr[i];
+          riII += rightnr[i] * Lr[i] - rightni[i] * Li[i];
+        }
+        for (int p = 0; p < 3; p++) {
+          float pw = wpol[p] * lt[t3 * T1 + t1];
+          RA(4 * p + 2, t3, t1) += rrII * pw;
+          RA(4 * p + 3, t3, t1) += riII * pw;
+        }
+      }
+    if (d3) /* diagram 3, :359-402 */
+      for (int t1 = 0; t1 < T1; t1++) {
+        const float *nr = t1nr + (size_t)t1 * N, *ni = t1ni + (size_t)t1 * N;
+        for (int i = 0; i < N; i++) l3r[i] = 0, l3i[i] = 0;
+        for (int i = 0; i < N; i++) l3r[i] += alpha4[i] * nr[i], l3i[i] += alpha4[i] * ni[i];
+        float rrII = 0, riII = 0;
+        for (int i = 0; i < N; i++) {
+          rrII += one * l3i[i] + zero * l3r[i];
+          riII += one * l3r[i] - zero * l3i[i];
+        }
+        for (int p = 0; p < 3; p++) {
+          float pw = wpol[p] * lt[t3 * T1 + t1];
+          RA(4 * p + 2, t3, t1) -= rrII * pw;
+          RA(4 * p + 3, t3, t1) -= riII * pw;
+        }
+      }
+    or_read_H(non, H_rec, H_stride, C_rec, tl, He, 0); /* :421, then propagate_1 */
+    if (d2) {
+      for (int t1 = 0; t1 < T1; t1++)
+        or_coupling_S_doubles_f32(N, He, ft1r + (size_t)t1 * nn2, ft1i + (size_t)t1 * nn2, ts, Anh,
+                                  anh, dt, cut);
+      or_coupling_S_f32(N, He, rightnr, rightni, ts, -1, dt, cut);
+    }
+    if (d3)
+      for (int t1 = 0; t1 < T1; t1++)
+        or_coupling_S_f32(N, He, t1nr + (size_t)t1 * N, t1ni + (size_t)t1 * N, ts, 1, dt, cut);
+    if (p1) {
+      or_coupling_S_doubles_f32(N, He, fr, fi, ts, Anh, anh, dt, cut);
+      for (int t1 = 0; t1 < T1; t1++)
+        or_coupling_S_f32(N, He, rightrr + (size_t)t1 * N, rightri + (size_t)t1 * N, ts, -1, dt, cut);
+    }
+  }
+#undef RA
+  free(He), free(Anh), free(over), free(leftrr), free(leftri), free(leftnr), free(leftni);
+  free(rightrr), free(rightri), free(rightnr), free(rightni), free(l3r), free(l3i), free(mut3r);
+  free(alpha4), free(fr), free(fi), free(ft1r), free(ft1i), free(t1nr), free(t1ni), free(lt);
   return 0;
 }

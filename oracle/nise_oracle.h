@@ -138,6 +138,16 @@ int or_2dir_item(const t_non *non, const float *H_rec, long long H_stride,
                  const float *A_rec, const float *over_rec, int sample,
                  int molPol, float *R12);
 
+/* src/NISE_subs.c:512-530 and src/polar.c:107-116 (as a return value) */
+void or_dipole_double_ground(int N, const float *dipole, float *fr, float *fi, float *over,
+                             float anharmonicity);
+int or_raman_pol(int p2, int p3);
+/* One 2DIR-Raman work item, src/calc_2DIRraman.c:176-441 (propagation 1,
+   fixed anharmonicity); see nise_oracle.c for the two changes */
+int or_2dirraman_item(const t_non *non, const float *H_rec, long long H_stride,
+                      const float *mu_rec, long long mu_stride, const float *C_rec,
+                      const float *alpha_rec, int sample, int molPol, float *R12);
+
 #ifdef __cplusplus
 }
 #endif

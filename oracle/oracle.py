@@ -76,6 +76,10 @@ def oracle():
                                        C.c_float, C.c_float, C.c_int]
     L.or_2dir_item.argtypes = [P, _vp, C.c_longlong, _vp, C.c_longlong, _vp, _vp, _vp, C.c_int,
                                C.c_int, _f32]
+    L.or_dipole_double_ground.argtypes = [C.c_int, _f32, _f32, _f32, _f32, C.c_float]
+    L.or_raman_pol.argtypes = [C.c_int, C.c_int]
+    L.or_2dirraman_item.argtypes = [P, _vp, C.c_longlong, _vp, C.c_longlong, _vp, _vp, C.c_int, C.c_int,
+                                    _f32]
     _o = L
     return L
 
@@ -123,6 +127,10 @@ def _bind_ref(L):
     L.ref_absorption_files_geo.argtypes = [C.c_int, cs, cs, cs, cs, cs, cs, cs, C.c_float, C.c_float,
                                            C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int,
                                            C.c_int, C.c_int, _f32, _f32]
+    if hasattr(L, "ref_dipole_double_ground"):
+        L.ref_dipole_double_ground.argtypes = [C.c_int, _f32, _f32, _f32, _f32, C.c_float]
+        L.ref_find_raman_pol.argtypes = [C.c_int, C.c_int]
+        L.ref_read_alpha.argtypes = [C.c_int, cs, C.c_int, C.c_int, _f32]
     return L
 
 
@@ -190,6 +198,32 @@ def twodir(non, H, mu, C_rec=None, A_rec=None, over_rec=None, acc64=False):
                 R[...] = 0
             rc = oracle().or_2dir_item(C.byref(non), ptr(H), H.shape[1], ptr(mu), mu.shape[1],
                                        ptr(Cr), ptr(Ar), ptr(Or), s, m, R)
+            assert rc == 0
+            if acc64:
+                R64 += R
+    return R64 if acc64 else R
+
+
+def twodir_raman(non, H, mu, alpha, C_rec=None, acc64=False):
+    """Oracle 2DIR-Raman: the 12 response planes (12, tmax3, tmax1) summed over
+    samples begin..end-1 and the 21 items in the reference's work order, as
+    twodir(); alpha holds the Raman tensor records (frames, 1 + 6N)."""
+    H = np.ascontiguousarray(H, np.float32)
+    mu = np.ascontiguousarray(mu, np.float32)
+    al = np.ascontiguousarray(alpha, np.float32)
+    Cr = None if C_rec is None else np.ascontiguousarray(C_rec, np.float32)
+    non.shifte = np.float32((np.float32(non.max1) + np.float32(non.min1)) / np.float32(2))
+    R = np.zeros((12, non.tmax3, non.tmax1), np.float32)
+    R64 = np.zeros(R.shape, np.float64) if acc64 else None
+    end = non.end
+    if end == 0:
+        end = (H.shape[0] - non.tmax1 - non.tmax2 - non.tmax3 - 1) // non.sample + 1
+    for s in range(non.begin, end):
+        for m in range(21):
+            if acc64:
+                R[...] = 0
+            rc = oracle().or_2dirraman_item(C.byref(non), ptr(H), H.shape[1], ptr(mu), mu.shape[1], ptr(Cr),
+                                            ptr(al), s, m, R)
             assert rc == 0
             if acc64:
                 R64 += R

@@ -96,6 +96,31 @@ EXPORT void ref_dipole_double_last_ES(int N, float *dipole, float *cr, float *ci
   dipole_double_last_ES(&non, dipole, cr, ci, fr, fi);
 }
 
+/* dipole_double_ground (src/NISE_subs.c:512-530), with its write-back of */
+/* over for a fixed anharmonicity */
+EXPORT void ref_dipole_double_ground(int N, float *dipole, float *fr, float *fi, float *over,
+                                     float anharmonicity) {
+  t_non non;
+  init_non(&non, N, 1, 0);
+  non.anharmonicity = anharmonicity;
+  dipole_double_ground(&non, dipole, fr, fi, over);
+}
+
+/* find_raman_pol (src/polar.c:107-116) as its return value */
+EXPORT int ref_find_raman_pol(int p2, int p3) { return find_raman_pol(p2, p3, -1); }
+
+/* read_alpha (src/read_trajectory.c:380-399): component x of frame pos of */
+/* a Raman tensor file; returns its control word */
+EXPORT int ref_read_alpha(int N, const char *afile, int pos, int x, float *alpha) {
+  t_non non;
+  init_non(&non, N, 1, 0);
+  FILE *f = fopen(afile, "rb");
+  if (!f) return -1;
+  int c = read_alpha(&non, alpha, f, pos, x);
+  fclose(f);
+  return c;
+}
+
 EXPORT float ref_polarweight(int pol, int x) { return polarweight(pol, x); }
 EXPORT void ref_polar(int *xp, int x) { polar(xp, x); }
 EXPORT int ref_sindex(int a, int b, int N) { return Sindex(a, b, N); }

@@ -158,6 +158,29 @@ int main(void) {
             for (int m = 0; m < 21; m++) or_2dir_item(&n2, H, hs, mu, ms, NULL, A, O, s, m, R12);
           free(R12);
         }
+    /* 2DIR-Raman items: every technique variant, Raman tensor records */
+    {
+      float *al = malloc((size_t)T * (1 + 6 * N) * sizeof(float));
+      for (int f = 0; f < T; f++) {
+        al[(size_t)f * (1 + 6 * N)] = 0;
+        for (int i = 0; i < 6 * N; i++) al[(size_t)f * (1 + 6 * N) + 1 + i] = nrand(1);
+      }
+      const char *rt[] = {"2DIRraman", "2DIRraman1", "2DIRraman2", "2DIRraman3", "2DIRramanI", "2DIRramanII"};
+      for (int k = 0; k < 6; k++) {
+        t_non n2 = base_non(N);
+        strcpy(n2.technique, rt[k]);
+        n2.tmax1 = n2.tmax3 = 5, n2.sample = 4, n2.begin = 0, n2.end = 2, n2.anharmonicity = 16.0f;
+        float *R12 = calloc((size_t)12 * 25, sizeof(float));
+        for (int s = 0; s < 2; s++)
+          for (int m = 0; m < 21; m++) or_2dirraman_item(&n2, H, hs, mu, ms, NULL, al, s, m, R12);
+        free(R12);
+      }
+      float dg[16], dgi[16], ov[5], dp[5] = {1, 2, 3, 4, 5};
+      or_dipole_double_ground(5, dp, dg, dgi, ov, 16.0f);
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) (void)or_raman_pol(a, b);
+      free(al);
+    }
     free(H), free(mu), free(A), free(O);
   }
   printf("sanitized oracle run completed\n");

@@ -177,6 +177,33 @@ def main():
         out[f"de{i}_out"] = np.concatenate([fr, fi])
         out[f"dle{i}_in"] = np.concatenate([gr, gi])
         out[f"dle{i}_out"] = np.concatenate([lr, li])
+    # 2DIR-Raman primitives: dipole_double_ground (with its write-back of
+    # over), find_raman_pol's return value, read_alpha's record layout
+    rng3 = np.random.default_rng(20261204)
+    for i, (N, anh) in enumerate([(5, 16.0), (7, 0.0)]):
+        N2 = N * (N + 1) // 2
+        dip = rng3.normal(size=N).astype(np.float32)
+        over = rng3.normal(size=N).astype(np.float32)
+        fr = rng3.normal(size=N2).astype(np.float32)
+        fi = rng3.normal(size=N2).astype(np.float32)
+        ov = over.copy()
+        R.ref_dipole_double_ground(N, dip, fr, fi, ov, anh)
+        out[f"dg{i}_args"] = np.array([N, anh])
+        out[f"dg{i}_in"] = np.concatenate([dip, over])
+        out[f"dg{i}_out"] = np.concatenate([fr, fi, ov])
+    out["raman_pol"] = np.array([[R.ref_find_raman_pol(a, b) for b in range(3)] for a in range(3)], np.int32)
+    N, nf = 6, 4
+    rec = np.zeros((nf, 1 + 6 * N), np.float32)
+    rec[:, 0] = np.arange(nf)
+    rec[:, 1:] = rng3.normal(size=(nf, 6 * N)).astype(np.float32)
+    with tempfile.TemporaryDirectory() as d:
+        fn = os.path.join(d, "Alpha.bin")
+        rec.tofile(fn)
+        got = np.zeros((nf, 6, N), np.float32)
+        for f in range(nf):
+            for x in range(6):
+                assert R.ref_read_alpha(N, fn.encode(), f, x, got[f, x]) == 1
+    out["alpha_rec"], out["alpha_read"] = rec, got
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {len(out)} arrays)")
 

@@ -22,7 +22,8 @@ polarisations, the same plan code):
 Tolerances: absorption 1e-5 relative L2 on the response (the Trotter
 propagation follows the reference's FP32 operation order; sums are FP64
 here, FP32 running sums in the oracle), exact-exponential systems 1e-5
-(FP64 eigensystems on both sides, FP32 vectors); adjoint 1e-3 relative
+(FP64 matrix exponential on the GPU, FP64 eigensystem in the oracle, FP32
+vectors on both sides); adjoint 1e-3 relative
 against finite differences; 2DIR 1e-4 relative L2 over the 12 planes.
 """
 import os
