@@ -188,6 +188,17 @@ void calc_2DIR(t_non *non, int parentRank, int parentSize, int subRank,
 void calc_2DES(t_non *non, int parentRank, int parentSize, int subRank,
                int subSize, MPI_Comm subComm, MPI_Comm rootComm);
 
+/* Replaces src/calc_2DIRraman.c:22-663 (prototype src/calc_2DIRraman.h:5),
+ * the techniques 2DIRraman, 2DIRraman1, 2DIRraman2, 2DIRraman3, 2DIRramanI
+ * and 2DIRramanII: reads the Hamiltonian, dipole and Raman tensor (Alphafile)
+ * trajectories and writes R{par,per,cro}{I,II}.IRraman.dat.  The Raman
+ * component of each item is find_raman_pol's return value (the reference
+ * discards it, src/calc_2DIRraman.c:319), lifetime 0 means no decay; the
+ * Trotter scheme with a fixed anharmonicity only (the reference exits for a
+ * per-frame one).  Rank 0 runs every work item; other ranks return. */
+void calc_2DIRraman(t_non *non, int parentRank, int parentSize, int subRank,
+                    int subSize, MPI_Comm subComm, MPI_Comm rootComm);
+
 /* ------------------------------------------------------------------------ */
 /* 2. Batched entry points (no counterpart in the reference).                */
 /* ------------------------------------------------------------------------ */
@@ -351,6 +362,21 @@ int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H_stride,
                       const float *mu_rec, int64_t mu_stride, int64_t nframes,
                       const float *coupling_rec, const float *anh_rec,
                       const float *over_rec, int where, float *R12);
+
+/* 2DIR-Raman in one call, the computation of calc_2DIRraman
+ * (src/calc_2DIRraman.c:176-441 per work item) for every sample and item:
+ * alpha_rec holds the Raman tensor records (1 + 6N words per frame:
+ * header, then the xx, xy, xz, yy, yz, zz components), the other records
+ * as for nise_2dir_batched.  R12 receives the sums over samples and items
+ * of rrIpar, riIpar, rrIIpar, riIIpar, then the per and cro planes,
+ * un-normalised.  The technique selects the variant (2DIRraman,
+ * 2DIRraman1, 2DIRraman2, 2DIRraman3, 2DIRramanI, 2DIRramanII); the
+ * Raman component of an item is find_raman_pol's return value; propagation
+ * 1 with a fixed anharmonicity only (NISE_ERR_ARG otherwise). */
+int nise_2dirraman_batched(const t_non *non, const float *H_rec, int64_t H_stride,
+                           const float *mu_rec, int64_t mu_stride, int64_t nframes,
+                           const float *coupling_rec, const float *alpha_rec, int where,
+                           float *R12);
 
 /* One system of a multi-system batch. */
 typedef struct {

@@ -134,6 +134,8 @@ def lib():
     L.nise_dft2d.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp]
     L.nise_2dir_batched.argtypes = [P, _vp, C.c_int64, _vp, C.c_int64, C.c_int64, _vp, _vp, _vp,
                                     C.c_int, _fp]
+    L.nise_2dirraman_batched.argtypes = [P, _vp, C.c_int64, _vp, C.c_int64, C.c_int64, _vp, _vp, C.c_int,
+                                         np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")]
     for name in ("propagate_vec_coupling_S",):
         getattr(L, name).argtypes = [P, _fp, _fp, _fp, C.c_int, C.c_int]
         getattr(L, name).restype = None
@@ -296,6 +298,23 @@ def absorption_multi(systems, where=None):
         out.append((re[o:o + t].copy(), im[o:o + t].copy()))
         o += t
     return out, done
+
+
+def twodir_raman_batched(non, H, mu, alpha, nframes=None, coupling_rec=None, where=None):
+    """nise_2dirraman_batched: the 12 2DIR-Raman planes (12, tmax3, tmax1)
+    float32 summed over samples and polarisation items (un-normalised);
+    alpha holds the Raman tensor records (frames, 1 + 6N)."""
+    if where is None:
+        where = NISE_DEVICE if hasattr(H, "data_ptr") and H.is_cuda else NISE_HOST
+    if where == NISE_HOST:
+        for a in (H, mu, alpha, coupling_rec):
+            if a is not None and a.dtype != np.float32:
+                raise ValueError("records must be float32")
+    R = np.zeros((12, non.tmax3, non.tmax1), np.float32)
+    nfr = H.shape[0] if nframes is None else nframes
+    check(lib().nise_2dirraman_batched(C.byref(non), ptr(H), H.shape[1], ptr(mu), mu.shape[1], nfr,
+                                       ptr(coupling_rec), ptr(alpha), where, R))
+    return R
 
 
 def twodir_batched(non, H, mu, nframes=None, coupling_rec=None, anh_rec=None, over_rec=None,

@@ -13,7 +13,7 @@ LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(LIBDIR, "libnise_hip.so")
 
-SOURCES = ["runtime.hip", "trotter.hip", "api_singles.hip", "dia.hip", "doubles.hip", "driver.hip", "twodir.hip", "adjoint.hip", "rows.hip", "multi.hip", "fft2d.hip", "noalloc.hip", "twodir_dia.hip"]
+SOURCES = ["runtime.hip", "trotter.hip", "api_singles.hip", "dia.hip", "doubles.hip", "driver.hip", "twodir.hip", "adjoint.hip", "rows.hip", "multi.hip", "fft2d.hip", "noalloc.hip", "twodir_dia.hip", "twodir_raman.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(64) doubles_step_kernel(DoublesArgs A) {
   float4 *csL = reinterpret_cast<float4 *>(smem + (((size_t)N2 * (VW + d2l) * 8 + 15) & ~(size_t)15));  // [ch]
   uint32_t *abL = reinterpret_cast<uint32_t *>(csL + ch);                                              // [ch]
   const int g = blockIdx.x, t = threadIdx.x;
-  const int v0 = A.vec_begin[g], nv = A.vec_begin[g + 1] - v0;
+  const int v0 = A.vec_begin[g], nv = A.vec_count ? A.vec_count[g] : A.vec_begin[g + 1] - v0;
   const int64_t row = A.rows[g];
   const float4 *cs2 = A.cs2 + row * K2;
   const float2 *d2L = psi + N2 * VW, *d2g = A.D2 + row * N2;  // the D2 row in LDS (d2l) or device memory

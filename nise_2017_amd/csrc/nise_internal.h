@@ -383,7 +383,9 @@ struct DoublesArgs {
   const float2 *D2 = nullptr;      // [rows][N2]
   const float4 *cs2 = nullptr;     // [rows][K2] (co, si, co2, si2)
   float2 *state = nullptr;         // [vectors][N2]
-  const int *vec_begin = nullptr;  // [groups + 1]
+  const int *vec_begin = nullptr;  // [groups + 1], or [groups] with vec_count
+  const int *vec_count = nullptr;  // [groups] vectors of each group (groups need
+                                   // not be adjacent), or null: vec_begin[g+1] - vec_begin[g]
   const int64_t *rows = nullptr;   // [groups] table row of each group
   int es = 0;                      // two-level sites (src/propagate.c:1051-1181):
                                    // no rotation for c == a or c == b
@@ -535,10 +537,31 @@ int ensure_device();
 hipStream_t default_stream();
 hipStream_t aux_stream();
 
+// src/polar.c:36-105 (directions) and :12-33 (weights)
+static __constant__ int c_polar[21][4] = {
+    {0, 0, 0, 0}, {1, 1, 1, 1}, {2, 2, 2, 2}, {0, 0, 1, 1}, {0, 0, 2, 2}, {1, 1, 0, 0},
+    {1, 1, 2, 2}, {2, 2, 0, 0}, {2, 2, 1, 1}, {0, 1, 0, 1}, {0, 2, 0, 2}, {1, 0, 1, 0},
+    {1, 2, 1, 2}, {2, 0, 2, 0}, {2, 1, 2, 1}, {0, 1, 1, 0}, {0, 2, 2, 0}, {1, 0, 0, 1},
+    {1, 2, 2, 1}, {2, 0, 0, 2}, {2, 1, 1, 2}};
+
+inline float polarweight_h(int pol, int x) {
+  const float ithirty = 1.0 / 30;
+  float w = 0;
+  if (pol == 0) w = (x <= 2) ? 6 * ithirty : 2 * ithirty;
+  if (pol == 1) w = (x <= 2) ? 2 * ithirty : (x <= 8) ? 4 * ithirty : -1 * ithirty;
+  if (pol == 2) w = (x <= 8) ? 0.0f : (x <= 14) ? (float)(2.5 * ithirty) : (float)(-2.5 * ithirty);
+  return w;
+}
+
 // Host file helpers (driver.hip).
 // Read records [first, first+count) of a GROBIN file into dst; false if the
 // file holds fewer records.
 bool read_records(const char *name, int64_t rec_words, int64_t first, int64_t count, float *dst);
+
+// The 2D file drop-ins (twodir.hip): print the message and exit with code;
+// write one print2D file (src/MPI_subs.c:20-34), dividing R and I in place.
+[[noreturn]] void die2d(int code, const char *msg);
+void print2D_file(const char *name, float *R, float *I, const t_non *non, int sampleCount);
 
 // Pinned host staging buffer (hipHostMalloc) with RAII release.
 struct Pinned {

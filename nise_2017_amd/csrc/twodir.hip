@@ -37,22 +37,6 @@
 
 namespace nise {
 
-// src/polar.c:36-105 (directions) and :12-33 (weights)
-__constant__ int c_polar[21][4] = {
-    {0, 0, 0, 0}, {1, 1, 1, 1}, {2, 2, 2, 2}, {0, 0, 1, 1}, {0, 0, 2, 2}, {1, 1, 0, 0},
-    {1, 1, 2, 2}, {2, 2, 0, 0}, {2, 2, 1, 1}, {0, 1, 0, 1}, {0, 2, 0, 2}, {1, 0, 1, 0},
-    {1, 2, 1, 2}, {2, 0, 2, 0}, {2, 1, 2, 1}, {0, 1, 1, 0}, {0, 2, 2, 0}, {1, 0, 0, 1},
-    {1, 2, 2, 1}, {2, 0, 0, 2}, {2, 1, 1, 2}};
-
-static float polarweight_h(int pol, int x) {
-  const float ithirty = 1.0 / 30;
-  float w = 0;
-  if (pol == 0) w = (x <= 2) ? 6 * ithirty : 2 * ithirty;
-  if (pol == 1) w = (x <= 2) ? 2 * ithirty : (x <= 8) ? 4 * ithirty : -1 * ithirty;
-  if (pol == 2) w = (x <= 8) ? 0.0f : (x <= 14) ? (float)(2.5 * ithirty) : (float)(-2.5 * ithirty);
-  return w;
-}
-
 struct TwoDirArgs {
   int N, T1, T3, S;         // sites, t1 and t3 points, samples in this chunk
   int T2;                   // waiting time steps: tk = tj + T2
@@ -360,76 +344,6 @@ __global__ void dd_kernel(TwoDirArgs A) {
   if (j >= N2) return;
   A.dbl[((int64_t)s * (9 + 9 * A.T1) + vec) * N2 + j] = dd_value(A, s, vec, j);
 }
-
-// Element k of dipole_double_last(dip, f) (src/NISE_subs.c:552-575) for NIT
-// dipole sets at once, in the reference's order: sum over i < k of
-// dip_i f_ik, then over_k f_kk, then dip_j f_kj for j > k, i.e. one pass over
-// i = 0..N-1 with the weight over_k at i = k.  Each element f is read once
-// for all NIT sets.  f_ik with i <= k sits at k + rs(i), rs(i) = i(2N-i-1)/2,
-// and f_ki with i > k at rs(k) + i (src/NISE_subs.h:54-65).  Element j of
-// f is read at f[j * FS].
-template <int NIT, int FS = 1>
-__device__ __forceinline__ void ddl_multi(const float2 *f, int N, int k, const float *dip0, const float *dip1,
-                                 const float *dip2, const float ov[NIT], float2 out[NIT]) {
-  const float *dip[3] = {dip0, dip1, dip2};
-  float r[NIT], q[NIT];
-#pragma unroll
-  for (int it = 0; it < NIT; ++it) r[it] = 0.f, q[it] = 0.f;
-  // state (i, k): index k + rs(i) for i <= k, rs(k) + i above, rs(i) =
-  // i (2N - i - 1) / 2; stepped from i = 0 (index k)
-  const char *fb = reinterpret_cast<const char *>(f);
-  int ob = k * FS * 8;
-  for (int i = 0; i < N; ++i) {
-    const float2 v = *reinterpret_cast<const float2 *>(fb + ob);
-    const bool d = i == k;
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      // dip[i] is wave-uniform: read it once into a scalar register (a
-      // convergent read, so it is not turned into a masked per-lane load)
-      const float dv = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(dip[it][i])));
-      const float w = d ? ov[it] : dv;
-      r[it] = __fadd_rn(r[it], __fmul_rn(w, v.x)), q[it] = __fadd_rn(q[it], __fmul_rn(w, v.y));
-    }
-    ob += (i < k) ? (N - i - 1) * FS * 8 : FS * 8;
-  }
-#pragma unroll
-  for (int it = 0; it < NIT; ++it) out[it] = make_float2(r[it], q[it]);
-}
-
-// ddl_multi for N % 4 == 0 with 16-byte aligned dipole rows in LDS: one
-// broadcast 128-bit read per item covers four i, and the state address is
-// stepped from i = 0 (index k; + N - i - 1 while i < k, + 1 after).
-template <int NIT, int FS>
-__device__ __forceinline__ void ddl_multi4(const float2 *f, int N, int k, const float *dip0, const float *dip1,
-                                           const float *dip2, const float ov[NIT], float2 out[NIT]) {
-  const float *dip[3] = {dip0, dip1, dip2};
-  float r[NIT], q[NIT];
-#pragma unroll
-  for (int it = 0; it < NIT; ++it) r[it] = 0.f, q[it] = 0.f;
-  const char *fb = reinterpret_cast<const char *>(f);
-  int ob = k * FS * 8;
-  for (int i0 = 0; i0 < N; i0 += 4) {
-    float4 d4[NIT];
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) d4[it] = *reinterpret_cast<const float4 *>(dip[it] + i0);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int i = i0 + u;
-      const float2 v = *reinterpret_cast<const float2 *>(fb + ob);
-      const bool d = i == k;
-#pragma unroll
-      for (int it = 0; it < NIT; ++it) {
-        const float dv = u == 0 ? d4[it].x : u == 1 ? d4[it].y : u == 2 ? d4[it].z : d4[it].w;
-        const float w = d ? ov[it] : dv;
-        r[it] = __fadd_rn(r[it], __fmul_rn(w, v.x)), q[it] = __fadd_rn(q[it], __fmul_rn(w, v.y));
-      }
-      ob += (i < k) ? (N - i - 1) * FS * 8 : FS * 8;
-    }
-  }
-#pragma unroll
-  for (int it = 0; it < NIT; ++it) out[it] = make_float2(r[it], q[it]);
-}
-
 
 // dip_x(tl) and the overtone dipoles over_x(tl) for x = 0..2 into LDS
 // (src/calc_2DIR.c:450-454; sqrt(2) mu for a fixed anharmonicity,
@@ -1532,15 +1446,14 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
 // ---------------------------------------------------------------------------
 // File-level drop-in
 // ---------------------------------------------------------------------------
-namespace {
-[[noreturn]] void die2d(int code, const char *msg) {
+[[noreturn]] void nise::die2d(int code, const char *msg) {
   printf("%s", msg);
   fflush(stdout);
   exit(code);
 }
 
 // src/MPI_subs.c:20-34
-void print2D_file(const char *name, float *R, float *I, const t_non *non, int sampleCount) {
+void nise::print2D_file(const char *name, float *R, float *I, const t_non *non, int sampleCount) {
   FILE *out = fopen(name, "w");
   if (!out) return;
   const int T1 = non->tmax1;
@@ -1556,7 +1469,6 @@ void print2D_file(const char *name, float *R, float *I, const t_non *non, int sa
   }
   fclose(out);
 }
-}  // namespace
 
 namespace {
 // The file flow of src/calc_2DIR.c:22-629 and src/calc_2DES.c:22-586 (the
