@@ -363,6 +363,20 @@ int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H_stride,
                       const float *coupling_rec, const float *anh_rec,
                       const float *over_rec, int where, float *R12);
 
+/* Absorption streamed from the trajectory files named in non (Full or
+ * Coupling type, propagation 1 or 0), for frame ranges beyond device or host
+ * memory: the samples begin..end-1 (end resolved by the caller) run in
+ * windows of at most window_samples start points (0: sized to a quarter of
+ * the free device memory); a reader thread reads the next window's records
+ * into pinned host memory and uploads them on a copy stream while the
+ * current window runs as nise_absorption_batched; the windows' responses
+ * are summed in FP64.  No counterpart in the reference, which reads one
+ * frame per step (src/read_trajectory.c:138-173); absorption() uses it when
+ * the frame range exceeds a third of the free device memory.  re_S/im_S:
+ * tmax1 floats (host), un-normalised as src/absorption.c:112-146 leaves
+ * them. */
+int nise_absorption_stream(const t_non *non, int window_samples, float *re_S, float *im_S);
+
 /* 2DIR-Raman in one call, the computation of calc_2DIRraman
  * (src/calc_2DIRraman.c:176-441 per work item) for every sample and item:
  * alpha_rec holds the Raman tensor records (1 + 6N words per frame:

@@ -134,6 +134,7 @@ def lib():
     L.nise_dft2d.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp]
     L.nise_2dir_batched.argtypes = [P, _vp, C.c_int64, _vp, C.c_int64, C.c_int64, _vp, _vp, _vp,
                                     C.c_int, _fp]
+    L.nise_absorption_stream.argtypes = [P, C.c_int, _fp, _fp]
     L.nise_2dirraman_batched.argtypes = [P, _vp, C.c_int64, _vp, C.c_int64, C.c_int64, _vp, _vp, C.c_int,
                                          np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")]
     for name in ("propagate_vec_coupling_S",):

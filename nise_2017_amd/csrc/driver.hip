@@ -8,11 +8,14 @@
 // apodisation and the discrete Fourier transform in double precision on the
 // GPU and writes the file on the host.  Messages and exit codes on bad input
 // follow the reference.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "nise_internal.h"
@@ -202,6 +205,129 @@ extern "C" void do_1DFFT(t_non *non, char fname[], float *re_S_1, float *im_S_1,
   fclose(out);
 }
 
+// ---------------------------------------------------------------------------
+// Streamed absorption: the samples in windows, each window's frames read and
+// uploaded while the previous window runs
+// ---------------------------------------------------------------------------
+namespace {
+struct StreamWindow {
+  int b = 0, e = 0;            // samples [b, e)
+  int64_t f_lo = 0, nfr = 0;   // their frames
+};
+
+// Frames of a window and the bytes of one frame's records.
+int64_t window_frames(int samples, int sample, int tmax) { return (int64_t)(samples - 1) * sample + tmax; }
+}  // namespace
+
+extern "C" int nise_absorption_stream(const t_non *non, int window_samples, float *re_S, float *im_S) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (!non || !re_S || !im_S) return set_error("stream: null argument"), NISE_ERR_ARG;
+  const int N = non->singles, T = non->tmax1, sr = non->sample;
+  if (N < 1 || T < 1 || sr < 1) return set_error("stream: bad sizes"), NISE_ERR_ARG;
+  const bool ctype = is_coupling(non);
+  if (!ctype && strcmp(non->hamiltonian, "Full") != 0)
+    return set_error("stream: HamiltonianType %s is not streamed (Full and Coupling are)", non->hamiltonian),
+           NISE_ERR_ARG;
+  const int begin = non->begin, end = non->end;
+  if (end <= begin) return set_error("stream: no samples (begin %d, end %d)", begin, end), NISE_ERR_RANGE;
+  const int64_t D = non->doubles;
+  const int64_t hw = ctype ? 1 + N : 1 + (int64_t)N * (N + 1) / 2 + D * (D + 1) / 2;
+  const int64_t mw = 1 + 3 * (int64_t)N + 3 * (int64_t)N * D;
+  const int64_t cw = 1 + (int64_t)N * (N + 1) / 2 + D * (D + 1) / 2;
+  // window: the given sample count, else what a quarter of the free device
+  // memory holds in the two frame buffers and the run's per-frame tables
+  int W = window_samples;
+  if (W <= 0) {
+    size_t fr = 0, tot = 0;
+    NISE_HIP_CHECK(hipMemGetInfo(&fr, &tot));
+    const double per_frame = 4.0 * (2 * hw + (ctype ? 0 : 2 * mw)) + 8.0 * N * (N + 1);
+    const int64_t F = (int64_t)((double)fr / 4 / per_frame);
+    W = F < T ? 1 : (int)std::min<int64_t>(end - begin, (F - T) / sr + 1);
+  }
+  W = std::max(1, std::min(W, end - begin));
+  std::vector<StreamWindow> win;
+  for (int b = begin; b < end; b += W) {
+    StreamWindow w;
+    w.b = b, w.e = std::min(end, b + W);
+    w.f_lo = (int64_t)b * sr, w.nfr = window_frames(w.e - b, sr, T);
+    win.push_back(w);
+  }
+  const int64_t fmax = window_frames(W, sr, T);
+  // static dipoles and couplings of the Coupling type, once
+  Pinned hmu[2], hH[2], hC, hmu0;
+  DevBuf dH[2], dmu[2], dC, dmu0;
+  for (int i = 0; i < 2 && i < (int)win.size(); ++i) {
+    if (!hH[i].alloc((size_t)(fmax * hw)) || (!ctype && !hmu[i].alloc((size_t)(fmax * mw))))
+      return set_error("stream: could not allocate pinned host memory"), NISE_ERR_NOMEM;
+    if ((rc = dH[i].alloc((size_t)(fmax * hw * 4))) || (!ctype && (rc = dmu[i].alloc((size_t)(fmax * mw * 4)))))
+      return rc;
+  }
+  if (ctype) {
+    if (!hC.alloc((size_t)cw) || !hmu0.alloc((size_t)mw))
+      return set_error("stream: could not allocate pinned host memory"), NISE_ERR_NOMEM;
+    if (!read_records(non->couplingFName, cw, 0, 1, hC.p))
+      return set_error("stream: coupling file %s missing or too short", non->couplingFName), NISE_ERR_RANGE;
+    if (!read_records(non->dipoleFName, mw, 0, 1, hmu0.p))
+      return set_error("stream: dipole file %s missing or too short", non->dipoleFName), NISE_ERR_RANGE;
+    if ((rc = dC.alloc((size_t)cw * 4)) || (rc = dmu0.alloc((size_t)mw * 4))) return rc;
+    NISE_HIP_CHECK(hipMemcpy(dC.p, hC.p, (size_t)cw * 4, hipMemcpyHostToDevice));
+    NISE_HIP_CHECK(hipMemcpy(dmu0.p, hmu0.p, (size_t)mw * 4, hipMemcpyHostToDevice));
+  }
+  hipStream_t cp = nullptr;
+  NISE_HIP_CHECK(hipStreamCreate(&cp));
+  // read window k into buffer k % 2 and upload it on the copy stream; the
+  // message of a failure is returned, not set (it runs on a reader thread)
+  auto fetch = [&](int k, std::string &err) {
+    const StreamWindow &w = win[k];
+    const int i = k % 2;
+    if (!read_records(non->energyFName, hw, w.f_lo, w.nfr, hH[i].p)) {
+      err = std::string("Hamiltonian trajectory file ") + non->energyFName + " too short for frames up to " +
+            std::to_string(w.f_lo + w.nfr);
+      return;
+    }
+    if (!ctype && !read_records(non->dipoleFName, mw, w.f_lo, w.nfr, hmu[i].p)) {
+      err = std::string("Dipole trajectory file ") + non->dipoleFName + " too short for frames up to " +
+            std::to_string(w.f_lo + w.nfr);
+      return;
+    }
+    if (hipMemcpyAsync(dH[i].p, hH[i].p, (size_t)(w.nfr * hw * 4), hipMemcpyHostToDevice, cp) != hipSuccess ||
+        (!ctype &&
+         hipMemcpyAsync(dmu[i].p, hmu[i].p, (size_t)(w.nfr * mw * 4), hipMemcpyHostToDevice, cp) != hipSuccess) ||
+        hipStreamSynchronize(cp) != hipSuccess)
+      err = "upload of a window failed";
+  };
+  std::vector<double> acc_r(T, 0.0), acc_i(T, 0.0);
+  std::vector<float> wr(T), wi(T);
+  std::string err;
+  fetch(0, err);
+  for (size_t k = 0; k < win.size() && err.empty(); ++k) {
+    std::string next_err;
+    std::thread reader;
+    if (k + 1 < win.size()) reader = std::thread([&, k] { fetch((int)k + 1, next_err); });
+    const StreamWindow &w = win[k];
+    const int i = (int)(k % 2);
+    t_non loc = *non;  // the window's records start at its first frame
+    loc.begin = 0, loc.end = w.e - w.b;
+    const int r = nise_absorption_batched(&loc, dH[i].as<float>(), hw, ctype ? dmu0.as<float>() : dmu[i].as<float>(),
+                                          mw, w.nfr, ctype ? dC.as<float>() : nullptr, NISE_DEVICE, 0, wr.data(),
+                                          wi.data());
+    if (reader.joinable()) reader.join();
+    if (r) {
+      (void)hipStreamDestroy(cp);
+      return r;
+    }
+    for (int t = 0; t < T; ++t) acc_r[t] += wr[t], acc_i[t] += wi[t];
+    err = next_err;
+  }
+  (void)hipStreamDestroy(cp);
+  if (!err.empty()) return set_error("stream: %s", err.c_str()), NISE_ERR_RANGE;
+  for (int t = 0; t < T; ++t) re_S[t] = (float)acc_r[t], im_S[t] = (float)acc_i[t];
+  return NISE_OK;
+}
+
+static void write_absorption(t_non *non, float *re, float *im);
+
 // src/absorption.c:18-202
 extern "C" void absorption(t_non *non) {
   const int N = non->singles;
@@ -241,6 +367,17 @@ extern "C" void absorption(t_non *non) {
   const int64_t f_lo = (int64_t)non->begin * non->sample;
   const int64_t f_hi = (int64_t)(non->end - 1) * non->sample + non->tmax1;
   const int64_t nfr = f_hi - f_lo;
+  // a frame range beyond a third of the free device memory runs in windows
+  // of samples, streamed from the files (Full and Coupling types)
+  size_t dfree = 0, dtot = 0;
+  if (!onfly && hipMemGetInfo(&dfree, &dtot) == hipSuccess &&
+      (double)nfr * (hw + (ctype ? 0 : mw)) * 4 > (double)dfree / 3) {
+    std::vector<float> re(non->tmax1), im(non->tmax1);
+    if (nise_absorption_stream(non, 0, re.data(), im.data()))
+      die(1, "absorption: GPU run failed: %s\n", nise_last_error());
+    write_absorption(non, re.data(), im.data());
+    return;
+  }
   Pinned H, mu, Cc;
   if (!H.alloc((size_t)(nfr * hw)) || !mu.alloc((size_t)((ctype ? 1 : nfr) * mw)))
     die(1, "absorption: could not allocate pinned host memory\n");
@@ -291,6 +428,11 @@ extern "C" void absorption(t_non *non) {
                                  im.data());
   }
   if (rc) die(1, "absorption: GPU run failed: %s\n", nise_last_error());
+  write_absorption(non, re.data(), im.data());
+}
+
+// the output files of src/absorption.c:157-201
+static void write_absorption(t_non *non, float *re, float *im) {
   const int samples = non->end;  // src/absorption.c:185,192: loop exit value
   FILE *log = fopen("NISE.log", "a");
   if (log) {
@@ -305,7 +447,7 @@ extern "C" void absorption(t_non *non) {
     fprintf(out, "\n");
   }
   fclose(out);
-  do_1DFFT(non, (char *)"Absorption.dat", re.data(), im.data(), samples);
+  do_1DFFT(non, (char *)"Absorption.dat", re, im, samples);
   printf("----------------------------------------------\n");
   printf(" Absorption calculation succesfully completed\n");
   printf("----------------------------------------------\n\n");
