@@ -187,6 +187,8 @@ def multi_leg(args, torch):
     torch.cuda.synchronize()
     abi.absorption_multi(systems)  # warm
     torch.cuda.synchronize()
+    tuned, cands = abi.absorption_multi_tune(systems)
+    log(f"multi leg: row counts tuned {[(r, round(ms, 2)) for r, ms in cands]}, kept {tuned}")
     walls = []
     for _ in range(5):
         t0 = time.perf_counter()
@@ -200,6 +202,7 @@ def multi_leg(args, torch):
            "systems_dia": int((Ns <= 64).sum()), "systems_trotter": int((Ns > 64).sum()),
            "site_steps": site_steps, "wall_ms": round(wall * 1e3, 3),
            "wall_ms_calls": [round(w * 1e3, 3) for w in walls],
+           "rows_tuned": {"kept": tuned, "ms_per_call": {str(r): round(ms, 3) for r, ms in cands}},
            "site_steps_per_s": site_steps / wall,
            "completion_ms_p50": float(np.percentile(done, 50)),
            "completion_ms_p99": float(np.percentile(done, 99)),

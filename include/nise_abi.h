@@ -415,6 +415,19 @@ typedef struct {
 int nise_absorption_multi(const nise_system *systems, int nsys, int where,
                           float *re_S, float *im_S, double *done_ms);
 
+/* Autotuner of the multi-system batch: runs the batch once, then times it
+ * with each row count of the row conveyor (1, 2, 4, 8, 16; best of two runs,
+ * counts the batch cannot take skipped) and keeps the fastest for later
+ * nise_absorption_multi calls of this thread on a batch of the same shape
+ * (per system N, tmax, samples, spacing, scheme, Trotter steps); without it
+ * the count is the work-weighted median of the systems' own.  R_best
+ * receives the count kept; ms / codes (n entries, optional) each count's
+ * host wall time per call and the count.  Returns the number of counts
+ * timed, or minus an error code.  No counterpart in the reference (its
+ * static work split, src/MPI_subs.c:36-110). */
+int nise_absorption_multi_tune(const nise_system *systems, int nsys, int where, int *R_best,
+                               float *ms, int *codes, int n);
+
 /* Replaces the 2DFFT post-processing program (src/2DFFT.c:20-830, a
  * separate executable in the reference, run on the NISE input file) in
  * process.  Reads the response files calc_2DIR writes in the current

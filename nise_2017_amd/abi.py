@@ -135,6 +135,8 @@ def lib():
     L.nise_2dir_batched.argtypes = [P, _vp, C.c_int64, _vp, C.c_int64, C.c_int64, _vp, _vp, _vp,
                                     C.c_int, _fp]
     L.nise_absorption_stream.argtypes = [P, C.c_int, _fp, _fp]
+    L.nise_absorption_multi_tune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_void_p,
+                                             C.c_void_p, C.c_int]
     L.nise_2dirraman_batched.argtypes = [P, _vp, C.c_int64, _vp, C.c_int64, C.c_int64, _vp, _vp, C.c_int,
                                          np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")]
     for name in ("propagate_vec_coupling_S",):
@@ -299,6 +301,26 @@ def absorption_multi(systems, where=None):
         out.append((re[o:o + t].copy(), im[o:o + t].copy()))
         o += t
     return out, done
+
+
+def absorption_multi_tune(systems, where=None):
+    """nise_absorption_multi_tune over the same list as absorption_multi:
+    returns (row count kept, [(row count, ms per call), ...])."""
+    if where is None:
+        H0 = systems[0][1]
+        where = NISE_DEVICE if hasattr(H0, "data_ptr") and H0.is_cuda else NISE_HOST
+    arr = (NiseSystem * len(systems))()
+    keep = []
+    for k, (non, H, mu, Cr) in enumerate(systems):
+        arr[k] = NiseSystem(C.pointer(non), ptr(H), H.shape[1], ptr(mu), mu.shape[1], H.shape[0], ptr(Cr))
+        keep.append(non)
+    best = C.c_int()
+    ms = (C.c_float * 8)()
+    codes = (C.c_int * 8)()
+    k = lib().nise_absorption_multi_tune(C.cast(arr, C.c_void_p), len(systems), where, C.byref(best), ms, codes, 8)
+    if k < 0:
+        check(-k)
+    return best.value, [(codes[i], ms[i]) for i in range(k)]
 
 
 def twodir_raman_batched(non, H, mu, alpha, nframes=None, coupling_rec=None, where=None):

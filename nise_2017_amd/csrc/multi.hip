@@ -13,6 +13,7 @@
 // system's completion latency is the end of its last workgroup measured from
 // the first start of the run.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -519,15 +520,32 @@ struct SysState {
 struct MultiCache {
   std::vector<std::unique_ptr<SysState>> sys;
   DevBuf mk, rsys, rwm, dsys, dwm, times, out;
+  // the row count nise_absorption_multi_tune chose for a batch of this shape
+  std::vector<int64_t> tune_sig;
+  int tune_R = 0;
 };
+thread_local MultiCache g_multi;
+
+// The shape of a batch: per system N, tmax, samples, spacing, scheme, Trotter.
+void batch_signature(const nise_system *systems, int nsys, std::vector<int64_t> &sig) {
+  sig.clear();
+  for (int k = 0; k < nsys; ++k) {
+    const t_non *n = systems[k].non;
+    sig.insert(sig.end(), {(int64_t)n->singles, (int64_t)n->tmax1, (int64_t)n->begin, (int64_t)n->end,
+                           (int64_t)n->sample, (int64_t)n->propagation, (int64_t)n->ts});
+  }
+}
 }  // namespace
 
 }  // namespace nise
 
 using namespace nise;
 
-extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int where, float *re_S,
-                                     float *im_S, double *done_ms) {
+// One batch; R_force > 0 runs the Trotter systems with that row count, else
+// the tuned count of a batch of this shape, else the work-weighted median of
+// the systems' own counts.
+static int multi_run(const nise_system *systems, int nsys, int where, float *re_S, float *im_S, double *done_ms,
+                     int R_force) {
   int rc = ensure_device();
   if (rc) return rc;
   if (nsys < 1 || !systems || !re_S || !im_S) return set_error("multi: bad arguments"), NISE_ERR_ARG;
@@ -535,7 +553,7 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
   // U and their conveyor) on a second one, so that the two overlap
   hipStream_t st = default_stream(), sd = aux_stream();
   if (!sd) return set_error("multi: no second stream"), NISE_ERR_HIP;
-  static thread_local MultiCache cache;
+  MultiCache &cache = g_multi;
   // a call that returned early may have left work queued on the buffers
   NISE_HIP_CHECK(hipStreamSynchronize(st));
   NISE_HIP_CHECK(hipStreamSynchronize(sd));
@@ -663,7 +681,13 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
       rw.push_back({rows_for(p->N, p->sch_asap.Kreal, p->sch_asap.nlayers()),
                     (double)p->nvec * p->T * std::max(p->sch_asap.Kreal, 1)});
   int R = 16;
-  if (!rw.empty()) {
+  std::vector<int64_t> sig;
+  if (R_force <= 0 && cache.tune_R > 0) batch_signature(systems, nsys, sig);
+  if (R_force > 0) {
+    R = R_force;
+  } else if (cache.tune_R > 0 && sig == cache.tune_sig) {
+    R = cache.tune_R;
+  } else if (!rw.empty()) {
     std::sort(rw.begin(), rw.end());
     double tot = 0, acc = 0;
     for (auto &e : rw) tot += e.second;
@@ -788,4 +812,49 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
     for (int k = 0; k < nsys; ++k) done_ms[k] = khz > 0 ? (double)(done[k] - t0) / khz : -1.0;
   }
   return NISE_OK;
+}
+
+extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int where, float *re_S,
+                                     float *im_S, double *done_ms) {
+  return multi_run(systems, nsys, where, re_S, im_S, done_ms, 0);
+}
+
+// Times the batch with every row count of the row conveyor (1, 2, 4, 8, 16;
+// the best of two runs each, after a warm run) and keeps the fastest for
+// later nise_absorption_multi calls on a batch of the same shape in this
+// thread.
+extern "C" int nise_absorption_multi_tune(const nise_system *systems, int nsys, int where, int *R_best, float *ms,
+                                          int *codes, int n) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (nsys < 1 || !systems) return set_error("multi tune: bad arguments"), NISE_ERR_ARG;
+  int64_t Tsum = 0;
+  for (int k = 0; k < nsys; ++k) Tsum += systems[k].non ? systems[k].non->tmax1 : 0;
+  std::vector<float> re((size_t)std::max<int64_t>(Tsum, 1)), im(re.size());
+  g_multi.tune_R = 0;
+  if ((rc = multi_run(systems, nsys, where, re.data(), im.data(), nullptr, 0))) return rc;
+  int best = 0, cnt = 0;
+  double best_ms = 0;
+  for (int R : {1, 2, 4, 8, 16}) {
+    double t = -1;
+    for (int rep = 0; rep < 2; ++rep) {
+      const auto t0 = std::chrono::steady_clock::now();
+      const int r = multi_run(systems, nsys, where, re.data(), im.data(), nullptr, R);
+      const double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      if (r) {  // a row count the batch cannot take (LDS): skipped
+        t = -1;
+        break;
+      }
+      t = t < 0 ? dt : std::min(t, dt);
+    }
+    if (t < 0) continue;
+    if (ms && codes && cnt < n) ms[cnt] = (float)t, codes[cnt] = R;
+    ++cnt;
+    if (!best || t < best_ms) best = R, best_ms = t;
+  }
+  if (!best) return set_error("multi tune: no row count ran"), NISE_ERR_ARG;
+  batch_signature(systems, nsys, g_multi.tune_sig);
+  g_multi.tune_R = best;
+  if (R_best) *R_best = best;
+  return cnt;
 }

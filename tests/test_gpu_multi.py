@@ -63,6 +63,21 @@ def test_dia_conveyor_shapes(orc, sr):
         assert e < 1e-5
 
 
+def test_tuned_row_count(orc):
+    """nise_absorption_multi_tune times every row count and keeps the
+    fastest; later calls on the same shape use it and still match the
+    oracle."""
+    specs = [(30, 1, 51), (70, 1, 52), (130, 1, 53), (20, 0, 54)]
+    systems = [system(N, prop, seed, T=32, ns=8, sr=5) for N, prop, seed in specs]
+    best, cands = abi.absorption_multi_tune(systems)
+    say(f"tuned row count {best}: {cands}")
+    assert best in (1, 2, 4, 8, 16) and len(cands) >= 2 and best == min(cands, key=lambda c: c[1])[0]
+    out, _ = abi.absorption_multi(systems)
+    for (non, H, mu, Cr), (re, im) in zip(systems, out):
+        o_re, o_im = orc.absorption(non, H, mu, C_rec=Cr)
+        assert rel(re + 1j * im, o_re + 1j * o_im) < 1e-5
+
+
 def test_single_dia_system_through_batched(orc):
     non, H, mu, Cr = system(20, 0, 11, T=32, ns=9, sr=5)
     import ctypes as C
