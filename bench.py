@@ -19,6 +19,8 @@ GPU kernel on a subset of samples, and the CPU restatement of the reference
 FP32 path timed on this host's cores (baseline only).
 """
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -28,6 +30,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+@contextlib.contextmanager
+def no_gc():
+    """Python's collector paused over a timed region (a collection pause
+    between two library calls is the interpreter's, not the GPU path's)."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
 
 
 def log(msg):
@@ -190,11 +204,12 @@ def multi_leg(args, torch):
     tuned, cands = abi.absorption_multi_tune(systems)
     log(f"multi leg: row counts tuned {[(r, round(ms, 2)) for r, ms in cands]}, kept {tuned}")
     walls = []
-    for _ in range(5):
-        t0 = time.perf_counter()
-        out, done = abi.absorption_multi(systems)
-        torch.cuda.synchronize()
-        walls.append(time.perf_counter() - t0)
+    with no_gc():
+        for _ in range(5):
+            t0 = time.perf_counter()
+            out, done = abi.absorption_multi(systems)
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - t0)
     wall = float(np.median(walls))
     site_steps = float(sum(3 * ns * T * int(N) for N in Ns))
     res = {"workload": f"{len(Ns)} systems, N log-uniform in [8, 1024] (sum {int(Ns.sum())}), "
@@ -560,11 +575,12 @@ def main():
     for _ in range(args.warmup):
         plan.execute(flags=2, fetch=False)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.execute(flags=2, fetch=False)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    with no_gc():
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.execute(flags=2, fetch=False)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     re_g, im_g = plan.execute(flags=0)
     site_steps = ns * 3 * T * N
