@@ -282,10 +282,11 @@ def twodir_leg(args, torch):
     Hd, mud = torch.from_numpy(H).cuda(), torch.from_numpy(mu).cuda()
     abi.twodir_batched(mk(ns), Hd, mud)  # warm: the run's device buffers are kept for the next call
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    R = abi.twodir_batched(mk(ns), Hd, mud)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    with no_gc():
+        t0 = time.perf_counter()
+        R = abi.twodir_batched(mk(ns), Hd, mud)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
     items = ns * 21
     # the reference's propagation work per item (src/calc_2DIR.c:184-506, t1 = t3 = T): singles vector-steps
     # sum_t1 t1 (GB seeds) + T (GB t3) + (T + 1) T (SE) + (T + 1) T (EA right-hand vectors), and doubles
@@ -462,11 +463,12 @@ def chain16_leg(args, torch):
         plan.execute(flags=2, fetch=False)
     torch.cuda.synchronize()
     reps = 50
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        plan.execute(flags=2, fetch=False)
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / reps
+    with no_gc():
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plan.execute(flags=2, fetch=False)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
     plan.execute(flags=1, fetch=False)
     t = plan.timing()
     info = plan.info()
