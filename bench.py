@@ -295,8 +295,28 @@ def twodir_leg(args, torch):
     ref_site_steps = items * (singles * N + (T + 1) * T * (N * (N + 1) // 2))
     res = {"workload": f"N={N} ring, t1=t3={T}, {ns} samples x 21 items, anharmonicity 16",
            "wall_ms": round(wall * 1e3, 3), "items_per_s": items / wall,
-           "reference_site_steps_per_s": ref_site_steps / wall}
+           "reference_equivalent_site_steps_per_s": ref_site_steps / wall,
+           "note": "reference_equivalent_site_steps_per_s counts the reference's per-item propagations (each "
+                   "item repeats vectors another item shares), not work this run does"}
     log(f"2DIR leg: {wall * 1e3:.1f} ms for {items} work items")
+    # 2DIR-Raman on the same system with Raman tensors xx..zz of 0.6 + N(0, 0.3) per site and frame
+    rng = np.random.default_rng(20261202 + 3)
+    al = np.zeros((nfr, 1 + 6 * N), np.float32)
+    al[:, 1:] = (0.6 + 0.3 * rng.normal(size=(nfr, 6 * N))).astype(np.float32)
+    ald = torch.from_numpy(al).cuda()
+    mkr = lambda end: abi.make_non(singles=N, tmax1=T, tmax2=0, tmax3=T, sample=sr, begin=0, end=end,  # noqa: E731
+                                   ts=1, propagation=1, deltat=2.0, min1=lo, max1=hi, technique="2DIRraman",
+                                   anharmonicity=16.0)
+    abi.twodir_raman_batched(mkr(ns), Hd, mud, ald)
+    torch.cuda.synchronize()
+    with no_gc():
+        t0 = time.perf_counter()
+        abi.twodir_raman_batched(mkr(ns), Hd, mud, ald)
+        torch.cuda.synchronize()
+        wr = time.perf_counter() - t0
+    res["raman"] = {"workload": f"2DIRraman, the same system and sizes, {ns} samples x 21 items",
+                    "wall_ms": round(wr * 1e3, 3), "items_per_s": items / wr}
+    log(f"2DIR-Raman: {wr * 1e3:.1f} ms for {items} work items")
     if not args.no_cpu:
         from oracle import oracle
         k = 2
@@ -309,6 +329,14 @@ def twodir_leg(args, torch):
         res["cpu_baseline"] = {"value": k * 21 / dt, "unit": "items/s", "cores": 1, "kind": "port",
                                "sample": f"{k} samples x 21 items, oracle item loop, {dt:.2f} s"}
         log(f"2DIR parity {err:.1e}, oracle {k * 21 / dt:.2f} items/s")
+        t0 = time.perf_counter()
+        refr = oracle.twodir_raman(mkr(k), H, mu, al, acc64=True)
+        dtr = time.perf_counter() - t0
+        Rr = abi.twodir_raman_batched(mkr(k), Hd, mud, ald)
+        res["raman"]["parity_rel_l2_vs_oracle"] = float(np.linalg.norm(Rr - refr) / np.linalg.norm(refr))
+        res["raman"]["cpu_baseline"] = {"value": k * 21 / dtr, "unit": "items/s", "cores": 1, "kind": "port",
+                                        "sample": f"{k} samples x 21 items, oracle item loop, {dtr:.2f} s"}
+        log(f"2DIR-Raman parity {res['raman']['parity_rel_l2_vs_oracle']:.1e}, oracle {k * 21 / dtr:.2f} items/s")
     return res
 
 
