@@ -557,7 +557,8 @@ struct PersistLayout {
 // cb NP + c0 in a wave's vectors; lanes with no rotation point both
 // offsets at a zero state after the N2 states of the wave (rotating zeros
 // leaves zeros), and two such rows end the table.  cs: the frame's angle
-// row.  psi: N2 + 1 states of VW vectors per wave.
+// row.  psi: N2 + 2 states of VW vectors per wave (the zero state, then a
+// spare state that takes writes nobody reads).
 __host__ __device__ inline PersistLayout persist_layout(int N, int VW, int K2, int npw) {
   const size_t N2 = (size_t)N * (N + 1) / 2;
   const int NP = 64 / VW, nb = (N + NP - 1) / NP;
@@ -569,7 +570,7 @@ __host__ __device__ inline PersistLayout persist_layout(int N, int VW, int K2, i
   L.dip = o, o += (size_t)3 * N * 4;
   L.ov = o, o += (size_t)3 * N * 4, o = (o + 15) & ~(size_t)15;
   L.cs = o, o += (size_t)K2 * 16;
-  L.psi = o, o += (size_t)npw * (N2 + 1) * VW * 8;
+  L.psi = o, o += (size_t)npw * (N2 + 2) * VW * 8;
   L.lv = o, o += (size_t)npw * VW * 3 * N * 8;
   L.total = o;
   return L;
@@ -616,6 +617,18 @@ __device__ __forceinline__ void persist_ft(const TwoDirArgs &A, int lane, int s,
   // :457-459, :468-469); one lane per (c, item) then adds the terms in
   // ascending k, the reference's order
   const int x1v[3] = {c_polar[mi.x][1], c_polar[mi.y][1], NIT > 2 ? c_polar[mi.z][1] : 0};
+  // the E entries of this lane's (vector, item) (NIT nv <= 24 lanes), read
+  // before the contraction so that their latency hides behind it
+  const int64_t plane = (int64_t)A.T3 * T1;
+  const bool sum_lane = lane < NIT * nv;
+  float *Ej = nullptr;
+  float e0 = 0.f, e1 = 0.f;
+  if (sum_lane) {
+    const int it = lane % NIT, c = lane / NIT;
+    const int m = it == 0 ? mi.x : it == 1 ? mi.y : mi.z;
+    Ej = A.E + (((int64_t)s * 21 + m) * 4 + 2) * plane + (int64_t)t3 * T1 + t1_0 + c;
+    e0 = gld(Ej), e1 = gld(Ej + plane);
+  }
   for (int u = lane; u < nv * N; u += 64) {
     const int c = u / N, k = u % N;
     float ov[NIT];
@@ -635,18 +648,15 @@ __device__ __forceinline__ void persist_ft(const TwoDirArgs &A, int lane, int s,
     }
   }
   asm volatile("" ::: "memory");
-  const int64_t plane = (int64_t)A.T3 * T1;
-  for (int j = lane; j < NIT * nv; j += 64) {
-    const int it = j % NIT, c = j / NIT, t1 = t1_0 + c;
-    const int m = it == 0 ? mi.x : it == 1 ? mi.y : mi.z;
+  if (sum_lane) {
+    const int it = lane % NIT, c = lane / NIT;
     const float2 *Tm = LvL + (c * NIT + it) * N;
     float rr = 0.f, ri = 0.f;
     for (int k = 0; k < N; ++k) {
       const float2 v = Tm[k];
       rr = __fadd_rn(rr, v.x), ri = __fadd_rn(ri, v.y);
     }
-    float *E = A.E + (((int64_t)s * 21 + m) * 4 + 2) * plane + (int64_t)t3 * T1 + t1;
-    E[0] += rr, E[plane] += ri;
+    gst(Ej, __fadd_rn(e0, rr)), gst(Ej + plane, __fadd_rn(e1, ri));
   }
   asm volatile("" ::: "memory");
 }
@@ -685,30 +695,37 @@ __device__ __forceinline__ void persist_fr(const TwoDirArgs &A, int lane, int s,
 // The rotations of one substep (as dbl_rotate) with the per-lane state
 // offsets from the table ix, each lane's next entry read one rotation ahead.
 // The pair words are wave-uniform and read by scalar loads; the angles come
-// from the staged row csL by broadcast reads.  The lane of c == a also
-// rotates (|ab>, |bb>) (not for two-level sites); the offset of |bb> and the
-// test c == a use the scalar pair word.  Every lane rotates: lanes with no
-// rotation and vector slots past nv hold zeros.
+// from the staged row csL by broadcast reads.  Coupling (a, b) rotates
+// (|ca>, |cb>) for every c, and |ab> twice in the reference's order: with
+// |aa> (c = a), then with |bb> (c = b), both with the sqrt2 angles.  When a
+// and b fall in the same site block, the c = b lane does the second of
+// these in the same round as the c = a lane does the first: it forms the
+// c = a lane's |ab> output itself from the same inputs and operations, and
+// the c = a lane writes its copy to a spare state.  When they fall in
+// different blocks, the c = a lane writes |ab> and the c = b lane's later
+// block rotates it with |bb>.  Two-level sites have no c = a or c = b
+// rotation.  Every lane rotates: lanes with no rotation and vector slots
+// past nv hold zeros.
 template <int NP>
 __device__ __forceinline__ void rot_tab_one(char *base, int N, bool es, uint32_t e, uint32_t p, float4 cv, int cb,
-                                            int c0) {
+                                            int c0, char *spare) {
   constexpr int VW = 64 / NP;
   const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
-  const int o3 = (b + ((b * (2 * N - b - 1)) >> 1)) * (VW * 8);
-  const bool ca = !es && c0 == a - cb * NP;
+  const int c = c0 + cb * NP;
+  const bool ca = !es && c == a, cbb = !es && c == b, same = a / NP == b / NP;
   float2 *a1 = reinterpret_cast<float2 *>(base + (e & 0xffffu));
   float2 *a2 = reinterpret_cast<float2 *>(base + (e >> 16));
-  float2 *a3 = reinterpret_cast<float2 *>(base + o3);
-  const float cl = ca ? cv.z : cv.x, sl = ca ? cv.w : cv.y;
-  float2 p1 = *a1, p2 = *a2, p3;
-  if (ca) p3 = *a3;
+  float2 p1 = *a1, p2 = *a2;
+  if (cbb && same) {  // |ab> after its rotation with |aa>, as the c = a lane forms it
+    const float2 q = *reinterpret_cast<const float2 *>(base + (a + ((a * (2 * N - a - 1)) >> 1)) * (VW * 8));
+    p1 = make_float2(__fsub_rn(__fmul_rn(cv.z, p1.x), __fmul_rn(cv.w, q.y)),
+                     __fadd_rn(__fmul_rn(cv.z, p1.y), __fmul_rn(cv.w, q.x)));
+  }
+  const bool s2 = ca || cbb;
+  const float cl = s2 ? cv.z : cv.x, sl = s2 ? cv.w : cv.y;
   rot2(p1, p2, cl, sl);
   *a1 = p1;
-  if (ca) {
-    rot2(p2, p3, cv.z, cv.w);
-    *a3 = p3;
-  }
-  *a2 = p2;
+  *reinterpret_cast<float2 *>((ca && same) ? spare : reinterpret_cast<char *>(a2)) = p2;
 }
 template <int NP>
 __device__ __forceinline__ void dbl_rotate_tab(float2 *psi, int N, bool es, int K2, const uint32_t *ix,
@@ -716,6 +733,7 @@ __device__ __forceinline__ void dbl_rotate_tab(float2 *psi, int N, bool es, int 
   const int nb = (N + NP - 1) / NP, total = K2 * nb;
   if (total == 0) return;
   char *base = reinterpret_cast<char *>(psi) + q * 8;
+  char *spare = base + (size_t)(N * (N + 1) / 2 + 1) * (64 / NP) * 8;  // after the zero state
   int k = 0, cb = 0;
   uint32_t e0 = ix[c0];
   uint32_t p0 = cld(ab2);
@@ -725,13 +743,13 @@ __device__ __forceinline__ void dbl_rotate_tab(float2 *psi, int N, bool es, int 
     const int cb1 = cb + 1 == nb ? 0 : cb + 1, k1 = cb + 1 == nb ? k + 1 : k;
     const uint32_t e1 = ix[(j + 1) * NP + c0], p1 = cld(ab2 + k1);
     const float4 v1 = csL[k1];
-    rot_tab_one<NP>(base, N, es, e0, p0, v0, cb, c0);
+    rot_tab_one<NP>(base, N, es, e0, p0, v0, cb, c0, spare);
     const int cb2 = cb1 + 1 == nb ? 0 : cb1 + 1, k2 = min(cb1 + 1 == nb ? k1 + 1 : k1, K2 - 1);
     e0 = ix[(j + 2) * NP + c0], p0 = cld(ab2 + k2), v0 = csL[k2];
-    rot_tab_one<NP>(base, N, es, e1, p1, v1, cb1, c0);
+    rot_tab_one<NP>(base, N, es, e1, p1, v1, cb1, c0, spare);
     cb = cb2, k = k2;
   }
-  if (j < total) rot_tab_one<NP>(base, N, es, e0, p0, v0, cb, c0);
+  if (j < total) rot_tab_one<NP>(base, N, es, e0, p0, v0, cb, c0, spare);
   asm volatile("" ::: "memory");
 }
 
@@ -768,7 +786,7 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
   float *dipL = reinterpret_cast<float *>(smem + Ly.dip);
   float *ovL = reinterpret_cast<float *>(smem + Ly.ov);
   float4 *csL = reinterpret_cast<float4 *>(smem + Ly.cs);
-  float2 *psi = reinterpret_cast<float2 *>(smem + Ly.psi) + (size_t)wave * (N2 + 1) * VW;
+  float2 *psi = reinterpret_cast<float2 *>(smem + Ly.psi) + (size_t)wave * (N2 + 2) * VW;
   float2 *LvL = reinterpret_cast<float2 *>(smem + Ly.lv) + (size_t)wave * VW * 3 * N;
   const DblGroup gd = P.groups[(size_t)blockIdx.x * npw + wave];
   const int s = P.groups[(size_t)blockIdx.x * npw].s;  // every wave of the workgroup: one sample
@@ -778,8 +796,8 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
   for (int w = 0; w < VW; ++w)
     for (int j = lane; j < N2; j += 64) psi[j * VW + w] = (w < nv) ? dd_value(A, s, v0 + w, j) : make_float2(0.f, 0.f);
   if (lane < VW) psi[N2 * VW + lane] = make_float2(0.f, 0.f);  // the zero state
-  // the rotation index table: every c except b, ascending; c == a also does
-  // the c == b rotation (|ab>, |bb>); two-level sites skip c == a
+  // the rotation index table: every c, ascending (c == a: |aa>, |ab>; c ==
+  // b: |ab>, |bb>); two-level sites skip c == a and c == b
   {
     const int nb = (N + NP - 1) / NP;
     for (int e = t; e < (K2 * nb + 2) * NP; e += nt) {
@@ -787,7 +805,7 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
       const uint32_t p = k < K2 ? P.ab2[k] : 0u;
       const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
       uint32_t v = (uint32_t)(N2 * VW * 8) * 0x10001u;  // the zero state
-      if (k < K2 && c < N && c != b && !(A.es && c == a)) {
+      if (k < K2 && c < N && !(A.es && (c == a || c == b))) {
         const int rs_c = (c * (2 * N - c - 1)) >> 1;
         const int rs_a = (a * (2 * N - a - 1)) >> 1, rs_b = (b * (2 * N - b - 1)) >> 1;
         const int i1 = c <= a ? a + rs_c : c + rs_a, i2 = c <= b ? b + rs_c : c + rs_b;
@@ -1351,7 +1369,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
       if (score > best * 1.0001) best = score, npw = w;
     }
     const size_t psmem = npw ? persist_layout(N, pvw, K2, npw).total : 0;
-    if (do_ea && prop == 1 && npw && (size_t)(N2 + 1) * pvw * 8 <= 65536) {
+    if (do_ea && prop == 1 && npw && (size_t)(N2 + 2) * pvw * 8 <= 65536) {
       // every doubles vector stays in the LDS of its workgroup for all t3
       // (and is formed there: no doubles vectors in device memory)
       std::vector<DblGroup> gh;
