@@ -552,24 +552,26 @@ struct PersistLayout {
 };
 // Workgroups of npw waves: the waves hold groups of the same sample and
 // share the staged frame tables, dipoles and M rows.
-// ix: per coupling k, site block cb and lane c0 the packed byte offsets
-// (o1 | o2 << 16, o = state index x VW x 8) of the rotation of site c =
-// cb NP + c0 in a wave's vectors; lanes with no rotation point both
-// offsets at a zero state after the N2 states of the wave (rotating zeros
-// leaves zeros), and two such rows end the table.  cs: the frame's angle
-// row.  psi: N2 + 2 states of VW vectors per wave (the zero state, then a
-// spare state that takes writes nobody reads).
+// ix: per coupling k, site block cb and lane c0 the rotation of site c =
+// cb NP + c0 as two words: the byte offsets (o1 | o2 << 16, o = state index
+// x VW x 8) of its two states in a wave's vectors, and (o3 | r << 16): the
+// offset of the state that pre-rotates the first (|aa> on the c = b lane,
+// else a neutral state) and the byte offset r of its coefficient row in
+// cs.  Lanes with no rotation point o1 and o2 at a zero state after the N2
+// states of the wave (rotating zeros leaves zeros); two such rows end the
+// table.  cs: the frame's coefficient rows, three per coupling.  psi: N2 +
+// 2 states of VW vectors per wave (the zero state and the neutral state).
 __host__ __device__ inline PersistLayout persist_layout(int N, int VW, int K2, int npw) {
   const size_t N2 = (size_t)N * (N + 1) / 2;
   const int NP = 64 / VW, nb = (N + NP - 1) / NP;
   PersistLayout L;
   size_t o = 0;
-  L.ix = o, o += ((size_t)K2 * nb + 2) * NP * 4, o = (o + 15) & ~(size_t)15;
+  L.ix = o, o += ((size_t)K2 * nb + 2) * NP * 8, o = (o + 15) & ~(size_t)15;
   L.d2 = o, o += N2 * 8;
   L.rn = o, o += (size_t)3 * N * 8;
   L.dip = o, o += (size_t)3 * N * 4;
   L.ov = o, o += (size_t)3 * N * 4, o = (o + 15) & ~(size_t)15;
-  L.cs = o, o += (size_t)K2 * 16;
+  L.cs = o, o += (size_t)K2 * 48;
   L.psi = o, o += (size_t)npw * (N2 + 2) * VW * 8;
   L.lv = o, o += (size_t)npw * VW * 3 * N * 8;
   L.total = o;
@@ -692,64 +694,49 @@ __device__ __forceinline__ void persist_fr(const TwoDirArgs &A, int lane, int s,
   asm volatile("" ::: "memory");
 }
 
-// The rotations of one substep (as dbl_rotate) with the per-lane state
-// offsets from the table ix, each lane's next entry read one rotation ahead.
-// The pair words are wave-uniform and read by scalar loads; the angles come
-// from the staged row csL by broadcast reads.  Coupling (a, b) rotates
-// (|ca>, |cb>) for every c, and |ab> twice in the reference's order: with
-// |aa> (c = a), then with |bb> (c = b), both with the sqrt2 angles.  When a
-// and b fall in the same site block, the c = b lane does the second of
-// these in the same round as the c = a lane does the first: it forms the
-// c = a lane's |ab> output itself from the same inputs and operations, and
-// the c = a lane writes its copy to a spare state.  When they fall in
-// different blocks, the c = a lane writes |ab> and the c = b lane's later
-// block rotates it with |bb>.  Two-level sites have no c = a or c = b
-// rotation.  Every lane rotates: lanes with no rotation and vector slots
-// past nv hold zeros.
-template <int NP>
-__device__ __forceinline__ void rot_tab_one(char *base, int N, bool es, uint32_t e, uint32_t p, float4 cv, int cb,
-                                            int c0, char *spare) {
-  constexpr int VW = 64 / NP;
-  const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
-  const int c = c0 + cb * NP;
-  const bool ca = !es && c == a, cbb = !es && c == b, same = a / NP == b / NP;
-  float2 *a1 = reinterpret_cast<float2 *>(base + (e & 0xffffu));
-  float2 *a2 = reinterpret_cast<float2 *>(base + (e >> 16));
+// The rotations of one substep (as dbl_rotate), each lane's from its table
+// entries (read one rotation ahead).  Coupling (a, b) rotates (|ca>, |cb>)
+// for every c, and |ab> twice in the reference's order: with |aa> (c = a),
+// then with |bb> (c = b), both with the sqrt2 angles.  When a and b fall in
+// the same site block, the c = b lane does its rotation in the same round
+// as the c = a lane: it first pre-rotates |ab> with |aa> itself, forming the
+// c = a lane's |ab> output from the same inputs and operations, and the
+// c = a lane's copy is overwritten: the rounds store the second state first.
+// Every other lane's pre-rotation is an exact identity (coefficients (1, 0)
+// on a neutral state (-0, +0): x - 0 * 0 and y + 0 * (-0) return x and y,
+// signed zeros included).  In different blocks the c = a lane's block
+// writes |ab> and the c = b lane's later block rotates it with |bb>.
+// Coefficient rows per coupling: (1, 0, co, si) for the plain rotations,
+// (1, 0, co2, si2) for the c = a lane and a c = b lane in another block,
+// (co2, si2, co2, si2) for a c = b lane in the same block.
+__device__ __forceinline__ void rot_entry(char *base, const char *cs, uint2 e) {
+  float2 *a1 = reinterpret_cast<float2 *>(base + (e.x & 0xffffu));
+  float2 *a2 = reinterpret_cast<float2 *>(base + (e.x >> 16));
+  const float2 q = *reinterpret_cast<const float2 *>(base + (e.y & 0xffffu));
+  const float4 cf = *reinterpret_cast<const float4 *>(cs + (e.y >> 16));
   float2 p1 = *a1, p2 = *a2;
-  if (cbb && same) {  // |ab> after its rotation with |aa>, as the c = a lane forms it
-    const float2 q = *reinterpret_cast<const float2 *>(base + (a + ((a * (2 * N - a - 1)) >> 1)) * (VW * 8));
-    p1 = make_float2(__fsub_rn(__fmul_rn(cv.z, p1.x), __fmul_rn(cv.w, q.y)),
-                     __fadd_rn(__fmul_rn(cv.z, p1.y), __fmul_rn(cv.w, q.x)));
-  }
-  const bool s2 = ca || cbb;
-  const float cl = s2 ? cv.z : cv.x, sl = s2 ? cv.w : cv.y;
-  rot2(p1, p2, cl, sl);
+  p1 = make_float2(__fsub_rn(__fmul_rn(cf.x, p1.x), __fmul_rn(cf.y, q.y)),
+                   __fadd_rn(__fmul_rn(cf.x, p1.y), __fmul_rn(cf.y, q.x)));
+  rot2(p1, p2, cf.z, cf.w);
+  *a2 = p2;
   *a1 = p1;
-  *reinterpret_cast<float2 *>((ca && same) ? spare : reinterpret_cast<char *>(a2)) = p2;
 }
 template <int NP>
-__device__ __forceinline__ void dbl_rotate_tab(float2 *psi, int N, bool es, int K2, const uint32_t *ix,
-                                               const uint32_t *ab2, const float4 *csL, int q, int c0) {
-  const int nb = (N + NP - 1) / NP, total = K2 * nb;
+__device__ __forceinline__ void dbl_rotate_tab(float2 *psi, int total, const uint2 *ix, const float4 *csL, int q,
+                                               int c0) {
   if (total == 0) return;
   char *base = reinterpret_cast<char *>(psi) + q * 8;
-  char *spare = base + (size_t)(N * (N + 1) / 2 + 1) * (64 / NP) * 8;  // after the zero state
-  int k = 0, cb = 0;
-  uint32_t e0 = ix[c0];
-  uint32_t p0 = cld(ab2);
-  float4 v0 = csL[0];
+  const char *cs = reinterpret_cast<const char *>(csL);
+  const uint2 *e = ix + c0;
+  uint2 e0 = e[0];
   int j = 0;
   for (; j + 1 < total; j += 2) {
-    const int cb1 = cb + 1 == nb ? 0 : cb + 1, k1 = cb + 1 == nb ? k + 1 : k;
-    const uint32_t e1 = ix[(j + 1) * NP + c0], p1 = cld(ab2 + k1);
-    const float4 v1 = csL[k1];
-    rot_tab_one<NP>(base, N, es, e0, p0, v0, cb, c0, spare);
-    const int cb2 = cb1 + 1 == nb ? 0 : cb1 + 1, k2 = min(cb1 + 1 == nb ? k1 + 1 : k1, K2 - 1);
-    e0 = ix[(j + 2) * NP + c0], p0 = cld(ab2 + k2), v0 = csL[k2];
-    rot_tab_one<NP>(base, N, es, e1, p1, v1, cb1, c0, spare);
-    cb = cb2, k = k2;
+    const uint2 e1 = e[(j + 1) * NP];
+    rot_entry(base, cs, e0);
+    e0 = e[(j + 2) * NP];
+    rot_entry(base, cs, e1);
   }
-  if (j < total) rot_tab_one<NP>(base, N, es, e0, p0, v0, cb, c0, spare);
+  if (j < total) rot_entry(base, cs, e0);
   asm volatile("" ::: "memory");
 }
 
@@ -780,7 +767,7 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
   const int wave = t >> 6, lane = t & 63;
   const int npw = P.npw, nt = blockDim.x;
   const PersistLayout Ly = persist_layout(N, VW, K2, npw);
-  uint32_t *ixL = reinterpret_cast<uint32_t *>(smem + Ly.ix);
+  uint2 *ixL = reinterpret_cast<uint2 *>(smem + Ly.ix);
   float2 *d2 = reinterpret_cast<float2 *>(smem + Ly.d2);
   float2 *RnL = reinterpret_cast<float2 *>(smem + Ly.rn);
   float *dipL = reinterpret_cast<float *>(smem + Ly.dip);
@@ -795,21 +782,28 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
   // and the history (dd_value)
   for (int w = 0; w < VW; ++w)
     for (int j = lane; j < N2; j += 64) psi[j * VW + w] = (w < nv) ? dd_value(A, s, v0 + w, j) : make_float2(0.f, 0.f);
-  if (lane < VW) psi[N2 * VW + lane] = make_float2(0.f, 0.f);  // the zero state
-  // the rotation index table: every c, ascending (c == a: |aa>, |ab>; c ==
-  // b: |ab>, |bb>); two-level sites skip c == a and c == b
+  if (lane < VW) psi[N2 * VW + lane] = make_float2(0.f, 0.f);            // the zero state
+  if (lane < VW) psi[(N2 + 1) * VW + lane] = make_float2(-0.f, 0.f);  // the neutral state
+  // the rotation table: every c, ascending; two-level sites have no c = a
+  // and no c = b rotation
   {
     const int nb = (N + NP - 1) / NP;
+    const uint32_t zs = (uint32_t)(N2 * VW * 8), ns = (uint32_t)((N2 + 1) * VW * 8);
     for (int e = t; e < (K2 * nb + 2) * NP; e += nt) {
       const int k = e / (nb * NP), c = e % (nb * NP);
       const uint32_t p = k < K2 ? P.ab2[k] : 0u;
       const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
-      uint32_t v = (uint32_t)(N2 * VW * 8) * 0x10001u;  // the zero state
+      uint2 v = make_uint2(zs * 0x10001u, ns | ((uint32_t)(k < K2 ? k * 48 : 0) << 16));
       if (k < K2 && c < N && !(A.es && (c == a || c == b))) {
         const int rs_c = (c * (2 * N - c - 1)) >> 1;
         const int rs_a = (a * (2 * N - a - 1)) >> 1, rs_b = (b * (2 * N - b - 1)) >> 1;
         const int i1 = c <= a ? a + rs_c : c + rs_a, i2 = c <= b ? b + rs_c : c + rs_b;
-        v = (uint32_t)(i1 * VW * 8) | ((uint32_t)(i2 * VW * 8) << 16);
+        const bool same = a / NP == b / NP;
+        uint32_t o3 = ns, row = 0;
+        if (!A.es && c == a) row = 1;
+        if (!A.es && c == b) row = same ? 2 : 1, o3 = same ? (uint32_t)((a + rs_a) * VW * 8) : ns;
+        v = make_uint2((uint32_t)(i1 * VW * 8) | ((uint32_t)(i2 * VW * 8) << 16),
+                       o3 | ((uint32_t)(k * 48 + row * 16) << 16));
       }
       ixL[e] = v;
     }
@@ -827,7 +821,7 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
         ++nit;
       }
   }
-  const int q = lane / NP, c0 = lane % NP;
+  const int q = lane % VW, c0 = lane / VW;
   const int64_t tk = tk_of(A, s);
   for (int t3 = 0; t3 < T3; ++t3) {
     const int64_t tl = tk + t3;
@@ -845,11 +839,16 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
     if (t3 + 1 < T3) {  // the doubles step of frame tk + t3 (the state after the last t3 is not used)
       const int64_t row = tk + t3 - P.f_lo;
       stage_lds<8>(d2, P.D2 + row * N2, N2);
-      for (int e = t; e < K2; e += nt) csL[e] = gld(P.cs2 + row * K2 + e);
+      for (int e = t; e < K2; e += nt) {  // the coefficient rows of coupling e
+        const float4 c = gld(P.cs2 + row * K2 + e);
+        csL[3 * e] = make_float4(1.f, 0.f, c.x, c.y);
+        csL[3 * e + 1] = make_float4(1.f, 0.f, c.z, c.w);
+        csL[3 * e + 2] = make_float4(c.z, c.w, c.z, c.w);
+      }
       __syncthreads();
       for (int sub = 0; sub < (nv > 0 ? P.m : 0); ++sub) {
         dbl_diag<VW>(psi, d2, N2, lane);
-        dbl_rotate_tab<NP>(psi, N, A.es != 0, K2, ixL, P.ab2, csL, q, c0);
+        dbl_rotate_tab<NP>(psi, K2 * ((N + NP - 1) / NP), ixL, csL, q, c0);
         dbl_diag<VW>(psi, d2, N2, lane);
       }
     }
@@ -1369,7 +1368,7 @@ extern "C" int nise_2dir_batched(const t_non *non, const float *H_rec, int64_t H
       if (score > best * 1.0001) best = score, npw = w;
     }
     const size_t psmem = npw ? persist_layout(N, pvw, K2, npw).total : 0;
-    if (do_ea && prop == 1 && npw && (size_t)(N2 + 2) * pvw * 8 <= 65536) {
+    if (do_ea && prop == 1 && npw && (size_t)(N2 + 2) * pvw * 8 <= 65536 && (size_t)K2 * 48 <= 65536) {
       // every doubles vector stays in the LDS of its workgroup for all t3
       // (and is formed there: no doubles vectors in device memory)
       std::vector<DblGroup> gh;
