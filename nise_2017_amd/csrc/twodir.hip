@@ -558,22 +558,28 @@ struct PersistLayout {
 // offset of the state that pre-rotates the first (|aa> on the c = b lane,
 // else a neutral state) and the byte offset r of its coefficient row in
 // cs.  Lanes with no rotation point o1 and o2 at a zero state after the N2
-// states of the wave (rotating zeros leaves zeros); two such rows end the
+// states of the wave (rotating zeros leaves zeros); one such row ends the
 // table.  cs: the frame's coefficient rows, three per coupling.  psi: N2 +
 // 2 states of VW vectors per wave (the zero state and the neutral state).
+// The EA phase of a t3 (rn, dip, ov, lv) and its doubles step (d2, cs) use
+// the same bytes: a barrier separates them.
 __host__ __device__ inline PersistLayout persist_layout(int N, int VW, int K2, int npw) {
   const size_t N2 = (size_t)N * (N + 1) / 2;
   const int NP = 64 / VW, nb = (N + NP - 1) / NP;
   PersistLayout L;
   size_t o = 0;
-  L.ix = o, o += ((size_t)K2 * nb + 2) * NP * 8, o = (o + 15) & ~(size_t)15;
-  L.d2 = o, o += N2 * 8;
+  L.ix = o, o += ((size_t)K2 * nb + 1) * NP * 8, o = (o + 15) & ~(size_t)15;
+  const size_t u0 = o;
   L.rn = o, o += (size_t)3 * N * 8;
   L.dip = o, o += (size_t)3 * N * 4;
   L.ov = o, o += (size_t)3 * N * 4, o = (o + 15) & ~(size_t)15;
-  L.cs = o, o += (size_t)K2 * 48;
-  L.psi = o, o += (size_t)npw * (N2 + 2) * VW * 8;
   L.lv = o, o += (size_t)npw * VW * 3 * N * 8;
+  const size_t ea_end = o;
+  o = u0;
+  L.d2 = o, o += N2 * 8, o = (o + 15) & ~(size_t)15;
+  L.cs = o, o += (size_t)K2 * 48;
+  o = (o > ea_end ? o : ea_end), o = (o + 15) & ~(size_t)15;
+  L.psi = o, o += (size_t)npw * (N2 + 2) * VW * 8;
   L.total = o;
   return L;
 }
@@ -789,7 +795,7 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
   {
     const int nb = (N + NP - 1) / NP;
     const uint32_t zs = (uint32_t)(N2 * VW * 8), ns = (uint32_t)((N2 + 1) * VW * 8);
-    for (int e = t; e < (K2 * nb + 2) * NP; e += nt) {
+    for (int e = t; e < (K2 * nb + 1) * NP; e += nt) {
       const int k = e / (nb * NP), c = e % (nb * NP);
       const uint32_t p = k < K2 ? P.ab2[k] : 0u;
       const int a = (int)(p & 0xffffu), b = (int)(p >> 16);
@@ -838,6 +844,7 @@ __global__ void __launch_bounds__(1024) dbl_ea_kernel(PersistArgs P) {
     }
     if (t3 + 1 < T3) {  // the doubles step of frame tk + t3 (the state after the last t3 is not used)
       const int64_t row = tk + t3 - P.f_lo;
+      __syncthreads();  // d2 and cs take the bytes of the EA phase's rows
       stage_lds<8>(d2, P.D2 + row * N2, N2);
       for (int e = t; e < K2; e += nt) {  // the coefficient rows of coupling e
         const float4 c = gld(P.cs2 + row * K2 + e);
