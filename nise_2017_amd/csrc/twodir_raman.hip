@@ -225,26 +225,53 @@ __global__ void raman_d2_kernel(RamanArgs A, int t3) {
 // Diagram 3 (src/calc_2DIRraman.c:359-402): for every (s, item, t3, t1) the
 // sum over sites of alpha_c(tl) t1n(t3), formed as the reference does (leftn3
 // = 0 + alpha t1n, then 1 leftn3i + 0 leftn3r against the ground-state right
-// side)
-__global__ void raman_d3_kernel(RamanArgs A) {
+// side).  Workgroup (s, x0, x1, t3): the items with this (x0, x1) share the
+// t1n rows, which are staged through LDS rc rows at a time by coalesced
+// loads (row pitch N + 1, so that the threads' walks along their rows fall
+// in different banks); thread t1 then sums its row for every such item.
+size_t raman_d3_smem(int N, int rc) { return (((size_t)3 * N * 4 + 15) & ~(size_t)15) + (size_t)rc * (N + 1) * 8; }
+__global__ void raman_d3_kernel(RamanArgs A, int rc) {
+  extern __shared__ __align__(16) unsigned char smem[];
   const int N = A.N, T1 = A.T1, T3 = A.T3;
-  const int s = blockIdx.x / 21, m = blockIdx.x % 21, t3 = blockIdx.y;
-  const int x0 = c_polar[m][0], x1 = c_polar[m][1], c = raman_pol(c_polar[m][2], c_polar[m][3]);
-  const float *al = r_al(A, r_tj(A, s) + t3, c);
-  const int64_t plane = (int64_t)T3 * T1;
+  const int b = blockIdx.x, s = b / 9, x0 = (b / 3) % 3, x1 = b % 3, t3 = blockIdx.y;
+  float *alL = reinterpret_cast<float *>(smem);  // [item][N]
+  float2 *rows = reinterpret_cast<float2 *>(smem + (((size_t)3 * N * 4 + 15) & ~(size_t)15));
+  int ms[3] = {0, 0, 0}, nm = 0;
+  for (int m = 0; m < 21; ++m)
+    if (c_polar[m][0] == x0 && c_polar[m][1] == x1 && nm < 3) ms[nm++] = m;
+  const int64_t tl = r_tj(A, s) + t3, plane = (int64_t)T3 * T1;
+  for (int e = threadIdx.x; e < nm * N; e += blockDim.x) {
+    const int j = e / N;
+    alL[e] = r_al(A, tl, raman_pol(c_polar[ms[j]][2], c_polar[ms[j]][3]))[e % N];
+  }
+  // row t1 of (s, x0, x1) at t3
+  const float2 *src = A.h3 + ((((int64_t)s * 3 + x0) * 3 + x1) * T1 * T3 + t3) * N;
   const float one = 1.f, zero = 0.f;
-  for (int t1 = threadIdx.x; t1 < T1; t1 += blockDim.x) {
-    const float2 *v = A.h3 + (((((int64_t)s * 3 + x0) * 3 + x1) * T1 + t1) * T3 + t3) * N;
-    float rr = 0.f, ri = 0.f;
-    for (int i = 0; i < N; ++i) {
-      const float a = al[i];
-      const float2 p = gld(v + i);
-      const float l3r = __fadd_rn(0.f, __fmul_rn(a, p.x)), l3i = __fadd_rn(0.f, __fmul_rn(a, p.y));
-      rr = __fadd_rn(rr, __fadd_rn(__fmul_rn(one, l3i), __fmul_rn(zero, l3r)));
-      ri = __fadd_rn(ri, __fsub_rn(__fmul_rn(one, l3r), __fmul_rn(zero, l3i)));
+  for (int t1c = 0; t1c < T1; t1c += rc) {
+    const int nr = min(rc, T1 - t1c);
+    __syncthreads();  // the previous chunk's rows are consumed (and the alpha rows written)
+    for (int e = threadIdx.x; e < nr * N; e += blockDim.x) {
+      const int r = e / N, i = e % N;
+      rows[r * (N + 1) + i] = gld(src + ((int64_t)(t1c + r) * T3) * N + i);
     }
-    float *E = A.E + (((int64_t)s * 21 + m) * 6 + 4) * plane + (int64_t)t3 * T1 + t1;
-    E[0] = rr, E[plane] = ri;
+    __syncthreads();
+    for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+      const int t1 = t1c + r;
+      const float2 *v = rows + r * (N + 1);
+      for (int j = 0; j < nm; ++j) {
+        const float *al = alL + j * N;
+        float rr = 0.f, ri = 0.f;
+        for (int i = 0; i < N; ++i) {
+          const float a = al[i];
+          const float2 p = v[i];
+          const float l3r = __fadd_rn(0.f, __fmul_rn(a, p.x)), l3i = __fadd_rn(0.f, __fmul_rn(a, p.y));
+          rr = __fadd_rn(rr, __fadd_rn(__fmul_rn(one, l3i), __fmul_rn(zero, l3r)));
+          ri = __fadd_rn(ri, __fsub_rn(__fmul_rn(one, l3r), __fmul_rn(zero, l3i)));
+        }
+        float *E = A.E + (((int64_t)s * 21 + ms[j]) * 6 + 4) * plane + (int64_t)t3 * T1 + t1;
+        E[0] = rr, E[plane] = ri;
+      }
+    }
   }
 }
 
@@ -607,7 +634,10 @@ extern "C" int nise_2dirraman_batched(const t_non *non, const float *H_rec, int6
       a.slots = B.slots3.as<SlotDesc>(), a.wg_slot_begin = B.wgb3.as<int>(), a.nwg = (int)wb.size() - 1;
       a.state = B.t1n.as<float2>(), a.writeback = 0, a.hist = B.h3.as<float2>();
       if ((rc = launch_conveyor(a, vp, block, st))) return rc;
-      hipLaunchKernelGGL(raman_d3_kernel, dim3(S * 21, T3), dim3(64), 0, st, A);
+      // t1n rows per LDS chunk: 64 at most, within 64 KB (N <= 200 here gives 40 or more)
+      const int rc = (int)std::max<size_t>(1, std::min<size_t>({(size_t)64, (size_t)T1,
+                                                                 (65536 - raman_d3_smem(N, 0)) / ((size_t)(N + 1) * 8)}));
+      hipLaunchKernelGGL(raman_d3_kernel, dim3(S * 9, T3), dim3(64), raman_d3_smem(N, rc), st, A, rc);
       NISE_HIP_CHECK(hipGetLastError());
     }
     // (5) this chunk into the 12 planes
