@@ -87,14 +87,17 @@ int launch_doubles_tables(const FrameSource &src, int N, int K2, const uint32_t 
 }
 
 // LDS bytes of a doubles step: states [N2][vectors per wave], the D2 row
-// when d2_lds, then the coupling chunk.  The D2 row is staged when
-// everything fits.
+// when d2_lds, then the coupling chunk.  The D2 row is staged when a wave
+// holds four or more vectors (N <= 16) and everything fits; with one or two
+// vectors it would take a half or a third of the workgroup's LDS, and reading
+// it through the cache keeps more waves resident (the N = 32 ring's 2DIR-
+// Raman run: 72.2 -> 69.1 ms).
 __host__ __device__ inline size_t doubles_smem(int N, int np, int K2, bool d2_lds) {
   const size_t N2 = (size_t)N * (N + 1) / 2, vw = (size_t)(64 / np) * dbl_vpl(np);
   return ((N2 * (vw + (d2_lds ? 1 : 0)) * 8 + 15) & ~(size_t)15) + (size_t)dbl_chunk(K2) * 20;
 }
 __host__ __device__ inline bool doubles_d2_lds(int N, int np, int K2) {
-  return doubles_smem(N, np, K2, true) <= 160 * 1024;
+  return (64 / np) * dbl_vpl(np) >= 4 && doubles_smem(N, np, K2, true) <= 160 * 1024;
 }
 
 // One frame of m Trotter substeps for groups of vectors.  Group g (one
@@ -102,8 +105,9 @@ __host__ __device__ inline bool doubles_d2_lds(int N, int np, int K2) {
 // (at most 64/NP x VPL of them) and uses table row rows[g].  The states,
 // the frame's D2 row and its coupling rows are staged in LDS first (loads in
 // flight together), so the ordered rotation chain waits on LDS only.  Lane
-// group q = t / NP owns vectors q VPL .. q VPL + VPL - 1, lane c0 = t % NP
-// the rotation of site c.
+// group q = t % (64 / NP) owns vectors q VPL .. q VPL + VPL - 1, and c0 =
+// t / (64 / NP) the rotation of site c: the vectors of one state sit in
+// adjacent lanes, so a round's contiguous states fill the LDS banks.
 template <int NP, int VPL>
 __global__ void __launch_bounds__(64) doubles_step_kernel(DoublesArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -164,7 +168,7 @@ __global__ void __launch_bounds__(64) doubles_step_kernel(DoublesArgs A) {
       if (t + 64 * u < K2) abL[t + 64 * u] = abv[u];
   }
   __syncthreads();
-  const int q = t / NP, c0 = t % NP;
+  const int q = t % (64 / NP), c0 = t / (64 / NP);
   const int rs0 = (c0 * (2 * N - c0 - 1)) >> 1;  // row start of |c0 c0> in the packed basis
   for (int sub = 0; sub < A.m; ++sub) {
     diag();

@@ -36,10 +36,10 @@ __host__ __device__ inline int dbl_chunk(int K2) {
 __host__ __device__ constexpr int dbl_vpl(int) { return 1; }
 // The rotations of couplings [0, nk) (pair words abL, angle rows csL, both
 // in LDS) on the states psi[j * VW + w] of the vectors w < nv, VW = 64 / NP
-// * VPL: lane group q = t / NP owns vectors q VPL .. q VPL + VPL - 1 and
-// lane c0 = t % NP the rotations of sites c = c0, c0 + NP, ...  rs0 is the
-// row start of |c0 c0> in the packed basis.  es: two-level sites, no
-// rotation for c == a or c == b.
+// * VPL: lane group q owns vectors q VPL .. q VPL + VPL - 1 and lane c0 the
+// rotations of sites c = c0, c0 + NP, ... (the caller maps lanes to (q,
+// c0)).  rs0 is the row start of |c0 c0> in the packed basis.  es: two-level
+// sites, no rotation for c == a or c == b.
 template <int NP, int VPL>
 __device__ __forceinline__ void dbl_rotate(float2 *psi, int N, int nv, int nk, const uint32_t *abL,
                                            const float4 *csL, int es, int q, int c0, int rs0) {
