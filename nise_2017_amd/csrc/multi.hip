@@ -174,13 +174,24 @@ __device__ void trig_poly(const double *B, const double *B2, double *X, const Ex
 // pairs = 0: U[m][a][b] row-major; pairs = 1: U[m][b / 2][a] as float4
 // (U_a,b, U_a,b+1), the second half zero past the last column (the DIA
 // conveyor's layout: one 16-byte read gives a site two columns).
-__global__ void __launch_bounds__(256) expm_kernel(FrameSource src, int N, int64_t f_first, float fac, float thres,
-                                                   int pairs, float2 *U) {
-  extern __shared__ __align__(16) double xs[];
+// One system's frames for expm_jobs_kernel; b is set by launch_expm_jobs.
+struct ExpmJob {
+  FrameSource src;
+  int N = 0, pairs = 0;
+  int64_t f_first = 0, nfr = 0, b = 0;
+  float fac = 0.f, thres = 0.f;
+  float2 *U = nullptr;
+};
+__host__ __device__ inline int64_t expm_per(int N, int pairs) {
+  return pairs ? (int64_t)2 * ((N + 1) >> 1) * N : (int64_t)N * N;
+}
+__host__ __device__ inline size_t expm_smem(int N) { return (size_t)4 * N * (N + 1) * 8 + 64; }
+// U of frame f into Um (the frame's block of U), xs: expm_smem(N) bytes.
+__device__ __forceinline__ void expm_frame(const FrameSource &src, int N, int64_t f, float fac, float thres,
+                                           int pairs, float2 *Um, double *xs) {
   const int t = threadIdx.x, L = N + 1, NL = N * L;
   double *A = xs, *B = A + NL, *B2 = B + NL, *X = B2 + NL;
   double *red = X + NL;  // [4]
-  const int64_t m = blockIdx.x, f = f_first + m;
   const ExpmTile g{N, L, t >> 4, t & 15};
   for (int e = t; e < N * N; e += 256) {
     const int a = e / N, b = e - a * N;
@@ -241,7 +252,6 @@ __global__ void __launch_bounds__(256) expm_kernel(FrameSource src, int N, int64
       for (int j = 0; j < 4; ++j) ur[i][j] = rr[i][j] - ii[i][j], ui[i][j] = ri[i][j] + ir[i][j];
   }
   const int Pr = (N + 1) >> 1;
-  float2 *Um = U + m * (pairs ? (int64_t)2 * Pr * N : (int64_t)N * N);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -256,18 +266,82 @@ __global__ void __launch_bounds__(256) expm_kernel(FrameSource src, int N, int64
     for (int a = t; a < N; a += 256) Um[((Pr - 1) * N + a) * 2 + 1] = make_float2(0.f, 0.f);
 }
 
+__global__ void __launch_bounds__(256) expm_kernel(FrameSource src, int N, int64_t f_first, float fac, float thres,
+                                                   int pairs, float2 *U) {
+  extern __shared__ __align__(16) double xs[];
+  const int64_t m = blockIdx.x;
+  expm_frame(src, N, f_first + m, fac, thres, pairs, U + m * expm_per(N, pairs), xs);
+}
+
+// The frames of several systems in one launch: workgroup b0 + blockIdx.x
+// computes frame b - jobs[j].b of job j, the last job with jobs[j].b <= b.
+__global__ void __launch_bounds__(256) expm_jobs_kernel(const ExpmJob *jobs, int njobs, int64_t b0) {
+  extern __shared__ __align__(16) double xs[];
+  const int64_t b = b0 + blockIdx.x;
+  int lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (jobs[mid].b <= b) lo = mid;
+    else hi = mid;
+  }
+  const ExpmJob J = jobs[lo];
+  const int64_t m = b - J.b;
+  expm_frame(J.src, J.N, J.f_first + m, J.fac, J.thres, J.pairs, J.U + m * expm_per(J.N, J.pairs), xs);
+}
+
 int launch_expm_tables(const FrameSource &src, int N, int64_t f_first, int64_t nfr, float fac, float thres,
                        int pairs, float2 *U, hipStream_t st) {
   if (N < 1 || N > kExpmMaxN) return set_error("expm: N=%d outside 1..%d", N, kExpmMaxN), NISE_ERR_ARG;
-  const size_t smem = (size_t)4 * N * (N + 1) * 8 + 64;
+  const size_t smem = expm_smem(N);
   NISE_HIP_CHECK(hipFuncSetAttribute((const void *)expm_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)smem));
   for (int64_t d0 = 0; d0 < nfr; d0 += 65535) {
     const int64_t nd = std::min<int64_t>(65535, nfr - d0);
-    const int64_t per = pairs ? (int64_t)2 * ((N + 1) >> 1) * N : (int64_t)N * N;
+    const int64_t per = expm_per(N, pairs);
     hipLaunchKernelGGL(expm_kernel, dim3((unsigned)nd), dim3(256), smem, st, src, N, f_first + d0, fac, thres,
                        pairs, U + d0 * per);
     NISE_HIP_CHECK(hipGetLastError());
+  }
+  return NISE_OK;
+}
+
+// The jobs' frames in one launch per size class (N <= 16, 32, 48, 64: each
+// launch takes the LDS of its largest N), larger N first within a class.
+// host and dev hold the job table between the upload and the launches.
+int launch_expm_jobs(std::vector<ExpmJob> &jobs, std::vector<ExpmJob> &host, DevBuf &dev, hipStream_t st) {
+  host.clear();
+  std::vector<std::pair<int, int>> cls;  // (class, first job of the class in host)
+  for (int c = 3; c >= 0; --c) {
+    std::vector<ExpmJob> in;
+    for (const ExpmJob &J : jobs) {
+      if (J.N < 1 || J.N > kExpmMaxN) return set_error("expm: N=%d outside 1..%d", J.N, kExpmMaxN), NISE_ERR_ARG;
+      if ((J.N - 1) / 16 == c) in.push_back(J);
+    }
+    if (in.empty()) continue;
+    std::stable_sort(in.begin(), in.end(), [](const ExpmJob &x, const ExpmJob &y) { return x.N > y.N; });
+    cls.push_back({c, (int)host.size()});
+    host.insert(host.end(), in.begin(), in.end());
+  }
+  if (host.empty()) return NISE_OK;
+  cls.push_back({-1, (int)host.size()});
+  for (size_t k = 0; k + 1 < cls.size(); ++k) {  // b: the job's first workgroup within its class
+    int64_t b = 0;
+    for (int j = cls[k].second; j < cls[k + 1].second; ++j) host[j].b = b, b += host[j].nfr;
+  }
+  int rc = dev.alloc(host.size() * sizeof(ExpmJob));
+  if (rc) return rc;
+  NISE_HIP_CHECK(hipMemcpyAsync(dev.p, host.data(), host.size() * sizeof(ExpmJob), hipMemcpyHostToDevice, st));
+  for (size_t k = 0; k + 1 < cls.size(); ++k) {
+    const int j0 = cls[k].second, nj = cls[k + 1].second - j0;
+    const size_t smem = expm_smem(16 * (cls[k].first + 1));
+    NISE_HIP_CHECK(hipFuncSetAttribute((const void *)expm_jobs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)smem));
+    const int64_t nb = host[j0 + nj - 1].b + host[j0 + nj - 1].nfr;
+    for (int64_t d0 = 0; d0 < nb; d0 += 65535) {
+      hipLaunchKernelGGL(expm_jobs_kernel, dim3((unsigned)std::min<int64_t>(65535, nb - d0)), dim3(256), smem, st,
+                         (const ExpmJob *)dev.as<ExpmJob>() + j0, nj, d0);
+      NISE_HIP_CHECK(hipGetLastError());
+    }
   }
   return NISE_OK;
 }
@@ -519,7 +593,8 @@ struct SysState {
 // call).  Every call ends with both streams synchronised.
 struct MultiCache {
   std::vector<std::unique_ptr<SysState>> sys;
-  DevBuf mk, rsys, rwm, dsys, dwm, times, out;
+  DevBuf mk, rsys, rwm, dsys, dwm, times, out, ejobs;
+  std::vector<ExpmJob> ejobs_h;  // the job table of the last upload
   // the row count nise_absorption_multi_tune chose for a batch of this shape
   std::vector<int64_t> tune_sig;
   int tune_R = 0;
@@ -617,12 +692,14 @@ static int multi_run(const nise_system *systems, int nsys, int where, float *re_
     }
   }
   // 2. the DIA systems' U per frame (the matrix exponential of each frame,
-  //    expm_kernel) on the second stream, queued before the host builds the
-  //    Trotter schedules so that the two overlap
+  //    expm_jobs_kernel: one launch per size class for all systems) on the
+  //    second stream, queued before the host builds the Trotter schedules so
+  //    that the two overlap
   std::vector<RowsArgs> rows;
   std::vector<int> rows_of;
   std::vector<DiaSys> dias;
   std::vector<int> dia_of;
+  std::vector<ExpmJob> jobs;
   for (int k = 0; k < nsys; ++k) {
     SysState &s = *S[k];
     const t_non *non = systems[k].non;
@@ -633,7 +710,10 @@ static int multi_run(const nise_system *systems, int nsys, int where, float *re_
       volatile float fv = non->deltat * kIcm2ifs;  // src/propagate.c:69
       fv = fv * kTwoPi;
       const float fac = fv;
-      if ((rc = launch_expm_tables(s.src, N, s.f_lo, nfr, fac, s.thres, 1, s.U.as<float2>(), sd))) return rc;
+      ExpmJob J;
+      J.src = s.src, J.N = N, J.pairs = 1, J.f_first = s.f_lo, J.nfr = nfr, J.fac = fac, J.thres = s.thres;
+      J.U = s.U.as<float2>();
+      jobs.push_back(J);
       DiaSys d;
       d.N = N, d.Np = 1;
       while (d.Np < N) d.Np <<= 1;
@@ -646,6 +726,7 @@ static int multi_run(const nise_system *systems, int nsys, int where, float *re_
       dias.push_back(d);
       dia_of.push_back(k);
   }
+  if ((rc = launch_expm_jobs(jobs, cache.ejobs_h, cache.ejobs, sd))) return rc;
   // the coupling masks of all Trotter systems in one device buffer, one
   // copy back (no synchronisation per system)
   {
