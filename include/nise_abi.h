@@ -285,10 +285,10 @@ int nise_abs_plan_create_onfly(nise_abs_plan **plan, const t_non *non,
 int nise_abs_plan_set_shape(nise_abs_plan *plan, int vp, int samples_per_wg,
                             int block);
 /* Row conveyor (the default): one wave per workgroup holding 64/rows
- * vectors, rows rotations of a layer at a time (1, 2, 4, 8 or 16; 0 picks
- * the count from the coupling structure). */
+ * vectors, rows rotations of a layer at a time (1, 2, 4, 8, 16, 32 or 64; 0
+ * picks the count from the coupling structure). */
 int nise_abs_plan_set_rows(nise_abs_plan *plan, int rows);
-/* Autotuner: times each candidate configuration (row conveyor with 1..16
+/* Autotuner: times each candidate configuration (row conveyor with 1..64
  * rows, slot-conveyor shapes) on the plan's own workload and keeps the
  * fastest.  ms / codes (host, n entries each, optional) receive each timed
  * candidate's conveyor time and code (rows R as R; slot shapes as
@@ -416,15 +416,16 @@ int nise_absorption_multi(const nise_system *systems, int nsys, int where,
                           float *re_S, float *im_S, double *done_ms);
 
 /* Autotuner of the multi-system batch: runs the batch once, then times it
- * with each row count of the row conveyor (1, 2, 4, 8, 16; best of two runs,
- * counts the batch cannot take skipped) and keeps the fastest for later
- * nise_absorption_multi calls of this thread on a batch of the same shape
- * (per system N, tmax, samples, spacing, scheme, Trotter steps); without it
- * the count is the work-weighted median of the systems' own.  R_best
- * receives the count kept; ms / codes (n entries, optional) each count's
- * host wall time per call and the count.  Returns the number of counts
- * timed, or minus an error code.  No counterpart in the reference (its
- * static work split, src/MPI_subs.c:36-110). */
+ * with each system's own row count (code 0: the power of two nearest its
+ * mean layer width, up to 64) and with every common row count of the row
+ * conveyor (1, 2, 4, ..., 64; best of two runs, counts the batch cannot take
+ * skipped) and keeps the fastest for later nise_absorption_multi calls of
+ * this thread on a batch of the same shape (per system N, tmax, samples,
+ * spacing, scheme, Trotter steps); without it each system takes its own
+ * count.  R_best receives the code kept; ms / codes (n entries, optional)
+ * each candidate's host wall time per call and its code.  Returns the number
+ * of candidates timed, or minus an error code.  No counterpart in the
+ * reference (its static work split, src/MPI_subs.c:36-110). */
 int nise_absorption_multi_tune(const nise_system *systems, int nsys, int where, int *R_best,
                                float *ms, int *codes, int n);
 

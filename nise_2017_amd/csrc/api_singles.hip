@@ -83,7 +83,7 @@ bool is_coupling_type(const t_non *non) { return std::strcmp(non->hamiltonian, "
 // copy and workgroup map; refreshed when checkpoints are switched on).
 int upload_rows_desc(nise_abs_plan *P) {
   RowsArgs r;
-  r.N = P->N, r.tmax = P->tmax, r.m = P->m, r.nlayers = P->sch.nlayers(), r.K = P->sch.K;
+  r.N = P->N, r.tmax = P->tmax, r.m = P->m, r.nlayers = P->sch.nlayers(), r.K = P->sch.K, r.rows = P->rows;
   r.f_lo = P->f_lo, r.nvec = 3 * (P->end - P->begin), r.begin = P->begin, r.sample = P->sample;
   r.steps = P->row_steps.as<uint2>(), r.cs = P->cs.as<float2>(), r.diag = P->diag.as<float2>();
   r.mu_rec = P->mu_dev, r.mu_stride = P->mu_stride, r.partial = P->partial.as<double2>();
@@ -388,8 +388,8 @@ extern "C" int nise_abs_plan_set_shape(nise_abs_plan *P, int vp, int samples_per
 extern "C" int nise_abs_plan_set_rows(nise_abs_plan *P, int R) {
   if (!P || P->kernel != 0) return set_error("no conveyor plan"), NISE_ERR_ARG;
   if (R == 0) R = rows_for(P->N, P->sch_asap.Kreal, P->sch_asap.nlayers());
-  if (!(R == 1 || R == 2 || R == 4 || R == 8 || R == 16))
-    return set_error("rows must be 1, 2, 4, 8 or 16, got %d", R), NISE_ERR_ARG;
+  if (!(R == 1 || R == 2 || R == 4 || R == 8 || R == 16 || R == 32 || R == 64))
+    return set_error("rows must be 1, 2, 4, 8, 16, 32 or 64, got %d", R), NISE_ERR_ARG;
   if (rows_smem(P->N, R) > 160 * 1024)
     return set_error("N=%d does not fit the row conveyor with %d rows", P->N, R), NISE_ERR_ARG;
   if (R != P->rows || P->sch_rows.ab.empty()) build_schedule_rows(P->N, P->mask, R, P->sch_rows);
@@ -407,7 +407,7 @@ extern "C" int nise_abs_plan_execute(nise_abs_plan *P, float *re_S, float *im_S,
 
 // Autotuner: times every candidate conveyor configuration on the plan's own
 // workload (two event-timed runs each, the faster counts) and keeps the
-// fastest.  Candidates: the row conveyor with 1..16 rows (those whose LDS
+// fastest.  Candidates: the row conveyor with 1..64 rows (those whose LDS
 // image fits) and slot-conveyor shapes.  ms[i] (optional, n entries)
 // receives the candidates' times in the order of codes[i]: rows R as R,
 // slot shapes as -(1000*vp + 10*spw) - ... see nise_abs_plan_info for the
@@ -416,7 +416,7 @@ extern "C" int nise_abs_plan_autotune(nise_abs_plan *P, float *ms, int *codes, i
   if (!P || P->kernel != 0) return set_error("no conveyor plan"), -NISE_ERR_ARG;
   struct Cand { int rows, vp, spw, block; };
   std::vector<Cand> cands;
-  for (int R : {1, 2, 4, 8, 16})
+  for (int R : {1, 2, 4, 8, 16, 32, 64})
     if (rows_smem(P->N, R) <= 160 * 1024) cands.push_back({R, 0, 0, 0});
   const Cand slot[] = {{0, 8, 2, 256}, {0, 16, 5, 256}, {0, 8, 2, 64}};
   for (const Cand &c : slot) cands.push_back(c);

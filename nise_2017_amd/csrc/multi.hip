@@ -597,7 +597,7 @@ struct MultiCache {
   std::vector<ExpmJob> ejobs_h;  // the job table of the last upload
   // the row count nise_absorption_multi_tune chose for a batch of this shape
   std::vector<int64_t> tune_sig;
-  int tune_R = 0;
+  int tune_R = 0;  // 0: none; > 0 that count for every system; -1 each its own
 };
 thread_local MultiCache g_multi;
 
@@ -616,9 +616,9 @@ void batch_signature(const nise_system *systems, int nsys, std::vector<int64_t> 
 
 using namespace nise;
 
-// One batch; R_force > 0 runs the Trotter systems with that row count, else
-// the tuned count of a batch of this shape, else the work-weighted median of
-// the systems' own counts.
+// One batch; R_force > 0 runs every Trotter system with that row count,
+// R_force < 0 each with its own; R_force = 0 takes the tuned choice for a
+// batch of this shape, else each system's own count.
 static int multi_run(const nise_system *systems, int nsys, int where, float *re_S, float *im_S, double *done_ms,
                      int R_force) {
   int rc = ensure_device();
@@ -755,28 +755,16 @@ static int multi_run(const nise_system *systems, int nsys, int where, float *re_
       s.asap_mask = s.mask, s.asap_N = s.N;
     }
   }
-  // 3. one row count for the Trotter launch: the work-weighted median
-  std::vector<std::pair<int, double>> rw;
-  for (int k = 0; k < nsys; ++k)
-    if (const SysState *p = S[k].get(); !p->dia)
-      rw.push_back({rows_for(p->N, p->sch_asap.Kreal, p->sch_asap.nlayers()),
-                    (double)p->nvec * p->T * std::max(p->sch_asap.Kreal, 1)});
-  int R = 16;
+  // 3. row counts of the Trotter launch
+  // R > 0: every Trotter system with R rows; 0: each system its own count,
+  // the power of two nearest its mean layer width up to 64 (systems with
+  // many couplings per layer take wide layers, which shortens the chain of
+  // layers each of their workgroups walks)
+  int R = 0;
   std::vector<int64_t> sig;
-  if (R_force <= 0 && cache.tune_R > 0) batch_signature(systems, nsys, sig);
-  if (R_force > 0) {
-    R = R_force;
-  } else if (cache.tune_R > 0 && sig == cache.tune_sig) {
-    R = cache.tune_R;
-  } else if (!rw.empty()) {
-    std::sort(rw.begin(), rw.end());
-    double tot = 0, acc = 0;
-    for (auto &e : rw) tot += e.second;
-    for (auto &e : rw) {
-      acc += e.second;
-      if (acc >= 0.5 * tot) { R = e.first; break; }
-    }
-  }
+  if (R_force == 0 && cache.tune_R != 0) batch_signature(systems, nsys, sig);
+  if (R_force > 0) R = R_force;
+  else if (R_force == 0 && cache.tune_R > 0 && sig == cache.tune_sig) R = cache.tune_R;
   // 4. tables of the Trotter systems
   for (int k = 0; k < nsys; ++k) {
     SysState &s = *S[k];
@@ -784,17 +772,18 @@ static int multi_run(const nise_system *systems, int nsys, int where, float *re_
     const int N = s.N;
     const int64_t nfr = s.f_hi - s.f_lo;
     if (s.dia) continue;
-      if (s.rows_N != N || s.rows_R != R || s.rows_mask != s.mask) {
+      const int Rk = R > 0 ? R : rows_for(N, s.sch_asap.Kreal, s.sch_asap.nlayers(), 64);
+      if (s.rows_N != N || s.rows_R != Rk || s.rows_mask != s.mask) {
         s.rows_N = 0;  // until the tables below are on the device
-        build_schedule_rows(N, s.mask, R, s.sch);
+        build_schedule_rows(N, s.mask, Rk, s.sch);
         std::vector<uint2> tab;
-        rows_step_table(s.sch, R, tab);
+        rows_step_table(s.sch, Rk, tab);
         std::vector<uint32_t> abd;  // the frame tables follow the step table's order
-        rows_device_order(s.sch, R, abd);
+        rows_device_order(s.sch, Rk, abd);
         if ((rc = up(s.ab, abd.data(), abd.size() * 4)) ||
             (rc = up(s.steps, tab.data(), tab.size() * sizeof(uint2))))
           return rc;
-        s.rows_mask = s.mask, s.rows_R = R, s.rows_N = N;
+        s.rows_mask = s.mask, s.rows_R = Rk, s.rows_N = N;
       }
       if ((rc = s.diag.alloc((size_t)nfr * N * 8)) || (rc = s.cs.alloc((size_t)nfr * s.sch.K * 8))) return rc;
       if ((rc = launch_frame_tables(s.src, N, s.sch, s.ab.as<uint32_t>(), trotter_factor(non->deltat, 1, non->ts),
@@ -804,6 +793,7 @@ static int multi_run(const nise_system *systems, int nsys, int where, float *re_
       if ((rc = s.partial.alloc((size_t)s.nwg * s.T * sizeof(double2)))) return rc;
       RowsArgs r;
       r.N = N, r.tmax = s.T, r.m = non->ts, r.nlayers = s.sch.nlayers(), r.K = s.sch.K, r.f_lo = s.f_lo;
+      r.rows = Rk;
       r.nvec = s.nvec, r.begin = non->begin, r.sample = non->sample;
       r.steps = s.steps.as<uint2>(), r.cs = s.cs.as<float2>(), r.diag = s.diag.as<float2>();
       r.mu_rec = s.mu_dev, r.mu_stride = s.mu_stride, r.partial = s.partial.as<double2>();
@@ -900,7 +890,8 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
   return multi_run(systems, nsys, where, re_S, im_S, done_ms, 0);
 }
 
-// Times the batch with every row count of the row conveyor (1, 2, 4, 8, 16;
+// Times the batch with each system's own row count and with every common
+// row count of the row conveyor (1, 2, ..., 64;
 // the best of two runs each, after a warm run) and keeps the fastest for
 // later nise_absorption_multi calls on a batch of the same shape in this
 // thread.
@@ -914,13 +905,13 @@ extern "C" int nise_absorption_multi_tune(const nise_system *systems, int nsys, 
   std::vector<float> re((size_t)std::max<int64_t>(Tsum, 1)), im(re.size());
   g_multi.tune_R = 0;
   if ((rc = multi_run(systems, nsys, where, re.data(), im.data(), nullptr, 0))) return rc;
-  int best = 0, cnt = 0;
+  int best = -2, cnt = 0;
   double best_ms = 0;
-  for (int R : {1, 2, 4, 8, 16}) {
+  for (int R : {0, 1, 2, 4, 8, 16, 32, 64}) {  // 0: each system its own count
     double t = -1;
     for (int rep = 0; rep < 2; ++rep) {
       const auto t0 = std::chrono::steady_clock::now();
-      const int r = multi_run(systems, nsys, where, re.data(), im.data(), nullptr, R);
+      const int r = multi_run(systems, nsys, where, re.data(), im.data(), nullptr, R ? R : -1);
       const double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
       if (r) {  // a row count the batch cannot take (LDS): skipped
         t = -1;
@@ -931,11 +922,11 @@ extern "C" int nise_absorption_multi_tune(const nise_system *systems, int nsys, 
     if (t < 0) continue;
     if (ms && codes && cnt < n) ms[cnt] = (float)t, codes[cnt] = R;
     ++cnt;
-    if (!best || t < best_ms) best = R, best_ms = t;
+    if (best == -2 || t < best_ms) best = R, best_ms = t;
   }
-  if (!best) return set_error("multi tune: no row count ran"), NISE_ERR_ARG;
+  if (best == -2) return set_error("multi tune: no row count ran"), NISE_ERR_ARG;
   batch_signature(systems, nsys, g_multi.tune_sig);
-  g_multi.tune_R = best;
+  g_multi.tune_R = best ? best : -1;
   if (R_best) *R_best = best;
   return cnt;
 }

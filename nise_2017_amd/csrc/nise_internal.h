@@ -402,6 +402,7 @@ int doubles_vw(int N);  // vectors per doubles group (one wave)
 // sample.  partial holds one row of tmax per vector of the system.
 struct RowsArgs {
   int N = 0, tmax = 0, m = 1, nlayers = 0;  // nlayers: multiple of 8
+  int rows = 0;                  // this system's row count (launches with R = 0)
   int64_t K = 0;                 // entries per sweep (= nlayers * R)
   int64_t f_lo = 0;
   int nvec = 0, begin = 0, sample = 1;
@@ -485,7 +486,7 @@ int launch_sparse_doubles(const SparseDblArgs &a, int ngroups, hipStream_t st);
 int launch_dia_tables(const FrameSource &src, int N, int64_t f_first, int64_t nfr, float fac, float thres,
                       float2 *U, DevBuf &A, DevBuf &V, DevBuf &w, int *status, hipStream_t st);
 size_t rows_smem(int N, int R);
-int rows_for(int N, int Kreal, int nlayers_asap);
+int rows_for(int N, int Kreal, int nlayers_asap, int cap = 16);
 void rows_device_order(const Schedule &s, int R, std::vector<uint32_t> &ab);
 void rows_step_table(const Schedule &s, int R, std::vector<uint2> &tab);
 

@@ -185,7 +185,7 @@ extern "C" int nise_absorption_batched_noalloc(const t_non *non, const float *H_
     return rc;
 
   RowsArgs r;
-  r.N = N, r.tmax = T, r.m = m, r.nlayers = sch.nlayers(), r.K = sch.K;
+  r.N = N, r.tmax = T, r.m = m, r.nlayers = sch.nlayers(), r.K = sch.K, r.rows = R;
   r.f_lo = f_lo, r.nvec = nvec, r.begin = begin, r.sample = sample;
   r.steps = steps, r.cs = cs, r.diag = diag;
   r.mu_rec = mu_rec, r.mu_stride = ctype ? 0 : mu_stride, r.partial = partial;

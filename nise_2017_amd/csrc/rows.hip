@@ -11,15 +11,17 @@
 // vector is propagated with the reference's FP32 operations in the
 // reference's order.
 //
-// Each quad of vectors (g / 4) runs frame-synchronously: its 4 vectors see
-// the same frame f at the same time, vector g being at relative time
-// f - fs_g (its sample's start frame fs_g) and active while that lies in
-// [0, tmax); quads run time-skewed against each other.  A frame's rotation
-// table is therefore the same for the 4 lanes of a row within a quad, so
-// each lane loads one entry of a block of 4 layers (lane & 3 picks the
-// layer) and the quad broadcasts it (DPP quad_perm, folded into the
-// arithmetic that uses it): one 8-byte load per lane covers 4 layers.  With
-// R = 16 (4 vectors per wave) the table address is wave-uniform.  An
+// Each group of B = min(G, 4) consecutive vectors (g / B) runs frame-
+// synchronously: its vectors see the same frame f at the same time, vector
+// g being at relative time f - fs_g (its sample's start frame fs_g) and
+// active while that lies in [0, tmax); groups run time-skewed against each
+// other.  A frame's rotation table is therefore the same for the B lanes of
+// a row within a group, so each lane loads one entry of a block of B layers
+// (lane % B picks the layer) and the group broadcasts it (DPP quad_perm,
+// folded into the arithmetic that uses it): one 8-byte load per lane covers
+// B layers.  With R = 16 (4 vectors per wave) the table address is
+// wave-uniform; R = 32 and 64 (2 and 1 vectors per wave) shorten each
+// wave's chain of layers for systems with many couplings per layer.  An
 // 8-layer register ring prefetches the entries.
 //
 // Per step: the per-site pass applies the pending half-phase of frame
@@ -37,8 +39,7 @@
 
 namespace nise {
 
-constexpr int kRingG = 2;           // ring slots per lane, 4 layers each
-constexpr int kRingL = 4 * kRingG;  // layers in flight (nlayers is a multiple)
+constexpr int kRingL = 8;  // layers in flight (nlayers is a multiple)
 constexpr int kSiteU = 4;           // sites per batch of HBM loads in the per-site pass
 
 // One rotation (src/propagate.c:883-887) of the pair at LDS byte offsets
@@ -52,27 +53,47 @@ __device__ __forceinline__ void rot_step(unsigned char *smem, uint32_t gofs, uin
   *pb_ = pb;
 }
 
-// Layer j of a ring slot: the entry loaded by quad lane j.
-template <int j>
-__device__ __forceinline__ void rot_slot(unsigned char *smem, uint32_t gofs, uint2 e, float2 c) {
-  rot_step(smem, gofs, make_uint2(quad_bcast<j>(e.x), quad_bcast<j>(e.y)),
-           make_float2(quad_bcast<j>(c.x), quad_bcast<j>(c.y)));
+// The value of lane j of each group of B lanes (B = 4: quad_bcast; 2: lanes
+// 0, 1 of the group from lane j and lanes 2, 3 from lane 2 + j; 1: the lane's
+// own value).
+template <int B, int j>
+__device__ __forceinline__ uint32_t grp_bcast(uint32_t x) {
+  if constexpr (B == 4) return quad_bcast<j>(x);
+  else if constexpr (B == 2)
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, j | (j << 2) | ((2 + j) << 4) | ((2 + j) << 6), 0xF, 0xF,
+                                                 false);
+  else return x;
+}
+template <int B, int j>
+__device__ __forceinline__ float grp_bcast(float x) {
+  return __uint_as_float(grp_bcast<B, j>(__float_as_uint(x)));
 }
 
-// Workgroup w runs the vectors wmap[w].y .. wmap[w].y + G - 1 of system
-// wmap[w].x (systems of different sizes share one launch).
+// Layer j of a ring slot: the entry loaded by group lane j.
+template <int B, int j>
+__device__ __forceinline__ void rot_slot(unsigned char *smem, uint32_t gofs, uint2 e, float2 c) {
+  rot_step(smem, gofs, make_uint2(grp_bcast<B, j>(e.x), grp_bcast<B, j>(e.y)),
+           make_float2(grp_bcast<B, j>(c.x), grp_bcast<B, j>(c.y)));
+}
+// The B layers of a ring slot.
+template <int B>
+__device__ __forceinline__ void rot_block(unsigned char *smem, uint32_t gofs, uint2 e, float2 c) {
+  rot_slot<B, 0>(smem, gofs, e, c);
+  if constexpr (B > 1) rot_slot<B, 1>(smem, gofs, e, c);
+  if constexpr (B > 2) {
+    rot_slot<B, 2>(smem, gofs, e, c);
+    rot_slot<B, 3>(smem, gofs, e, c);
+  }
+}
+
+// The workgroup's vectors wm.y .. wm.y + G - 1 of system A (wm.x).
 template <int R>
-__global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ sys, const int2 *__restrict__ wmap,
-                                                  unsigned long long *times) {
+__device__ __forceinline__ void rows_run(const RowsArgs &A, int2 wm) {
   constexpr int G = 64 / R;
-  static_assert(G >= 4, "a row needs a whole quad of lanes");
+  constexpr int B = G < 4 ? G : 4;  // vectors per frame group, layers per table block
+  constexpr int kRingS = kRingL / B;  // ring slots per lane
+  static_assert(G == 1 || G == 2 || G % 4 == 0, "vectors per wave: 1, 2 or a multiple of 4");
   extern __shared__ __align__(16) unsigned char smem[];
-  const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
-  // wave-uniform: the descriptor is read with scalar loads and kept in
-  // registers (the kernel's own stores cannot alias it)
-  const int2 wm0 = wmap[blockIdx.x];
-  const int2 wm = make_int2(__builtin_amdgcn_readfirstlane(wm0.x), __builtin_amdgcn_readfirstlane(wm0.y));
-  const RowsArgs A = sys[wm.x];
   const int N = A.N, T = A.tmax, nl = A.nlayers, m = A.m;
   const int64_t K = A.K;
   // lane = q * G + g: the G vectors of a row are adjacent, so a row's LDS
@@ -84,27 +105,27 @@ __global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ s
   const int64_t fs = has ? (int64_t)(A.begin + v / 3) * A.sample : LLONG_MAX / 4;
   const uint32_t gofs = (uint32_t)g * 8u;
   const int64_t hist0 = (A.hist && has) ? (int64_t)v * A.nck : -1;
-  // Vector quad h = g / 4 (4 consecutive vectors) shares a frame: at wave
+  // Vector group h = g / B (B consecutive vectors) shares a frame: at wave
   // time tau it is at frame F0(h) + tau, F0(h) being its first vector's
-  // start; quads run time-skewed, so the wave runs for the longest quad span
-  // (its last vector's start - F0(h) + tmax).
+  // start; groups run time-skewed, so the wave runs for the longest group
+  // span (its last vector's start - F0(h) + tmax).
   const int vlast = min(wm.y + G, A.nvec) - 1;
   auto start_of = [&](int vec) { return (int64_t)(A.begin + vec / 3) * A.sample; };
   int64_t span = 0;
-  for (int h = 0; 4 * h < G && wm.y + 4 * h <= vlast; ++h)
-    span = max(span, start_of(min(wm.y + 4 * h + 3, vlast)) - start_of(wm.y + 4 * h) + T);
-  const int vq = min(wm.y + (g & ~3), vlast);  // this lane's quad's first vector
+  for (int h = 0; B * h < G && wm.y + B * h <= vlast; ++h)
+    span = max(span, start_of(min(wm.y + B * h + B - 1, vlast)) - start_of(wm.y + B * h) + T);
+  const int vq = min(wm.y + (g & ~(B - 1)), vlast);  // this lane's group's first vector
   const int64_t F0 = start_of(vq);
-  const int64_t Fend = start_of(min(vq + 3, vlast)) + T - 1;  // the quad's last frame
+  const int64_t Fend = start_of(min(vq + B - 1, vlast)) + T - 1;  // the group's last frame
 
   for (int i = lane; i < (N + R) * G; i += 64) reinterpret_cast<float2 *>(smem)[i] = make_float2(0.f, 0.f);
 
-  // ring: slot k holds, in quad lane j, entry q of layer 4k + j of the
-  // block in flight; loff is the lane's entry within a block of 4 layers
+  // ring: slot k holds, in group lane j, entry q of layer B k + j of the
+  // block in flight; loff is the lane's entry within a block of B layers
   // (device order, rows_device_order)
-  const uint32_t loff = (uint32_t)(4 * q + (lane & 3));
-  uint2 re[kRingG];
-  float2 rc[kRingG];
+  const uint32_t loff = (uint32_t)(B * q + (lane & (B - 1)));
+  uint2 re[kRingS];
+  float2 rc[kRingS];
   int64_t ring_key = -1;  // sweep (tau * m + sub) whose first block the ring holds
 
   for (int64_t tau = 0; tau < span; ++tau) {
@@ -114,14 +135,14 @@ __global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ s
     if (!__ballot(act)) {  // no vector of the wave is at its frame: jump
       int64_t nt = span;
       for (int k = 0; k < G && wm.y + k <= vlast; ++k) {
-        const int64_t o = start_of(wm.y + k) - start_of(wm.y + (k & ~3));
+        const int64_t o = start_of(wm.y + k) - start_of(wm.y + (k & ~(B - 1)));
         if (o > tau) nt = min(nt, o);
       }
       tau = nt - 1;
       continue;
     }
-    // table row of the frame, clamped to the quad's frames (a finished
-    // quad keeps loading valid rows)
+    // table row of the frame, clamped to the group's frames (a finished
+    // group keeps loading valid rows)
     const int64_t row = min(f, Fend) - A.f_lo;
     const bool last = t == T - 1;
     float rr = 0.f, ri = 0.f;
@@ -197,9 +218,9 @@ __global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ s
       const float2 *csf = A.cs + row * K + loff;
       if (ring_key != tau * m + sub) {  // the previous sweep prefetched another one
 #pragma unroll
-        for (int k = 0; k < kRingG; ++k) {
-          re[k] = gld(A.steps + loff + 4 * k * R);
-          rc[k] = gld(csf + 4 * k * R);
+        for (int k = 0; k < kRingS; ++k) {
+          re[k] = gld(A.steps + loff + B * k * R);
+          rc[k] = gld(csf + B * k * R);
         }
       }
       // Each ring slot is consumed and refilled with the entries kRingL
@@ -210,31 +231,60 @@ __global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ s
       const float2 *ca = csf + kRingL * R;
       for (int b0 = 0; b0 < nl - kRingL; b0 += kRingL, sa += kRingL * R, ca += kRingL * R) {
 #pragma unroll
-        for (int k = 0; k < kRingG; ++k) {
-          rot_slot<0>(smem, gofs, re[k], rc[k]);
-          rot_slot<1>(smem, gofs, re[k], rc[k]);
-          rot_slot<2>(smem, gofs, re[k], rc[k]);
-          rot_slot<3>(smem, gofs, re[k], rc[k]);
-          re[k] = gld(sa + 4 * k * R);
-          rc[k] = gld(ca + 4 * k * R);
+        for (int k = 0; k < kRingS; ++k) {
+          rot_block<B>(smem, gofs, re[k], rc[k]);
+          re[k] = gld(sa + B * k * R);
+          rc[k] = gld(ca + B * k * R);
           __builtin_amdgcn_sched_barrier(0);  // the refill stays here
         }
       }
       const float2 *cn = A.cs + (sub + 1 < m ? row : min(f + 1, Fend) - A.f_lo) * K + loff;
 #pragma unroll
-      for (int k = 0; k < kRingG; ++k) {
-        rot_slot<0>(smem, gofs, re[k], rc[k]);
-        rot_slot<1>(smem, gofs, re[k], rc[k]);
-        rot_slot<2>(smem, gofs, re[k], rc[k]);
-        rot_slot<3>(smem, gofs, re[k], rc[k]);
-        re[k] = gld(A.steps + loff + 4 * k * R);
-        rc[k] = gld(cn + 4 * k * R);
+      for (int k = 0; k < kRingS; ++k) {
+        rot_block<B>(smem, gofs, re[k], rc[k]);
+        re[k] = gld(A.steps + loff + B * k * R);
+        rc[k] = gld(cn + B * k * R);
         __builtin_amdgcn_sched_barrier(0);
       }
       ring_key = tau * m + sub + 1;
     }
   }
-  if (lane == 0 && times) {  // device wall clock at start and end
+}
+
+// Workgroup w runs the vectors wmap[w].y .. wmap[w].y + G - 1 of system
+// wmap[w].x (systems of different sizes share one launch), all systems with
+// R rows; rows_mixed_kernel takes each system's own count (sys[k].rows).
+// The descriptor is wave-uniform: it is read with scalar loads and kept in
+// registers (the kernel's own stores cannot alias it).
+template <int R>
+__global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ sys, const int2 *__restrict__ wmap,
+                                                  unsigned long long *times) {
+  const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
+  const int2 wm0 = wmap[blockIdx.x];
+  const int2 wm = make_int2(__builtin_amdgcn_readfirstlane(wm0.x), __builtin_amdgcn_readfirstlane(wm0.y));
+  const RowsArgs A = sys[wm.x];
+  rows_run<R>(A, wm);
+  if (threadIdx.x == 0 && times) {  // device wall clock at start and end
+    times[2 * blockIdx.x] = t_start;
+    times[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
+  }
+}
+__global__ void __launch_bounds__(64) rows_mixed_kernel(const RowsArgs *__restrict__ sys,
+                                                        const int2 *__restrict__ wmap, unsigned long long *times) {
+  const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
+  const int2 wm0 = wmap[blockIdx.x];
+  const int2 wm = make_int2(__builtin_amdgcn_readfirstlane(wm0.x), __builtin_amdgcn_readfirstlane(wm0.y));
+  const RowsArgs A = sys[wm.x];
+  switch (A.rows) {
+    case 1: rows_run<1>(A, wm); break;
+    case 2: rows_run<2>(A, wm); break;
+    case 4: rows_run<4>(A, wm); break;
+    case 8: rows_run<8>(A, wm); break;
+    case 16: rows_run<16>(A, wm); break;
+    case 32: rows_run<32>(A, wm); break;
+    default: rows_run<64>(A, wm); break;  // rows_prepare admits 1, 2, ..., 64 only
+  }
+  if (threadIdx.x == 0 && times) {
     times[2 * blockIdx.x] = t_start;
     times[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
   }
@@ -247,32 +297,38 @@ static const void *rows_fn(int R) {
        : R == 2 ? (const void *)rows_kernel<2>
        : R == 4 ? (const void *)rows_kernel<4>
        : R == 8 ? (const void *)rows_kernel<8>
-       : R == 16 ? (const void *)rows_kernel<16> : nullptr;
+       : R == 16 ? (const void *)rows_kernel<16>
+       : R == 32 ? (const void *)rows_kernel<32>
+       : R == 64 ? (const void *)rows_kernel<64> : nullptr;
 }
 
-// Checks the systems against the row count and builds the workgroup map
-// (G consecutive vectors of one system per workgroup) and the LDS size.
+// Checks the systems against the row count (R = 0: each system's own,
+// sys[k].rows) and builds the workgroup map (G = 64 / R consecutive vectors
+// of one system per workgroup) and the LDS size.
 int rows_prepare(const RowsArgs *sys, int nsys, int R, std::vector<int2> &wmap, size_t &smem) {
-  if (!rows_fn(R)) return set_error("row conveyor: unsupported row count %d", R), NISE_ERR_ARG;
+  if (R && !rows_fn(R)) return set_error("row conveyor: unsupported row count %d", R), NISE_ERR_ARG;
   smem = 0;
   wmap.clear();
-  const int G = 64 / R;
   for (int k = 0; k < nsys; ++k) {
     const RowsArgs &a = sys[k];
+    const int Rk = R ? R : a.rows;
+    if (!rows_fn(Rk)) return set_error("row conveyor: unsupported row count %d", Rk), NISE_ERR_ARG;
+    const int G = 64 / Rk;
     if (a.nlayers < kRingL || a.nlayers % kRingL)
       return set_error("row conveyor: layer count %d is not a multiple of %d", a.nlayers, kRingL), NISE_ERR_ARG;
-    if (a.K != (int64_t)a.nlayers * R) return set_error("row conveyor: schedule built for other rows"), NISE_ERR_ARG;
-    smem = std::max(smem, rows_smem(a.N, R));
+    if (a.K != (int64_t)a.nlayers * Rk) return set_error("row conveyor: schedule built for other rows"), NISE_ERR_ARG;
+    smem = std::max(smem, rows_smem(a.N, Rk));
     for (int v = 0; v < a.nvec; v += G) wmap.push_back(make_int2(k, v));
   }
   if (smem > 160 * 1024) return set_error("row conveyor: %zu B of LDS needed", smem), NISE_ERR_ARG;
   return NISE_OK;
 }
 
-// sys / wmap: device copies (rows_prepare), nwg workgroups.
+// sys / wmap: device copies (rows_prepare), nwg workgroups; R = 0: each
+// system's own row count.
 int launch_rows(const RowsArgs *sys, const int2 *wmap, int nwg, int R, size_t smem, hipStream_t st,
                 unsigned long long *times) {
-  const void *fn = rows_fn(R);
+  const void *fn = R ? rows_fn(R) : (const void *)rows_mixed_kernel;
   if (!fn) return set_error("row conveyor: unsupported row count %d", R), NISE_ERR_ARG;
   if (nwg == 0) return NISE_OK;
   NISE_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
@@ -282,25 +338,26 @@ int launch_rows(const RowsArgs *sys, const int2 *wmap, int nwg, int R, size_t sm
 }
 
 // Row count for a schedule: the power of two closest to the mean number of
-// rotations per layer of the unconstrained layering, raised while the LDS
-// image would exceed 80 KiB (so that two waves fit on a CU).
-int rows_for(int N, int Kreal, int nlayers_asap) {
+// rotations per layer of the unconstrained layering, at most cap, raised
+// while the LDS image would exceed 80 KiB (so that two waves fit on a CU).
+int rows_for(int N, int Kreal, int nlayers_asap, int cap) {
   const double per = nlayers_asap ? (double)Kreal / nlayers_asap : 1.0;
   int R = 1;
-  while (R < 16 && R * 1.5 < per) R <<= 1;
-  while (R < 16 && rows_smem(N, R) > 80 * 1024) R <<= 1;
+  while (R < cap && R * 1.5 < per) R <<= 1;
+  while (R < cap && rows_smem(N, R) > 80 * 1024) R <<= 1;
   return R;
 }
 
 // Device order of a row schedule's entries (the order of the step table and
-// of every per-frame table built from it): entry q of layer l sits at
-// (l / 4) * 4R + 4q + l % 4, so the 4 layers a quad loads at once are one
-// contiguous 32-byte run and a wave's load of a 4-layer block is contiguous.
+// of every per-frame table built from it): with B = min(64 / R, 4), entry q
+// of layer l sits at (l / B) * B R + B q + l % B, so the B layers a group
+// loads at once are one contiguous run and a wave's load of a B-layer block
+// is contiguous.
 void rows_device_order(const Schedule &s, int R, std::vector<uint32_t> &ab) {
   ab.resize(s.ab.size());
-  const int nl = s.nlayers();
+  const int nl = s.nlayers(), B = std::min(64 / R, 4);
   for (int l = 0; l < nl; ++l)
-    for (int q = 0; q < R; ++q) ab[(size_t)(l / 4) * 4 * R + 4 * q + l % 4] = s.ab[(size_t)l * R + q];
+    for (int q = 0; q < R; ++q) ab[(size_t)(l / B) * B * R + B * q + l % B] = s.ab[(size_t)l * R + q];
 }
 
 // Static step table of a row schedule in device order: LDS byte offsets of

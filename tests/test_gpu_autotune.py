@@ -1,5 +1,5 @@
 """The autotuner (nise_abs_plan_autotune) on a small lattice: it times every
-candidate (row conveyor with 1..16 rows, slot-conveyor shapes), keeps the
+candidate (row conveyor with 1..64 rows, slot-conveyor shapes), keeps the
 fastest, and the plan's result afterwards matches the oracle within 1e-5
 (the batched path's bar).  Each candidate set explicitly gives the same
 response within 1e-6 (the propagation is the reference's FP32 order in

@@ -188,7 +188,7 @@ def test_absorption_shapes_agree(orc, nise_lib):
         if base is None:
             base = re + 1j * im
         assert rel_l2(re + 1j * im, base) < 1e-6
-    for rows in (1, 2, 4, 8, 16, 0):
+    for rows in (1, 2, 4, 8, 16, 32, 64, 0):
         p = abi.AbsPlan(non, H, H.shape[1], mu, mu.shape[1], H.shape[0])
         p.set_rows(rows)
         re, im = p.execute()
@@ -196,7 +196,7 @@ def test_absorption_shapes_agree(orc, nise_lib):
         assert rel_l2(re + 1j * im, base) < 1e-6, rows
 
 
-@pytest.mark.parametrize("rows", [1, 4, 16])
+@pytest.mark.parametrize("rows", [1, 4, 16, 32, 64])
 def test_row_conveyor_vs_oracle(orc, nise_lib, rows):
     """Row conveyor on a lattice (wide layers) and a chain (one rotation per
     layer), Trotter 1 and 3, against the FP32 oracle."""
