@@ -928,5 +928,8 @@ extern "C" int nise_absorption_multi_tune(const nise_system *systems, int nsys, 
   batch_signature(systems, nsys, g_multi.tune_sig);
   g_multi.tune_R = best ? best : -1;
   if (R_best) *R_best = best;
+  // one run with the kept choice, so that the next call finds its schedules
+  // and tables in the cache (the last candidate timed left its own there)
+  if ((rc = multi_run(systems, nsys, where, re.data(), im.data(), nullptr, 0))) return rc;
   return cnt;
 }
