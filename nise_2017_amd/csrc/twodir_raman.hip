@@ -10,7 +10,9 @@
 //                conjugate of the sign +1 step bit for bit
 //   M[x1]        mu_x1(tj) through t3 (history): rightnr(t3) = conj(M(t3))
 //   ft1[x0][t1]  |aa> = sqrt2 mu_x0,a(tj - t1) (dipole_double_ground)
-//                through frames tj - t1 .. tj - 1, then through t3 (kII)
+//                through frames tj - t1 .. tj - 1, then through t3 (kII;
+//                held in LDS for all of it by raman_persist_kernel, which
+//                also forms diagram 2 and the t1n rows)
 //   fr[x1]       |aa> = sqrt2 mu_x1,a(tj) through t3 (kI)
 //   t1n[x0][x1][t1]  dipole_double_last(mu_x1(tj), ft1[x0][t1](tj))
 //                through t3 (+1, history)
@@ -39,6 +41,7 @@
 
 #include "nise_internal.h"
 #include "doubles_dev.h"
+#include "persist_dev.h"
 
 namespace nise {
 
@@ -219,6 +222,182 @@ __global__ void raman_d2_kernel(RamanArgs A, int t3) {
     }
     float *E = A.E + (((int64_t)s * 21 + m) * 6 + 2) * plane + (int64_t)t3 * T1 + t1;
     E[0] = rr, E[plane] = ri;
+  }
+}
+
+// The kII doubles vectors in LDS for all their frames (src/calc_2DIRraman.c
+// :203-246 and :333-356).  Workgroup (s, t1 block) holds three waves, x0 = 0,
+// 1, 2, each with the vectors ft1[s][x0][t1] of t1 = t1a .. t1a + nv - 1 in
+// VW slots; the three waves share every frame.  From the earliest start
+// frame tj - (t1a + nv - 1) to tj, vector t1 is formed at frame tj - t1
+// (dipole_double_ground: sqrt2 mu_x0,a on |aa>, every other state 0, before
+// that frame's step; a slot holds zeros until then, and rotating zeros keeps
+// zeros) and every formed vector takes the step of frames tj - t1 .. tj - 1.
+// At tj: t1n[x0][x1][t1] = dipole_double_last(mu_x1(tj), ft1(tj)) when d3.
+// Then per t3 (when d2): the contraction with the six Raman components at
+// tj + t3 (dipole_double_last with alpha_c and overtone sqrt2 alpha_c), for
+// every item with this x0 the k-th rounded dot term against conj(M[x1](t3))
+// at once, the sums over k in ascending order (the reference's operation
+// sequence, as raman_d2_kernel), then the doubles step of frame tj + t3.
+struct RamanPersistArgs {
+  RamanArgs A;
+  int K2 = 0, m = 1;
+  const uint32_t *ab2 = nullptr;  // kept couplings (lexicographic)
+  const float2 *D2 = nullptr;     // [frame - f_lo][N2]
+  const float4 *cs2 = nullptr;    // [frame - f_lo][K2]
+  int64_t f_lo = 0;
+};
+constexpr int kRamanItemBatch = 2;  // items whose dot terms share the LDS rows
+struct RamanLayout {
+  size_t ix, al, ov, rn, lv, d2, cs, psi, total;
+};
+// ix: the rotation table; then the EA rows (alpha and overtone rows of the
+// six components, or the dipole rows at tj; the M rows; the dot terms of
+// kRamanItemBatch items per vector) sharing bytes with the doubles step's
+// rows (D2, coefficient rows); then the three waves' states.
+__host__ __device__ inline RamanLayout raman_layout(int N, int VW, int K2) {
+  const size_t N2 = (size_t)N * (N + 1) / 2;
+  RamanLayout L;
+  size_t o = 0;
+  L.ix = o, o += rot_table_bytes(N, 64 / VW, K2);
+  const size_t u0 = o;
+  L.al = o, o += ((size_t)6 * N * 4 + 15) & ~(size_t)15;
+  L.ov = o, o += ((size_t)6 * N * 4 + 15) & ~(size_t)15;
+  L.rn = o, o += (size_t)3 * N * 8;
+  L.lv = o, o += (size_t)3 * VW * kRamanItemBatch * N * 8;
+  const size_t ea_end = o;
+  o = u0;
+  L.d2 = o, o += N2 * 8, o = (o + 15) & ~(size_t)15;
+  L.cs = o, o += (size_t)K2 * 48;
+  o = (o > ea_end ? o : ea_end), o = (o + 15) & ~(size_t)15;
+  L.psi = o, o += (size_t)3 * (N2 + 2) * VW * 8;
+  L.total = o;
+  return L;
+}
+
+template <int NP>
+__global__ void __launch_bounds__(192) raman_persist_kernel(RamanPersistArgs P) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int VW = 64 / NP;
+  const RamanArgs &A = P.A;
+  const int N = A.N, N2 = N * (N + 1) / 2, T1 = A.T1, T3 = A.T3, K2 = P.K2, t = threadIdx.x;
+  const int wave = t >> 6, lane = t & 63, x0 = wave;
+  const int nblk = (T1 + VW - 1) / VW, s = blockIdx.x / nblk, t1a = (blockIdx.x % nblk) * VW;
+  const int nv = min(VW, T1 - t1a);
+  const RamanLayout Ly = raman_layout(N, VW, K2);
+  uint2 *ixL = reinterpret_cast<uint2 *>(smem + Ly.ix);
+  float *alL = reinterpret_cast<float *>(smem + Ly.al), *ovL = reinterpret_cast<float *>(smem + Ly.ov);
+  float2 *RnL = reinterpret_cast<float2 *>(smem + Ly.rn);
+  float2 *LvL = reinterpret_cast<float2 *>(smem + Ly.lv) + (size_t)wave * VW * kRamanItemBatch * N;
+  float2 *d2 = reinterpret_cast<float2 *>(smem + Ly.d2);
+  float4 *csL = reinterpret_cast<float4 *>(smem + Ly.cs);
+  float2 *psi = reinterpret_cast<float2 *>(smem + Ly.psi) + (size_t)wave * (N2 + 2) * VW;
+  const int q = lane % VW, c0 = lane / VW, total = K2 * ((N + NP - 1) / NP);
+  for (int j = lane; j < N2 * VW; j += 64) psi[j] = make_float2(0.f, 0.f);
+  init_special_states<VW>(psi, N2, lane);
+  build_rot_table<NP>(ixL, P.ab2, N, K2, false);
+  const int64_t tj = r_tj(A, s);
+  // one doubles step of frame f on every slot (unformed slots hold zeros)
+  auto step = [&](int64_t f) {
+    const int64_t row = f - P.f_lo;
+    __syncthreads();  // the rows of the previous phase are consumed
+    stage_step_rows(d2, csL, P.D2 + row * N2, P.cs2 + row * K2, N2, K2);
+    __syncthreads();
+    for (int sub = 0; sub < P.m; ++sub) {
+      dbl_diag<VW>(psi, d2, N2, lane);
+      dbl_rotate_tab<NP>(psi, total, ixL, csL, q, c0);
+      dbl_diag<VW>(psi, d2, N2, lane);
+    }
+  };
+  // (1) formation and the t1 steps
+  for (int64_t f = tj - (t1a + nv - 1); f <= tj; ++f) {
+    const int qi = (int)(tj - f) - t1a;  // the slot formed at f
+    if (qi >= 0 && qi < nv) {
+      for (int j = lane; j < N2; j += 64) psi[j * VW + qi] = make_float2(0.f, 0.f);
+      const float *mu = r_mu(A, f, x0);
+      for (int a = lane; a < N; a += 64)
+        psi[((a * (2 * N - a + 1)) >> 1) * VW + qi] = make_float2(__fmul_rn(kSqrt2, mu[a]), 0.f);
+      asm volatile("" ::: "memory");
+    }
+    if (f < tj) step(f);
+  }
+  // (2) t1n at tj: the dipole rows mu_x1(tj) and their sqrt2 overtone rows
+  if (A.d3) {
+    __syncthreads();
+    for (int e = t; e < 3 * N; e += blockDim.x) {
+      const float v = r_mu(A, tj, e / N)[e % N];
+      alL[e] = v, ovL[e] = __fmul_rn(kSqrt2, v);
+    }
+    __syncthreads();
+    if (lane < nv * N) {
+      const int c = lane / N, k = lane % N;
+      const float ov[3] = {ovL[k], ovL[N + k], ovL[2 * N + k]};
+      float2 o[3];
+      if ((N & 3) == 0) ddl_multi4<3, VW>(psi + c, N, k, alL, alL + N, alL + 2 * N, ov, o);
+      else ddl_multi<3, VW>(psi + c, N, k, alL, alL + N, alL + 2 * N, ov, o);
+      for (int x1 = 0; x1 < 3; ++x1) A.t1n[((((int64_t)s * 3 + x0) * 3 + x1) * T1 + t1a + c) * N + k] = o[x1];
+    }
+  }
+  if (!A.d2) return;
+  // (3) the t3 steps with diagram 2: the items with this x0
+  int mj[7], cj[7], x1j[7], nm = 0;
+  for (int m = 0; m < 21; ++m)
+    if (c_polar[m][0] == x0 && nm < 7) mj[nm] = m, cj[nm] = raman_pol(c_polar[m][2], c_polar[m][3]), x1j[nm++] = c_polar[m][1];
+  const int nrow = 3 * T1 + 3;
+  const int64_t plane = (int64_t)T3 * T1;
+  for (int t3 = 0; t3 < T3; ++t3) {
+    const int64_t tl = tj + t3;
+    __syncthreads();  // the previous step's rows are consumed
+    for (int e = t; e < 6 * N; e += blockDim.x) {
+      const float v = r_al(A, tl, e / N)[e % N];
+      alL[e] = v, ovL[e] = __fmul_rn(kSqrt2, v);
+    }
+    for (int e = t; e < 3 * N; e += blockDim.x)
+      RnL[e] = gld(A.hist + (((int64_t)s * nrow + 3 * T1 + e / N) * T3 + t3) * N + e % N);
+    __syncthreads();
+    // lane (c, k): the six contractions, then per batch of items the k-th
+    // dot terms; one lane per (c, item) adds them in ascending k
+    float2 o[6];
+    const int c = lane / N, k = lane % N;
+    if (lane < nv * N) {
+      const float ova[3] = {ovL[k], ovL[N + k], ovL[2 * N + k]}, ovb[3] = {ovL[3 * N + k], ovL[4 * N + k], ovL[5 * N + k]};
+      if ((N & 3) == 0) {
+        ddl_multi4<3, VW>(psi + c, N, k, alL, alL + N, alL + 2 * N, ova, o);
+        ddl_multi4<3, VW>(psi + c, N, k, alL + 3 * N, alL + 4 * N, alL + 5 * N, ovb, o + 3);
+      } else {
+        ddl_multi<3, VW>(psi + c, N, k, alL, alL + N, alL + 2 * N, ova, o);
+        ddl_multi<3, VW>(psi + c, N, k, alL + 3 * N, alL + 4 * N, alL + 5 * N, ovb, o + 3);
+      }
+    }
+    for (int j0 = 0; j0 < nm; j0 += kRamanItemBatch) {
+      const int nb = min(kRamanItemBatch, nm - j0);
+      if (lane < nv * N)
+        for (int jj = 0; jj < nb; ++jj) {
+          const int j = j0 + jj;
+          float2 l = o[0];
+#pragma unroll
+          for (int cc = 1; cc < 6; ++cc)
+            if (cj[j] == cc) l = o[cc];
+          const float2 Mv = RnL[x1j[j] * N + k];
+          const float nr = Mv.x, ni = -Mv.y;
+          LvL[(c * kRamanItemBatch + jj) * N + k] = make_float2(__fadd_rn(__fmul_rn(nr, l.y), __fmul_rn(ni, l.x)),
+                                                             __fsub_rn(__fmul_rn(nr, l.x), __fmul_rn(ni, l.y)));
+        }
+      asm volatile("" ::: "memory");
+      if (lane < nv * nb) {
+        const int cs = lane / nb, jj = lane % nb;
+        const float2 *Tm = LvL + (cs * kRamanItemBatch + jj) * N;
+        float rr = 0.f, ri = 0.f;
+        for (int kk = 0; kk < N; ++kk) {
+          const float2 v = Tm[kk];
+          rr = __fadd_rn(rr, v.x), ri = __fadd_rn(ri, v.y);
+        }
+        float *E = A.E + (((int64_t)s * 21 + mj[j0 + jj]) * 6 + 2) * plane + (int64_t)t3 * T1 + t1a + cs;
+        E[0] = rr, E[plane] = ri;
+      }
+      asm volatile("" ::: "memory");
+    }
+    if (t3 + 1 < T3) step(tl);
   }
 }
 
@@ -468,6 +647,11 @@ extern "C" int nise_2dirraman_batched(const t_non *non, const float *H_rec, int6
                             8.0 * 9 * T1 * T3 * N + 4.0 * 126 * plane;
   const int chunk = (int)std::max(1.0, std::min<double>({(double)total, 1024.0, 8.0e9 / per_sample}));
   const int vw = doubles_vw(N);
+  // the kII vectors stay in LDS for all their frames (raman_persist_kernel)
+  // when their states, rotation table and rows fit
+  const int pnp = doubles_np(N);
+  const size_t psm = raman_layout(N, 64 / pnp, (int)lex.size()).total;
+  const bool pers = k2 && rot_table_fits(N, 64 / pnp, (int)lex.size()) && psm <= 160 * 1024;
   const size_t csm = raman_contract_smem(N);
   NISE_HIP_CHECK(hipFuncSetAttribute((const void *)raman_d1_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)csm));
   NISE_HIP_CHECK(hipFuncSetAttribute((const void *)raman_d2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)csm));
@@ -545,7 +729,19 @@ extern "C" int nise_2dirraman_batched(const t_non *non, const float *H_rec, int6
     //     (frame tj - T1 + r) the vectors of t1 = T1 - r are formed, then
     //     every vector of t1 >= T1 - r takes the step (src/calc_2DIRraman.c
     //     :203-246); fr from mu_x1(tj) (:255-258)
-    if (k2) {
+    if (k2 && pers) {
+      RamanPersistArgs pa;
+      pa.A = A, pa.K2 = (int)lex.size(), pa.m = non->ts, pa.ab2 = B.ab2.as<uint32_t>();
+      pa.D2 = B.D2.as<float2>(), pa.cs2 = B.cs2.as<float4>(), pa.f_lo = f_lo;
+      const void *fp = pnp == 8 ? (const void *)raman_persist_kernel<8>
+                     : pnp == 16 ? (const void *)raman_persist_kernel<16>
+                     : pnp == 32 ? (const void *)raman_persist_kernel<32>
+                                 : (const void *)raman_persist_kernel<64>;
+      NISE_HIP_CHECK(hipFuncSetAttribute(fp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)psm));
+      void *kargs[] = {&pa};
+      const int nblk = (T1 + 64 / pnp - 1) / (64 / pnp);
+      NISE_HIP_CHECK(hipLaunchKernel(fp, dim3((unsigned)(S * nblk)), dim3(192), kargs, psm, st));
+    } else if (k2) {
       std::vector<int> vb, vc;
       std::vector<int64_t> rows;
       std::vector<size_t> off(T1 + 1, 0);
@@ -576,7 +772,7 @@ extern "C" int nise_2dirraman_batched(const t_non *non, const float *H_rec, int6
       hipLaunchKernelGGL(raman_inject_kernel, dim3(S * 3), dim3(256), 0, st, A, 0, 1);
       NISE_HIP_CHECK(hipGetLastError());
     }
-    if (d3) {
+    if (d3 && !pers) {
       hipLaunchKernelGGL(raman_t1n_kernel, dim3(S * 3 * T1), dim3(64), csm, st, A);
       NISE_HIP_CHECK(hipGetLastError());
     }
@@ -584,7 +780,7 @@ extern "C" int nise_2dirraman_batched(const t_non *non, const float *H_rec, int6
     //     for ft1 (d2) and fr (p1)
     std::vector<int> gvb, gvc;
     for (int s = 0; s < S; ++s) {
-      if (d2) split_groups((int64_t)s * T1 * 3, 3 * T1, vw, gvb, gvc);
+      if (d2 && !pers) split_groups((int64_t)s * T1 * 3, 3 * T1, vw, gvb, gvc);
       if (p1 && k1) split_groups((int64_t)S * T1 * 3 + s * 3, 3, vw, gvb, gvc);
     }
     const int G = (int)gvb.size();
@@ -592,7 +788,7 @@ extern "C" int nise_2dirraman_batched(const t_non *non, const float *H_rec, int6
     {
       int g = 0;
       for (int s = 0; s < S; ++s) {
-        const int ng = (d2 ? (3 * T1 + vw - 1) / vw : 0) + (p1 && k1 ? (3 + vw - 1) / vw : 0);
+        const int ng = (d2 && !pers ? (3 * T1 + vw - 1) / vw : 0) + (p1 && k1 ? (3 + vw - 1) / vw : 0);
         for (int j = 0; j < ng; ++j, ++g)
           for (int t3 = 0; t3 + 1 < T3; ++t3) grow[(size_t)t3 * G + g] = tj_of(s) + t3 - f_lo;
       }
@@ -606,7 +802,7 @@ extern "C" int nise_2dirraman_batched(const t_non *non, const float *H_rec, int6
         hipLaunchKernelGGL(raman_d1_kernel, dim3(S * 3), dim3(64), csm, st, A, t3);
         NISE_HIP_CHECK(hipGetLastError());
       }
-      if (d2) {
+      if (d2 && !pers) {
         hipLaunchKernelGGL(raman_d2_kernel, dim3(S * 3 * T1), dim3(64), csm, st, A, t3);
         NISE_HIP_CHECK(hipGetLastError());
       }

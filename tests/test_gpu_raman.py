@@ -65,9 +65,10 @@ def test_batched_against_oracle(orc, tech):
     assert e < 1e-5
 
 
-@pytest.mark.parametrize("N,T1,T3,ts,T2", [(12, 8, 7, 2, 0), (5, 1, 4, 1, 3), (9, 6, 1, 1, 0)])
+@pytest.mark.parametrize("N,T1,T3,ts,T2", [(12, 8, 7, 2, 0), (5, 1, 4, 1, 3), (9, 6, 1, 1, 0), (40, 5, 4, 1, 0)])
 def test_shapes_against_oracle(orc, N, T1, T3, ts, T2):
-    """Trotter 2, a waiting time in the input, t1 or t3 of one step."""
+    """Trotter 2, a waiting time in the input, t1 or t3 of one step, and
+    N = 40: one kII vector per wave (64 lanes per vector), an odd t1 count."""
     sr, ns = 4, 3
     nfr = (ns - 1) * sr + T1 + T2 + T3 + 2
     H, mu, al = system(N, nfr, seed=400 + N)
