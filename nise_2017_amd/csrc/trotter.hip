@@ -32,26 +32,50 @@ namespace nise {
 // ---------------------------------------------------------------------------
 // Union of kept couplings over a frame range (one byte per a<b pair).
 // ---------------------------------------------------------------------------
-__global__ void union_mask_kernel(FrameSource s, int N, int64_t f_lo, int64_t nfr, uint8_t *mask) {
-  int64_t npair = (int64_t)N * (N - 1) / 2;
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npair) return;
-  // invert the lexicographic pair index p -> (a, b)
-  int a = 0;
-  int64_t rem = p;
-  while (rem >= N - 1 - a) { rem -= N - 1 - a; ++a; }
-  int b = a + 1 + (int)rem;
+// Workgroup of 256 threads: kMaskPairs consecutive pairs (a row segment of
+// a frame per load) x kMaskLanes frame lanes; lane fl reads frames fl, fl +
+// kMaskLanes, ..., kMaskInflight of them in flight, and stops at the first
+// kept coupling.  The lanes' flags are ORed in LDS.
+constexpr int kMaskPairs = 32, kMaskLanes = 8, kMaskInflight = 8;
+__global__ void __launch_bounds__(256) union_mask_kernel(FrameSource s, int N, int64_t f_lo, int64_t nfr,
+                                                         uint8_t *mask) {
+  __shared__ uint8_t kk[kMaskLanes][kMaskPairs];
+  const int tp = threadIdx.x % kMaskPairs, fl = threadIdx.x / kMaskPairs;
+  const int64_t npair = (int64_t)N * (N - 1) / 2;
+  const int64_t p = (int64_t)blockIdx.x * kMaskPairs + tp;
   uint8_t keep = 0;
-  int64_t nf = s.coupling_type ? 1 : nfr;
-  for (int64_t f = 0; f < nf && !keep; ++f) keep = kept_coupling(s, N, f_lo + f, a, b) != 0.f;
-  mask[p] = keep;
+  if (p < npair) {
+    // invert the lexicographic pair index p -> (a, b)
+    int a = 0;
+    int64_t rem = p;
+    while (rem >= N - 1 - a) { rem -= N - 1 - a; ++a; }
+    const int b = a + 1 + (int)rem;
+    const int64_t nf = s.coupling_type ? 1 : nfr;
+    for (int64_t f0 = fl; f0 < nf && !keep; f0 += (int64_t)kMaskLanes * kMaskInflight) {
+      float v[kMaskInflight];
+#pragma unroll
+      for (int u = 0; u < kMaskInflight; ++u) {
+        const int64_t f = f0 + (int64_t)u * kMaskLanes;
+        v[u] = f < nf ? kept_coupling(s, N, f_lo + f, a, b) : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < kMaskInflight; ++u) keep |= (uint8_t)(v[u] != 0.f);
+    }
+  }
+  kk[fl][tp] = keep;
+  __syncthreads();
+  if (fl == 0 && p < npair) {
+    uint8_t r = 0;
+    for (int l = 0; l < kMaskLanes; ++l) r |= kk[l][tp];
+    mask[p] = r;
+  }
 }
 
 int launch_union_mask(const FrameSource &src, int N, int64_t f_lo, int64_t nfr, uint8_t *mask_dev,
                       hipStream_t st) {
   int64_t npair = (int64_t)N * (N - 1) / 2;
   if (npair == 0) return NISE_OK;
-  dim3 grid((unsigned)((npair + 255) / 256));
+  dim3 grid((unsigned)((npair + kMaskPairs - 1) / kMaskPairs));
   hipLaunchKernelGGL(union_mask_kernel, grid, dim3(256), 0, st, src, N, f_lo, nfr, mask_dev);
   NISE_HIP_CHECK(hipGetLastError());
   return NISE_OK;

@@ -454,19 +454,23 @@ __global__ void raman_d3_kernel(RamanArgs A, int rc) {
   }
 }
 
-// R[4p + q] += over samples and items, in ascending (s, m) order, of the
-// reference's FP32 products E pw, pw = polarweight(p, m) lt(t3, t1): q = 0,
-// 1 diagram 1; q = 2, 3 diagram 2 minus diagram 3 (src/calc_2DIRraman.c
-// :324-329, :350-355, :396-401); FP64 sums
+// part[c][4p + q] = sum over the samples of chunk c (kRamanRedS samples)
+// and the items, in ascending (s, m) order, of the reference's FP32 products
+// E pw, pw = polarweight(p, m) lt(t3, t1): q = 0, 1 diagram 1; q = 2, 3
+// diagram 2 minus diagram 3 (src/calc_2DIRraman.c :324-329, :350-355,
+// :396-401); FP64 sums.  raman_reduce_final_kernel adds the chunks to R in
+// ascending order.
+constexpr int kRamanRedS = 16;
 __global__ void raman_reduce_kernel(const float *E, int S, int T3, int T1, const float *wpl, const float *lt,
-                                    double *R) {
+                                    double *part) {
   const int64_t plane = (int64_t)T3 * T1;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y, p = k / 4, q = k % 4;
+  const int k = blockIdx.y, p = k / 4, q = k % 4, c = blockIdx.z;
   if (e >= plane) return;
   const float l = lt[e];
-  double acc = R[(int64_t)k * plane + e];
-  for (int s = 0; s < S; ++s)
+  double acc = 0.0;
+  const int s1 = min(S, (c + 1) * kRamanRedS);
+  for (int s = c * kRamanRedS; s < s1; ++s)
     for (int m = 0; m < 21; ++m) {
       const float pw = __fmul_rn(wpl[p * 21 + m], l);
       const float *Em = E + ((int64_t)s * 21 + m) * 6 * plane + e;
@@ -477,7 +481,14 @@ __global__ void raman_reduce_kernel(const float *E, int S, int T3, int T1, const
         acc -= (double)__fmul_rn(gld(Em + (int64_t)(q + 2) * plane), pw);
       }
     }
-  R[(int64_t)k * plane + e] = acc;
+  part[((int64_t)c * 12 + k) * plane + e] = acc;
+}
+__global__ void raman_reduce_final_kernel(const double *part, int nc, int64_t n, double *R) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  double acc = 0.0;
+  for (int c = 0; c < nc; ++c) acc += part[(int64_t)c * n + idx];
+  R[idx] += acc;
 }
 
 }  // namespace nise
@@ -496,7 +507,7 @@ bool tech_eq(const t_non *n, const char *t) { return strcmp(n->technique, t) == 
 // Device buffers of the calling thread's Raman runs, kept between calls.
 struct RamanBufs {
   DevBuf Hb, mub, Cb, Alb, mk, ab, lay, diag, cs, ab2, D2, cs2, stp, wpl, ltb, Rd;
-  DevBuf L0, hist, dbl, t1n, h3, E, slots, wgb, slots2, wgb2, slots3, wgb3, gvb, gvc, grow, tvb, tvc, trow;
+  DevBuf L0, hist, dbl, t1n, h3, E, part, slots, wgb, slots2, wgb2, slots3, wgb3, gvb, gvc, grow, tvb, tvc, trow;
 };
 
 // Groups of at most vw consecutive vectors [first, first + n): begins and
@@ -837,9 +848,14 @@ extern "C" int nise_2dirraman_batched(const t_non *non, const float *H_rec, int6
       NISE_HIP_CHECK(hipGetLastError());
     }
     // (5) this chunk into the 12 planes
-    hipLaunchKernelGGL(raman_reduce_kernel, dim3((unsigned)((plane + 255) / 256), 12), dim3(256), 0, st,
+    const int nred = (S + kRamanRedS - 1) / kRamanRedS;
+    if ((rc = B.part.alloc((size_t)nred * 12 * plane * 8))) return rc;
+    hipLaunchKernelGGL(raman_reduce_kernel, dim3((unsigned)((plane + 255) / 256), 12, nred), dim3(256), 0, st,
                        (const float *)B.E.as<float>(), S, T3, T1, (const float *)B.wpl.as<float>(),
-                       (const float *)B.ltb.as<float>(), B.Rd.as<double>());
+                       (const float *)B.ltb.as<float>(), B.part.as<double>());
+    NISE_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(raman_reduce_final_kernel, dim3((unsigned)((12 * plane + 255) / 256)), dim3(256), 0, st,
+                       (const double *)B.part.as<double>(), nred, 12 * plane, B.Rd.as<double>());
     NISE_HIP_CHECK(hipGetLastError());
   }
   std::vector<double> Rh((size_t)12 * plane);
