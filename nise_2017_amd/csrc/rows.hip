@@ -40,7 +40,13 @@
 namespace nise {
 
 constexpr int kRingL = 8;  // layers in flight (nlayers is a multiple)
-constexpr int kSiteU = 4;           // sites per batch of HBM loads in the per-site pass
+// Sites per batch of HBM loads in the per-site pass (all loads of a batch
+// are in flight together): a lane walks N / R sites per frame, so waves
+// with few rows take larger batches.
+template <int R>
+constexpr int site_batch() {
+  return R >= 4 ? 4 : R == 2 ? 8 : 16;
+}
 
 // One rotation (src/propagate.c:883-887) of the pair at LDS byte offsets
 // e.x, e.y (plus the lane's vector column) with (cos, sin) = c.
@@ -91,6 +97,7 @@ template <int R>
 __device__ __forceinline__ void rows_run(const RowsArgs &A, int2 wm) {
   constexpr int G = 64 / R;
   constexpr int B = G < 4 ? G : 4;  // vectors per frame group, layers per table block
+  constexpr int kSiteU = site_batch<R>();
   constexpr int kRingS = kRingL / B;  // ring slots per lane
   static_assert(G == 1 || G == 2 || G % 4 == 0, "vectors per wave: 1, 2 or a multiple of 4");
   extern __shared__ __align__(16) unsigned char smem[];
