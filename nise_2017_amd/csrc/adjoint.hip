@@ -193,7 +193,13 @@ __global__ void adjoint_reduce_kernel(const double *pd, const double *pc, int nw
 // shuffles and added to an FP64 LDS accumulator by the g == 0 lane of each
 // row.
 constexpr int kAdjRingG = 2;              // ring slots per lane, 4 layers each
-constexpr int kAdjSiteU = 4;              // sites per batch of HBM loads
+// Sites per batch of HBM loads in the site pass, whose gradient butterflies
+// also run interleaved: a lane walks N / R sites per frame, so waves with
+// few rows take larger batches.
+template <int R>
+constexpr int adj_site_batch() {
+  return R >= 4 ? 4 : R == 2 ? 8 : 16;
+}
 
 // Layer j of a ring block, backwards (src/propagate.c:883-887 inverted):
 // the coupling-scale gradient term and R^dagger on phi and chi.
@@ -215,6 +221,7 @@ template <int R>
 __global__ void __launch_bounds__(64) adjoint_rows_kernel(AdjRowsArgs A) {
   constexpr int G = 64 / R;
   static_assert(G >= 4, "a row needs a whole quad of lanes");
+  constexpr int kAdjSiteU = adj_site_batch<R>();
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = A.N, T = A.T, nl = A.nlayers, NS = N + R, m = A.m;
   const int64_t K = A.K;
@@ -277,6 +284,7 @@ __global__ void __launch_bounds__(64) adjoint_rows_kernel(AdjRowsArgs A) {
         cv[u] = reset ? gld(A.ck + ckr * N + ii) : make_float2(0.f, 0.f);
       }
       __builtin_amdgcn_sched_barrier(0);
+      float ccu[kAdjSiteU];
 #pragma unroll
       for (int u = 0; u < kAdjSiteU; ++u) {
         const int i = i0 + u * R;
@@ -300,9 +308,18 @@ __global__ void __launch_bounds__(64) adjoint_rows_kernel(AdjRowsArgs A) {
           c = cmul_conj_d(c, da[u]);
         }
         if (ok) phi[is * G + g] = p, chi[is * G + g] = c;
+        ccu[u] = cc;
+      }
+      // each site's sum over the wave's vectors, the butterflies of the
+      // batch's sites interleaved (each site's own order unchanged)
 #pragma unroll
-        for (int off = 1; off < G; off <<= 1) cc += __shfl_xor(cc, off);
-        if (ok && g == 0) gd[i] += (double)cc;
+      for (int off = 1; off < G; off <<= 1)
+#pragma unroll
+        for (int u = 0; u < kAdjSiteU; ++u) ccu[u] += __shfl_xor(ccu[u], off);
+#pragma unroll
+      for (int u = 0; u < kAdjSiteU; ++u) {
+        const int i = i0 + u * R;
+        if (i < N && g == 0) gd[i] += (double)ccu[u];
       }
     }
   };
