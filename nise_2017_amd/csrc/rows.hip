@@ -92,12 +92,16 @@ __device__ __forceinline__ void rot_block(unsigned char *smem, uint32_t gofs, ui
   }
 }
 
-// The workgroup's vectors wm.y .. wm.y + G - 1 of system A (wm.x).
-template <int R>
+// The workgroup's vectors wm.y .. wm.y + G - 1 of system A (wm.x).  WIDE:
+// the per-site pass's batches for few rows (site_batch); they take
+// registers, so the mixed-count kernel, whose register count all its row
+// counts share, keeps batches of 4.  (Loading each batch while the previous
+// one was used ran slower for the chains: 0.69 vs 0.60 ms at N = 16.)
+template <int R, bool WIDE = false>
 __device__ __forceinline__ void rows_run(const RowsArgs &A, int2 wm) {
   constexpr int G = 64 / R;
   constexpr int B = G < 4 ? G : 4;  // vectors per frame group, layers per table block
-  constexpr int kSiteU = site_batch<R>();
+  constexpr int kSiteU = WIDE ? site_batch<R>() : 4;
   constexpr int kRingS = kRingL / B;  // ring slots per lane
   static_assert(G == 1 || G == 2 || G % 4 == 0, "vectors per wave: 1, 2 or a multiple of 4");
   extern __shared__ __align__(16) unsigned char smem[];
@@ -164,32 +168,41 @@ __device__ __forceinline__ void rows_run(const RowsArgs &A, int2 wm) {
       }
       // kSiteU sites per batch: all HBM loads of a batch are issued before
       // the first is used (indexes past N load site 0 and are not applied)
-      for (int i0 = q; i0 < N; i0 += kSiteU * R) {
+      struct SiteBatch {
         float mv[kSiteU];
         float2 dp[kSiteU], dc[kSiteU];
+      };
+      auto load = [&](SiteBatch &b, int i0) {
 #pragma unroll
         for (int u = 0; u < kSiteU; ++u) {
           const int i = i0 + u * R, ii = i < N ? i : 0;
-          mv[u] = gld(mu + ii);
-          dp[u] = t ? gld(Dp + ii) : make_float2(0.f, 0.f);
-          dc[u] = last ? make_float2(0.f, 0.f) : gld(Dc + ii);
+          b.mv[u] = gld(mu + ii);
+          b.dp[u] = t ? gld(Dp + ii) : make_float2(0.f, 0.f);
+          b.dc[u] = last ? make_float2(0.f, 0.f) : gld(Dc + ii);
         }
         __builtin_amdgcn_sched_barrier(0);  // keep the batch of loads together
+      };
+      auto use = [&](const SiteBatch &b, int i0) {
 #pragma unroll
         for (int u = 0; u < kSiteU; ++u) {
           const int i = i0 + u * R;
           const bool ok = i < N;
           float2 *pp = reinterpret_cast<float2 *>(smem + ((uint32_t)(ok ? i : q) * G * 8u + gofs));
-          const float mm = mv[u];
-          float2 p = t == 0 ? make_float2(mm, 0.f) : cmul_diag(*pp, dp[u]);
+          const float mm = b.mv[u];
+          float2 p = t == 0 ? make_float2(mm, 0.f) : cmul_diag(*pp, b.dp[u]);
           if (ok) {
             rr = __fadd_rn(rr, __fmul_rn(mm, p.x));
             ri = __fadd_rn(ri, __fmul_rn(mm, p.y));
             if (hr >= 0) gst(A.hist + hr * N + i, p);
-            if (!last) p = cmul_diag(p, dc[u]);
+            if (!last) p = cmul_diag(p, b.dc[u]);
             *pp = p;
           }
         }
+      };
+      for (int i0 = q; i0 < N; i0 += kSiteU * R) {
+        SiteBatch b;
+        load(b, i0);
+        use(b, i0);
       }
     }
     // response of each vector at its relative time: sum over its rows
@@ -270,7 +283,7 @@ __global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ s
   const int2 wm0 = wmap[blockIdx.x];
   const int2 wm = make_int2(__builtin_amdgcn_readfirstlane(wm0.x), __builtin_amdgcn_readfirstlane(wm0.y));
   const RowsArgs A = sys[wm.x];
-  rows_run<R>(A, wm);
+  rows_run<R, true>(A, wm);
   if (threadIdx.x == 0 && times) {  // device wall clock at start and end
     times[2 * blockIdx.x] = t_start;
     times[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
