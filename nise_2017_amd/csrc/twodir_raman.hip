@@ -343,6 +343,17 @@ __global__ void __launch_bounds__(192) raman_persist_kernel(RamanPersistArgs P) 
   int mj[7], cj[7], x1j[7], nm = 0;
   for (int m = 0; m < 21; ++m)
     if (c_polar[m][0] == x0 && nm < 7) mj[nm] = m, cj[nm] = raman_pol(c_polar[m][2], c_polar[m][3]), x1j[nm++] = c_polar[m][1];
+  // the components these items use (five of the six), and item j's
+  // position pj[j] among them
+  int comp[6] = {0, 0, 0, 0, 0, 0}, ncomp = 0, pj[7];
+  for (int cc = 0; cc < 6; ++cc) {
+    bool used = false;
+    for (int j = 0; j < nm; ++j) used |= cj[j] == cc;
+    if (used) comp[ncomp++] = cc;
+  }
+  for (int j = 0; j < nm; ++j)
+    for (int p = 0; p < ncomp; ++p)
+      if (comp[p] == cj[j]) pj[j] = p;
   const int nrow = 3 * T1 + 3;
   const int64_t plane = (int64_t)T3 * T1;
   for (int t3 = 0; t3 < T3; ++t3) {
@@ -357,17 +368,27 @@ __global__ void __launch_bounds__(192) raman_persist_kernel(RamanPersistArgs P) 
     __syncthreads();
     // lane (c, k): the six contractions, then per batch of items the k-th
     // dot terms; one lane per (c, item) adds them in ascending k
+    // o[j]: component comp[j]; only the components this x0's items use
     float2 o[6];
     const int c = lane / N, k = lane % N;
     if (lane < nv * N) {
-      const float ova[3] = {ovL[k], ovL[N + k], ovL[2 * N + k]}, ovb[3] = {ovL[3 * N + k], ovL[4 * N + k], ovL[5 * N + k]};
-      if ((N & 3) == 0) {
-        ddl_multi4<3, VW>(psi + c, N, k, alL, alL + N, alL + 2 * N, ova, o);
-        ddl_multi4<3, VW>(psi + c, N, k, alL + 3 * N, alL + 4 * N, alL + 5 * N, ovb, o + 3);
-      } else {
-        ddl_multi<3, VW>(psi + c, N, k, alL, alL + N, alL + 2 * N, ova, o);
-        ddl_multi<3, VW>(psi + c, N, k, alL + 3 * N, alL + 4 * N, alL + 5 * N, ovb, o + 3);
-      }
+      auto group = [&](int j0, int n) {  // components comp[j0 .. j0 + n), n <= 3
+        const float *r0 = alL + comp[j0] * N, *r1 = alL + comp[min(j0 + 1, ncomp - 1)] * N,
+                    *r2 = alL + comp[min(j0 + 2, ncomp - 1)] * N;
+        const float ov[3] = {ovL[comp[j0] * N + k], ovL[comp[min(j0 + 1, ncomp - 1)] * N + k],
+                             ovL[comp[min(j0 + 2, ncomp - 1)] * N + k]};
+        if ((N & 3) == 0) {
+          if (n == 3) ddl_multi4<3, VW>(psi + c, N, k, r0, r1, r2, ov, o + j0);
+          else if (n == 2) ddl_multi4<2, VW>(psi + c, N, k, r0, r1, r2, ov, o + j0);
+          else ddl_multi4<1, VW>(psi + c, N, k, r0, r1, r2, ov, o + j0);
+        } else {
+          if (n == 3) ddl_multi<3, VW>(psi + c, N, k, r0, r1, r2, ov, o + j0);
+          else if (n == 2) ddl_multi<2, VW>(psi + c, N, k, r0, r1, r2, ov, o + j0);
+          else ddl_multi<1, VW>(psi + c, N, k, r0, r1, r2, ov, o + j0);
+        }
+      };
+      group(0, min(3, ncomp));
+      if (ncomp > 3) group(3, ncomp - 3);
     }
     for (int j0 = 0; j0 < nm; j0 += kRamanItemBatch) {
       const int nb = min(kRamanItemBatch, nm - j0);
@@ -377,7 +398,7 @@ __global__ void __launch_bounds__(192) raman_persist_kernel(RamanPersistArgs P) 
           float2 l = o[0];
 #pragma unroll
           for (int cc = 1; cc < 6; ++cc)
-            if (cj[j] == cc) l = o[cc];
+            if (pj[j] == cc) l = o[cc];
           const float2 Mv = RnL[x1j[j] * N + k];
           const float nr = Mv.x, ni = -Mv.y;
           LvL[(c * kRamanItemBatch + jj) * N + k] = make_float2(__fadd_rn(__fmul_rn(nr, l.y), __fmul_rn(ni, l.x)),
