@@ -41,11 +41,13 @@ namespace nise {
 
 constexpr int kRingL = 8;  // layers in flight (nlayers is a multiple)
 // Sites per batch of HBM loads in the per-site pass (all loads of a batch
-// are in flight together): a lane walks N / R sites per frame, so waves
-// with few rows take larger batches.
+// are in flight together): a lane walks N / R sites per frame, and each
+// batch waits for its loads, so up to 16 rows take batches of 16 (at the
+// N = 256 lattice with 16 rows, one batch per lane and frame: conveyor 13.7
+// -> 11.8 ms; 8 gained nothing).  Wider rows leave few sites per lane.
 template <int R>
 constexpr int site_batch() {
-  return R >= 4 ? 4 : R == 2 ? 8 : 16;
+  return R <= 16 ? 16 : 4;
 }
 
 // One rotation (src/propagate.c:883-887) of the pair at LDS byte offsets
