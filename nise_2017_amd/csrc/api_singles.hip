@@ -39,6 +39,7 @@ struct nise_abs_plan {
   int rows = 0;                // rows of the row conveyor, 0: slot conveyor
   DevBuf ab, layer_off, diag, cs, slots, wg_begin, partial, out, steps, row_steps, row_sys, row_wmap;
   size_t row_smem = 0;
+  bool row_pf = false;  // the row conveyor prefetches each frame's first site batch
   int nsteps = 0;
   DevBuf ws_cs, resp_plain;
   int vp = 16, spw = 5, nwg = 0, block = 256;
@@ -89,7 +90,7 @@ int upload_rows_desc(nise_abs_plan *P) {
   r.mu_rec = P->mu_dev, r.mu_stride = P->mu_stride, r.partial = P->partial.as<double2>();
   if (P->every) r.hist = P->ck.as<float2>(), r.hist_every = P->every, r.nck = P->nck;
   std::vector<int2> wmap;
-  int rc = rows_prepare(&r, 1, P->rows, wmap, P->row_smem);
+  int rc = rows_prepare(&r, 1, P->rows, wmap, P->row_smem, P->row_pf);
   if (rc) return rc;
   if ((rc = upload(P->row_sys, &r, sizeof(r), P->st))) return rc;
   return upload(P->row_wmap, wmap.data(), wmap.size() * sizeof(int2), P->st);
@@ -158,7 +159,7 @@ int run_tables(nise_abs_plan *P) {
 
 int run_conveyor(nise_abs_plan *P) {
   if (P->rows)
-    return launch_rows(P->row_sys.as<RowsArgs>(), P->row_wmap.as<int2>(), P->nwg, P->rows, P->row_smem,
+    return launch_rows(P->row_sys.as<RowsArgs>(), P->row_wmap.as<int2>(), P->nwg, P->rows, P->row_pf, P->row_smem,
                        P->st);
   ConveyorArgs a;
   a.N = P->N, a.K = P->sch.K, a.nlayers = P->sch.nlayers(), a.m = P->m;

@@ -804,7 +804,8 @@ static int multi_run(const nise_system *systems, int nsys, int where, float *re_
   //    for all DIA systems), timed on the device clock
   std::vector<int2> rmap, dmap;
   size_t rsmem = 0, dsmem = 0;
-  if (!rows.empty() && (rc = rows_prepare(rows.data(), (int)rows.size(), R, rmap, rsmem))) return rc;
+  bool rpf = false;
+  if (!rows.empty() && (rc = rows_prepare(rows.data(), (int)rows.size(), R, rmap, rsmem, rpf))) return rc;
   // longest workgroups first (layers x steps x Trotter substeps of their
   // system), so that the short ones fill the end of the launch
   std::stable_sort(rmap.begin(), rmap.end(), [&](const int2 &x, const int2 &y) {
@@ -824,7 +825,7 @@ static int multi_run(const nise_system *systems, int nsys, int where, float *re_
       (rc = up(dwm, dmap.data(), dmap.size() * sizeof(int2))) || (rc = times.alloc(nt * 16 + 16)))
     return rc;
   unsigned long long *tm = times.as<unsigned long long>();
-  if (!rmap.empty() && (rc = launch_rows(rsys.as<RowsArgs>(), rwm.as<int2>(), (int)rmap.size(), R, rsmem, st, tm)))
+  if (!rmap.empty() && (rc = launch_rows(rsys.as<RowsArgs>(), rwm.as<int2>(), (int)rmap.size(), R, rpf, rsmem, st, tm)))
     return rc;
   if (!dmap.empty()) {
     NISE_HIP_CHECK(hipFuncSetAttribute((const void *)dia_conveyor_kernel,

@@ -415,9 +415,9 @@ struct RowsArgs {
   float2 *hist = nullptr;        // checkpoints (hist_every, nck per vector)
   int hist_every = 1, nck = 0;
 };
-int rows_prepare(const RowsArgs *sys, int nsys, int R, std::vector<int2> &wmap, size_t &smem);
+int rows_prepare(const RowsArgs *sys, int nsys, int R, std::vector<int2> &wmap, size_t &smem, bool &pf);
 // times (optional): per workgroup (start, end) on the device wall clock.
-int launch_rows(const RowsArgs *sys, const int2 *wmap, int nwg, int R, size_t smem, hipStream_t st,
+int launch_rows(const RowsArgs *sys, const int2 *wmap, int nwg, int R, bool pf, size_t smem, hipStream_t st,
                 unsigned long long *times = nullptr);
 
 // DIA conveyor (multi.hip): one descriptor per system, vp consecutive

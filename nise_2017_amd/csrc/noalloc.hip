@@ -191,11 +191,12 @@ extern "C" int nise_absorption_batched_noalloc(const t_non *non, const float *H_
   r.mu_rec = mu_rec, r.mu_stride = ctype ? 0 : mu_stride, r.partial = partial;
   std::vector<int2> wmap;
   size_t smem = 0;
-  if ((rc = rows_prepare(&r, 1, R, wmap, smem))) return rc;
+  bool pf = false;
+  if ((rc = rows_prepare(&r, 1, R, wmap, smem, pf))) return rc;
   if ((int)wmap.size() != nwg) return set_error("row conveyor: workgroup map mismatch"), NISE_ERR_ARG;
   NISE_HIP_CHECK(hipMemcpy(sysd, &r, sizeof(r), hipMemcpyHostToDevice));
   NISE_HIP_CHECK(hipMemcpy(wmapd, wmap.data(), wmap.size() * sizeof(int2), hipMemcpyHostToDevice));
-  if ((rc = launch_rows(sysd, wmapd, nwg, R, smem, st))) return rc;
+  if ((rc = launch_rows(sysd, wmapd, nwg, R, pf, smem, st))) return rc;
   if ((rc = launch_reduce_partials(partial, nvec, T, out, out + T, st))) return rc;
   NISE_HIP_CHECK(hipMemcpy(re_S, out, T * sizeof(float), hipMemcpyDeviceToHost));
   NISE_HIP_CHECK(hipMemcpy(im_S, out + T, T * sizeof(float), hipMemcpyDeviceToHost));

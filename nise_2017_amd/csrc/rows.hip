@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "nise_internal.h"
@@ -97,13 +98,24 @@ __device__ __forceinline__ void rot_block(unsigned char *smem, uint32_t gofs, ui
 // The workgroup's vectors wm.y .. wm.y + G - 1 of system A (wm.x).  WIDE:
 // the per-site pass's batches for few rows (site_batch); they take
 // registers, so the mixed-count kernel, whose register count all its row
-// counts share, keeps batches of 4.  (Loading each batch while the previous
-// one was used ran slower for the chains: 0.69 vs 0.60 ms at N = 16.)
-template <int R, bool WIDE = false>
+// counts share, keeps batches of 4.  PF: the launch prefetches each frame's
+// first batch (rows_prepare decides).  (Loading each batch while the
+// previous one was used, in the per-site form, ran slower for the chains:
+// 0.69 vs 0.60 ms at N = 16.)
+template <int R, bool WIDE = false, bool PF = false>
 __device__ __forceinline__ void rows_run(const RowsArgs &A, int2 wm) {
   constexpr int G = 64 / R;
   constexpr int B = G < 4 ? G : 4;  // vectors per frame group, layers per table block
   constexpr int kSiteU = WIDE ? site_batch<R>() : 4;
+  // The per-site pass in one of two forms (results identical).  Branch-free
+  // batches (every load and LDS read of a batch in flight at once) and the
+  // prefetch of each frame's first batch suit few rows, where a lane walks
+  // many sites per frame: the N = 16 chain 0.62 -> 0.38 ms, the N = 128
+  // chain 7.9 -> 5.3 ms.  With 16 rows and more (one batch per lane and
+  // frame) the per-site form measured 3% faster at the N = 256 lattice
+  // (11.9 vs 12.3 ms), and the prefetch gained nothing there.
+  constexpr bool kPerSite = WIDE && R >= 16;
+  constexpr bool kPF = PF && !kPerSite;
   constexpr int kRingS = kRingL / B;  // ring slots per lane
   static_assert(G == 1 || G == 2 || G % 4 == 0, "vectors per wave: 1, 2 or a multiple of 4");
   extern __shared__ __align__(16) unsigned char smem[];
@@ -140,6 +152,38 @@ __device__ __forceinline__ void rows_run(const RowsArgs &A, int2 wm) {
   uint2 re[kRingS];
   float2 rc[kRingS];
   int64_t ring_key = -1;  // sweep (tau * m + sub) whose first block the ring holds
+  // Per-site pass: batches of kSiteU sites, sites i0 + u R of the lane's
+  // frame; every load of a batch is issued (from valid rows) before the
+  // first is used.  load() clamps the frame to the group's frames (a
+  // finished group loads valid rows); TAIL: the batch runs past N (indexes
+  // past N load site 0 and are not applied).
+  struct SiteBatch {
+    float mv[kSiteU];
+    float2 dp[kSiteU], dc[kSiteU];
+  };
+  auto load = [&](SiteBatch &b, int64_t fr, int i0, auto tail) {
+    constexpr bool TAIL = decltype(tail)::value;
+    const int64_t fc = min(fr, Fend), rw = fc - A.f_lo;
+    const float *mu = A.mu_rec + (A.mu_stride ? fc * A.mu_stride : 0) + 1 + (int64_t)x * N;
+    const float2 *Dc = A.diag + rw * N;
+    // the previous frame's phases (rw - 1 >= 0 whenever t > 0; at t = 0 a
+    // valid row is loaded and not used)
+    const float2 *Dp = A.diag + (rw > 0 ? rw - 1 : 0) * N;
+#pragma unroll
+    for (int u = 0; u < kSiteU; ++u) {
+      const int i = i0 + u * R, ii = (!TAIL || i < N) ? i : 0;
+      b.mv[u] = gld(mu + ii);
+      b.dp[u] = gld(Dp + ii);
+      b.dc[u] = gld(Dc + ii);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the batch of loads together
+  };
+  // kPF (the branch-free form of a PF launch: N is a multiple of kSiteU R,
+  // so no batch runs past N, and no history rows): nb holds the first batch
+  // of the wave's frame, loaded before the previous frame's sweeps (or
+  // before the loop, or by a jump)
+  SiteBatch nb;
+  if constexpr (kPF) load(nb, F0, q, std::false_type{});
 
   for (int64_t tau = 0; tau < span; ++tau) {
     const int64_t f = F0 + tau;  // this lane's frame
@@ -151,6 +195,7 @@ __device__ __forceinline__ void rows_run(const RowsArgs &A, int2 wm) {
         const int64_t o = start_of(wm.y + k) - start_of(wm.y + (k & ~(B - 1)));
         if (o > tau) nt = min(nt, o);
       }
+      if constexpr (kPF) load(nb, F0 + nt, q, std::false_type{});
       tau = nt - 1;
       continue;
     }
@@ -159,7 +204,9 @@ __device__ __forceinline__ void rows_run(const RowsArgs &A, int2 wm) {
     const int64_t row = min(f, Fend) - A.f_lo;
     const bool last = t == T - 1;
     float rr = 0.f, ri = 0.f;
-    if (act) {
+    if (kPerSite && act) {
+      // per-site form: each site's LDS read and phases wait for its own
+      // loads, with t == 0 and the last frame branching around them
       const float *mu = A.mu_rec + (A.mu_stride ? f * A.mu_stride : 0) + 1 + (int64_t)x * N;
       const float2 *Dc = A.diag + row * N;
       const float2 *Dp = Dc - N;
@@ -168,13 +215,7 @@ __device__ __forceinline__ void rows_run(const RowsArgs &A, int2 wm) {
         if (t % A.hist_every == 0) hr = hist0 + t / A.hist_every;
         else if (last) hr = hist0 + t / A.hist_every + 1;
       }
-      // kSiteU sites per batch: all HBM loads of a batch are issued before
-      // the first is used (indexes past N load site 0 and are not applied)
-      struct SiteBatch {
-        float mv[kSiteU];
-        float2 dp[kSiteU], dc[kSiteU];
-      };
-      auto load = [&](SiteBatch &b, int i0) {
+      auto load_ps = [&](SiteBatch &b, int i0) {
 #pragma unroll
         for (int u = 0; u < kSiteU; ++u) {
           const int i = i0 + u * R, ii = i < N ? i : 0;
@@ -182,9 +223,9 @@ __device__ __forceinline__ void rows_run(const RowsArgs &A, int2 wm) {
           b.dp[u] = t ? gld(Dp + ii) : make_float2(0.f, 0.f);
           b.dc[u] = last ? make_float2(0.f, 0.f) : gld(Dc + ii);
         }
-        __builtin_amdgcn_sched_barrier(0);  // keep the batch of loads together
+        __builtin_amdgcn_sched_barrier(0);
       };
-      auto use = [&](const SiteBatch &b, int i0) {
+      auto use_ps = [&](const SiteBatch &b, int i0) {
 #pragma unroll
         for (int u = 0; u < kSiteU; ++u) {
           const int i = i0 + u * R;
@@ -203,9 +244,59 @@ __device__ __forceinline__ void rows_run(const RowsArgs &A, int2 wm) {
       };
       for (int i0 = q; i0 < N; i0 += kSiteU * R) {
         SiteBatch b;
-        load(b, i0);
-        use(b, i0);
+        load_ps(b, i0);
+        use_ps(b, i0);
       }
+    } else if (act) {
+      // branch-free form: t == 0 and the last frame select their values;
+      // HIST: the launch writes history rows (wave-uniform)
+      int64_t hr = -1;
+      if (hist0 >= 0) {
+        if (t % A.hist_every == 0) hr = hist0 + t / A.hist_every;
+        else if (last) hr = hist0 + t / A.hist_every + 1;
+      }
+      auto use = [&](const SiteBatch &b, int i0, auto tail, auto hist) {
+        constexpr bool TAIL = decltype(tail)::value, HIST = decltype(hist)::value;
+#pragma unroll
+        for (int u = 0; u < kSiteU; ++u) {
+          const int i = i0 + u * R;
+          if (!TAIL || i < N) {
+            const float mm = b.mv[u];
+            // both values first, then a select (a conditional expression
+            // would branch around the LDS read)
+            const float2 pv = *reinterpret_cast<const float2 *>(smem + ((uint32_t)i * G * 8u + gofs));
+            const float2 pd = cmul_diag(pv, b.dp[u]);
+            float2 p = t == 0 ? make_float2(mm, 0.f) : pd;
+            rr = __fadd_rn(rr, __fmul_rn(mm, p.x));
+            ri = __fadd_rn(ri, __fmul_rn(mm, p.y));
+            if (HIST && hr >= 0) gst(A.hist + hr * N + i, p);
+            if (!last) p = cmul_diag(p, b.dc[u]);
+            *reinterpret_cast<float2 *>(smem + ((uint32_t)i * G * 8u + gofs)) = p;
+          }
+        }
+      };
+      auto pass = [&](auto hist) {
+        int b0 = 0;
+        if constexpr (kPF) {  // the first batch is in nb
+          use(nb, q, std::false_type{}, hist);
+          b0 = kSiteU * R;
+        }
+        for (; b0 + kSiteU * R <= N; b0 += kSiteU * R) {
+          SiteBatch b;
+          load(b, f, b0 + q, std::false_type{});
+          use(b, b0 + q, std::false_type{}, hist);
+        }
+        if constexpr (!kPF) {
+          if (b0 < N) {
+            SiteBatch b;
+            load(b, f, b0 + q, std::true_type{});
+            use(b, b0 + q, std::true_type{}, hist);
+          }
+        }
+      };
+      if constexpr (PF) pass(std::false_type{});  // PF launches write no history
+      else if (A.hist) pass(std::true_type{});
+      else pass(std::false_type{});
     }
     // response of each vector at its relative time: sum over its rows
 #pragma unroll
@@ -214,6 +305,9 @@ __device__ __forceinline__ void rows_run(const RowsArgs &A, int2 wm) {
       ri += __shfl_xor(ri, off);
     }
     if (act && q == 0) gst(A.partial + (int64_t)v * T + t, make_double2((double)rr, (double)ri));
+    // kPF: the next frame's first batch, in flight during the sweeps (the
+    // frame clamped to the group's frames: every lane loads valid rows)
+    if constexpr (kPF) load(nb, f + 1, q, std::false_type{});
     if (!__ballot(act && !last)) continue;  // nothing propagates past this frame
 
     for (int sub = 0; sub < m; ++sub) {
@@ -277,15 +371,18 @@ __device__ __forceinline__ void rows_run(const RowsArgs &A, int2 wm) {
 // wmap[w].x (systems of different sizes share one launch), all systems with
 // R rows; rows_mixed_kernel takes each system's own count (sys[k].rows).
 // The descriptor is wave-uniform: it is read with scalar loads and kept in
-// registers (the kernel's own stores cannot alias it).
-template <int R>
-__global__ void __launch_bounds__(64) rows_kernel(const RowsArgs *__restrict__ sys, const int2 *__restrict__ wmap,
-                                                  unsigned long long *times) {
+// registers (the kernel's own stores cannot alias it).  At most 168 VGPRs
+// (three waves per SIMD): the prefetched batch otherwise takes the chain
+// kernels to 174-183 and two waves per SIMD (at 2 the N = 256 lattice ran
+// 16.3 ms); the few values this spills are reloaded once per frame.
+template <int R, bool PF>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+rows_kernel(const RowsArgs *__restrict__ sys, const int2 *__restrict__ wmap, unsigned long long *times) {
   const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
   const int2 wm0 = wmap[blockIdx.x];
   const int2 wm = make_int2(__builtin_amdgcn_readfirstlane(wm0.x), __builtin_amdgcn_readfirstlane(wm0.y));
   const RowsArgs A = sys[wm.x];
-  rows_run<R, true>(A, wm);
+  rows_run<R, true, PF>(A, wm);
   if (threadIdx.x == 0 && times) {  // device wall clock at start and end
     times[2 * blockIdx.x] = t_start;
     times[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
@@ -314,23 +411,37 @@ __global__ void __launch_bounds__(64) rows_mixed_kernel(const RowsArgs *__restri
 
 size_t rows_smem(int N, int R) { return (size_t)(N + R) * (64 / R) * 8; }
 
-static const void *rows_fn(int R) {
-  return R == 1 ? (const void *)rows_kernel<1>
-       : R == 2 ? (const void *)rows_kernel<2>
-       : R == 4 ? (const void *)rows_kernel<4>
-       : R == 8 ? (const void *)rows_kernel<8>
-       : R == 16 ? (const void *)rows_kernel<16>
-       : R == 32 ? (const void *)rows_kernel<32>
-       : R == 64 ? (const void *)rows_kernel<64> : nullptr;
+template <bool PF>
+static const void *rows_fn_pf(int R) {
+  return R == 1 ? (const void *)rows_kernel<1, PF>
+       : R == 2 ? (const void *)rows_kernel<2, PF>
+       : R == 4 ? (const void *)rows_kernel<4, PF>
+       : R == 8 ? (const void *)rows_kernel<8, PF>
+       : R == 16 ? (const void *)rows_kernel<16, PF>
+       : R == 32 ? (const void *)rows_kernel<32, PF>
+       : R == 64 ? (const void *)rows_kernel<64, PF> : nullptr;
+}
+static const void *rows_fn(int R, bool pf = false) { return pf ? rows_fn_pf<true>(R) : rows_fn_pf<false>(R); }
+// the sites a wave's first per-site batch spans (site_batch(R) * R)
+static int site_span(int R) {
+  const int u = R == 1 ? site_batch<1>() : R == 2 ? site_batch<2>() : R == 4 ? site_batch<4>()
+              : R == 8 ? site_batch<8>() : R == 16 ? site_batch<16>() : R == 32 ? site_batch<32>()
+              : site_batch<64>();
+  return u * R;
 }
 
 // Checks the systems against the row count (R = 0: each system's own,
 // sys[k].rows) and builds the workgroup map (G = 64 / R consecutive vectors
 // of one system per workgroup) and the LDS size.
-int rows_prepare(const RowsArgs *sys, int nsys, int R, std::vector<int2> &wmap, size_t &smem) {
+// pf: R > 0, every system's N a multiple of the per-site batch span and
+// no history rows (the launch then takes the kernel that prefetches each
+// frame's first batch).
+int rows_prepare(const RowsArgs *sys, int nsys, int R, std::vector<int2> &wmap, size_t &smem, bool &pf) {
   if (R && !rows_fn(R)) return set_error("row conveyor: unsupported row count %d", R), NISE_ERR_ARG;
   smem = 0;
   wmap.clear();
+  pf = R > 0;
+  for (int k = 0; k < nsys; ++k) pf = pf && sys[k].N % site_span(R) == 0 && !sys[k].hist;
   for (int k = 0; k < nsys; ++k) {
     const RowsArgs &a = sys[k];
     const int Rk = R ? R : a.rows;
@@ -347,10 +458,10 @@ int rows_prepare(const RowsArgs *sys, int nsys, int R, std::vector<int2> &wmap, 
 }
 
 // sys / wmap: device copies (rows_prepare), nwg workgroups; R = 0: each
-// system's own row count.
-int launch_rows(const RowsArgs *sys, const int2 *wmap, int nwg, int R, size_t smem, hipStream_t st,
+// system's own row count; pf as rows_prepare set it.
+int launch_rows(const RowsArgs *sys, const int2 *wmap, int nwg, int R, bool pf, size_t smem, hipStream_t st,
                 unsigned long long *times) {
-  const void *fn = R ? rows_fn(R) : (const void *)rows_mixed_kernel;
+  const void *fn = R ? rows_fn(R, pf) : (const void *)rows_mixed_kernel;
   if (!fn) return set_error("row conveyor: unsupported row count %d", R), NISE_ERR_ARG;
   if (nwg == 0) return NISE_OK;
   NISE_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
