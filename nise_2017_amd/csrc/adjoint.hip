@@ -222,6 +222,11 @@ __global__ void __launch_bounds__(64) adjoint_rows_kernel(AdjRowsArgs A) {
   constexpr int G = 64 / R;
   static_assert(G >= 4, "a row needs a whole quad of lanes");
   constexpr int kAdjSiteU = adj_site_batch<R>();
+  // Site pass without per-site branches for one and two rows (a lane walks
+  // N / R sites per frame; the N = 128 chain's adjoint ran 15.4 ms with the
+  // branches); wider rows keep the branches, which take fewer registers
+  // (at 4-16 rows 140-162 VGPRs against 177-198).
+  constexpr bool kAdjSelect = R <= 2;
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = A.N, T = A.T, nl = A.nlayers, NS = N + R, m = A.m;
   const int64_t K = A.K;
@@ -268,7 +273,10 @@ __global__ void __launch_bounds__(64) adjoint_rows_kernel(AdjRowsArgs A) {
     const bool reset = bfin && (tb % A.every == 0);
     // response and checkpoint of the closing phase of step t+1
     const float2 gv = bfin ? gld(A.g + tb) : make_float2(0.f, 0.f);
-    const float *mu = A.mu_rec + (A.mu_stride ? (f + 1) * A.mu_stride : 0) + 1 + (int64_t)x * N;
+    // (f + 1 clamped to the quad's frames F0..Fmax: lanes outside their
+    // window load valid rows they do not use)
+    const int64_t fmu = max(min(f + 1, Fmax), F0);
+    const float *mu = A.mu_rec + (A.mu_stride ? fmu * A.mu_stride : 0) + 1 + (int64_t)x * N;
     const int64_t ckr = ck0 + (reset ? tb / A.every : 0);
     const float2 *Da = A.diag + row * N;
     const float2 *Db = sub == 0 ? A.diag + (max(min(f + 1, Fmax), F0) - A.f_lo) * N : Da;
@@ -280,35 +288,63 @@ __global__ void __launch_bounds__(64) adjoint_rows_kernel(AdjRowsArgs A) {
         const int i = i0 + u * R, ii = i < N ? i : 0;
         da[u] = gld(Da + ii);
         db[u] = gld(Db + ii);
-        mv[u] = bfin ? gld(mu + ii) : 0.f;
-        cv[u] = reset ? gld(A.ck + ckr * N + ii) : make_float2(0.f, 0.f);
+        mv[u] = gld(mu + ii);
+        cv[u] = reset ? gld(A.ck + ckr * N + ii) : make_float2(0.f, 0.f);  // padding lanes have no rows
       }
       __builtin_amdgcn_sched_barrier(0);
       float ccu[kAdjSiteU];
+      if constexpr (kAdjSelect) {
+        // no branch per site: both phases are computed and selected, so the
+        // LDS reads of a batch are in flight together (the selected values
+        // are those of the branches: (0 + x) and x agree, -0 included)
 #pragma unroll
-      for (int u = 0; u < kAdjSiteU; ++u) {
-        const int i = i0 + u * R;
-        const bool ok = i < N;
-        const int is = ok ? i : q;
-        float2 p = phi[is * G + g], c = chi[is * G + g];
-        float cc = 0.f;
-        if (bflag) {
-          cc = hf * im_cj(c, p);
-          p = cmul_conj_d(p, db[u]);
-          c = cmul_conj_d(c, db[u]);
-          if (bfin) {
-            c.x = __fadd_rn(c.x, __fmul_rn(gv.x, mv[u]));
-            c.y = __fadd_rn(c.y, __fmul_rn(gv.y, mv[u]));
-            if (reset) p = cv[u];
+        for (int u = 0; u < kAdjSiteU; ++u) {
+          const int i = i0 + u * R;
+          const bool ok = i < N;
+          const int is = ok ? i : q;
+          const float2 p0 = phi[is * G + g], c0 = chi[is * G + g];
+          // B: closing phase of the previous sweep
+          const float ccb = hf * im_cj(c0, p0);
+          const float2 pb = cmul_conj_d(p0, db[u]);
+          float2 cb = cmul_conj_d(c0, db[u]);
+          const float2 cbf = make_float2(__fadd_rn(cb.x, __fmul_rn(gv.x, mv[u])), __fadd_rn(cb.y, __fmul_rn(gv.y, mv[u])));
+          cb = bfin ? cbf : cb;
+          const float2 p1 = bflag ? (reset ? cv[u] : pb) : p0;
+          const float2 c1 = bflag ? cb : c0;
+          const float cc1 = bflag ? ccb : 0.f;
+          // A: opening phase of this sweep
+          const float cca = hf * im_cj(c1, p1);
+          const float2 pa = cmul_conj_d(p1, da[u]), ca = cmul_conj_d(c1, da[u]);
+          const float2 p = step ? pa : p1, c = step ? ca : c1;
+          ccu[u] = step ? cc1 + cca : cc1;
+          if (ok) phi[is * G + g] = p, chi[is * G + g] = c;
+        }
+      } else {  // per-site branches
+#pragma unroll
+        for (int u = 0; u < kAdjSiteU; ++u) {
+          const int i = i0 + u * R;
+          const bool ok = i < N;
+          const int is = ok ? i : q;
+          float2 p = phi[is * G + g], c = chi[is * G + g];
+          float cc = 0.f;
+          if (bflag) {
+            cc = hf * im_cj(c, p);
+            p = cmul_conj_d(p, db[u]);
+            c = cmul_conj_d(c, db[u]);
+            if (bfin) {
+              c.x = __fadd_rn(c.x, __fmul_rn(gv.x, mv[u]));
+              c.y = __fadd_rn(c.y, __fmul_rn(gv.y, mv[u]));
+              if (reset) p = cv[u];
+            }
           }
+          if (step) {
+            cc += hf * im_cj(c, p);
+            p = cmul_conj_d(p, da[u]);
+            c = cmul_conj_d(c, da[u]);
+          }
+          if (ok) phi[is * G + g] = p, chi[is * G + g] = c;
+          ccu[u] = cc;
         }
-        if (step) {
-          cc += hf * im_cj(c, p);
-          p = cmul_conj_d(p, da[u]);
-          c = cmul_conj_d(c, da[u]);
-        }
-        if (ok) phi[is * G + g] = p, chi[is * G + g] = c;
-        ccu[u] = cc;
       }
       // each site's sum over the wave's vectors, the butterflies of the
       // batch's sites interleaved (each site's own order unchanged)
