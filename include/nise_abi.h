@@ -416,13 +416,13 @@ int nise_absorption_multi(const nise_system *systems, int nsys, int where,
                           float *re_S, float *im_S, double *done_ms);
 
 /* Autotuner of the multi-system batch: runs the batch once, then times it
- * with each system's own row count (code 0: the power of two nearest its
- * mean layer width, up to 64) and with every common row count of the row
- * conveyor (1, 2, 4, ..., 64; best of two runs, counts the batch cannot take
- * skipped) and keeps the fastest for later nise_absorption_multi calls of
- * this thread on a batch of the same shape (per system N, tmax, samples,
- * spacing, scheme, Trotter steps); without it each system takes its own
- * count.  R_best receives the code kept; ms / codes (n entries, optional)
+ * with each system's own row count (the power of two nearest its mean layer
+ * width, up to 32: code 0; up to 16 or 64: codes -16, -64) and with every
+ * common row count of the row conveyor (1, 2, 4, ..., 64; best of two runs,
+ * counts the batch cannot take skipped) and keeps the fastest for later
+ * nise_absorption_multi calls of this thread on a batch of the same shape
+ * (per system N, tmax, samples, spacing, scheme, Trotter steps); without it
+ * each system takes its own count, up to 32.  R_best receives the code kept; ms / codes (n entries, optional)
  * each candidate's host wall time per call and its code.  Returns the number
  * of candidates timed, or minus an error code.  No counterpart in the
  * reference (its static work split, src/MPI_subs.c:36-110). */

@@ -315,12 +315,12 @@ def absorption_multi_tune(systems, where=None):
         arr[k] = NiseSystem(C.pointer(non), ptr(H), H.shape[1], ptr(mu), mu.shape[1], H.shape[0], ptr(Cr))
         keep.append(non)
     best = C.c_int()
-    ms = (C.c_float * 8)()
-    codes = (C.c_int * 8)()
-    k = lib().nise_absorption_multi_tune(C.cast(arr, C.c_void_p), len(systems), where, C.byref(best), ms, codes, 8)
+    ms = (C.c_float * 16)()
+    codes = (C.c_int * 16)()
+    k = lib().nise_absorption_multi_tune(C.cast(arr, C.c_void_p), len(systems), where, C.byref(best), ms, codes, 16)
     if k < 0:
         check(-k)
-    return best.value, [(codes[i], ms[i]) for i in range(k)]
+    return best.value, [(codes[i], ms[i]) for i in range(min(k, 16))]
 
 
 def twodir_raman_batched(non, H, mu, alpha, nframes=None, coupling_rec=None, where=None):

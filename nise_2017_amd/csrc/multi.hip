@@ -597,8 +597,15 @@ struct MultiCache {
   std::vector<ExpmJob> ejobs_h;  // the job table of the last upload
   // the row count nise_absorption_multi_tune chose for a batch of this shape
   std::vector<int64_t> tune_sig;
-  int tune_R = 0;  // 0: none; > 0 that count for every system; -1 each its own
+  // 0: none; > 0 that count for every system; -1 each its own (capped at
+  // kMultiRowCap); < -1 each its own capped at -tune_R
+  int tune_R = 0;
 };
+// Cap of a system's own row count: 32 took the 96-system batch from 71.0 to
+// 65.5 ms against 64 (the largest systems, N = 900-1000, then take 32 rows
+// and two vectors per wave instead of one: twice the layers, half the
+// waves, and half the VMEM loads per layer-vector).
+constexpr int kMultiRowCap = 32;
 thread_local MultiCache g_multi;
 
 // The shape of a batch: per system N, tmax, samples, spacing, scheme, Trotter.
@@ -617,7 +624,8 @@ void batch_signature(const nise_system *systems, int nsys, std::vector<int64_t> 
 using namespace nise;
 
 // One batch; R_force > 0 runs every Trotter system with that row count,
-// R_force < 0 each with its own; R_force = 0 takes the tuned choice for a
+// R_force = -1 each with its own (capped at kMultiRowCap), R_force < -1 each
+// with its own capped at -R_force; R_force = 0 takes the tuned choice for a
 // batch of this shape, else each system's own count.
 static int multi_run(const nise_system *systems, int nsys, int where, float *re_S, float *im_S, double *done_ms,
                      int R_force) {
@@ -760,11 +768,15 @@ static int multi_run(const nise_system *systems, int nsys, int where, float *re_
   // the power of two nearest its mean layer width up to 64 (systems with
   // many couplings per layer take wide layers, which shortens the chain of
   // layers each of their workgroups walks)
-  int R = 0;
+  int R = 0, cap = kMultiRowCap;
   std::vector<int64_t> sig;
   if (R_force == 0 && cache.tune_R != 0) batch_signature(systems, nsys, sig);
   if (R_force > 0) R = R_force;
-  else if (R_force == 0 && cache.tune_R > 0 && sig == cache.tune_sig) R = cache.tune_R;
+  else if (R_force < -1) cap = -R_force;
+  else if (R_force == 0 && cache.tune_R != 0 && sig == cache.tune_sig) {
+    if (cache.tune_R > 0) R = cache.tune_R;
+    else if (cache.tune_R < -1) cap = -cache.tune_R;
+  }
   // 4. tables of the Trotter systems
   for (int k = 0; k < nsys; ++k) {
     SysState &s = *S[k];
@@ -772,7 +784,7 @@ static int multi_run(const nise_system *systems, int nsys, int where, float *re_
     const int N = s.N;
     const int64_t nfr = s.f_hi - s.f_lo;
     if (s.dia) continue;
-      const int Rk = R > 0 ? R : rows_for(N, s.sch_asap.Kreal, s.sch_asap.nlayers(), 64);
+      const int Rk = R > 0 ? R : rows_for(N, s.sch_asap.Kreal, s.sch_asap.nlayers(), cap);
       if (s.rows_N != N || s.rows_R != Rk || s.rows_mask != s.mask) {
         s.rows_N = 0;  // until the tables below are on the device
         build_schedule_rows(N, s.mask, Rk, s.sch);
@@ -891,11 +903,11 @@ extern "C" int nise_absorption_multi(const nise_system *systems, int nsys, int w
   return multi_run(systems, nsys, where, re_S, im_S, done_ms, 0);
 }
 
-// Times the batch with each system's own row count and with every common
-// row count of the row conveyor (1, 2, ..., 64;
-// the best of two runs each, after a warm run) and keeps the fastest for
-// later nise_absorption_multi calls on a batch of the same shape in this
-// thread.
+// Times the batch with each system's own row count (capped at 32, 16 and
+// 64: codes 0, -16, -64) and with every common row count of the row
+// conveyor (1, 2, ..., 64; the best of two runs each, after a warm run) and
+// keeps the fastest for later nise_absorption_multi calls on a batch of the
+// same shape in this thread.
 extern "C" int nise_absorption_multi_tune(const nise_system *systems, int nsys, int where, int *R_best, float *ms,
                                           int *codes, int n) {
   int rc = ensure_device();
@@ -908,11 +920,11 @@ extern "C" int nise_absorption_multi_tune(const nise_system *systems, int nsys, 
   if ((rc = multi_run(systems, nsys, where, re.data(), im.data(), nullptr, 0))) return rc;
   int best = -2, cnt = 0;
   double best_ms = 0;
-  for (int R : {0, 1, 2, 4, 8, 16, 32, 64}) {  // 0: each system its own count
+  for (int R : {0, -16, -64, 1, 2, 4, 8, 16, 32, 64}) {  // <= 0: each system its own count
     double t = -1;
     for (int rep = 0; rep < 2; ++rep) {
       const auto t0 = std::chrono::steady_clock::now();
-      const int r = multi_run(systems, nsys, where, re.data(), im.data(), nullptr, R ? R : -1);
+      const int r = multi_run(systems, nsys, where, re.data(), im.data(), nullptr, R ? R : -1);  // -1: cap 32
       const double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
       if (r) {  // a row count the batch cannot take (LDS): skipped
         t = -1;
@@ -927,7 +939,7 @@ extern "C" int nise_absorption_multi_tune(const nise_system *systems, int nsys, 
   }
   if (best == -2) return set_error("multi tune: no row count ran"), NISE_ERR_ARG;
   batch_signature(systems, nsys, g_multi.tune_sig);
-  g_multi.tune_R = best ? best : -1;
+  g_multi.tune_R = best ? best : -1;  // -16, -64: those caps
   if (R_best) *R_best = best;
   // one run with the kept choice, so that the next call finds its schedules
   // and tables in the cache (the last candidate timed left its own there)

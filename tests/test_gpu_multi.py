@@ -71,7 +71,8 @@ def test_tuned_row_count(orc):
     systems = [system(N, prop, seed, T=32, ns=8, sr=5) for N, prop, seed in specs]
     best, cands = abi.absorption_multi_tune(systems)
     say(f"tuned row count {best}: {cands}")
-    assert best in (0, 1, 2, 4, 8, 16, 32, 64) and len(cands) >= 2 and best == min(cands, key=lambda c: c[1])[0]
+    assert best in (0, -16, -64, 1, 2, 4, 8, 16, 32, 64) and len(cands) >= 2
+    assert best == min(cands, key=lambda c: c[1])[0]
     out, _ = abi.absorption_multi(systems)
     for (non, H, mu, Cr), (re, im) in zip(systems, out):
         o_re, o_im = orc.absorption(non, H, mu, C_rec=Cr)
