@@ -128,12 +128,26 @@ __device__ __forceinline__ void tile_store(double *X, const double q[4][4], cons
       if (g.in(i, j)) X[g.at(i, j)] = q[i][j];
 }
 
+// 1 / n! for n < 32, by the divisions 1 / 2 / 3 / ... / n in turn (the
+// values the series coefficients had when each thread divided at run time).
+struct InvFact {
+  double v[32];
+  constexpr InvFact() : v() {
+    for (int n = 0; n < 32; ++n) {
+      double c = 1.0;
+      for (int d = 2; d <= n; ++d) c /= d;
+      v[n] = c;
+    }
+  }
+};
+__constant__ constexpr InvFact kInvFact{};
+
 // Coefficient k of the cos (odd = 0) or sin (odd = 1) series in B = A^2:
-// (-1)^k / (2k + odd)!, zero past the last power kept.
+// (-1)^k / (2k + odd)!, zero past the last power kept (K < 30, so
+// 2k + odd < 32).
 __device__ __forceinline__ double trig_coef(int k, int odd, int kmax) {
   if (k > kmax) return 0.0;
-  double c = 1.0;
-  for (int n = 2; n <= 2 * k + odd; ++n) c /= n;
+  const double c = kInvFact.v[2 * k + odd];
   return (k & 1) ? -c : c;
 }
 
@@ -193,10 +207,35 @@ __device__ __forceinline__ void expm_frame(const FrameSource &src, int N, int64_
   double *A = xs, *B = A + NL, *B2 = B + NL, *X = B2 + NL;
   double *red = X + NL;  // [4]
   const ExpmTile g{N, L, t >> 4, t & 15};
-  for (int e = t; e < N * N; e += 256) {
-    const int a = e / N, b = e - a * N;
-    const float h = a == b ? site_energy(src, N, f, a) : he_coupling(src, N, f, a < b ? a : b, a < b ? b : a);
-    A[a * L + b] = (double)h * (double)fac;
+  if (!src.onfly) {
+    // couplings and energies from the records: every load of the thread in
+    // flight before the first use (per element, with the on-the-fly
+    // coupling code beside it, each load's latency showed), indexes past
+    // the matrix clamped to its last element and not stored
+    constexpr int kLd = (kExpmMaxN * kExpmMaxN + 255) / 256;
+    const float *rec = src.H_rec + f * src.H_stride + 1;
+    const float *crec = src.coupling_type ? src.C_rec + 1 : rec;
+    float hv[kLd];
+#pragma unroll
+    for (int u = 0; u < kLd; ++u) {
+      const int e = min(t + 256 * u, N * N - 1), a = e / N, b = e - a * N;
+      hv[u] = a == b ? gld(rec + (src.coupling_type ? a : sindex(a, a, N)))
+                     : gld(crec + sindex(a < b ? a : b, a < b ? b : a, N));
+    }
+#pragma unroll
+    for (int u = 0; u < kLd; ++u) {
+      const int e = t + 256 * u, a = e / N, b = e - a * N;
+      if (e < N * N) {
+        const float h = a == b ? shift_diag(src, a, hv[u]) : (fabsf(hv[u]) < src.cut ? 0.f : hv[u]);
+        A[a * L + b] = (double)h * (double)fac;
+      }
+    }
+  } else {
+    for (int e = t; e < N * N; e += 256) {
+      const int a = e / N, b = e - a * N;
+      const float h = a == b ? site_energy(src, N, f, a) : he_coupling(src, N, f, a < b ? a : b, a < b ? b : a);
+      A[a * L + b] = (double)h * (double)fac;
+    }
   }
   __syncthreads();
   // 1-norm (largest column sum of |A|)
@@ -837,6 +876,17 @@ static int multi_run(const nise_system *systems, int nsys, int where, float *re_
       (rc = up(dwm, dmap.data(), dmap.size() * sizeof(int2))) || (rc = times.alloc(nt * 16 + 16)))
     return rc;
   unsigned long long *tm = times.as<unsigned long long>();
+  // The row conveyor starts after the matrix exponentials: its long-lived
+  // waves fill the CUs' LDS, and an exponential launch queued behind them
+  // (75-133 KB of LDS per workgroup) waited for them to drain (the N = 33-48
+  // class took 50 ms instead of 0.6; the batch 65.6 -> 64.2 ms with the wait).
+  {
+    hipEvent_t ev;
+    NISE_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipError_t e1 = hipEventRecord(ev, sd), e2 = e1 == hipSuccess ? hipStreamWaitEvent(st, ev, 0) : e1;
+    (void)hipEventDestroy(ev);
+    NISE_HIP_CHECK(e2);
+  }
   if (!rmap.empty() && (rc = launch_rows(rsys.as<RowsArgs>(), rwm.as<int2>(), (int)rmap.size(), R, rpf, rsmem, st, tm)))
     return rc;
   if (!dmap.empty()) {

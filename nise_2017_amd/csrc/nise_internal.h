@@ -173,13 +173,16 @@ __device__ inline float raw_coupling(const FrameSource &s, int N, int64_t f, int
 // Diagonal element as read_Hamiltonian leaves it: minus shifte
 // (src/read_trajectory.c:153,176,337), then the single-site shifts
 // (src/hamshift.c:59-66) in list order.
-__device__ inline float site_energy(const FrameSource &s, int N, int64_t f, int a) {
-  float h = __fsub_rn(raw_diag(s, N, f, a), s.shifte);
+__device__ inline float shift_diag(const FrameSource &s, int a, float raw) {
+  float h = __fsub_rn(raw, s.shifte);
   if (s.apply_shift) {
     for (int i = 0; i < s.nshift; ++i)
       if (s.shift_site[i] == a) h = __fadd_rn(__fmul_rn(h, s.shift_ab[2 * i]), s.shift_ab[2 * i + 1]);
   }
   return h;
+}
+__device__ inline float site_energy(const FrameSource &s, int N, int64_t f, int a) {
+  return shift_diag(s, a, raw_diag(s, N, f, a));
 }
 
 // Coupling as kept by read_He (|J| < cut -> 0, src/read_trajectory.c:305-311)
