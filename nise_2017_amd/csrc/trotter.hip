@@ -261,21 +261,22 @@ __global__ void __launch_bounds__(BLOCK) conveyor_kernel(ConveyorArgs A) {
     const int64_t hr = hist_row(sd, rel, resp, A.hist_every);
     if (starts || prev_prop || resp || prop) {
       for (int i = q; i < N; i += kRows) {
-        float2 p;
-        if (starts) {
-          p = (sd.init == 0) ? make_float2(mu[i], 0.f) : A.state[(int64_t)sd.out * N + i];
-        } else {
-          p = psi[i * VP + v];
-          if (prev_prop) p = cmul_diag(p, Dprev[i]);
-        }
-        if (resp) {
-          const float m = mu[i];
-          rr = __fadd_rn(rr, __fmul_rn(m, p.x));
-          ri = __fadd_rn(ri, __fmul_rn(m, p.y));
-          if (hr >= 0) A.hist[hr * N + i] = p;
-        }
-        if (prop) p = cmul_diag(p, Dcur[i]);
-        psi[i * VP + v] = p;
+        // the LDS values and both half-phases first, then selects: branches
+        // around the LDS read made each site one LDS round trip (a starting
+        // slot's state load and the history store stay conditional)
+        const float2 ps = psi[i * VP + v];
+        const float m = mu[i];
+        const float2 pd = cmul_diag(ps, Dprev[i]);
+        float2 sv = make_float2(m, 0.f);
+        if (starts && sd.init != 0) sv = A.state[(int64_t)sd.out * N + i];
+        float2 p = prev_prop ? pd : ps;
+        p = starts ? sv : p;
+        if (resp && hr >= 0) A.hist[hr * N + i] = p;
+        const float rx = __fadd_rn(rr, __fmul_rn(m, p.x)), ry = __fadd_rn(ri, __fmul_rn(m, p.y));
+        rr = resp ? rx : rr;
+        ri = resp ? ry : ri;
+        const float2 pc = cmul_diag(p, Dcur[i]);
+        psi[i * VP + v] = prop ? pc : p;
       }
     }
     if (A.tmax_acc > 0) {
@@ -477,21 +478,22 @@ __global__ void __launch_bounds__(64) conveyor_w1_kernel(ConveyorArgs A) {
     const int64_t hr = hist_row(sd, rel, resp, A.hist_every);
     if (starts || prev_prop || resp || prop) {
       for (int i = q; i < N; i += kRows) {
-        float2 p;
-        if (starts) {
-          p = (sd.init == 0) ? make_float2(mu[i], 0.f) : A.state[(int64_t)sd.out * N + i];
-        } else {
-          p = psi[i * VP + v];
-          if (prev_prop) p = cmul_diag(p, Dprev[i]);
-        }
-        if (resp) {
-          const float m = mu[i];
-          rr = __fadd_rn(rr, __fmul_rn(m, p.x));
-          ri = __fadd_rn(ri, __fmul_rn(m, p.y));
-          if (hr >= 0) A.hist[hr * N + i] = p;
-        }
-        if (prop) p = cmul_diag(p, Dcur[i]);
-        psi[i * VP + v] = p;
+        // the LDS values and both half-phases first, then selects: branches
+        // around the LDS read made each site one LDS round trip (a starting
+        // slot's state load and the history store stay conditional)
+        const float2 ps = psi[i * VP + v];
+        const float m = mu[i];
+        const float2 pd = cmul_diag(ps, Dprev[i]);
+        float2 sv = make_float2(m, 0.f);
+        if (starts && sd.init != 0) sv = A.state[(int64_t)sd.out * N + i];
+        float2 p = prev_prop ? pd : ps;
+        p = starts ? sv : p;
+        if (resp && hr >= 0) A.hist[hr * N + i] = p;
+        const float rx = __fadd_rn(rr, __fmul_rn(m, p.x)), ry = __fadd_rn(ri, __fmul_rn(m, p.y));
+        rr = resp ? rx : rr;
+        ri = resp ? ry : ri;
+        const float2 pc = cmul_diag(p, Dcur[i]);
+        psi[i * VP + v] = prop ? pc : p;
       }
     }
     if (A.tmax_acc > 0) {
