@@ -546,7 +546,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the sharding plan across GPUs is described in DESIGN.md)", file=sys.stderr)
+              f"(one GPU's share of the sample range is the unit that would shard)", file=sys.stderr)
         sys.exit(2)
 
     import ctypes as C
