@@ -153,9 +153,9 @@ __device__ __forceinline__ double trig_coef(int k, int odd, int kmax) {
 
 // R = p(B) = sum_{k <= kmax} coef_k B^k for the thread's elements, by Horner
 // in B^2 over the chunks coef_2j I + coef_2j+1 B (the top chunk also takes
-// B^2); X is an LDS buffer for the running value.  Every iteration ends with
-// a barrier.
-__device__ void trig_poly(const double *B, const double *B2, double *X, const ExpmTile &g, int odd, int kmax,
+// B^2); Bv: the thread's elements of B; B2 (LDS) the product operand; X an
+// LDS buffer for the running value.  Every iteration ends with a barrier.
+__device__ void trig_poly(const double Bv[4][4], const double *B2, double *X, const ExpmTile &g, int odd, int kmax,
                           double R[4][4]) {
   const int J = kmax >= 1 ? (kmax - 1) / 2 : 0;  // top chunk: powers 2J .. kmax (at most 3)
   auto chunk = [&](int j, int top, double Y[4][4]) {
@@ -167,7 +167,7 @@ __device__ void trig_poly(const double *B, const double *B2, double *X, const Ex
       for (int l = 0; l < 4; ++l) {
         const bool in = g.in(i, l);
         const int e = in ? g.at(i, l) : 0;
-        Y[i][l] = (g.ry + 16 * i == g.cx + 16 * l ? a0 : 0.0) + (in ? a1 * B[e] + a2 * B2[e] : 0.0);
+        Y[i][l] = (g.ry + 16 * i == g.cx + 16 * l ? a0 : 0.0) + (in ? a1 * Bv[i][l] + a2 * B2[e] : 0.0);
       }
   };
   chunk(J, 1, R);
@@ -199,50 +199,58 @@ struct ExpmJob {
 __host__ __device__ inline int64_t expm_per(int N, int pairs) {
   return pairs ? (int64_t)2 * ((N + 1) >> 1) * N : (int64_t)N * N;
 }
-__host__ __device__ inline size_t expm_smem(int N) { return (size_t)4 * N * (N + 1) * 8 + 64; }
+__host__ __device__ inline size_t expm_smem(int N) { return (size_t)2 * N * (N + 1) * 8 + 64; }
 // U of frame f into Um (the frame's block of U), xs: expm_smem(N) bytes.
 __device__ __forceinline__ void expm_frame(const FrameSource &src, int N, int64_t f, float fac, float thres,
                                            int pairs, float2 *Um, double *xs) {
+  // Two N x N matrices in LDS (M0, M1): A, then B = A^2 (the thread's
+  // elements kept in registers), B^2 and the running value of the series,
+  // then A again (reloaded from the records) for sin A = A (sin A / A).
+  // Four matrices took 133 KB at N = 64, one workgroup per CU.
   const int t = threadIdx.x, L = N + 1, NL = N * L;
-  double *A = xs, *B = A + NL, *B2 = B + NL, *X = B2 + NL;
-  double *red = X + NL;  // [4]
+  double *M0 = xs, *M1 = M0 + NL;
+  double *red = M1 + NL;  // [4]
   const ExpmTile g{N, L, t >> 4, t & 15};
-  if (!src.onfly) {
-    // couplings and energies from the records: every load of the thread in
-    // flight before the first use (per element, with the on-the-fly
-    // coupling code beside it, each load's latency showed), indexes past
-    // the matrix clamped to its last element and not stored
-    constexpr int kLd = (kExpmMaxN * kExpmMaxN + 255) / 256;
-    const float *rec = src.H_rec + f * src.H_stride + 1;
-    const float *crec = src.coupling_type ? src.C_rec + 1 : rec;
-    float hv[kLd];
+  // M[a][b] = (H_ab fac) mul
+  auto fill = [&](double *M, double mul) {
+    if (!src.onfly) {
+      // couplings and energies from the records: every load of the thread
+      // in flight before the first use (per element, with the on-the-fly
+      // coupling code beside it, each load's latency showed), indexes past
+      // the matrix clamped to its last element and not stored
+      constexpr int kLd = (kExpmMaxN * kExpmMaxN + 255) / 256;
+      const float *rec = src.H_rec + f * src.H_stride + 1;
+      const float *crec = src.coupling_type ? src.C_rec + 1 : rec;
+      float hv[kLd];
 #pragma unroll
-    for (int u = 0; u < kLd; ++u) {
-      const int e = min(t + 256 * u, N * N - 1), a = e / N, b = e - a * N;
-      hv[u] = a == b ? gld(rec + (src.coupling_type ? a : sindex(a, a, N)))
-                     : gld(crec + sindex(a < b ? a : b, a < b ? b : a, N));
-    }
+      for (int u = 0; u < kLd; ++u) {
+        const int e = min(t + 256 * u, N * N - 1), a = e / N, b = e - a * N;
+        hv[u] = a == b ? gld(rec + (src.coupling_type ? a : sindex(a, a, N)))
+                       : gld(crec + sindex(a < b ? a : b, a < b ? b : a, N));
+      }
 #pragma unroll
-    for (int u = 0; u < kLd; ++u) {
-      const int e = t + 256 * u, a = e / N, b = e - a * N;
-      if (e < N * N) {
-        const float h = a == b ? shift_diag(src, a, hv[u]) : (fabsf(hv[u]) < src.cut ? 0.f : hv[u]);
-        A[a * L + b] = (double)h * (double)fac;
+      for (int u = 0; u < kLd; ++u) {
+        const int e = t + 256 * u, a = e / N, b = e - a * N;
+        if (e < N * N) {
+          const float h = a == b ? shift_diag(src, a, hv[u]) : (fabsf(hv[u]) < src.cut ? 0.f : hv[u]);
+          M[a * L + b] = ((double)h * (double)fac) * mul;
+        }
+      }
+    } else {
+      for (int e = t; e < N * N; e += 256) {
+        const int a = e / N, b = e - a * N;
+        const float h = a == b ? site_energy(src, N, f, a) : he_coupling(src, N, f, a < b ? a : b, a < b ? b : a);
+        M[a * L + b] = ((double)h * (double)fac) * mul;
       }
     }
-  } else {
-    for (int e = t; e < N * N; e += 256) {
-      const int a = e / N, b = e - a * N;
-      const float h = a == b ? site_energy(src, N, f, a) : he_coupling(src, N, f, a < b ? a : b, a < b ? b : a);
-      A[a * L + b] = (double)h * (double)fac;
-    }
-  }
+  };
+  fill(M0, 1.0);
   __syncthreads();
   // 1-norm (largest column sum of |A|)
   double cm = 0.0;
   for (int c = t; c < N; c += 256) {
     double sum = 0.0;
-    for (int r = 0; r < N; ++r) sum += fabs(A[r * L + c]);
+    for (int r = 0; r < N; ++r) sum += fabs(M0[r * L + c]);
     cm = fmax(cm, sum);
   }
   for (int o = 32; o > 0; o >>= 1) cm = fmax(cm, __shfl_xor(cm, o));
@@ -257,25 +265,29 @@ __device__ __forceinline__ void expm_frame(const FrameSource &src, int N, int64_
   __syncthreads();  // every thread has read red
   for (int e = t; e < N * N; e += 256) {
     const int a = e / N, b = e - a * N;
-    A[a * L + b] *= scale;
+    M0[a * L + b] *= scale;
   }
   __syncthreads();
-  double q[4][4];
-  tile_mm(A, A, g, q), tile_store(B, q, g);
+  double q[4][4], bv[4][4];
+  tile_mm(M0, M0, g, bv), tile_store(M1, bv, g);  // B
   __syncthreads();
-  tile_mm(B, B, g, q), tile_store(B2, q, g);
+  tile_mm(M1, M1, g, q);  // B^2
+  __syncthreads();        // every thread has read M0 (A) and M1 (B)
+  tile_store(M0, q, g);
   __syncthreads();
   double ur[4][4], ui[4][4];
-  trig_poly(B, B2, X, g, 0, K / 2, ur);        // cos A
-  trig_poly(B, B2, X, g, 1, (K - 1) / 2, ui);  // sin A / A
-  tile_store(X, ui, g);
+  trig_poly(bv, M0, M1, g, 0, K / 2, ur);        // cos A
+  trig_poly(bv, M0, M1, g, 1, (K - 1) / 2, ui);  // sin A / A
+  __syncthreads();  // the products have read M0 (B^2) and M1
+  fill(M0, scale);  // A
+  tile_store(M1, ui, g);
   __syncthreads();
-  tile_mm(A, X, g, q);
+  tile_mm(M0, M1, g, q);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) ui[i][j] = -q[i][j];  // exp(-i A) = cos A - i sin A
-  double *P = B, *Q = B2;
+  double *P = M0, *Q = M1;
   for (int it = 0; it < sq; ++it) {  // U <- U U (complex)
     __syncthreads();  // the previous products have read P, Q (and X)
     tile_store(P, ur, g), tile_store(Q, ui, g);
