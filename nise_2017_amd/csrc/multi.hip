@@ -97,8 +97,9 @@ constexpr int kExpmMaxN = 64;
 // B and the 4 row groups of a wave read rows in different banks.
 struct ExpmTile {
   int N, L, ry, cx;
-  __device__ bool in(int i, int j) const { return ry + 16 * i < N && cx + 16 * j < N; }
-  __device__ int at(int i, int j) const { return (ry + 16 * i) * L + cx + 16 * j; }
+  int S;  // stride between a thread's rows / columns (16 for 256 threads, 8 for 64)
+  __device__ bool in(int i, int j) const { return ry + S * i < N && cx + S * j < N; }
+  __device__ int at(int i, int j) const { return (ry + S * i) * L + cx + S * j; }
 };
 
 // q = P B for the thread's elements.
@@ -110,9 +111,9 @@ __device__ __forceinline__ void tile_mm(const double *P, const double *B, const 
   for (int c = 0; c < g.N; ++c) {
     double pr[4], bc[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) pr[i] = g.ry + 16 * i < g.N ? P[(g.ry + 16 * i) * g.L + c] : 0.0;
+    for (int i = 0; i < 4; ++i) pr[i] = g.ry + g.S * i < g.N ? P[(g.ry + g.S * i) * g.L + c] : 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) bc[j] = g.cx + 16 * j < g.N ? B[c * g.L + g.cx + 16 * j] : 0.0;
+    for (int j = 0; j < 4; ++j) bc[j] = g.cx + g.S * j < g.N ? B[c * g.L + g.cx + g.S * j] : 0.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -167,7 +168,7 @@ __device__ void trig_poly(const double Bv[4][4], const double *B2, double *X, co
       for (int l = 0; l < 4; ++l) {
         const bool in = g.in(i, l);
         const int e = in ? g.at(i, l) : 0;
-        Y[i][l] = (g.ry + 16 * i == g.cx + 16 * l ? a0 : 0.0) + (in ? a1 * Bv[i][l] + a2 * B2[e] : 0.0);
+        Y[i][l] = (g.ry + g.S * i == g.cx + g.S * l ? a0 : 0.0) + (in ? a1 * Bv[i][l] + a2 * B2[e] : 0.0);
       }
   };
   chunk(J, 1, R);
@@ -201,6 +202,9 @@ __host__ __device__ inline int64_t expm_per(int N, int pairs) {
 }
 __host__ __device__ inline size_t expm_smem(int N) { return (size_t)2 * N * (N + 1) * 8 + 64; }
 // U of frame f into Um (the frame's block of U), xs: expm_smem(N) bytes.
+// NT threads: 256 (a 16 x 16 grid, N <= 64) or 64 (8 x 8, N <= 32); each
+// thread holds the elements (ry + S i, cx + S j), i, j < 4.
+template <int NT>
 __device__ __forceinline__ void expm_frame(const FrameSource &src, int N, int64_t f, float fac, float thres,
                                            int pairs, float2 *Um, double *xs) {
   // Two N x N matrices in LDS (M0, M1): A, then B = A^2 (the thread's
@@ -210,7 +214,8 @@ __device__ __forceinline__ void expm_frame(const FrameSource &src, int N, int64_
   const int t = threadIdx.x, L = N + 1, NL = N * L;
   double *M0 = xs, *M1 = M0 + NL;
   double *red = M1 + NL;  // [4]
-  const ExpmTile g{N, L, t >> 4, t & 15};
+  constexpr int S = NT == 64 ? 8 : 16;
+  const ExpmTile g{N, L, t / S, t % S, S};
   // M[a][b] = (H_ab fac) mul
   auto fill = [&](double *M, double mul) {
     if (!src.onfly) {
@@ -218,26 +223,26 @@ __device__ __forceinline__ void expm_frame(const FrameSource &src, int N, int64_
       // in flight before the first use (per element, with the on-the-fly
       // coupling code beside it, each load's latency showed), indexes past
       // the matrix clamped to its last element and not stored
-      constexpr int kLd = (kExpmMaxN * kExpmMaxN + 255) / 256;
+      constexpr int kLd = (4 * S) * (4 * S) / NT;
       const float *rec = src.H_rec + f * src.H_stride + 1;
       const float *crec = src.coupling_type ? src.C_rec + 1 : rec;
       float hv[kLd];
 #pragma unroll
       for (int u = 0; u < kLd; ++u) {
-        const int e = min(t + 256 * u, N * N - 1), a = e / N, b = e - a * N;
+        const int e = min(t + NT * u, N * N - 1), a = e / N, b = e - a * N;
         hv[u] = a == b ? gld(rec + (src.coupling_type ? a : sindex(a, a, N)))
                        : gld(crec + sindex(a < b ? a : b, a < b ? b : a, N));
       }
 #pragma unroll
       for (int u = 0; u < kLd; ++u) {
-        const int e = t + 256 * u, a = e / N, b = e - a * N;
+        const int e = t + NT * u, a = e / N, b = e - a * N;
         if (e < N * N) {
           const float h = a == b ? shift_diag(src, a, hv[u]) : (fabsf(hv[u]) < src.cut ? 0.f : hv[u]);
           M[a * L + b] = ((double)h * (double)fac) * mul;
         }
       }
     } else {
-      for (int e = t; e < N * N; e += 256) {
+      for (int e = t; e < N * N; e += NT) {
         const int a = e / N, b = e - a * N;
         const float h = a == b ? site_energy(src, N, f, a) : he_coupling(src, N, f, a < b ? a : b, a < b ? b : a);
         M[a * L + b] = ((double)h * (double)fac) * mul;
@@ -248,7 +253,7 @@ __device__ __forceinline__ void expm_frame(const FrameSource &src, int N, int64_
   __syncthreads();
   // 1-norm (largest column sum of |A|)
   double cm = 0.0;
-  for (int c = t; c < N; c += 256) {
+  for (int c = t; c < N; c += NT) {
     double sum = 0.0;
     for (int r = 0; r < N; ++r) sum += fabs(M0[r * L + c]);
     cm = fmax(cm, sum);
@@ -256,14 +261,15 @@ __device__ __forceinline__ void expm_frame(const FrameSource &src, int N, int64_
   for (int o = 32; o > 0; o >>= 1) cm = fmax(cm, __shfl_xor(cm, o));
   if ((t & 63) == 0) red[t >> 6] = cm;
   __syncthreads();
-  double theta = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  double theta = red[0];
+  for (int w = 1; w < NT / 64; ++w) theta = fmax(theta, red[w]);
   int sq = 0;
   while (theta > 0.5) theta *= 0.5, ++sq;
   int K = 0;
   for (double term = 1.0; term >= 1e-15 && K < 30; ++K) term *= theta / (K + 1);
   const double scale = ldexp(1.0, -sq);
   __syncthreads();  // every thread has read red
-  for (int e = t; e < N * N; e += 256) {
+  for (int e = t; e < N * N; e += NT) {
     const int a = e / N, b = e - a * N;
     M0[a * L + b] *= scale;
   }
@@ -308,25 +314,27 @@ __device__ __forceinline__ void expm_frame(const FrameSource &src, int N, int64_
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (!g.in(i, j)) continue;
-      const int a = g.ry + 16 * i, b = g.cx + 16 * j;
+      const int a = g.ry + S * i, b = g.cx + S * j;
       float2 u = make_float2((float)ur[i][j], (float)ui[i][j]);
       if (thres > 0.f && !(__fadd_rn(__fmul_rn(u.x, u.x), __fmul_rn(u.y, u.y)) > thres)) u = make_float2(0.f, 0.f);
       Um[pairs ? ((b >> 1) * N + a) * 2 + (b & 1) : a * N + b] = u;
     }
   if (pairs && (N & 1))
-    for (int a = t; a < N; a += 256) Um[((Pr - 1) * N + a) * 2 + 1] = make_float2(0.f, 0.f);
+    for (int a = t; a < N; a += NT) Um[((Pr - 1) * N + a) * 2 + 1] = make_float2(0.f, 0.f);
 }
 
 __global__ void __launch_bounds__(256) expm_kernel(FrameSource src, int N, int64_t f_first, float fac, float thres,
                                                    int pairs, float2 *U) {
   extern __shared__ __align__(16) double xs[];
   const int64_t m = blockIdx.x;
-  expm_frame(src, N, f_first + m, fac, thres, pairs, U + m * expm_per(N, pairs), xs);
+  expm_frame<256>(src, N, f_first + m, fac, thres, pairs, U + m * expm_per(N, pairs), xs);
 }
 
 // The frames of several systems in one launch: workgroup b0 + blockIdx.x
-// computes frame b - jobs[j].b of job j, the last job with jobs[j].b <= b.
-__global__ void __launch_bounds__(256) expm_jobs_kernel(const ExpmJob *jobs, int njobs, int64_t b0) {
+// computes frame b - jobs[j].b of job j, the last job with jobs[j].b <= b
+// (NT threads: 64 for N <= 32).
+template <int NT>
+__global__ void __launch_bounds__(NT) expm_jobs_kernel(const ExpmJob *jobs, int njobs, int64_t b0) {
   extern __shared__ __align__(16) double xs[];
   const int64_t b = b0 + blockIdx.x;
   int lo = 0, hi = njobs;
@@ -337,7 +345,7 @@ __global__ void __launch_bounds__(256) expm_jobs_kernel(const ExpmJob *jobs, int
   }
   const ExpmJob J = jobs[lo];
   const int64_t m = b - J.b;
-  expm_frame(J.src, J.N, J.f_first + m, J.fac, J.thres, J.pairs, J.U + m * expm_per(J.N, J.pairs), xs);
+  expm_frame<NT>(J.src, J.N, J.f_first + m, J.fac, J.thres, J.pairs, J.U + m * expm_per(J.N, J.pairs), xs);
 }
 
 int launch_expm_tables(const FrameSource &src, int N, int64_t f_first, int64_t nfr, float fac, float thres,
@@ -385,13 +393,17 @@ int launch_expm_jobs(std::vector<ExpmJob> &jobs, std::vector<ExpmJob> &host, Dev
   for (size_t k = 0; k + 1 < cls.size(); ++k) {
     const int j0 = cls[k].second, nj = cls[k + 1].second - j0;
     const size_t smem = expm_smem(16 * (cls[k].first + 1));
-    NISE_HIP_CHECK(hipFuncSetAttribute((const void *)expm_jobs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)smem));
+    // N <= 32: one wave per frame (eight frames per CU instead of two at
+    // the kernel's register count)
+    const bool one_wave = cls[k].first <= 1;
+    const void *fn = one_wave ? (const void *)expm_jobs_kernel<64> : (const void *)expm_jobs_kernel<256>;
+    NISE_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     const int64_t nb = host[j0 + nj - 1].b + host[j0 + nj - 1].nfr;
     for (int64_t d0 = 0; d0 < nb; d0 += 65535) {
-      hipLaunchKernelGGL(expm_jobs_kernel, dim3((unsigned)std::min<int64_t>(65535, nb - d0)), dim3(256), smem, st,
-                         (const ExpmJob *)dev.as<ExpmJob>() + j0, nj, d0);
-      NISE_HIP_CHECK(hipGetLastError());
+      const ExpmJob *jd = (const ExpmJob *)dev.as<ExpmJob>() + j0;
+      void *kargs[] = {&jd, (void *)&nj, &d0};
+      NISE_HIP_CHECK(hipLaunchKernel(fn, dim3((unsigned)std::min<int64_t>(65535, nb - d0)), dim3(one_wave ? 64 : 256),
+                                     kargs, smem, st));
     }
   }
   return NISE_OK;
