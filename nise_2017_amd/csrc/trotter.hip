@@ -37,6 +37,30 @@ namespace nise {
 // kMaskLanes, ..., kMaskInflight of them in flight, and stops at the first
 // kept coupling.  The lanes' flags are ORed in LDS.
 constexpr int kMaskPairs = 32, kMaskLanes = 8, kMaskInflight = 8;
+// (a, b) of the lexicographic pair index p (a < b < N): the row from the
+// quadratic's root, then corrected in integers (row a starts at a (2N - a -
+// 1) / 2).  A walk along the rows took up to N steps per thread.
+__device__ __forceinline__ void pair_of(int64_t p, int N, int &a, int &b) {
+  auto start = [&](int64_t r) { return r * (2 * (int64_t)N - r - 1) / 2; };
+  const double c = 2.0 * N - 1.0;
+  int64_t r = (int64_t)((c - sqrt(fmax(c * c - 8.0 * (double)p, 0.0))) * 0.5);
+  r = r < 0 ? 0 : r > N - 2 ? N - 2 : r;
+  while (r > 0 && start(r) > p) --r;
+  while (r < N - 2 && start(r + 1) <= p) ++r;
+  a = (int)r;
+  b = a + 1 + (int)(p - start(r));
+}
+
+// Static couplings (one record): one thread per pair.
+__global__ void __launch_bounds__(256) union_mask_static_kernel(FrameSource s, int N, int64_t f_lo, uint8_t *mask) {
+  const int64_t npair = (int64_t)N * (N - 1) / 2;
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npair) return;
+  int a, b;
+  pair_of(p, N, a, b);
+  mask[p] = (uint8_t)(kept_coupling(s, N, f_lo, a, b) != 0.f);
+}
+
 __global__ void __launch_bounds__(256) union_mask_kernel(FrameSource s, int N, int64_t f_lo, int64_t nfr,
                                                          uint8_t *mask) {
   __shared__ uint8_t kk[kMaskLanes][kMaskPairs];
@@ -45,11 +69,8 @@ __global__ void __launch_bounds__(256) union_mask_kernel(FrameSource s, int N, i
   const int64_t p = (int64_t)blockIdx.x * kMaskPairs + tp;
   uint8_t keep = 0;
   if (p < npair) {
-    // invert the lexicographic pair index p -> (a, b)
-    int a = 0;
-    int64_t rem = p;
-    while (rem >= N - 1 - a) { rem -= N - 1 - a; ++a; }
-    const int b = a + 1 + (int)rem;
+    int a, b;
+    pair_of(p, N, a, b);
     const int64_t nf = s.coupling_type ? 1 : nfr;
     for (int64_t f0 = fl; f0 < nf && !keep; f0 += (int64_t)kMaskLanes * kMaskInflight) {
       float v[kMaskInflight];
@@ -75,6 +96,12 @@ int launch_union_mask(const FrameSource &src, int N, int64_t f_lo, int64_t nfr, 
                       hipStream_t st) {
   int64_t npair = (int64_t)N * (N - 1) / 2;
   if (npair == 0) return NISE_OK;
+  if (src.coupling_type) {
+    hipLaunchKernelGGL(union_mask_static_kernel, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, st, src, N, f_lo,
+                       mask_dev);
+    NISE_HIP_CHECK(hipGetLastError());
+    return NISE_OK;
+  }
   dim3 grid((unsigned)((npair + kMaskPairs - 1) / kMaskPairs));
   hipLaunchKernelGGL(union_mask_kernel, grid, dim3(256), 0, st, src, N, f_lo, nfr, mask_dev);
   NISE_HIP_CHECK(hipGetLastError());
